@@ -1,0 +1,171 @@
+/*
+ * gfm_hip.h -- C ABI of the MI355X (gfx950) implementation of the
+ * PhD-2D-GFM per-timestep hot path.
+ *
+ * Plain C types only.  Every entry point returns 0 on success or a negative
+ * GFM_E_* code; numerical faults found on the device are reported per
+ * instance through gfm_status() (GFM_ST_* bits), mirroring the reference's
+ * live asserts and its "Exact RS returned NaN" message.
+ *
+ * Host grids passed through this ABI use the reference's grideuler2type
+ * layout flattened row-major: (Ny+2g) rows x (Nx+2g) cells x 4 doubles,
+ * cell (i,j) component k at [((i*(Nx+2g))+j)*4+k]; i is the y (row) index.
+ * phi is (Ny+2g) x (Nx+2g) doubles; the velocity field is x2.
+ *
+ * A context holds one batch of instances that share the methods (flux
+ * solver, GFM variant, level-set order and reinitialisation schedule,
+ * precision).  Instances may differ in grid size, geometry, EOS, BCs, CFL,
+ * end time and step budget; each keeps its own dt, t and step count on the
+ * device, so a whole step is launched without host synchronisation and can
+ * be captured in one hipGraph.
+ */
+#ifndef GFM_HIP_H
+#define GFM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* boundary conditions (sim_info::BC_L/BC_R/BC_B/BC_T strings) */
+enum { GFM_BC_TRANSMISSIVE = 0, GFM_BC_REFLECTIVE = 1, GFM_BC_PERIODIC = 2 };
+/* sim types (GFM_settingsfile::sim_type) */
+enum { GFM_SIM_ONEFLUID = 0, GFM_SIM_TWOFLUID = 1 };
+/* flux solvers (GFM_settingsfile::FS) */
+enum { GFM_FS_MUSCL = 0, GFM_FS_GODUNOV = 1 };
+/* ghost fluid methods (GFM_settingsfile::GFM) */
+enum { GFM_ORIGINAL = 0, GFM_REAL = 1, GFM_RIEMANN = 2, GFM_MODIFIED = 3 };
+/* precision */
+enum { GFM_FP64 = 0, GFM_MIXED = 1 };
+
+/* error codes */
+enum {
+    GFM_OK = 0,
+    GFM_E_ARG = -1,
+    GFM_E_HIP = -2,
+    GFM_E_STATE = -3,
+    GFM_E_NOMEM = -4
+};
+
+/* device fault bits, per instance */
+enum {
+    GFM_ST_UNPHYSICAL = 1,      /* an assert of the reference would fire      */
+    GFM_ST_RS_FAIL = 2,         /* exact RS: no convergence, vacuum or NaN    */
+    GFM_ST_IFACE_OVERFLOW = 4,  /* interfacial list larger than its capacity  */
+    GFM_ST_IFACE_ROUNDS = 8     /* interfacial dependency rounds not drained  */
+};
+
+/* sim_info (external PhD-Common sim_info.hpp; fields used at
+ * sim_twofluid.cpp:343-360 and euler_bc.hpp:5-103) */
+typedef struct {
+    int Nx, Ny, numGC;
+    double x0, y0, dx, dy;
+    int BC_L, BC_R, BC_B, BC_T;
+} gfm_sim_info;
+
+/* binarySGparams (sim_twofluid.cpp:347-350) */
+typedef struct {
+    double gamma1, pinf1, gamma2, pinf2;
+} gfm_eos;
+
+/* methods shared by a context (sim_twofluid::set_sim_methods,
+ * sim_twofluid.cpp:495-594, plus the precision / reinitialisation knobs) */
+typedef struct {
+    int sim_type;      /* GFM_SIM_*                                  */
+    int fs;            /* GFM_FS_*                                   */
+    int gfm;           /* GFM_* (two-fluid only)                     */
+    int ls_order;      /* 1,3,5,7,9: HOUC order ("LS1".."LS9")       */
+    int reinit_every;  /* reinitialise phi every N steps (0 = never) */
+    int reinit_iters;  /* pseudo-time iterations per reinit          */
+    int precision;     /* GFM_FP64 or GFM_MIXED                      */
+} gfm_methods;
+
+typedef struct gfm_ctx gfm_ctx;
+
+/* context lifetime: create, add instances, commit (allocates), use, destroy */
+int gfm_ctx_create(const gfm_methods* methods, gfm_ctx** out);
+int gfm_ctx_destroy(gfm_ctx* ctx);
+
+/* sim_twofluid::run_sim set-up (sim_twofluid.cpp:27-45): one simulation.
+ * cfl and T as in GFM_settingsfile::CFL and sim_info::T; max_steps bounds
+ * the step loop in addition to t < T. */
+int gfm_instance_add(gfm_ctx* ctx, const gfm_sim_info* info, const gfm_eos* eos,
+                     double cfl, double T, int max_steps, int* inst_id);
+int gfm_ctx_commit(gfm_ctx* ctx);
+int gfm_num_instances(const gfm_ctx* ctx);
+
+/* state transfer (host FP64, reference layout).  grid2 and phi are ignored
+ * for one-fluid contexts and may be NULL. */
+int gfm_upload_state(gfm_ctx* ctx, int inst, const double* grid1, const double* grid2,
+                     const double* phi);
+int gfm_download_state(gfm_ctx* ctx, int inst, double* grid1, double* grid2, double* phi);
+int gfm_upload_vfield(gfm_ctx* ctx, int inst, const double* vfield);
+int gfm_download_vfield(gfm_ctx* ctx, int inst, double* vfield);
+
+/* time bookkeeping (t, step count, last dt) */
+int gfm_set_time(gfm_ctx* ctx, int inst, double t, int nstep);
+int gfm_get_time(gfm_ctx* ctx, int inst, double* t, int* nstep, double* dt);
+int gfm_set_dt(gfm_ctx* ctx, int inst, double dt);
+
+/* step phases over all instances, stream ordered, no host synchronisation.
+ * They follow the reference loop body sim_twofluid.cpp:64-88. */
+int gfm_compute_dt(gfm_ctx* ctx);             /* sim_twofluid::compute_dt :108-157, + mass sums */
+int gfm_set_ghost_states(gfm_ctx* ctx);       /* GFM_base::set_ghost_states, GFM_base.hpp:37-47 */
+int gfm_fs_update(gfm_ctx* ctx);              /* FS_base::pure_fluid_update, FS_base.hpp:23-32 (both fluids) */
+int gfm_ls_update(gfm_ctx* ctx);              /* ls->update_interface_coupled, sim_twofluid.cpp:72 (+ reinit) */
+int gfm_apply_bcs(gfm_ctx* ctx);              /* apply_BCs_euler, euler_bc.hpp:5-103, then t += dt */
+int gfm_extrapolate(gfm_ctx* ctx, int N);     /* extrapolate_vector, extrapolate_vector.hpp:8-15,
+                                                 on grids holding primitives */
+int gfm_ls_reinit(gfm_ctx* ctx, int iters);   /* reinitialise phi now */
+int gfm_get_bbox(gfm_ctx* ctx, int inst, int* bbox1, int* bbox2);   /* realcells1/2 */
+
+/* whole steps: the loop body above, nsteps times.  gfm_step launches
+ * eagerly; gfm_step_graph captures one step as a hipGraph and replays it. */
+int gfm_step(gfm_ctx* ctx, int nsteps);
+int gfm_step_graph(gfm_ctx* ctx, int nsteps);
+int gfm_sync(gfm_ctx* ctx);
+/* nsteps eager steps with HIP events around each phase on the context's
+ * stream; ms[0..4] receive the summed milliseconds of dt, ghost fill, flux
+ * update, level set and BCs (no counterpart in the reference, which only
+ * times the whole loop with omp_get_wtime, sim_twofluid.cpp:62,90) */
+int gfm_profile_steps(gfm_ctx* ctx, int nsteps, double* ms);
+
+int gfm_status(gfm_ctx* ctx, int inst, unsigned* flags);
+/* fluid masses after the last gfm_compute_dt: sim_twofluid::fluid_mass
+ * (sim_twofluid.cpp:1803-1827) for fluid 1 (phi<0) and fluid 2 (phi>0) */
+int gfm_mass_sums(gfm_ctx* ctx, int inst, double* out2);
+/* number of interfacial cells and dependency rounds of the last ghost fill */
+int gfm_iface_info(gfm_ctx* ctx, int inst, int* ncells, int* rounds);
+
+/* Riemann solvers over n faces / interfaces, host pointers, AoS x4.
+ * pure_RS_base::solve_RP (pure_RS_base.hpp:13-20): conserved L/R -> flux.
+ * mixed_RS_base::solve_mixed_RP (mixed_RS_base.hpp:12-24): conserved L/R
+ * -> (p*, u*, rho*_L, rho*_R). */
+int gfm_pure_rs_solve(int n, int paralleltox, double gamma, double pinf,
+                      const double* UL, const double* UR, double* flux, unsigned* status);
+int gfm_mixed_rs_solve(int n, double gammaL, double pinfL, double gammaR, double pinfR,
+                       const double* UL, const double* UR, double* out4, unsigned* status);
+
+/* initial conditions of the settings-file test cases (host code, no GPU):
+ * sim_onefluid::set_sim_ICs (sim_onefluid.cpp:196-422; test_case 1..5 =
+ * TTC1..5_x, 11..15 = TTC1..5_y, 100 = circularexplosion) and
+ * sim_twofluid::set_sim_ICs (shocked bubble :774-859, RMI :1245-1347,
+ * planar tubes :675-742), plus the Rankine-Hugoniot post-shock state. */
+int gfm_ic_onefluid(const gfm_sim_info* info, const gfm_eos* eos, int test_case, double* grid);
+int gfm_ic_bubble(const gfm_sim_info* info, const gfm_eos* eos, double xc, double yc, double R,
+                  const double* bubble_prims, const double* pre_prims, const double* post_prims,
+                  double shock_x, double* grid1, double* grid2, double* phi);
+int gfm_ic_rmi(const gfm_sim_info* info, const gfm_eos* eos, double x_if, double amp, int modes,
+               const double* light_prims, const double* post_prims, const double* heavy_prims,
+               double shock_x, double* grid1, double* grid2, double* phi);
+int gfm_ic_planar(const gfm_sim_info* info, const gfm_eos* eos, int axis, double pos,
+                  const double* prims1, const double* prims2, double* grid1, double* grid2, double* phi);
+int gfm_shock_state(double gamma, double pinf, const double* pre_prims, double mach, int direction,
+                    double* post_prims);
+
+/* last HIP error string (thread local) */
+const char* gfm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,793 @@
+// gfm_capi.hip -- host side of the C ABI declared in include/gfm_hip.h:
+// context and instance management, device memory layout, work tables,
+// step sequencing and hipGraph capture.  No CPU fallback: every numerical
+// phase runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gfm_hip.h"
+#include "gfm_kernels.hpp"
+
+using gfm::Inst;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char* what, hipError_t e) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return GFM_E_HIP;
+}
+int fail_arg(const char* what) {
+    g_err = what;
+    return GFM_E_ARG;
+}
+
+#define HC(call)                                   \
+    do {                                           \
+        hipError_t e_ = (call);                    \
+        if (e_ != hipSuccess) return fail(#call, e_); \
+    } while (0)
+
+struct InstHost {
+    gfm_sim_info info;
+    gfm_eos eos;
+    double cfl, T;
+    int max_steps;
+    void* mem = nullptr;   // one allocation per instance
+    Inst d;                // descriptor as uploaded (device pointers)
+};
+
+void houc_coeffs(int order, gfm::Houc& H) {
+    static const int c1[] = {-1, 1};
+    static const int c3[] = {1, -6, 3, 2};
+    static const int c5[] = {-2, 15, -60, 20, 30, -3};
+    static const int c7[] = {3, -28, 126, -420, 105, 252, -42, 4};
+    static const int c9[] = {-4, 45, -240, 840, -2520, 504, 1680, -360, 60, -5};
+    const int* src = c5;
+    double den = 60.0;
+    switch (order) {
+        case 1: src = c1; den = 1.0; break;
+        case 3: src = c3; den = 6.0; break;
+        case 7: src = c7; den = 420.0; break;
+        case 9: src = c9; den = 2520.0; break;
+        default: order = 5; break;
+    }
+    H.n = order + 1;
+    for (int m = 0; m < H.n; ++m) H.c[m] = src[m] / den;
+    for (int m = H.n; m < 10; ++m) H.c[m] = 0.0;
+    H.o0 = -(H.n / 2);
+}
+
+size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+}  // namespace
+
+struct gfm_ctx {
+    gfm_methods m;
+    std::vector<InstHost> inst;
+    bool committed = false;
+    hipStream_t stream = nullptr;
+    Inst* d_insts = nullptr;
+    int4* d_work = nullptr;
+    gfm::LaunchCtx L{};
+    gfm::Houc houc{};
+    int cur = 0;                       // current conserved buffer (one-fluid step path)
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    bool mixed() const { return m.precision == GFM_MIXED; }
+    bool twofluid() const { return m.sim_type == GFM_SIM_TWOFLUID; }
+    int nfluid() const { return twofluid() ? 2 : 1; }
+    size_t es() const { return mixed() ? 4 : 8; }
+};
+
+namespace {
+
+template <typename R>
+void dispatch_dt(gfm_ctx* c, int q) { gfm::launch_dt<R>(c->L, c->twofluid() ? 1 : 0, q); }
+
+// N Jacobi iterations of the extrapolation.  Source: conserved Q[.][qin]
+// (from_q) or primitives P[.][pstart]; destination: conserved Q[.][qout]
+// with W->U and the velocity field (to_q) or the P ping-pong.  Returns the
+// P buffer holding the result when !to_q.
+template <typename R>
+int extrap_chain(gfm_ctx* c, int N, bool from_q, int qin, int pstart, bool inject_first, bool to_q, int qout,
+                 int last_extra) {
+    int cur_p = pstart;
+    for (int k = 0; k < N; ++k) {
+        int mode = 0, bin, bout;
+        if (k == 0) {
+            bin = from_q ? qin : pstart;
+            if (from_q) mode |= 1;            // MF_FIRST
+            if (inject_first) mode |= 2;      // MF_INJECT
+        } else {
+            bin = cur_p;
+        }
+        if (k == N - 1 && to_q) {
+            mode |= 4 | last_extra;           // MF_LAST (| MF_OGFM)
+            bout = qout;
+        } else {
+            bout = (k == 0 && from_q) ? 0 : 1 - cur_p;
+            cur_p = bout;
+        }
+        gfm::launch_extrap<R>(c->L, mode, bin, bout);
+    }
+    return cur_p;
+}
+
+// GFM_base::set_ghost_states for the context's variant: Q[.][0] -> Q[.][1]
+template <typename R>
+void ghost_chain(gfm_ctx* c) {
+    switch (c->m.gfm) {
+        case GFM_ORIGINAL:   // GFM_original.cpp:34-165
+            extrap_chain<R>(c, 6, true, 0, 0, false, true, 1, 8);
+            break;
+        case GFM_REAL: {     // GFM_real.cpp:32-247
+            gfm::launch_iface<R>(c->L, gfm::V_REAL, 0);
+            extrap_chain<R>(c, 6, true, 0, 0, true, true, 1, 0);
+            break;
+        }
+        case GFM_RIEMANN: {  // GFM_riemann.cpp:33-187
+            const int p = extrap_chain<R>(c, 2, true, 0, 0, false, false, 0, 0);
+            gfm::launch_iface<R>(c->L, gfm::V_RIEMANN, p);
+            extrap_chain<R>(c, 6, false, 0, p, true, true, 1, 0);
+            break;
+        }
+        case GFM_MODIFIED: { // GFM_modified.cpp:37-239
+            const int p = extrap_chain<R>(c, 2, true, 0, 0, false, false, 0, 0);
+            gfm::launch_iface<R>(c->L, gfm::V_MODIFIED, p);
+            gfm::launch_mgfm_final<R>(c->L, 0, 1);
+            break;
+        }
+        default: break;
+    }
+}
+
+// one step of sim_twofluid::run_sim (sim_twofluid.cpp:64-88) or
+// sim_onefluid::run_sim (sim_onefluid.cpp:40-54)
+template <typename R>
+void step_launches(gfm_ctx* c, int cur) {
+    const bool muscl = c->m.fs == GFM_FS_MUSCL;
+    if (c->twofluid()) {
+        gfm::launch_dt<R>(c->L, 1, 0);
+        ghost_chain<R>(c);
+        gfm::launch_fs<R>(c->L, muscl, 1, 0);
+        gfm::launch_ls<R>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
+        gfm::launch_bc<R>(c->L, 0, 2, 1);
+    } else {
+        gfm::launch_dt<R>(c->L, 0, cur);
+        gfm::launch_fs<R>(c->L, muscl, cur, 1 - cur);
+        gfm::launch_bc<R>(c->L, 1 - cur, 1, 1);
+    }
+}
+
+int copy_q(gfm_ctx* c, int from, int to) {
+    for (auto& h : c->inst)
+        for (int f = 0; f < c->nfluid(); ++f)
+            HC(hipMemcpyAsync(h.d.Q[f][to], h.d.Q[f][from], 4 * h.d.plane * c->es(), hipMemcpyDeviceToDevice,
+                              c->stream));
+    return GFM_OK;
+}
+
+// XCD-aware order: MI355X dispatches workgroup k to XCD k % 8, so entry p
+// takes tile (p % 8) * ceil(n/8) + p / 8 and each XCD walks a contiguous run
+// of tiles whose halos share its L2.
+void xcd_order(std::vector<int4>& v, size_t begin) {
+    const size_t n = v.size() - begin;
+    if (n < 16) return;
+    const size_t per = (n + 7) / 8;
+    std::vector<int4> src(v.begin() + begin, v.end()), out;
+    out.reserve(n);
+    for (size_t p = 0; out.size() < n; ++p) {
+        const size_t t = (p % 8) * per + p / 8;
+        if (t < n) out.push_back(src[t]);
+        if (p > 16 * n) break;
+    }
+    std::copy(out.begin(), out.end(), v.begin() + begin);
+}
+
+}  // namespace
+
+static int check_inst(gfm_ctx* c, int inst) {
+    if (!c) return fail_arg("null ctx");
+    if (!c->committed) return fail_arg("context not committed");
+    if (inst < 0 || inst >= (int)c->inst.size()) return fail_arg("instance out of range");
+    return GFM_OK;
+}
+
+template <typename R>
+static int upload_planes(gfm_ctx* c, const InstHost& h, void* dst, const double* src, int ncomp) {
+    const Inst& d = h.d;
+    std::vector<R> buf((size_t)ncomp * d.plane, R(0));
+    for (int i = 0; i < d.H; ++i)
+        for (int j = 0; j < d.W; ++j)
+            for (int k = 0; k < ncomp; ++k)
+                buf[(size_t)k * d.plane + (size_t)i * d.pitch + j] = (R)src[((size_t)i * d.W + j) * ncomp + k];
+    HC(hipMemcpyAsync(dst, buf.data(), buf.size() * sizeof(R), hipMemcpyHostToDevice, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return GFM_OK;
+}
+
+template <typename R>
+static int download_planes(gfm_ctx* c, const InstHost& h, const void* src, double* dst, int ncomp) {
+    const Inst& d = h.d;
+    std::vector<R> buf((size_t)ncomp * d.plane);
+    HC(hipMemcpyAsync(buf.data(), src, buf.size() * sizeof(R), hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < d.H; ++i)
+        for (int j = 0; j < d.W; ++j)
+            for (int k = 0; k < ncomp; ++k)
+                dst[((size_t)i * d.W + j) * ncomp + k] = (double)buf[(size_t)k * d.plane + (size_t)i * d.pitch + j];
+    return GFM_OK;
+}
+
+static int xfer(gfm_ctx* c, int inst, void* dev, double* host, int ncomp, bool up) {
+    const InstHost& h = c->inst[inst];
+    if (c->mixed())
+        return up ? upload_planes<float>(c, h, dev, host, ncomp) : download_planes<float>(c, h, dev, host, ncomp);
+    return up ? upload_planes<double>(c, h, dev, host, ncomp) : download_planes<double>(c, h, dev, host, ncomp);
+}
+
+static int read_si(gfm_ctx* c, int inst, int* si) {
+    HC(hipMemcpyAsync(si, c->inst[inst].d.si, gfm::SI_N * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return GFM_OK;
+}
+
+static int ready(gfm_ctx* c) {
+    if (!c) return fail_arg("null ctx");
+    if (!c->committed) return fail_arg("context not committed");
+    return GFM_OK;
+}
+
+static int post_launch(gfm_ctx* c) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail("kernel launch", e);
+    return GFM_OK;
+}
+
+static int rs_common(int n, const double* UL, const double* UR, double* out, unsigned* status, bool mixed,
+                     int px, double a, double b, double cc, double d) {
+    if (n < 0 || (n > 0 && (!UL || !UR || !out))) return fail_arg("bad arguments");
+    if (n == 0) { if (status) *status = 0; return GFM_OK; }
+    double *dL = nullptr, *dR = nullptr, *dO = nullptr;
+    unsigned* dS = nullptr;
+    const size_t bytes = (size_t)n * 4 * sizeof(double);
+    HC(hipMalloc((void**)&dL, bytes));
+    HC(hipMalloc((void**)&dR, bytes));
+    HC(hipMalloc((void**)&dO, bytes));
+    HC(hipMalloc((void**)&dS, sizeof(unsigned)));
+    HC(hipMemcpy(dL, UL, bytes, hipMemcpyHostToDevice));
+    HC(hipMemcpy(dR, UR, bytes, hipMemcpyHostToDevice));
+    HC(hipMemset(dS, 0, sizeof(unsigned)));
+    if (mixed) gfm::launch_mixed_rs(nullptr, n, a, b, cc, d, dL, dR, dO, dS);
+    else gfm::launch_pure_rs(nullptr, n, px, a, b, dL, dR, dO, dS);
+    HC(hipGetLastError());
+    HC(hipDeviceSynchronize());
+    unsigned st = 0;
+    HC(hipMemcpy(out, dO, bytes, hipMemcpyDeviceToHost));
+    HC(hipMemcpy(&st, dS, sizeof(unsigned), hipMemcpyDeviceToHost));
+    hipFree(dL); hipFree(dR); hipFree(dO); hipFree(dS);
+    if (status) *status = st;
+    return GFM_OK;
+}
+
+extern "C" {
+
+const char* gfm_last_error(void) { return g_err.c_str(); }
+
+int gfm_ctx_create(const gfm_methods* methods, gfm_ctx** out) {
+    if (!methods || !out) return fail_arg("null argument");
+    if (methods->sim_type != GFM_SIM_ONEFLUID && methods->sim_type != GFM_SIM_TWOFLUID) return fail_arg("sim_type");
+    if (methods->fs != GFM_FS_MUSCL && methods->fs != GFM_FS_GODUNOV) return fail_arg("fs");
+    if (methods->gfm < GFM_ORIGINAL || methods->gfm > GFM_MODIFIED) return fail_arg("gfm");
+    if (methods->precision != GFM_FP64 && methods->precision != GFM_MIXED) return fail_arg("precision");
+    const int o = methods->ls_order;
+    if (o != 1 && o != 3 && o != 5 && o != 7 && o != 9) return fail_arg("ls_order must be 1,3,5,7 or 9");
+    if (methods->reinit_every < 0 || methods->reinit_iters < 0) return fail_arg("reinit");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail("hipGetDeviceCount (no GPU)", e == hipSuccess ? hipErrorNoDevice : e);
+    gfm_ctx* c = new gfm_ctx();
+    c->m = *methods;
+    houc_coeffs(o, c->houc);
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return fail("hipStreamCreate", e); }
+    *out = c;
+    return GFM_OK;
+}
+
+int gfm_ctx_destroy(gfm_ctx* c) {
+    if (!c) return GFM_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& g : c->gexec)
+        if (g) hipGraphExecDestroy(g);
+    for (auto& h : c->inst)
+        if (h.mem) hipFree(h.mem);
+    if (c->d_insts) hipFree(c->d_insts);
+    if (c->d_work) hipFree(c->d_work);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return GFM_OK;
+}
+
+int gfm_instance_add(gfm_ctx* c, const gfm_sim_info* info, const gfm_eos* eos, double cfl, double T, int max_steps,
+                     int* inst_id) {
+    if (!c || !info || !eos) return fail_arg("null argument");
+    if (c->committed) return fail_arg("context already committed");
+    if (info->Nx < 1 || info->Ny < 1) return fail_arg("Nx, Ny must be positive");
+    const int gmin = c->twofluid() ? 6 : 2;   // MUSCL stencil 2; HOUC-9 stencil 5 + normal 1
+    if (info->numGC < gmin) return fail_arg("numGC too small for the stencils");
+    if (c->twofluid() && info->numGC < (c->m.ls_order + 1) / 2 + 1) return fail_arg("numGC too small for HOUC order");
+    if (!(info->dx > 0.0) || !(info->dy > 0.0)) return fail_arg("dx, dy must be positive");
+    const int bcs[4] = {info->BC_L, info->BC_R, info->BC_B, info->BC_T};
+    for (int b : bcs)
+        if (b < GFM_BC_TRANSMISSIVE || b > GFM_BC_PERIODIC) return fail_arg("bad BC");
+    if ((info->BC_L == GFM_BC_PERIODIC) != (info->BC_R == GFM_BC_PERIODIC)) return fail_arg("periodic BCs come in L/R pairs");
+    if ((info->BC_B == GFM_BC_PERIODIC) != (info->BC_T == GFM_BC_PERIODIC)) return fail_arg("periodic BCs come in B/T pairs");
+    if (info->BC_L == GFM_BC_PERIODIC && info->Nx < info->numGC) return fail_arg("periodic x needs Nx >= numGC");
+    if (info->BC_B == GFM_BC_PERIODIC && info->Ny < info->numGC) return fail_arg("periodic y needs Ny >= numGC");
+    if (info->Nx < info->numGC || info->Ny < info->numGC) return fail_arg("mirror BCs need Nx, Ny >= numGC");
+    InstHost h;
+    h.info = *info;
+    h.eos = *eos;
+    h.cfl = cfl;
+    h.T = T;
+    h.max_steps = max_steps;
+    c->inst.push_back(h);
+    if (inst_id) *inst_id = (int)c->inst.size() - 1;
+    return GFM_OK;
+}
+
+int gfm_num_instances(const gfm_ctx* c) { return c ? (int)c->inst.size() : 0; }
+
+int gfm_ctx_commit(gfm_ctx* c) {
+    if (!c) return fail_arg("null ctx");
+    if (c->committed) return GFM_OK;
+    if (c->inst.empty()) return fail_arg("no instances");
+    const size_t es = c->es();
+    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst;
+    std::vector<Inst> descs;
+    for (size_t k = 0; k < c->inst.size(); ++k) {
+        InstHost& h = c->inst[k];
+        const gfm_sim_info& s = h.info;
+        Inst d{};
+        d.Nx = s.Nx; d.Ny = s.Ny; d.g = s.numGC;
+        d.W = s.Nx + 2 * s.numGC;
+        d.H = s.Ny + 2 * s.numGC;
+        d.pitch = (int)round_up(d.W, 64);
+        d.plane = (long long)round_up((size_t)d.pitch * d.H, 64);
+        d.x0 = s.x0; d.y0 = s.y0; d.dx = s.dx; d.dy = s.dy;
+        d.bc[0] = s.BC_L; d.bc[1] = s.BC_R; d.bc[2] = s.BC_B; d.bc[3] = s.BC_T;
+        d.gam[0] = h.eos.gamma1; d.pinf[0] = h.eos.pinf1;
+        d.gam[1] = c->twofluid() ? h.eos.gamma2 : h.eos.gamma1;
+        d.pinf[1] = c->twofluid() ? h.eos.pinf2 : h.eos.pinf1;
+        d.cfl = h.cfl; d.T = h.T; d.max_steps = h.max_steps;
+        d.use_bbox = c->twofluid() && c->m.gfm == GFM_ORIGINAL;
+        const int tix = (s.Nx + 63) / 64, tiy = (s.Ny + 3) / 4;
+        d.ntile_dt = tix * tiy;
+        d.icap = c->twofluid() ? std::min(s.Nx * s.Ny, 49152) : 1;
+        // layout of the single allocation
+        const size_t P = (size_t)d.plane;
+        const int nf = c->nfluid();
+        size_t off = 0;
+        auto take = [&](size_t bytes) { size_t o = off; off = round_up(off + bytes, 256); return o; };
+        size_t oQ[2][2] = {{0, 0}, {0, 0}}, oP[2][2] = {{0, 0}, {0, 0}};
+        for (int f = 0; f < nf; ++f)
+            for (int b = 0; b < 2; ++b) oQ[f][b] = take(4 * P * es);
+        size_t ophi = 0, oA = 0, oB = 0, ovel = 0, oimap = 0, oilist = 0, oside = 0;
+        if (c->twofluid()) {
+            for (int f = 0; f < 2; ++f)
+                for (int b = 0; b < 2; ++b) oP[f][b] = take(4 * P * es);
+            ophi = take(P * es); oA = take(P * es); oB = take(P * es);
+            ovel = take(2 * P * es);
+            oimap = take(P * sizeof(int));
+            oilist = take((size_t)d.icap * sizeof(int));
+            oside = take((size_t)d.icap * 8 * sizeof(double));
+        }
+        const size_t opart = take((size_t)d.ntile_dt * 4 * sizeof(double));
+        const size_t osd = take(gfm::SD_N * sizeof(double));
+        const size_t osi = take(gfm::SI_N * sizeof(int));
+        hipError_t e = hipMalloc(&h.mem, off);
+        if (e != hipSuccess) return fail("hipMalloc (instance state)", e);
+        HC(hipMemsetAsync(h.mem, 0, off, c->stream));
+        char* base = (char*)h.mem;
+        for (int f = 0; f < 2; ++f)
+            for (int b = 0; b < 2; ++b) {
+                d.Q[f][b] = base + oQ[f < nf ? f : 0][b];
+                d.P[f][b] = c->twofluid() ? (void*)(base + oP[f][b]) : nullptr;
+            }
+        if (c->twofluid()) {
+            d.phi = base + ophi; d.phiA = base + oA; d.phiB = base + oB; d.vel = base + ovel;
+            d.imap = (int*)(base + oimap); d.ilist = (int*)(base + oilist); d.side = (double*)(base + oside);
+            HC(hipMemsetAsync(d.imap, 0xff, P * sizeof(int), c->stream));   // -1 everywhere
+        }
+        d.partials = (double*)(base + opart);
+        d.sd = (double*)(base + osd);
+        d.si = (int*)(base + osi);
+        h.d = d;
+        descs.push_back(d);
+        // work tables
+        const int ki = (int)k;
+        const int fx = (d.W + 63) / 64, fy = (d.H + 3) / 4;
+        for (int ty = 0; ty < fy; ++ty)
+            for (int tx = 0; tx < fx; ++tx) w_full.push_back(make_int4(ki, ty, tx, 0));
+        int li = 0;
+        for (int ty = 0; ty < tiy; ++ty)
+            for (int tx = 0; tx < tix; ++tx) w_int.push_back(make_int4(ki, ty, tx, li++));
+        const int sx = (s.Nx + 31) / 32, sy = (s.Ny + 7) / 8;
+        for (int f = 0; f < nf; ++f)
+            for (int ty = 0; ty < sy; ++ty)
+                for (int tx = 0; tx < sx; ++tx) w_fs.push_back(make_int4(ki, ty, tx, f));
+        const int strip_n[4] = {d.g * d.W, d.g * d.W, d.g * d.Ny, d.g * d.Ny};
+        for (int st = 0; st < 4; ++st)
+            for (int o = 0; o < strip_n[st]; o += 256) w_frame.push_back(make_int4(ki, st, o, 0));
+        w_inst.push_back(make_int4(ki, 0, 0, 0));
+    }
+    xcd_order(w_full, 0);
+    xcd_order(w_fs, 0);
+    HC(hipMalloc((void**)&c->d_insts, descs.size() * sizeof(Inst)));
+    HC(hipMemcpyAsync(c->d_insts, descs.data(), descs.size() * sizeof(Inst), hipMemcpyHostToDevice, c->stream));
+    const size_t nw = w_full.size() + w_int.size() + w_fs.size() + w_frame.size() + w_inst.size();
+    HC(hipMalloc((void**)&c->d_work, nw * sizeof(int4)));
+    std::vector<int4> all;
+    all.reserve(nw);
+    all.insert(all.end(), w_full.begin(), w_full.end());
+    all.insert(all.end(), w_int.begin(), w_int.end());
+    all.insert(all.end(), w_fs.begin(), w_fs.end());
+    all.insert(all.end(), w_frame.begin(), w_frame.end());
+    all.insert(all.end(), w_inst.begin(), w_inst.end());
+    HC(hipMemcpyAsync(c->d_work, all.data(), nw * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    gfm::LaunchCtx& L = c->L;
+    L.stream = c->stream;
+    L.insts = c->d_insts;
+    size_t o = 0;
+    L.w_full = c->d_work + o; L.n_full = (int)w_full.size(); o += w_full.size();
+    L.w_int = c->d_work + o; L.n_int = (int)w_int.size(); o += w_int.size();
+    L.w_fs = c->d_work + o; L.n_fs = (int)w_fs.size(); o += w_fs.size();
+    L.w_frame = c->d_work + o; L.n_frame = (int)w_frame.size(); o += w_frame.size();
+    L.w_inst = c->d_work + o; L.n_inst = (int)w_inst.size();
+    // initial scalars: t = 0, step 0, active
+    for (auto& h : c->inst) {
+        int si[gfm::SI_N] = {0};
+        si[gfm::SI_ACTIVE] = 1;
+        HC(hipMemcpyAsync(h.d.si, si, sizeof(si), hipMemcpyHostToDevice, c->stream));
+    }
+    HC(hipStreamSynchronize(c->stream));
+    c->committed = true;
+    return GFM_OK;
+}
+
+
+
+
+
+
+
+
+
+int gfm_upload_state(gfm_ctx* c, int inst, const double* g1, const double* g2, const double* phi) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    if (!g1) return fail_arg("grid1 is NULL");
+    InstHost& h = c->inst[inst];
+    const int q = c->twofluid() ? 0 : c->cur;
+    if ((r = xfer(c, inst, h.d.Q[0][q], (double*)g1, 4, true))) return r;
+    if (c->twofluid()) {
+        if (!g2 || !phi) return fail_arg("two-fluid upload needs grid2 and phi");
+        if ((r = xfer(c, inst, h.d.Q[1][0], (double*)g2, 4, true))) return r;
+        if ((r = xfer(c, inst, h.d.phi, (double*)phi, 1, true))) return r;
+    }
+    return GFM_OK;
+}
+
+int gfm_download_state(gfm_ctx* c, int inst, double* g1, double* g2, double* phi) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    InstHost& h = c->inst[inst];
+    const int q = c->twofluid() ? 0 : c->cur;
+    if (g1 && (r = xfer(c, inst, h.d.Q[0][q], g1, 4, false))) return r;
+    if (c->twofluid()) {
+        if (g2 && (r = xfer(c, inst, h.d.Q[1][0], g2, 4, false))) return r;
+        if (phi && (r = xfer(c, inst, h.d.phi, phi, 1, false))) return r;
+    }
+    return GFM_OK;
+}
+
+int gfm_upload_vfield(gfm_ctx* c, int inst, const double* vf) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    if (!c->twofluid() || !vf) return fail_arg("velocity field needs a two-fluid context");
+    return xfer(c, inst, c->inst[inst].d.vel, (double*)vf, 2, true);
+}
+
+int gfm_download_vfield(gfm_ctx* c, int inst, double* vf) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    if (!c->twofluid() || !vf) return fail_arg("velocity field needs a two-fluid context");
+    return xfer(c, inst, c->inst[inst].d.vel, vf, 2, false);
+}
+
+int gfm_set_time(gfm_ctx* c, int inst, double t, int nstep) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    const Inst& d = c->inst[inst].d;
+    HC(hipMemcpyAsync(d.sd + gfm::SD_T, &t, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HC(hipMemcpyAsync(d.si + gfm::SI_NSTEP, &nstep, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return GFM_OK;
+}
+
+int gfm_set_dt(gfm_ctx* c, int inst, double dt) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    const Inst& d = c->inst[inst].d;
+    HC(hipMemcpyAsync(d.sd + gfm::SD_DT, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return GFM_OK;
+}
+
+int gfm_get_time(gfm_ctx* c, int inst, double* t, int* nstep, double* dt) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    const Inst& d = c->inst[inst].d;
+    double sd[gfm::SD_N];
+    int si[gfm::SI_N];
+    HC(hipMemcpyAsync(sd, d.sd, sizeof(sd), hipMemcpyDeviceToHost, c->stream));
+    HC(hipMemcpyAsync(si, d.si, sizeof(si), hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    if (t) *t = sd[gfm::SD_T];
+    if (nstep) *nstep = si[gfm::SI_NSTEP];
+    if (dt) *dt = sd[gfm::SD_DT];
+    return GFM_OK;
+}
+
+
+
+int gfm_status(gfm_ctx* c, int inst, unsigned* flags) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    int si[gfm::SI_N];
+    if ((r = read_si(c, inst, si))) return r;
+    if (flags) *flags = (unsigned)si[gfm::SI_STATUS];
+    return GFM_OK;
+}
+
+int gfm_iface_info(gfm_ctx* c, int inst, int* ncells, int* rounds) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    int si[gfm::SI_N];
+    if ((r = read_si(c, inst, si))) return r;
+    if (ncells) *ncells = si[gfm::SI_NIFACE];
+    if (rounds) *rounds = si[gfm::SI_ROUNDS];
+    return GFM_OK;
+}
+
+int gfm_get_bbox(gfm_ctx* c, int inst, int* b1, int* b2) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    int si[gfm::SI_N];
+    if ((r = read_si(c, inst, si))) return r;
+    const gfm_sim_info& s = c->inst[inst].info;
+    const int g = s.numGC;
+    int full[4] = {g, g + s.Ny, g, g + s.Nx};
+    for (int f = 0; f < 2; ++f) {
+        int* b = f == 0 ? b1 : b2;
+        if (!b) continue;
+        if (c->twofluid() && c->m.gfm == GFM_ORIGINAL) {
+            const int* v = si + gfm::SI_BBOX + 4 * f;
+            b[0] = std::max(v[0], g); b[1] = std::min(v[1], g + s.Ny);
+            b[2] = std::max(v[2], g); b[3] = std::min(v[3], g + s.Nx);
+        } else {
+            std::memcpy(b, full, sizeof(full));
+        }
+    }
+    return GFM_OK;
+}
+
+int gfm_mass_sums(gfm_ctx* c, int inst, double* out2) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    double sd[gfm::SD_N];
+    HC(hipMemcpyAsync(sd, c->inst[inst].d.sd, sizeof(sd), hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    out2[0] = sd[gfm::SD_MASS1];
+    out2[1] = sd[gfm::SD_MASS2];
+    return GFM_OK;
+}
+
+
+
+
+
+int gfm_compute_dt(gfm_ctx* c) {
+    int r = ready(c);
+    if (r) return r;
+    const int q = c->twofluid() ? 0 : c->cur;
+    if (c->mixed()) dispatch_dt<float>(c, q);
+    else dispatch_dt<double>(c, q);
+    return post_launch(c);
+}
+
+int gfm_set_ghost_states(gfm_ctx* c) {
+    int r = ready(c);
+    if (r) return r;
+    if (!c->twofluid()) return fail_arg("ghost states need a two-fluid context");
+    if (c->mixed()) { gfm::launch_reset<float>(c->L); ghost_chain<float>(c); }
+    else { gfm::launch_reset<double>(c->L); ghost_chain<double>(c); }
+    if ((r = post_launch(c))) return r;
+    // keep the phase-by-phase state in buffer 0 (but not the OGFM boxes,
+    // which live in the scalars)
+    return copy_q(c, 1, 0);
+}
+
+int gfm_fs_update(gfm_ctx* c) {
+    int r = ready(c);
+    if (r) return r;
+    const bool muscl = c->m.fs == GFM_FS_MUSCL;
+    const int q = c->twofluid() ? 0 : c->cur;
+    if (c->mixed()) gfm::launch_fs<float>(c->L, muscl, q, 1 - q);
+    else gfm::launch_fs<double>(c->L, muscl, q, 1 - q);
+    if ((r = post_launch(c))) return r;
+    return copy_q(c, 1 - q, q);
+}
+
+int gfm_ls_update(gfm_ctx* c) {
+    int r = ready(c);
+    if (r) return r;
+    if (!c->twofluid()) return fail_arg("level set needs a two-fluid context");
+    if (c->mixed()) gfm::launch_ls<float>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
+    else gfm::launch_ls<double>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
+    return post_launch(c);
+}
+
+int gfm_ls_reinit(gfm_ctx* c, int iters) {
+    int r = ready(c);
+    if (r) return r;
+    if (!c->twofluid()) return fail_arg("level set needs a two-fluid context");
+    if (iters <= 0) return GFM_OK;
+    if (c->mixed()) gfm::launch_reinit_now<float>(c->L, iters);
+    else gfm::launch_reinit_now<double>(c->L, iters);
+    return post_launch(c);
+}
+
+int gfm_apply_bcs(gfm_ctx* c) {
+    int r = ready(c);
+    if (r) return r;
+    const int q = c->twofluid() ? 0 : c->cur;
+    if (c->mixed()) gfm::launch_bc<float>(c->L, q, c->nfluid(), 1);
+    else gfm::launch_bc<double>(c->L, q, c->nfluid(), 1);
+    return post_launch(c);
+}
+
+int gfm_extrapolate(gfm_ctx* c, int N) {
+    int r = ready(c);
+    if (r) return r;
+    if (!c->twofluid()) return fail_arg("extrapolation needs a two-fluid context");
+    if (N <= 0) return GFM_OK;
+    int q = 0;
+    for (int k = 0; k < N; ++k) {
+        if (c->mixed()) { if (k == 0) gfm::launch_reset<float>(c->L); gfm::launch_extrap<float>(c->L, 16, q, 1 - q); }
+        else { if (k == 0) gfm::launch_reset<double>(c->L); gfm::launch_extrap<double>(c->L, 16, q, 1 - q); }
+        q = 1 - q;
+    }
+    if ((r = post_launch(c))) return r;
+    if (q != 0) return copy_q(c, 1, 0);
+    return GFM_OK;
+}
+
+int gfm_step(gfm_ctx* c, int nsteps) {
+    int r = ready(c);
+    if (r) return r;
+    for (int s = 0; s < nsteps; ++s) {
+        if (c->mixed()) step_launches<float>(c, c->cur);
+        else step_launches<double>(c, c->cur);
+        if (!c->twofluid()) c->cur = 1 - c->cur;
+    }
+    return post_launch(c);
+}
+
+int gfm_step_graph(gfm_ctx* c, int nsteps) {
+    int r = ready(c);
+    if (r) return r;
+    const int ng = c->twofluid() ? 1 : 2;
+    for (int k = 0; k < ng; ++k) {
+        if (c->gexec[k]) continue;
+        hipGraph_t g = nullptr;
+        HC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        if (c->mixed()) step_launches<float>(c, k);
+        else step_launches<double>(c, k);
+        HC(hipStreamEndCapture(c->stream, &g));
+        hipError_t e = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return fail("hipGraphInstantiate", e);
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        const int k = c->twofluid() ? 0 : c->cur;
+        HC(hipGraphLaunch(c->gexec[k], c->stream));
+        if (!c->twofluid()) c->cur = 1 - c->cur;
+    }
+    return GFM_OK;
+}
+
+int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
+    int r = ready(c);
+    if (r) return r;
+    if (!ms || nsteps < 0) return fail_arg("bad arguments");
+    for (int k = 0; k < 5; ++k) ms[k] = 0.0;
+    hipEvent_t ev[6];
+    for (auto& e : ev) HC(hipEventCreate(&e));
+    const bool muscl = c->m.fs == GFM_FS_MUSCL;
+    for (int s = 0; s < nsteps; ++s) {
+        const int cur = c->cur;
+        HC(hipEventRecord(ev[0], c->stream));
+        if (c->twofluid()) {
+            if (c->mixed()) {
+                gfm::launch_dt<float>(c->L, 1, 0);
+                HC(hipEventRecord(ev[1], c->stream));
+                ghost_chain<float>(c);
+                HC(hipEventRecord(ev[2], c->stream));
+                gfm::launch_fs<float>(c->L, muscl, 1, 0);
+                HC(hipEventRecord(ev[3], c->stream));
+                gfm::launch_ls<float>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
+                HC(hipEventRecord(ev[4], c->stream));
+                gfm::launch_bc<float>(c->L, 0, 2, 1);
+            } else {
+                gfm::launch_dt<double>(c->L, 1, 0);
+                HC(hipEventRecord(ev[1], c->stream));
+                ghost_chain<double>(c);
+                HC(hipEventRecord(ev[2], c->stream));
+                gfm::launch_fs<double>(c->L, muscl, 1, 0);
+                HC(hipEventRecord(ev[3], c->stream));
+                gfm::launch_ls<double>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
+                HC(hipEventRecord(ev[4], c->stream));
+                gfm::launch_bc<double>(c->L, 0, 2, 1);
+            }
+        } else {
+            if (c->mixed()) gfm::launch_dt<float>(c->L, 0, cur);
+            else gfm::launch_dt<double>(c->L, 0, cur);
+            HC(hipEventRecord(ev[1], c->stream));
+            HC(hipEventRecord(ev[2], c->stream));
+            if (c->mixed()) gfm::launch_fs<float>(c->L, muscl, cur, 1 - cur);
+            else gfm::launch_fs<double>(c->L, muscl, cur, 1 - cur);
+            HC(hipEventRecord(ev[3], c->stream));
+            HC(hipEventRecord(ev[4], c->stream));
+            if (c->mixed()) gfm::launch_bc<float>(c->L, 1 - cur, 1, 1);
+            else gfm::launch_bc<double>(c->L, 1 - cur, 1, 1);
+            c->cur = 1 - cur;
+        }
+        HC(hipEventRecord(ev[5], c->stream));
+        HC(hipEventSynchronize(ev[5]));
+        for (int k = 0; k < 5; ++k) {
+            float t = 0.f;
+            HC(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+            ms[k] += t;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return post_launch(c);
+}
+
+int gfm_sync(gfm_ctx* c) {
+    if (!c) return fail_arg("null ctx");
+    HC(hipStreamSynchronize(c->stream));
+    return GFM_OK;
+}
+
+
+
+int gfm_pure_rs_solve(int n, int paralleltox, double gamma, double pinf, const double* UL, const double* UR,
+                      double* flux, unsigned* status) {
+    return rs_common(n, UL, UR, flux, status, false, paralleltox, gamma, pinf, 0.0, 0.0);
+}
+
+int gfm_mixed_rs_solve(int n, double gL, double pL, double gR, double pR, const double* UL, const double* UR,
+                       double* out4, unsigned* status) {
+    return rs_common(n, UL, UR, out4, status, true, 0, gL, pL, gR, pR);
+}
+
+}  // extern "C"

@@ -1,0 +1,1095 @@
+// gfm_kernels.hip -- gfx950 kernels of the GFM hot path.
+//
+// Every kernel is batched: blockIdx.x indexes a work table of int4 entries
+// whose .x is the instance, so one launch advances every instance of a
+// context.  Kernels read the instance's dt / activity from device memory,
+// which keeps a whole step free of host round trips and graph-capturable.
+// R is the storage type (double, or float in mixed precision).
+#include "gfm_kernels.hpp"
+
+namespace gfm {
+
+template <typename R>
+__device__ __forceinline__ V4<R> ld4(const void* base, const Inst& I, long long c) {
+    const R* p = (const R*)base;
+    V4<R> a;
+    a.v[0] = p[c];
+    a.v[1] = p[c + I.plane];
+    a.v[2] = p[c + 2 * I.plane];
+    a.v[3] = p[c + 3 * I.plane];
+    return a;
+}
+
+template <typename R>
+__device__ __forceinline__ void st4(void* base, const Inst& I, long long c, const V4<R>& a) {
+    R* p = (R*)base;
+    p[c] = a.v[0];
+    p[c + I.plane] = a.v[1];
+    p[c + 2 * I.plane] = a.v[2];
+    p[c + 3 * I.plane] = a.v[3];
+}
+
+template <typename R>
+__device__ __forceinline__ R ld1(const void* base, long long c) { return ((const R*)base)[c]; }
+
+__device__ __forceinline__ void flag(const Inst& I, unsigned bits) {
+    if (bits) atomicOr((unsigned*)&I.si[SI_STATUS], bits);
+}
+
+// ---------------------------------------------------------------------------
+// A14 + A16: CFL dt (sim_twofluid.cpp:108-157, sim_onefluid.cpp:68-91) and
+// fluid masses (sim_twofluid.cpp:1803-1827).  Block partials, then the last
+// block of each instance reduces them in tile order (deterministic) and also
+// resets the per-step counters used later in the step.
+// work: {inst, tile_y, tile_x, local tile index}, 64x4 interior tiles.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                            int twofluid, int q) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    const int nstep = I.si[SI_NSTEP];
+    const double t = I.sd[SD_T];
+    const bool active = (t < I.T) && (nstep < I.max_steps);
+    __shared__ double red[4][256];
+    __shared__ int last;
+    double mu = 0.0, mv = 0.0, m1 = 0.0, m2 = 0.0;
+    if (active) {
+        const int i = I.g + w.y * 4 + (threadIdx.x >> 6);
+        const int j = I.g + w.z * 64 + (threadIdx.x & 63);
+        if (i < I.g + I.Ny && j < I.g + I.Nx) {
+            const long long c = cidx(I, i, j);
+            int f = 0;
+            double phi = 0.0;
+            if (twofluid) {
+                phi = (double)ld1<R>(I.phi, c);
+                f = phi < 0.0 ? 0 : 1;   // :128 strict
+            }
+            V4<R> Ur = ld4<R>(I.Q[f][q], I, c);
+            double U0 = Ur.v[0], U1 = Ur.v[1], U2 = Ur.v[2], U3 = Ur.v[3];
+            double rho = U0;
+            double u = U1 / rho, v = U2 / rho;
+            double e = (U3 / rho) - 0.5 * (u * u + v * v);
+            double p = eos_p(I.gam[f], I.pinf[f], e, rho);
+            double cs = eos_c(I.gam[f], I.pinf[f], p, rho);
+            mu = fabs(U1) / rho + cs;
+            mv = fabs(U2) / rho + cs;
+            if (twofluid) {
+                const double dA = I.dx * I.dy;
+                if (-phi > 0.0) m1 = (double)ld4<R>(I.Q[0][q], I, c).v[0] * dA;
+                if (phi > 0.0) m2 = (double)ld4<R>(I.Q[1][q], I, c).v[0] * dA;
+            }
+        }
+    }
+    red[0][threadIdx.x] = mu;
+    red[1][threadIdx.x] = mv;
+    red[2][threadIdx.x] = m1;
+    red[3][threadIdx.x] = m2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+            red[2][threadIdx.x] += red[2][threadIdx.x + s];
+            red[3][threadIdx.x] += red[3][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* pp = I.partials + 4 * (long long)w.w;
+        pp[0] = red[0][0]; pp[1] = red[1][0]; pp[2] = red[2][0]; pp[3] = red[3][0];
+        __threadfence();
+        int prev = atomicAdd(&I.si[SI_DTCOUNT], 1);
+        last = (prev == I.ntile_dt - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // last block of this instance: ordered reduction of the partials
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (threadIdx.x == 0) {
+        const volatile double* pp = I.partials;
+        for (int k = 0; k < I.ntile_dt; ++k) {
+            a0 = fmax(a0, pp[4 * k]);
+            a1 = fmax(a1, pp[4 * k + 1]);
+            a2 += pp[4 * k + 2];
+            a3 += pp[4 * k + 3];
+        }
+        I.si[SI_DTCOUNT] = 0;
+        I.si[SI_ACTIVE] = active ? 1 : 0;
+        I.si[SI_NIFACE] = 0;
+        I.si[SI_BBOX + 0] = I.g + I.Ny; I.si[SI_BBOX + 1] = I.g;
+        I.si[SI_BBOX + 2] = I.g + I.Nx; I.si[SI_BBOX + 3] = I.g;
+        I.si[SI_BBOX + 4] = I.g + I.Ny; I.si[SI_BBOX + 5] = I.g;
+        I.si[SI_BBOX + 6] = I.g + I.Nx; I.si[SI_BBOX + 7] = I.g;
+        if (active) {
+            const double cfl = nstep < 5 ? fmin(I.cfl, 0.2) : I.cfl;   // :66
+            double dt = cfl * fmin(I.dx / a0, I.dy / a1);
+            if (t + dt > I.T) dt = I.T - t;
+            I.sd[SD_DT] = dt;
+            I.sd[SD_MASS1] = a2;
+            I.sd[SD_MASS2] = a3;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// A4/A5: one unsplit MUSCL-Hancock (or first-order Godunov) update of one
+// fluid on a 32x8 tile (FS_MUSCL.cpp:148-327, FS_godunov.cpp:5-50).
+//  A  primitives of the tile + 2-cell halo -> LDS
+//  B  per cell (tile + 1 halo): minmod slopes, boundary-extrapolated values,
+//     cell-level positivity fallback, Hancock half step -> LDS
+//  C  x- and y-face exact Riemann fluxes (face-level fallback) -> registers
+//     -> LDS (aliasing B's storage)
+//  D  flux-difference update in the reference's accumulation order
+// work: {inst, tile_y, tile_x, fluid}
+// ---------------------------------------------------------------------------
+constexpr int FTX = 32, FTY = 8;
+constexpr int FWX = FTX + 4, FWY = FTY + 4;
+constexpr int FEX = FTX + 2, FEY = FTY + 2;
+constexpr int NFX = FTY * (FTX + 1);
+constexpr int NFY = (FTY + 1) * FTX;
+constexpr int NFACE = NFX + NFY;
+
+template <typename T>
+__device__ __forceinline__ T minmod1(T dL, T dR) {
+    return dR > T(0) ? fmax(T(0), fmin(dL, dR)) : fmin(T(0), fmax(dL, dR));
+}
+
+template <typename R, bool MUSCL>
+__global__ __launch_bounds__(256) void k_fs(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                            int qin, int qout) {
+    typedef R T;
+    __shared__ T sW[4][FWY][FWX];
+    __shared__ T sE[MUSCL ? 16 : 1][FEY][FEX];
+    __shared__ T sF[MUSCL ? 1 : 4 * NFACE];   // Godunov fluxes; MUSCL aliases sE
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE]) return;
+    const int f = w.w;
+    const T g = (T)I.gam[f], pinf = (T)I.pinf[f];
+    const double dtd = I.sd[SD_DT];
+    const T dtodx = (T)(dtd / I.dx), dtody = (T)(dtd / I.dy);
+    const int i0 = I.g + w.y * FTY, j0 = I.g + w.z * FTX;
+    int bb0 = I.g, bb1 = I.g + I.Ny, bb2 = I.g, bb3 = I.g + I.Nx;
+    if (I.use_bbox) {
+        const int* b = I.si + SI_BBOX + 4 * f;
+        bb0 = max(b[0], I.g); bb1 = min(b[1], I.g + I.Ny);
+        bb2 = max(b[2], I.g); bb3 = min(b[3], I.g + I.Nx);
+    }
+    const void* Qi = I.Q[f][qin];
+    void* Qo = I.Q[f][qout];
+    const bool tile_active = !(i0 + FTY <= bb0 || i0 >= bb1 || j0 + FTX <= bb2 || j0 >= bb3);
+    const int tid = threadIdx.x;
+    const int ci = i0 + (tid / FTX), cj = j0 + (tid % FTX);
+    if (!tile_active) {
+        if (ci < I.g + I.Ny && cj < I.g + I.Nx) {
+            const long long c = cidx(I, ci, cj);
+            st4<R>(Qo, I, c, ld4<R>(Qi, I, c));
+        }
+        return;
+    }
+    // A
+    for (int c = tid; c < FWY * FWX; c += 256) {
+        const int a = c / FWX, b = c % FWX;
+        const int i = min(max(i0 - 2 + a, 0), I.H - 1);
+        const int j = min(max(j0 - 2 + b, 0), I.W - 1);
+        V4<T> W = cons2prims(g, pinf, ld4<R>(Qi, I, cidx(I, i, j)));
+        sW[0][a][b] = W.v[0]; sW[1][a][b] = W.v[1]; sW[2][a][b] = W.v[2]; sW[3][a][b] = W.v[3];
+    }
+    __syncthreads();
+    const T hx = T(0.5) * dtodx, hy = T(0.5) * dtody;
+    if (MUSCL) {
+        // B
+        for (int c = tid; c < FEY * FEX; c += 256) {
+            const int a = c / FEX + 1, b = c % FEX + 1;   // position in sW
+            V4<T> Wc, BL, BR, BB, BT;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const T wc = sW[k][a][b];
+                const T sx = minmod1(wc - sW[k][a][b - 1], sW[k][a][b + 1] - wc);
+                const T sy = minmod1(wc - sW[k][a - 1][b], sW[k][a + 1][b] - wc);
+                Wc.v[k] = wc;
+                BL.v[k] = wc - T(0.5) * sx; BR.v[k] = wc + T(0.5) * sx;
+                BB.v[k] = wc - T(0.5) * sy; BT.v[k] = wc + T(0.5) * sy;
+            }
+            if (!phys_prims(g, pinf, BL) || !phys_prims(g, pinf, BR)) { BL = Wc; BR = Wc; }
+            if (!phys_prims(g, pinf, BB) || !phys_prims(g, pinf, BT)) { BB = Wc; BT = Wc; }
+            // Hancock increment hx*A(W)(BL-BR) + hy*B(W)(BB-BT) (euler_misc.hpp:149-180)
+            const T rho = Wc.v[0], u = Wc.v[1], v = Wc.v[2];
+            const T cs = eos_c(g, pinf, Wc.v[3], rho);
+            const T rc2 = rho * cs * cs;
+            const T ir = T(1) / rho;
+            const T x0 = BL.v[0] - BR.v[0], x1 = BL.v[1] - BR.v[1], x2 = BL.v[2] - BR.v[2], x3 = BL.v[3] - BR.v[3];
+            const T y0 = BB.v[0] - BT.v[0], y1 = BB.v[1] - BT.v[1], y2 = BB.v[2] - BT.v[2], y3 = BB.v[3] - BT.v[3];
+            T d[4];
+            d[0] = hx * (u * x0 + rho * x1) + hy * (v * y0 + rho * y2);
+            d[1] = hx * (u * x1 + ir * x3) + hy * (v * y1);
+            d[2] = hx * (u * x2) + hy * (v * y2 + ir * y3);
+            d[3] = hx * (u * x3 + rc2 * x1) + hy * (v * y3 + rc2 * y2);
+            const int ea = a - 1, eb = b - 1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                sE[k][ea][eb] = BL.v[k] + d[k];
+                sE[4 + k][ea][eb] = BR.v[k] + d[k];
+                sE[8 + k][ea][eb] = BB.v[k] + d[k];
+                sE[12 + k][ea][eb] = BT.v[k] + d[k];
+            }
+        }
+        __syncthreads();
+    }
+    // C
+    const int lim_i = min(bb1, i0 + FTY), lim_j = min(bb3, j0 + FTX);
+    unsigned st = 0;
+    V4<T> fl[3];
+    int fid[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int c = tid + 256 * r;
+        fid[r] = -1;
+        if (c >= NFACE) continue;
+        bool px;
+        int la, lb, ra, rb;   // sW coordinates of the left/lower and right/upper cell
+        int ci_lo, cj_lo, ci_hi, cj_hi;
+        if (c < NFX) {
+            const int rr = c / (FTX + 1), fc = c % (FTX + 1);
+            px = true;
+            la = rr + 2; lb = fc + 1; ra = rr + 2; rb = fc + 2;
+            ci_lo = i0 + rr; cj_lo = j0 + fc - 1; ci_hi = ci_lo; cj_hi = j0 + fc;
+        } else {
+            const int cc = c - NFX;
+            const int rr = cc / FTX, fc = cc % FTX;
+            px = false;
+            la = rr + 1; lb = fc + 2; ra = rr + 2; rb = fc + 2;
+            ci_lo = i0 + rr - 1; cj_lo = j0 + fc; ci_hi = i0 + rr; cj_hi = cj_lo;
+        }
+        // needed only if one side is an updated cell of this tile
+        const bool lo_upd = ci_lo >= max(bb0, i0) && ci_lo < lim_i && cj_lo >= max(bb2, j0) && cj_lo < lim_j;
+        const bool hi_upd = ci_hi >= max(bb0, i0) && ci_hi < lim_i && cj_hi >= max(bb2, j0) && cj_hi < lim_j;
+        if (!lo_upd && !hi_upd) continue;
+        V4<T> SL, SR, WL, WR;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { WL.v[k] = sW[k][la][lb]; WR.v[k] = sW[k][ra][rb]; }
+        if (MUSCL) {
+            const int lea = la - 1, leb = lb - 1, rea = ra - 1, reb = rb - 1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                SL.v[k] = px ? sE[4 + k][lea][leb] : sE[12 + k][lea][leb];   // evolved R / T of lower cell
+                SR.v[k] = px ? sE[k][rea][reb] : sE[8 + k][rea][reb];        // evolved L / B of upper cell
+            }
+            if (!phys_prims(g, pinf, SR) || !phys_prims(g, pinf, SL)) { SR = WR; SL = WL; }   // :276-285
+            fl[r] = pure_rs_flux<T>(px, g, pinf, prims2cons(g, pinf, SL), prims2cons(g, pinf, SR), st);
+        } else {
+            // Godunov passes the stored conserved states (FS_godunov.cpp:34-35)
+            const V4<R> UL = ld4<R>(Qi, I, cidx(I, ci_lo, cj_lo));
+            const V4<R> UR = ld4<R>(Qi, I, cidx(I, ci_hi, cj_hi));
+            fl[r] = pure_rs_flux<T>(px, g, pinf, UL, UR, st);
+        }
+        fid[r] = c;
+    }
+    flag(I, st);
+    __syncthreads();
+    T* sFl = MUSCL ? &sE[0][0][0] : &sF[0];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        if (fid[r] >= 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sFl[k * NFACE + fid[r]] = fl[r].v[k];
+        }
+    __syncthreads();
+    // D
+    if (ci < I.g + I.Ny && cj < I.g + I.Nx) {
+        const long long c = cidx(I, ci, cj);
+        V4<R> U = ld4<R>(Qi, I, c);
+        if (ci >= bb0 && ci < bb1 && cj >= bb2 && cj < bb3) {
+            if (!phys_state(U)) flag(I, ST_UNPHYS);   // FS_MUSCL.cpp:167
+            const int rr = tid / FTX, fc = tid % FTX;
+            const int fxl = rr * (FTX + 1) + fc, fxr = fxl + 1;
+            const int fyb = NFX + rr * FTX + fc, fyt = fyb + FTX;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const T* Fk = sFl + k * NFACE;
+                T u = (T)U.v[k];
+                u = u + (dtodx * Fk[fxl] + dtody * Fk[fyb]);   // :290
+                u = u - dtodx * Fk[fxr];                        // :291 of the next column
+                u = u - dtody * Fk[fyt];                        // :292 of the next row
+                U.v[k] = (R)u;
+            }
+        }
+        st4<R>(Qo, I, c, U);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// A12: one Jacobi iteration of extrapolate_vector (extrapolate_vector.cpp:45-130)
+// for both fluids, with the boundary conditions of :128-129 folded in as a
+// gather: every cell of the padded array computes the new value of its BC
+// source cell.  Mode bits:
+//   MF_FIRST   read conserved Q[.][qin] and convert (GFM U->W, e.g. GFM_riemann.cpp:33-44)
+//   MF_INJECT  interfacial star states from the side buffer replace the real
+//              fluid's value on read (the in-place writes of GFM_riemann.cpp:127-140)
+//   MF_LAST    write conserved Q[.][qout] and the velocity field (W->U and vfield)
+//   MF_OGFM    apply GFM_original's ghost-state rule (GFM_original.cpp:59-128)
+//   MF_RAW     read/write Q buffers as primitives (standalone extrapolation)
+// work: {inst, tile_y, tile_x, 0}, 64x4 tiles over the padded array
+// ---------------------------------------------------------------------------
+enum { MF_FIRST = 1, MF_INJECT = 2, MF_LAST = 4, MF_OGFM = 8, MF_RAW = 16 };
+
+template <typename R>
+__device__ __forceinline__ void unit_normal(const Inst& I, int i, int j, double& nx, double& ny) {
+    nx = ((double)ld1<R>(I.phi, cidx(I, i, j + 1)) - (double)ld1<R>(I.phi, cidx(I, i, j - 1))) / (2.0 * I.dx);
+    ny = ((double)ld1<R>(I.phi, cidx(I, i + 1, j)) - (double)ld1<R>(I.phi, cidx(I, i - 1, j))) / (2.0 * I.dy);
+    const double m = sqrt(nx * nx + ny * ny);
+    if (m < 1e-12) { nx = 1.0; ny = 0.0; }
+    else { nx /= m; ny /= m; }
+}
+
+template <typename R, int MODE>
+__device__ __forceinline__ V4<R> ex_load(const Inst& I, int f, int i, int j, const void* src) {
+    const long long c = cidx(I, i, j);
+    V4<R> W;
+    if (MODE & MF_FIRST) W = cons2prims<R>((R)I.gam[f], (R)I.pinf[f], ld4<R>(src, I, c));
+    else W = ld4<R>(src, I, c);
+    if (MODE & MF_INJECT) {
+        const int k = I.imap[c];
+        if (k >= 0) {
+            const int fr = ld1<R>(I.phi, c) <= R(0) ? 0 : 1;
+            if (fr == f) {
+                const double* s = I.side + 8 * (long long)k;
+                W.v[0] = (R)s[0]; W.v[1] = (R)s[1]; W.v[2] = (R)s[2]; W.v[3] = (R)s[3];
+            }
+        }
+    }
+    return W;
+}
+
+template <typename R, int MODE>
+__device__ __forceinline__ void extrap_cell(const Inst& I, int i, int j, int bin, int bout, int* sbb) {
+    bool fv, fu;
+    const int si = src_row(I, i, fv);
+    const int sj = src_col(I, j, fu);
+    const bool qsrc = (MODE & (MF_FIRST | MF_RAW)) != 0;
+    const long long cs = cidx(I, si, sj);
+    const R lsval = ld1<R>(I.phi, cs);
+    double nxd, nyd;
+    unit_normal<R>(I, si, sj, nxd, nyd);
+    const R nx = (R)nxd, ny = (R)nyd;
+    const int fr = lsval <= R(0) ? 0 : 1, fg = 1 - fr;
+    const void* in_r = qsrc ? I.Q[fr][bin] : I.P[fr][bin];
+    const void* in_g = qsrc ? I.Q[fg][bin] : I.P[fg][bin];
+    const R pdt = R(0.5 * fmin(I.dx, I.dy));
+    const R ax = pdt / (R)I.dx, ay = pdt / (R)I.dy;
+    V4<R> Wr = ex_load<R, MODE>(I, fr, si, sj, in_r);
+    V4<R> Wg;
+    {
+        const V4<R> c0 = ex_load<R, MODE>(I, fg, si, sj, in_g);
+        if (fg == 1) {   // fluid 2 ghost: information along -n (:74-97)
+            const V4<R> xo = ex_load<R, MODE>(I, fg, si, nx > R(0) ? sj + 1 : sj - 1, in_g);
+            const V4<R> yo = ex_load<R, MODE>(I, fg, ny > R(0) ? si + 1 : si - 1, sj, in_g);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const R dxv = nx > R(0) ? xo.v[k] - c0.v[k] : c0.v[k] - xo.v[k];
+                const R dyv = ny > R(0) ? yo.v[k] - c0.v[k] : c0.v[k] - yo.v[k];
+                Wg.v[k] = c0.v[k] + ax * nx * dxv + ay * ny * dyv;
+            }
+        } else {         // fluid 1 ghost: information along +n (:98-121)
+            const V4<R> xo = ex_load<R, MODE>(I, fg, si, nx > R(0) ? sj - 1 : sj + 1, in_g);
+            const V4<R> yo = ex_load<R, MODE>(I, fg, ny > R(0) ? si - 1 : si + 1, sj, in_g);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const R dxv = nx > R(0) ? c0.v[k] - xo.v[k] : xo.v[k] - c0.v[k];
+                const R dyv = ny > R(0) ? c0.v[k] - yo.v[k] : yo.v[k] - c0.v[k];
+                Wg.v[k] = c0.v[k] - ax * nx * dxv - ay * ny * dyv;
+            }
+        }
+        if (!(Wg.v[0] >= R(0))) flag(I, ST_UNPHYS);
+    }
+    const bool own = (si == i && sj == j);
+    if ((MODE & MF_OGFM) && own) {
+        // GFM_original.cpp:65-116: the ghost fluid takes the real fluid's
+        // velocity and pressure, density by isentropic extrapolation
+        const double gg = I.gam[fg], pg = I.pinf[fg];
+        const double rn = (double)Wg.v[0] * pow(((double)Wr.v[3] + pg) / ((double)Wg.v[3] + pg), 1.0 / gg);
+        if (!(rn >= 0.0)) flag(I, ST_UNPHYS);
+        Wg.v[0] = (R)rn; Wg.v[1] = Wr.v[1]; Wg.v[2] = Wr.v[2]; Wg.v[3] = Wr.v[3];
+        int* bb = sbb + 4 * fr;
+        atomicMin(&bb[0], i - 2); atomicMax(&bb[1], i + 2);
+        atomicMin(&bb[2], j - 2); atomicMax(&bb[3], j + 2);
+    }
+    if (fu) { Wr.v[1] = -Wr.v[1]; Wg.v[1] = -Wg.v[1]; }
+    if (fv) { Wr.v[2] = -Wr.v[2]; Wg.v[2] = -Wg.v[2]; }
+    V4<R> W1, W2;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        W1.v[k] = fr == 0 ? Wr.v[k] : Wg.v[k];
+        W2.v[k] = fr == 0 ? Wg.v[k] : Wr.v[k];
+    }
+    const long long c = cidx(I, i, j);
+    if (MODE & MF_LAST) {
+        // W->U of both grids and the stored velocity field of the fluid real
+        // in this cell (GFM_riemann.cpp:156-187)
+        st4<R>(I.Q[0][bout], I, c, prims2cons<R>((R)I.gam[0], (R)I.pinf[0], W1));
+        st4<R>(I.Q[1][bout], I, c, prims2cons<R>((R)I.gam[1], (R)I.pinf[1], W2));
+        R* vel = (R*)I.vel;
+        vel[c] = Wr.v[1];
+        vel[c + I.plane] = Wr.v[2];
+    } else if (MODE & MF_RAW) {
+        st4<R>(I.Q[0][bout], I, c, W1);
+        st4<R>(I.Q[1][bout], I, c, W2);
+    } else {
+        st4<R>(I.P[0][bout], I, c, W1);
+        st4<R>(I.P[1][bout], I, c, W2);
+    }
+}
+
+template <typename R, int MODE>
+__global__ __launch_bounds__(256) void k_extrap(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                int bin, int bout) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE]) return;
+    __shared__ int sbb[8];
+    if (MODE & MF_OGFM) {
+        if (threadIdx.x < 8) sbb[threadIdx.x] = (threadIdx.x & 1) ? INT_MIN : INT_MAX;
+        __syncthreads();
+    }
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i < I.H && j < I.W) extrap_cell<R, MODE>(I, i, j, bin, bout, sbb);
+    if (MODE & MF_OGFM) {
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            const int v = sbb[threadIdx.x];
+            int* dst = I.si + SI_BBOX + threadIdx.x;
+            if (threadIdx.x & 1) { if (v != INT_MIN) atomicMax(dst, v); }
+            else if (v != INT_MAX) atomicMin(dst, v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// interfacial cells (external Levelset::is_interfacial_cell, restated as a
+// 4-neighbour change of side with the side taken as phi <= 0).
+// work: {inst, tile_y, tile_x, tile index}, 64x4 interior tiles
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ bool is_iface(const Inst& I, int i, int j) {
+    const bool s = ld1<R>(I.phi, cidx(I, i, j)) <= R(0);
+    return ((ld1<R>(I.phi, cidx(I, i, j + 1)) <= R(0)) != s) || ((ld1<R>(I.phi, cidx(I, i, j - 1)) <= R(0)) != s) ||
+           ((ld1<R>(I.phi, cidx(I, i + 1, j)) <= R(0)) != s) || ((ld1<R>(I.phi, cidx(I, i - 1, j)) <= R(0)) != s);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_iface_mark(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE]) return;
+    const int i = I.g + w.y * 4 + (threadIdx.x >> 6);
+    const int j = I.g + w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.g + I.Ny || j >= I.g + I.Nx) return;
+    int k = -1;
+    if (is_iface<R>(I, i, j)) {
+        k = atomicAdd(&I.si[SI_NIFACE], 1);
+        if (k < I.icap) I.ilist[k] = i * I.W + j;
+        else { k = -1; flag(I, ST_IOVF); }
+    }
+    I.imap[cidx(I, i, j)] = k;
+}
+
+// ---------------------------------------------------------------------------
+// Interfacial mixed Riemann problems (GFM_real.cpp:52-208,
+// GFM_riemann.cpp:51-148, GFM_modified.cpp:65-200), one workgroup per
+// instance, FP64.
+//
+// GFM_real and GFM_riemann overwrite the real fluid's primitives in place
+// while they loop over rows i and columns j, so a cell whose probe reads an
+// interfacial cell that comes earlier in that order sees the earlier cell's
+// star state.  The kernel reproduces that order exactly: results go to a
+// side buffer; a cell is computed once every earlier interfacial cell it
+// reads has been computed (dependency rounds separated by barriers), and it
+// reads those cells' star states and every other cell's original value.
+// GFM_modified writes to separate arrays and has no such dependency.
+// ---------------------------------------------------------------------------
+constexpr int ICAP_LDS = 49152;
+
+struct Probe { int i0, j0; double tx, ty; };
+
+__device__ __forceinline__ Probe bilinear_stencil(const Inst& I, double x, double y) {
+    // external grid_bilinear_interpolate, restated: cell-centred, clamped
+    const double fj = (x - I.x0) / I.dx - 0.5 + I.g;
+    const double fi = (y - I.y0) / I.dy - 0.5 + I.g;
+    Probe p;
+    p.j0 = (int)floor(fj);
+    p.i0 = (int)floor(fi);
+    p.j0 = min(max(p.j0, 0), I.W - 2);
+    p.i0 = min(max(p.i0, 0), I.H - 2);
+    p.tx = fmin(fmax(fj - p.j0, 0.0), 1.0);
+    p.ty = fmin(fmax(fi - p.i0, 0.0), 1.0);
+    return p;
+}
+
+// probe positions of GFM_base.cpp:4-33 (fluid 2 at +n -> pL, fluid 1 at -n -> pR)
+template <typename R>
+__device__ __forceinline__ void probes(const Inst& I, int i, int j, Probe& p2, Probe& p1) {
+    const double cx = I.x0 + (j - I.g + 0.5) * I.dx, cy = I.y0 + (i - I.g + 0.5) * I.dy;
+    double lsval = (double)ld1<R>(I.phi, cidx(I, i, j));
+    double nx = ((double)ld1<R>(I.phi, cidx(I, i, j + 1)) - (double)ld1<R>(I.phi, cidx(I, i, j - 1))) / (2.0 * I.dx);
+    double ny = ((double)ld1<R>(I.phi, cidx(I, i + 1, j)) - (double)ld1<R>(I.phi, cidx(I, i - 1, j))) / (2.0 * I.dy);
+    double m = sqrt(nx * nx + ny * ny);
+    if (m < 1e-12) { nx = 1.0; ny = 0.0; m = 1.0; }
+    const double ds = fmin(I.dx, I.dy);
+    if (fabs(lsval) > ds) lsval = copysign(ds, lsval);
+    const double hx = nx / m, hy = ny / m;
+    const double ix = cx - lsval * hx, iy = cy - lsval * hy;
+    p2 = bilinear_stencil(I, ix + ds * hx, iy + ds * hy);
+    p1 = bilinear_stencil(I, ix - ds * hx, iy - ds * hy);
+}
+
+// dependency of a read of fluid f at cell (ii,jj) by the cell with linear index lin
+template <typename R>
+__device__ __forceinline__ int dep_of(const Inst& I, int ii, int jj, int f, int lin) {
+    const long long c = cidx(I, ii, jj);
+    const int k = I.imap[c];
+    if (k < 0) return -1;
+    if (ii * I.W + jj >= lin) return -1;
+    const int fr = ld1<R>(I.phi, c) <= R(0) ? 0 : 1;
+    return fr == f ? k : -1;
+}
+
+template <typename R, int VAR>
+__device__ __forceinline__ void read_prims(const Inst& I, int f, int ii, int jj, int lin, int pbuf, double* W) {
+    const long long c = cidx(I, ii, jj);
+    if (VAR != V_MODIFIED) {
+        const int d = dep_of<R>(I, ii, jj, f, lin);
+        if (d >= 0) {
+            const double* s = I.side + 8 * (long long)d;
+            W[0] = s[0]; W[1] = s[1]; W[2] = s[2]; W[3] = s[3];
+            return;
+        }
+    }
+    if (VAR == V_REAL) {
+        const V4<R> U = ld4<R>(I.Q[f][0], I, c);
+        const V4<R> P = cons2prims<R>((R)I.gam[f], (R)I.pinf[f], U);
+        W[0] = P.v[0]; W[1] = P.v[1]; W[2] = P.v[2]; W[3] = P.v[3];
+    } else {
+        const V4<R> P = ld4<R>(I.P[f][pbuf], I, c);
+        W[0] = P.v[0]; W[1] = P.v[1]; W[2] = P.v[2]; W[3] = P.v[3];
+    }
+}
+
+template <typename R, int VAR>
+__device__ __forceinline__ void bilinear_read(const Inst& I, const Probe& p, int f, int lin, int pbuf, double* out) {
+    double a[4], b[4], c[4], d[4];
+    read_prims<R, VAR>(I, f, p.i0, p.j0, lin, pbuf, a);
+    read_prims<R, VAR>(I, f, p.i0, p.j0 + 1, lin, pbuf, b);
+    read_prims<R, VAR>(I, f, p.i0 + 1, p.j0, lin, pbuf, c);
+    read_prims<R, VAR>(I, f, p.i0 + 1, p.j0 + 1, lin, pbuf, d);
+    for (int k = 0; k < 4; ++k)
+        out[k] = (1.0 - p.ty) * ((1.0 - p.tx) * a[k] + p.tx * b[k]) + p.ty * ((1.0 - p.tx) * c[k] + p.tx * d[k]);
+}
+
+// shared normal-frame mixed RP (e.g. GFM_riemann.cpp:76-121)
+__device__ __forceinline__ unsigned iface_mrp(const Inst& I, const double* ip1, const double* ip2, double nx, double ny,
+                                              Star& S, double* t1, double* t2) {
+    const double u1n = ip1[1] * nx + ip1[2] * ny;
+    const double u2n = ip2[1] * nx + ip2[2] * ny;
+    t1[0] = ip1[1] - u1n * nx; t1[1] = ip1[2] - u1n * ny;
+    t2[0] = ip2[1] - u2n * nx; t2[1] = ip2[2] - u2n * ny;
+    // Lprims/Rprims with the 1e-6 pressure floor, through W->U->W as the
+    // reference passes conserved states to the mixed solver
+    V4<double> Lp, Rp;
+    Lp.v[0] = ip1[0]; Lp.v[1] = u1n; Lp.v[2] = 0.0; Lp.v[3] = fmax(1e-6, ip1[3]);
+    Rp.v[0] = ip2[0]; Rp.v[1] = u2n; Rp.v[2] = 0.0; Rp.v[3] = fmax(1e-6, ip2[3]);
+    const V4<double> Ls = prims2cons(I.gam[0], I.pinf[0], Lp);
+    const V4<double> Rs = prims2cons(I.gam[1], I.pinf[1], Rp);
+    unsigned st = 0;
+    if (!phys_state(Ls) || !phys_state(Rs)) st |= ST_UNPHYS;
+    const V4<double> Lw = cons2prims(I.gam[0], I.pinf[0], Ls);
+    const V4<double> Rw = cons2prims(I.gam[1], I.pinf[1], Rs);
+    RSSide L, R;
+    rs_side(L, I.gam[0], I.pinf[0], Lw.v[0], Lw.v[1], Lw.v[3]);
+    rs_side(R, I.gam[1], I.pinf[1], Rw.v[0], Rw.v[1], Rw.v[3]);
+    st |= rs_star(L, R, S);
+    return st;
+}
+
+template <typename R, int VAR>
+__global__ __launch_bounds__(512) void k_iface_solve(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                     int pbuf) {
+    __shared__ unsigned char fl[ICAP_LDS];
+    __shared__ int pending[2];
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE]) return;
+    const int n = min(I.si[SI_NIFACE], I.icap);
+    if (n > ICAP_LDS) { if (threadIdx.x == 0) flag(I, ST_IOVF); return; }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) fl[k] = 0;
+    unsigned st = 0;
+    int round = 0;
+    for (;; ++round) {
+        if (threadIdx.x == 0) pending[round & 1] = 0;
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            if (fl[k]) continue;
+            const int lin = I.ilist[k];
+            const int i = lin / I.W, j = lin % I.W;
+            const double lsval = (double)ld1<R>(I.phi, cidx(I, i, j));
+            double nx, ny;
+            unit_normal<R>(I, i, j, nx, ny);
+            double ip1[4], ip2[4];
+            if (VAR == V_REAL) {
+                // GFM_real.cpp:77-128: neighbour across the interface whose
+                // normal is closest in angle (first minimum in x-major order)
+                double mind = 1e100;
+                int io = -1000, jo = -1000;
+                for (int xi = -1; xi <= 1; ++xi)
+                    for (int yi = -1; yi <= 1; ++yi) {
+                        const double o = (double)ld1<R>(I.phi, cidx(I, i + yi, j + xi));
+                        if (lsval * o <= 0.0) {
+                            double ox, oy;
+                            unit_normal<R>(I, i + yi, j + xi, ox, oy);
+                            double dp = nx * ox + ny * oy;
+                            dp = fmin(fmax(dp, -1.0), 1.0);
+                            const double ad = acos(dp);
+                            if (ad < mind) { io = i + yi; jo = j + xi; mind = ad; }
+                        }
+                    }
+                if (io == -1000) { fl[k] = 2; st |= ST_UNPHYS; continue; }
+                const int fo = lsval <= 0.0 ? 1 : 0;   // fluid read at the opposite cell
+                const int d = dep_of<R>(I, io, jo, fo, lin);
+                if (d >= 0 && fl[d] != 1) { atomicAdd(&pending[round & 1], 1); continue; }
+                if (lsval <= 0.0) {
+                    read_prims<R, VAR>(I, 0, i, j, lin, pbuf, ip1);
+                    read_prims<R, VAR>(I, 1, io, jo, lin, pbuf, ip2);
+                } else {
+                    read_prims<R, VAR>(I, 1, i, j, lin, pbuf, ip2);
+                    read_prims<R, VAR>(I, 0, io, jo, lin, pbuf, ip1);
+                }
+            } else {
+                Probe p2, p1;
+                probes<R>(I, i, j, p2, p1);
+                if (VAR == V_RIEMANN) {
+                    bool ready = true;
+                    for (int a = 0; a < 2 && ready; ++a)
+                        for (int b = 0; b < 2 && ready; ++b) {
+                            int d = dep_of<R>(I, p2.i0 + a, p2.j0 + b, 1, lin);
+                            if (d >= 0 && fl[d] != 1) ready = false;
+                            d = dep_of<R>(I, p1.i0 + a, p1.j0 + b, 0, lin);
+                            if (d >= 0 && fl[d] != 1) ready = false;
+                        }
+                    if (!ready) { atomicAdd(&pending[round & 1], 1); continue; }
+                }
+                bilinear_read<R, VAR>(I, p2, 1, lin, pbuf, ip2);
+                bilinear_read<R, VAR>(I, p1, 0, lin, pbuf, ip1);
+            }
+            Star S;
+            double t1[2], t2[2];
+            st |= iface_mrp(I, ip1, ip2, nx, ny, S, t1, t2);
+            double* out = I.side + 8 * (long long)k;
+            const long long c = cidx(I, i, j);
+            if (VAR == V_MODIFIED) {
+                // GFM_modified.cpp:149-191 (no-slip): real fluid keeps its
+                // state except the density, the ghost takes the star state
+                const double* tr = lsval <= 0.0 ? t1 : t2;
+                const double usx = S.u * nx + tr[0], usy = S.u * ny + tr[1];
+                const int fr = lsval <= 0.0 ? 0 : 1;
+                const double preal = (double)ld4<R>(I.P[fr][pbuf], I, c).v[3];
+                const double rs_real = fr == 0 ? S.rl : S.rr;
+                out[0] = rs_real * pow((preal + I.pinf[fr]) / (S.p + I.pinf[fr]), 1.0 / I.gam[fr]);
+                out[1] = fr == 0 ? S.rr : S.rl;
+                out[2] = usx;
+                out[3] = usy;
+                out[4] = S.p;
+            } else if (lsval <= 0.0) {
+                out[0] = S.rl; out[1] = S.u * nx + t1[0]; out[2] = S.u * ny + t1[1]; out[3] = S.p;
+            } else {
+                out[0] = S.rr; out[1] = S.u * nx + t2[0]; out[2] = S.u * ny + t2[1]; out[3] = S.p;
+            }
+            if (S.p != S.p) st |= ST_RS;
+            fl[k] = 2;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x)
+            if (fl[k] == 2) fl[k] = 1;
+        __syncthreads();
+        if (pending[round & 1] == 0) break;
+        if (round > n + 1) { if (threadIdx.x == 0) flag(I, ST_IROUNDS); break; }
+    }
+    flag(I, st);
+    if (threadIdx.x == 0) I.si[SI_ROUNDS] = round + 1;
+}
+
+// GFM_modified write-back (GFM_modified.cpp:37-47, :230-239): the grids are
+// W->U of the unextrapolated primitives with the interfacial changes; the
+// velocity field is the real fluid's (the 6-iteration extrapolation at :202
+// only changes ghost values and its result is not written back).
+// work: full-array 64x4 tiles
+template <typename R>
+__global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                    int qin, int qout) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE]) return;
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    const long long c = cidx(I, i, j);
+    V4<R> W1 = cons2prims<R>((R)I.gam[0], (R)I.pinf[0], ld4<R>(I.Q[0][qin], I, c));
+    V4<R> W2 = cons2prims<R>((R)I.gam[1], (R)I.pinf[1], ld4<R>(I.Q[1][qin], I, c));
+    const int fr = ld1<R>(I.phi, c) <= R(0) ? 0 : 1;
+    R* vel = (R*)I.vel;
+    vel[c] = fr == 0 ? W1.v[1] : W2.v[1];
+    vel[c + I.plane] = fr == 0 ? W1.v[2] : W2.v[2];
+    if (interior(I, i, j)) {
+        const int k = I.imap[c];
+        if (k >= 0) {
+            const double* s = I.side + 8 * (long long)k;
+            if (fr == 0) {
+                W1.v[0] = (R)s[0];
+                W2.v[0] = (R)s[1]; W2.v[1] = (R)s[2]; W2.v[2] = (R)s[3]; W2.v[3] = (R)s[4];
+            } else {
+                W2.v[0] = (R)s[0];
+                W1.v[0] = (R)s[1]; W1.v[1] = (R)s[2]; W1.v[2] = (R)s[3]; W1.v[3] = (R)s[4];
+            }
+        }
+    }
+    st4<R>(I.Q[0][qout], I, c, prims2cons<R>((R)I.gam[0], (R)I.pinf[0], W1));
+    st4<R>(I.Q[1][qout], I, c, prims2cons<R>((R)I.gam[1], (R)I.pinf[1], W2));
+}
+
+// ---------------------------------------------------------------------------
+// A15 level set: TVD-RK3 stages of HOUC advection with the stored velocity
+// (update_interface_coupled, sim_twofluid.cpp:72), BCs folded in as a gather.
+//   stage 1: A = phi + dt L(phi)
+//   stage 2: B = 3/4 phi + 1/4 (A + dt L(A))
+//   stage 3: A = 1/3 phi + 2/3 (B + dt L(B))
+// work: full-array 64x4 tiles
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ R ls_rhs(const Inst& I, const Houc& H, const R* psi, int i, int j) {
+    const long long c = cidx(I, i, j);
+    const R* vel = (const R*)I.vel;
+    const R u = vel[c], v = vel[c + I.plane];
+    R px = R(0), py = R(0);
+    if (u > R(0)) { for (int m = 0; m < H.n; ++m) px += (R)H.c[m] * psi[cidx(I, i, j + H.o0 + m)]; }
+    else { for (int m = 0; m < H.n; ++m) px -= (R)H.c[m] * psi[cidx(I, i, j - H.o0 - m)]; }
+    if (v > R(0)) { for (int m = 0; m < H.n; ++m) py += (R)H.c[m] * psi[cidx(I, i + H.o0 + m, j)]; }
+    else { for (int m = 0; m < H.n; ++m) py -= (R)H.c[m] * psi[cidx(I, i - H.o0 - m, j)]; }
+    return -(u * (px / (R)I.dx) + v * (py / (R)I.dy));
+}
+
+template <typename R, int STAGE>
+__global__ __launch_bounds__(256) void k_ls_stage(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                  Houc H) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE]) return;
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    bool f0, f1;
+    const int si = src_row(I, i, f0), sj = src_col(I, j, f1);
+    const R dt = (R)I.sd[SD_DT];
+    const R* phi = (const R*)I.phi;
+    const R* A = (const R*)I.phiA;
+    const R* B = (const R*)I.phiB;
+    const long long cs = cidx(I, si, sj);
+    R val;
+    if (STAGE == 1) val = phi[cs] + dt * ls_rhs<R>(I, H, phi, si, sj);
+    else if (STAGE == 2) val = R(0.75) * phi[cs] + R(0.25) * (A[cs] + dt * ls_rhs<R>(I, H, A, si, sj));
+    else val = R(1.0 / 3.0) * phi[cs] + R(2.0 / 3.0) * (B[cs] + dt * ls_rhs<R>(I, H, B, si, sj));
+    R* out = (R*)(STAGE == 2 ? I.phiB : I.phiA);
+    out[cidx(I, i, j)] = val;
+}
+
+__device__ __forceinline__ bool reinit_step(const Inst& I, int every) {
+    return every > 0 && ((I.si[SI_NSTEP] + 1) % every) == 0;
+}
+
+// copy A -> phi on steps without reinitialisation
+template <typename R>
+__global__ __launch_bounds__(256) void k_ls_copy(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                 int every) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE] || reinit_step(I, every)) return;
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    const long long c = cidx(I, i, j);
+    ((R*)I.phi)[c] = ((const R*)I.phiA)[c];
+}
+
+// Sussman-Smereka-Osher reinitialisation, first-order Godunov, Russo-Smereka
+// subcell fix near the interface; phi0 frozen in A.  Iteration k of K reads
+// (k==0 ? A : previous output) and writes phi when K-1-k is even, else B.
+// gate: 0 = always (explicit reinit), otherwise only on reinit steps.
+template <typename R>
+__global__ __launch_bounds__(256) void k_ls_reinit(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                   int k, int K, int gate_every, int force) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!force && (!I.si[SI_ACTIVE] || !reinit_step(I, gate_every))) return;
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    bool f0, f1;
+    const int si = src_row(I, i, f0), sj = src_col(I, j, f1);
+    const R* P0 = (const R*)I.phiA;
+    const bool out_phi = ((K - 1 - k) % 2) == 0;
+    const R* in = k == 0 ? P0 : (const R*)(out_phi ? I.phiB : I.phi);
+    R* out = (R*)(out_phi ? I.phi : I.phiB);
+    const double dx = I.dx, dy = I.dy, h = fmin(dx, dy), dtau = 0.5 * h;
+    const long long q = cidx(I, si, sj);
+    const R f0v = P0[q];
+    const R f0e = P0[cidx(I, si, sj + 1)], f0w = P0[cidx(I, si, sj - 1)];
+    const R f0n = P0[cidx(I, si + 1, sj)], f0s = P0[cidx(I, si - 1, sj)];
+    const bool s0 = f0v <= R(0);
+    const bool ifc = ((f0e <= R(0)) != s0) || ((f0w <= R(0)) != s0) || ((f0n <= R(0)) != s0) || ((f0s <= R(0)) != s0);
+    const R f = in[q];
+    R res;
+    if (ifc) {
+        const R gx = (f0e - f0w) / R(2.0 * dx), gy = (f0n - f0s) / R(2.0 * dy);
+        R gm = sqrt(gx * gx + gy * gy);
+        gm = fmax(gm, fabs(f0e - f0v) / R(dx));
+        gm = fmax(gm, fabs(f0v - f0w) / R(dx));
+        gm = fmax(gm, fabs(f0n - f0v) / R(dy));
+        gm = fmax(gm, fabs(f0v - f0s) / R(dy));
+        gm = fmax(gm, R(1e-12));
+        const R D = f0v / gm;
+        const R sg = f0v > R(0) ? R(1) : (f0v < R(0) ? R(-1) : R(0));
+        res = f - R(dtau / h) * (sg * fabs(f) - D);
+    } else {
+        const R a = (f - in[cidx(I, si, sj - 1)]) / R(dx);
+        const R b = (in[cidx(I, si, sj + 1)] - f) / R(dx);
+        const R c = (f - in[cidx(I, si - 1, sj)]) / R(dy);
+        const R d = (in[cidx(I, si + 1, sj)] - f) / R(dy);
+        R gsq;
+        if (f0v > R(0)) {
+            const R ap = fmax(a, R(0)), bm = fmin(b, R(0)), cp = fmax(c, R(0)), dm = fmin(d, R(0));
+            gsq = fmax(ap * ap, bm * bm) + fmax(cp * cp, dm * dm);
+        } else {
+            const R am = fmin(a, R(0)), bp = fmax(b, R(0)), cm = fmin(c, R(0)), dp = fmax(d, R(0));
+            gsq = fmax(am * am, bp * bp) + fmax(cm * cm, dp * dp);
+        }
+        const R S = f0v / sqrt(f0v * f0v + R(h * h));
+        res = f - R(dtau) * S * (sqrt(gsq) - R(1));
+    }
+    out[cidx(I, i, j)] = res;
+}
+
+// copy phi -> A (frozen phi0 for an explicit reinitialisation)
+template <typename R>
+__global__ __launch_bounds__(256) void k_phi_to_A(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    const long long c = cidx(I, i, j);
+    ((R*)I.phiA)[c] = ((const R*)I.phi)[c];
+}
+
+// ---------------------------------------------------------------------------
+// A13 at the end of the step (sim_twofluid.cpp:74-75): ghost frame of the
+// current conserved grids, then t += dt and the step count (:77-78).
+// work: {inst, strip, chunk start, 0}; strips 0 bottom, 1 top, 2 left, 3 right
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool frame_cell(const Inst& I, int strip, int idx, int& i, int& j) {
+    const int g = I.g;
+    if (strip <= 1) {
+        if (idx >= g * I.W) return false;
+        i = idx / I.W; j = idx % I.W;
+        if (strip == 1) i += g + I.Ny;
+    } else {
+        if (idx >= g * I.Ny) return false;
+        i = g + idx / g; j = idx % g;
+        if (strip == 3) j += g + I.Nx;
+    }
+    return true;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_bc(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                            int q, int nfluid, int advance) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE]) return;
+    int i, j;
+    if (frame_cell(I, w.y, w.z + threadIdx.x, i, j)) {
+        bool fv, fu;
+        const int si = src_row(I, i, fv), sj = src_col(I, j, fu);
+        for (int f = 0; f < nfluid; ++f) {
+            V4<R> U = ld4<R>(I.Q[f][q], I, cidx(I, si, sj));
+            if (fu) U.v[1] = -U.v[1];
+            if (fv) U.v[2] = -U.v[2];
+            st4<R>(I.Q[f][q], I, cidx(I, i, j), U);
+        }
+    }
+    if (advance && w.y == 0 && w.z == 0 && threadIdx.x == 0) {
+        I.sd[SD_T] += I.sd[SD_DT];
+        I.si[SI_NSTEP] += 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched Riemann solves for the pure_RS_base / mixed_RS_base drop-ins
+// ---------------------------------------------------------------------------
+__global__ void k_pure_rs(int n, int px, double g, double pinf, const double* UL, const double* UR, double* F,
+                          unsigned* status) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    V4<double> L, Rr;
+    for (int k = 0; k < 4; ++k) { L.v[k] = UL[4 * t + k]; Rr.v[k] = UR[4 * t + k]; }
+    unsigned st = 0;
+    const V4<double> f = pure_rs_flux<double>(px != 0, g, pinf, L, Rr, st);
+    for (int k = 0; k < 4; ++k) F[4 * t + k] = f.v[k];
+    if (st) atomicOr(status, st);
+}
+
+__global__ void k_mixed_rs(int n, double gL, double pL, double gR, double pR, const double* UL, const double* UR,
+                           double* out, unsigned* status) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    V4<double> L, Rr;
+    for (int k = 0; k < 4; ++k) { L.v[k] = UL[4 * t + k]; Rr.v[k] = UR[4 * t + k]; }
+    unsigned st = 0;
+    if (!phys_state(L) || !phys_state(Rr)) st |= ST_UNPHYS;
+    const V4<double> WL = cons2prims(gL, pL, L), WR = cons2prims(gR, pR, Rr);
+    RSSide a, b;
+    rs_side(a, gL, pL, WL.v[0], WL.v[1], WL.v[3]);
+    rs_side(b, gR, pR, WR.v[0], WR.v[1], WR.v[3]);
+    Star S;
+    st |= rs_star(a, b, S);
+    out[4 * t] = S.p; out[4 * t + 1] = S.u; out[4 * t + 2] = S.rl; out[4 * t + 3] = S.rr;
+    if (st) atomicOr(status, st);
+}
+
+// per-step counters for phase-by-phase use (the step path resets them in k_dt)
+__global__ void k_reset(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+    const Inst& I = insts[work[blockIdx.x].x];
+    if (threadIdx.x != 0) return;
+    I.si[SI_ACTIVE] = 1;
+    I.si[SI_NIFACE] = 0;
+    I.si[SI_DTCOUNT] = 0;
+    I.si[SI_BBOX + 0] = I.g + I.Ny; I.si[SI_BBOX + 1] = I.g;
+    I.si[SI_BBOX + 2] = I.g + I.Nx; I.si[SI_BBOX + 3] = I.g;
+    I.si[SI_BBOX + 4] = I.g + I.Ny; I.si[SI_BBOX + 5] = I.g;
+    I.si[SI_BBOX + 6] = I.g + I.Nx; I.si[SI_BBOX + 7] = I.g;
+}
+
+// ---------------------------------------------------------------------------
+// launchers (explicit instantiation per storage type)
+// ---------------------------------------------------------------------------
+#define GRID(n) dim3((unsigned)(n))
+
+template <typename R>
+void launch_dt(const LaunchCtx& L, int twofluid, int q) {
+    hipLaunchKernelGGL((k_dt<R>), GRID(L.n_int), dim3(256), 0, L.stream, L.insts, L.w_int, twofluid, q);
+}
+
+template <typename R>
+void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout) {
+    if (muscl) hipLaunchKernelGGL((k_fs<R, true>), GRID(L.n_fs), dim3(256), 0, L.stream, L.insts, L.w_fs, qin, qout);
+    else hipLaunchKernelGGL((k_fs<R, false>), GRID(L.n_fs), dim3(256), 0, L.stream, L.insts, L.w_fs, qin, qout);
+}
+
+template <typename R>
+void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout) {
+#define EXC(M)                                                                                              \
+    case M:                                                                                                 \
+        hipLaunchKernelGGL((k_extrap<R, M>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, bin, \
+                           bout);                                                                           \
+        break;
+    switch (mode) {
+        EXC(0)
+        EXC(MF_FIRST)
+        EXC(MF_INJECT)
+        EXC(MF_LAST)
+        EXC(MF_FIRST | MF_INJECT)
+        EXC(MF_FIRST | MF_LAST)
+        EXC(MF_LAST | MF_OGFM)
+        EXC(MF_FIRST | MF_LAST | MF_OGFM)
+        EXC(MF_INJECT | MF_LAST)
+        EXC(MF_FIRST | MF_INJECT | MF_LAST)
+        EXC(MF_RAW)
+        default: break;
+    }
+#undef EXC
+}
+
+template <typename R>
+void launch_iface(const LaunchCtx& L, int variant, int pbuf) {
+    hipLaunchKernelGGL((k_iface_mark<R>), GRID(L.n_int), dim3(256), 0, L.stream, L.insts, L.w_int);
+    switch (variant) {
+        case V_REAL:
+            hipLaunchKernelGGL((k_iface_solve<R, V_REAL>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst, pbuf);
+            break;
+        case V_RIEMANN:
+            hipLaunchKernelGGL((k_iface_solve<R, V_RIEMANN>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst, pbuf);
+            break;
+        case V_MODIFIED:
+            hipLaunchKernelGGL((k_iface_solve<R, V_MODIFIED>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst, pbuf);
+            break;
+        default: break;
+    }
+}
+
+template <typename R>
+void launch_mgfm_final(const LaunchCtx& L, int qin, int qout) {
+    hipLaunchKernelGGL((k_mgfm_final<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, qin, qout);
+}
+
+template <typename R>
+void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters) {
+    hipLaunchKernelGGL((k_ls_stage<R, 1>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
+    hipLaunchKernelGGL((k_ls_stage<R, 2>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
+    hipLaunchKernelGGL((k_ls_stage<R, 3>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
+    hipLaunchKernelGGL((k_ls_copy<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, every);
+    if (every > 0)
+        for (int k = 0; k < iters; ++k)
+            hipLaunchKernelGGL((k_ls_reinit<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, k, iters,
+                               every, 0);
+}
+
+template <typename R>
+void launch_reinit_now(const LaunchCtx& L, int iters) {
+    hipLaunchKernelGGL((k_phi_to_A<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full);
+    for (int k = 0; k < iters; ++k)
+        hipLaunchKernelGGL((k_ls_reinit<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, k, iters, 0, 1);
+}
+
+template <typename R>
+void launch_reset(const LaunchCtx& L) {
+    hipLaunchKernelGGL(k_reset, GRID(L.n_inst), dim3(64), 0, L.stream, L.insts, L.w_inst);
+}
+
+template <typename R>
+void launch_bc(const LaunchCtx& L, int q, int nfluid, int advance) {
+    hipLaunchKernelGGL((k_bc<R>), GRID(L.n_frame), dim3(256), 0, L.stream, L.insts, L.w_frame, q, nfluid, advance);
+}
+
+void launch_pure_rs(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR, double* F,
+                    unsigned* st) {
+    hipLaunchKernelGGL(k_pure_rs, dim3((n + 255) / 256), dim3(256), 0, s, n, px, g, pinf, UL, UR, F, st);
+}
+
+void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, double pR, const double* UL,
+                     const double* UR, double* out, unsigned* st) {
+    hipLaunchKernelGGL(k_mixed_rs, dim3((n + 255) / 256), dim3(256), 0, s, n, gL, pL, gR, pR, UL, UR, out, st);
+}
+
+#define INST(R)                                                               \
+    template void launch_dt<R>(const LaunchCtx&, int, int);                   \
+    template void launch_fs<R>(const LaunchCtx&, bool, int, int);             \
+    template void launch_extrap<R>(const LaunchCtx&, int, int, int);          \
+    template void launch_iface<R>(const LaunchCtx&, int, int);                \
+    template void launch_mgfm_final<R>(const LaunchCtx&, int, int);           \
+    template void launch_ls<R>(const LaunchCtx&, const Houc&, int, int);      \
+    template void launch_reinit_now<R>(const LaunchCtx&, int);                \
+    template void launch_bc<R>(const LaunchCtx&, int, int, int);               \
+    template void launch_reset<R>(const LaunchCtx&);
+INST(double)
+INST(float)
+
+}  // namespace gfm

@@ -1,0 +1,36 @@
+// gfm_kernels.hpp -- launch interface between the host context code and the
+// gfx950 kernels.
+#pragma once
+#include <climits>
+#include "gfm_device.hpp"
+
+namespace gfm {
+
+// work tables (device pointers) and their lengths for one context
+struct LaunchCtx {
+    hipStream_t stream;
+    const Inst* insts;
+    const int4* w_full;  int n_full;    // 64x4 tiles over the padded arrays
+    const int4* w_int;   int n_int;     // 64x4 tiles over the interior (.w = tile index in instance)
+    const int4* w_fs;    int n_fs;      // 32x8 interior tiles x fluids (.w = fluid)
+    const int4* w_frame; int n_frame;   // 256-cell chunks of the ghost frame
+    const int4* w_inst;  int n_inst;    // one entry per instance
+};
+
+struct Houc;
+
+template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q);
+template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout);
+template <typename R> void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout);
+template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf);
+template <typename R> void launch_mgfm_final(const LaunchCtx& L, int qin, int qout);
+template <typename R> void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters);
+template <typename R> void launch_reinit_now(const LaunchCtx& L, int iters);
+template <typename R> void launch_bc(const LaunchCtx& L, int q, int nfluid, int advance);
+template <typename R> void launch_reset(const LaunchCtx& L);
+void launch_pure_rs(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR,
+                    double* F, unsigned* st);
+void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, double pR, const double* UL,
+                     const double* UR, double* out, unsigned* st);
+
+}  // namespace gfm

@@ -1,0 +1,355 @@
+"""GPU parity: the HIP library (through its C ABI) against the FP64 CPU oracle
+on the same seeded inputs.
+
+Floating-point bar (FP64 GPU vs FP64 oracle).  Both follow the same
+operation order; they differ by FMA contraction on the GPU and by the
+last-ulp behaviour of pow/acos in the device math library, so:
+  - single calls (Riemann solvers, one ghost fill, one flux update, one
+    level-set update): max |gpu - oracle| <= 1e-12 x (max |oracle| of that
+    component), i.e. rounding-level;
+  - multi-step runs: per conserved variable, L_inf <= 1e-8 x scale and
+    L1 (cell-averaged) <= 1e-10 x scale; level-set zero contour identical
+    (no cell changes side).
+Integer results (interfacial cells, update boxes, step counts) are exact.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import gfm_hip as G
+import problems as P
+
+
+def ogrid(oracle, info):
+    return oracle.Grid(info.Nx, info.Ny, info.numGC, info.x0, info.y0, info.dx, info.dy,
+                       info.BC_L, info.BC_R, info.BC_B, info.BC_T)
+
+
+def oeos(oracle, eos):
+    return oracle.Eos(eos.gamma1, eos.pinf1, eos.gamma2, eos.pinf2)
+
+
+def scaled_err(a, b):
+    """max |a-b| per component divided by max |b| of that component"""
+    a = np.asarray(a).reshape(-1, a.shape[-1]) if np.ndim(a) > 1 else np.asarray(a).reshape(-1, 1)
+    b = np.asarray(b).reshape(a.shape)
+    scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
+    return (np.abs(a - b).max(axis=0) / scale).max(), (np.abs(a - b).mean(axis=0) / scale).max()
+
+
+def print_case(name):
+    print("CASE", name, flush=True)
+
+
+# ---------------------------------------------------------------- Riemann solvers
+def random_states(rng, n, g, pinf, pscale=1.0):
+    rho = rng.uniform(0.1, 5.0, n)
+    u = rng.normal(0.0, 1.0, n) * np.sqrt(pscale)
+    v = rng.normal(0.0, 1.0, n) * np.sqrt(pscale)
+    p = rng.uniform(0.05, 5.0, n) * pscale
+    W = np.stack([rho, u, v, p], 1)
+    e = (p + g * pinf) / ((g - 1.0) * rho)
+    U = np.stack([rho, rho * u, rho * v, rho * e + 0.5 * rho * (u * u + v * v)], 1)
+    return W, U
+
+
+@pytest.mark.parametrize("px", [True, False])
+@pytest.mark.parametrize("g,pinf", [(1.4, 0.0), (4.4, 6e8 / 1e9), (1.667, 0.0)])
+def test_pure_rs(oracle, px, g, pinf):
+    print_case("pure_rs %s %s %s" % (px, g, pinf))
+    rng = np.random.default_rng(1)
+    n = 2000
+    _, UL = random_states(rng, n, g, pinf)
+    _, UR = random_states(rng, n, g, pinf)
+    UR[:100] = UL[:100]          # identical states take the early exit
+    F, st = G.pure_rs_solve(px, g, pinf, UL, UR)
+    Fo = np.array([oracle.pure_rs_flux(px, g, pinf, UL[k], UR[k])[0] for k in range(n)])
+    ok = ~np.isnan(Fo).any(1)
+    err, _ = scaled_err(F[ok], Fo[ok])
+    assert err <= 1e-12, err
+
+
+def test_mixed_rs(oracle):
+    print_case("mixed_rs")
+    rng = np.random.default_rng(2)
+    n = 1000
+    for (gL, pL, gR, pR) in [(1.4, 0.0, 1.667, 0.0), (4.4, 6e8, 1.4, 0.0), (1.4, 0.0, 1.093, 0.0)]:
+        scale = 1e5 if pL > 0 else 1.0
+        _, UL = random_states(rng, n, gL, pL, scale)
+        _, UR = random_states(rng, n, gR, pR, scale)
+        out, st = G.mixed_rs_solve(gL, pL, gR, pR, UL, UR)
+        ref = np.array([oracle.mixed_rs(gL, pL, gR, pR, UL[k], UR[k])[0] for k in range(n)])
+        ok = ~np.isnan(ref).any(1) & ~np.isnan(out).any(1)
+        assert ok.mean() > 0.9
+        err, _ = scaled_err(out[ok], ref[ok])
+        assert err <= 1e-12, (gL, gR, err)
+
+
+# ---------------------------------------------------------------- one fluid
+@pytest.mark.parametrize("fs", [G.FS_MUSCL, G.FS_GODUNOV])
+def test_onefluid_one_step(oracle, fs):
+    print_case("onefluid_one_step fs=%d" % fs)
+    info, eos, grid = P.circular_explosion(101)
+    ctx = G.Context(G.methods(sim_type=G.SIM_ONEFLUID, fs=fs))
+    k = ctx.add(info, eos, 0.8, 0.25)
+    ctx.commit()
+    ctx.upload(k, grid)
+    ctx.step(1)
+    ctx.sync()
+    out = ctx.download(k)
+    t, n, dt = ctx.get_time(k)
+    Go = ogrid(oracle, info)
+    ref = grid.copy()
+    scratch = np.zeros_like(ref)
+    tt = np.zeros(1)
+    dto = np.zeros(1)
+    oracle.lib().or_onefluid_step(Go, 1.4, 0.0, fs, 0.2, 0.25, oracle._p(tt), oracle._p(ref),
+                                  oracle._p(scratch), oracle._p(dto))
+    assert abs(dt - dto[0]) <= 1e-14 * dto[0]
+    assert n == 1
+    err, _ = scaled_err(out, ref)
+    assert err <= 1e-12, err
+    assert ctx.status(k) == 0
+
+
+def test_onefluid_circular_explosion_full(oracle):
+    """circular explosion 101x101, MUSCL + exact RS, CFL 0.8, to t = 0.25"""
+    print_case("circular_explosion_full")
+    info, eos, grid = P.circular_explosion(101)
+    ctx = G.Context(G.methods(sim_type=G.SIM_ONEFLUID, fs=G.FS_MUSCL))
+    k = ctx.add(info, eos, 0.8, 0.25)
+    ctx.commit()
+    ctx.upload(k, grid)
+    for _ in range(400):
+        ctx.step_graph(10)
+        t, n, _ = ctx.get_time(k)
+        if t >= 0.25:
+            break
+    out = ctx.download(k)
+    ref = grid.copy()
+    steps = np.zeros(1, dtype=np.int32)
+    to = np.zeros(1)
+    oracle.lib().or_onefluid_run(ogrid(oracle, info), 1.4, 0.0, G.FS_MUSCL, 0.8, 0.25, 100000,
+                                 oracle._p(ref), oracle._p(steps), oracle._p(to))
+    assert n == steps[0]
+    assert abs(t - to[0]) <= 1e-12
+    ii = P.interior(info, out)
+    ir = P.interior(info, ref)
+    linf, l1 = scaled_err(ii, ir)
+    assert linf <= 1e-8 and l1 <= 1e-10, (linf, l1)
+    # mirror symmetry about x = 1 and y = 1 (exact for the oracle up to rounding)
+    rho = ir[..., 0]
+    assert np.abs(rho - rho[::-1, :]).max() <= 1e-10
+    assert np.abs(rho - rho[:, ::-1]).max() <= 1e-10
+
+
+# ---------------------------------------------------------------- two fluid helpers
+def bubble_state(oracle, variant, Nx=160, Ny=44, steps=0, **kw):
+    info, eos, (g1, g2, phi) = P.shocked_bubble(Nx, Ny, **kw)
+    if steps:
+        run = oracle.Run(variant, 0, 5, 5, 10, 0.6, 1e300, steps)
+        s = np.zeros(1, dtype=np.int32)
+        t = np.zeros(1)
+        oracle.lib().or_twofluid_run(ogrid(oracle, info), oeos(oracle, eos), run, oracle._p(g1), oracle._p(g2),
+                                     oracle._p(phi), oracle._p(s), oracle._p(t))
+    return info, eos, g1, g2, phi
+
+
+def two_ctx(info, eos, variant, precision=G.FP64, cfl=0.6, max_steps=1 << 30, reinit_every=5, order=5):
+    ctx = G.Context(G.methods(sim_type=G.SIM_TWOFLUID, fs=G.FS_MUSCL, gfm=variant, ls_order=order,
+                              reinit_every=reinit_every, reinit_iters=10, precision=precision))
+    k = ctx.add(info, eos, cfl, 1e300, max_steps)
+    ctx.commit()
+    return ctx, k
+
+
+VARIANTS = [G.GFM_ORIGINAL, G.GFM_REAL, G.GFM_RIEMANN, G.GFM_MODIFIED]
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_ghost_states(oracle, variant):
+    print_case("ghost_states variant=%d" % variant)
+    info, eos, g1, g2, phi = bubble_state(oracle, G.GFM_RIEMANN, steps=30)
+    ctx, k = two_ctx(info, eos, variant)
+    ctx.upload(k, g1, g2, phi)
+    ctx.set_ghost_states()
+    ctx.sync()
+    o1, o2, _ = ctx.download(k)
+    vf = ctx.download_vfield(k)
+    b1, b2 = ctx.bbox(k)
+    r1, r2 = g1.copy(), g2.copy()
+    bb1 = np.zeros(4, dtype=np.int32)
+    bb2 = np.zeros(4, dtype=np.int32)
+    vo = np.zeros(phi.shape + (2,))
+    st = oracle.lib().or_set_ghost_states(ogrid(oracle, info), oeos(oracle, eos), variant, oracle._p(phi),
+                                          oracle._p(r1), oracle._p(r2), oracle._p(bb1), oracle._p(bb2),
+                                          oracle._p(vo))
+    assert st == 0
+    assert ctx.status(k) == 0
+    e1, _ = scaled_err(o1, r1)
+    e2, _ = scaled_err(o2, r2)
+    ev, _ = scaled_err(P.interior(info, vf), P.interior(info, vo))
+    assert max(e1, e2) <= 1e-12, (e1, e2)
+    assert ev <= 1e-12, ev
+    assert list(bb1) == b1 and list(bb2) == b2
+    n, rounds = ctx.iface_info(k)
+    assert n > 0 or variant == G.GFM_ORIGINAL
+
+
+def test_extrapolate(oracle):
+    print_case("extrapolate")
+    info, eos, g1, g2, phi = bubble_state(oracle, G.GFM_RIEMANN, steps=20)
+    rng = np.random.default_rng(3)
+    p1 = g1.copy()
+    p2 = g2.copy()
+    p1[..., 0] *= 1 + 0.1 * rng.random(phi.shape)
+    p2[..., 3] *= 1 + 0.1 * rng.random(phi.shape)
+    ctx, k = two_ctx(info, eos, G.GFM_RIEMANN)
+    ctx.upload(k, p1, p2, phi)
+    ctx.extrapolate(5)
+    ctx.sync()
+    o1, o2, _ = ctx.download(k)
+    r1, r2 = p1.copy(), p2.copy()
+    oracle.lib().or_extrapolate_vector(ogrid(oracle, info), oracle._p(phi), 5, oracle._p(r1), oracle._p(r2), 0)
+    e1, _ = scaled_err(o1, r1)
+    e2, _ = scaled_err(o2, r2)
+    assert max(e1, e2) <= 1e-12, (e1, e2)
+
+
+@pytest.mark.parametrize("order", [1, 3, 5, 9])
+def test_ls_advect_and_reinit(oracle, order):
+    print_case("ls order=%d" % order)
+    info, eos, g1, g2, phi = bubble_state(oracle, G.GFM_RIEMANN)
+    H, W = phi.shape
+    yy, xx = np.mgrid[0:H, 0:W]
+    vf = np.stack([300 * np.sin(xx / 7.0), -200 * np.cos(yy / 5.0)], -1).astype(np.float64)
+    dt = 2e-6
+    ctx, k = two_ctx(info, eos, G.GFM_RIEMANN, reinit_every=0, order=order)
+    ctx.upload(k, g1, g2, phi)
+    ctx.upload_vfield(k, np.ascontiguousarray(vf))
+    ctx.set_dt(k, dt)
+    ctx.ls_update()
+    ctx.sync()
+    _, _, po = ctx.download(k)
+    ref = phi.copy()
+    oracle.lib().or_ls_advect(ogrid(oracle, info), order, oracle._p(ref), oracle._p(np.ascontiguousarray(vf)), dt)
+    err, _ = scaled_err(po, ref)
+    assert err <= 1e-12, err
+    # reinitialisation of the advected field
+    ctx.ls_reinit(7)
+    ctx.sync()
+    _, _, pr = ctx.download(k)
+    oracle.lib().or_ls_reinit(ogrid(oracle, info), oracle._p(ref), 7)
+    err, _ = scaled_err(pr, ref)
+    assert err <= 1e-12, err
+
+
+def test_compute_dt_and_mass(oracle):
+    print_case("dt_mass")
+    info, eos, g1, g2, phi = bubble_state(oracle, G.GFM_RIEMANN, steps=10)
+    ctx, k = two_ctx(info, eos, G.GFM_RIEMANN)
+    ctx.upload(k, g1, g2, phi)
+    ctx.set_time(k, 0.0, 7)
+    ctx.compute_dt()
+    ctx.sync()
+    _, _, dt = ctx.get_time(k)
+    dto = oracle.lib().or_compute_dt_twofluid(ogrid(oracle, info), oeos(oracle, eos), 0.6, oracle._p(phi),
+                                              oracle._p(g1), oracle._p(g2), 1e300, 0.0)
+    assert abs(dt - dto) <= 1e-15 * dto
+    m = ctx.mass(k)
+    m1 = oracle.lib().or_fluid_mass(ogrid(oracle, info), -1.0, oracle._p(g1), oracle._p(phi))
+    m2 = oracle.lib().or_fluid_mass(ogrid(oracle, info), 1.0, oracle._p(g2), oracle._p(phi))
+    assert abs(m[0] - m1) <= 1e-13 * m1 and abs(m[1] - m2) <= 1e-13 * m2
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_twofluid_steps(oracle, variant):
+    print_case("twofluid_steps variant=%d" % variant)
+    nsteps = 60
+    info, eos, g1, g2, phi = bubble_state(oracle, variant)
+    ctx, k = two_ctx(info, eos, variant)
+    ctx.upload(k, g1, g2, phi)
+    ctx.step_graph(nsteps)
+    ctx.sync()
+    o1, o2, op = ctx.download(k)
+    t, n, _ = ctx.get_time(k)
+    assert n == nsteps
+    run = oracle.Run(variant, 0, 5, 5, 10, 0.6, 1e300, nsteps)
+    s = np.zeros(1, dtype=np.int32)
+    to = np.zeros(1)
+    r1, r2, rp = g1.copy(), g2.copy(), phi.copy()
+    st = oracle.lib().or_twofluid_run(ogrid(oracle, info), oeos(oracle, eos), run, oracle._p(r1), oracle._p(r2),
+                                      oracle._p(rp), oracle._p(s), oracle._p(to))
+    assert st == 0 and ctx.status(k) == 0
+    assert abs(t - to[0]) <= 1e-12 * to[0]
+    for a, b in ((o1, r1), (o2, r2)):
+        linf, l1 = scaled_err(P.interior(info, a), P.interior(info, b))
+        assert linf <= 1e-8 and l1 <= 1e-10, (linf, l1)
+    pi, pr = P.interior(info, op), P.interior(info, rp)
+    assert np.array_equal(pi <= 0, pr <= 0)
+    assert np.abs(pi - pr).max() <= 1e-8 * np.abs(pr).max()
+
+
+def test_graph_matches_eager():
+    print_case("graph_vs_eager")
+    info, eos, (g1, g2, phi) = P.shocked_bubble(128, 36)
+    outs = []
+    for graph in (False, True):
+        ctx, k = two_ctx(info, eos, G.GFM_RIEMANN)
+        ctx.upload(k, g1, g2, phi)
+        for _ in range(3):
+            (ctx.step_graph if graph else ctx.step)(7)
+            ctx.sync()
+            ctx.get_time(k)
+        outs.append(ctx.download(k))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_batch_matches_single():
+    print_case("batch_vs_single")
+    cases = [dict(Nx=128, Ny=36), dict(Nx=200, Ny=56, bubble_gamma=1.2, bubble_rho=3.0, mach=2.0, R=0.03)]
+    singles = []
+    for c in cases:
+        info, eos, st = P.shocked_bubble(**c)
+        ctx, k = two_ctx(info, eos, G.GFM_RIEMANN)
+        ctx.upload(k, *st)
+        ctx.step_graph(40)
+        ctx.sync()
+        singles.append(ctx.download(k))
+    ctx = G.Context(G.methods(sim_type=G.SIM_TWOFLUID, gfm=G.GFM_RIEMANN))
+    sts = []
+    for c in cases:
+        info, eos, st = P.shocked_bubble(**c)
+        ctx.add(info, eos, 0.6)
+        sts.append(st)
+    ctx.commit()
+    for k, st in enumerate(sts):
+        ctx.upload(k, *st)
+    ctx.step_graph(40)
+    ctx.sync()
+    for k in range(len(cases)):
+        for a, b in zip(ctx.download(k), singles[k]):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("variant", [G.GFM_RIEMANN, G.GFM_MODIFIED])
+def test_mixed_precision_error(variant):
+    """FP32 storage/sweeps with FP64 Newton: L1 density error vs the FP64 GPU
+    path, relative to the mean density, <= 1e-4 after 200 steps."""
+    print_case("mixed variant=%d" % variant)
+    info, eos, st = P.shocked_bubble(256, 72)
+    res = []
+    for prec in (G.FP64, G.MIXED):
+        ctx, k = two_ctx(info, eos, variant, precision=prec)
+        ctx.upload(k, *st)
+        ctx.step_graph(200)
+        ctx.sync()
+        assert ctx.status(k) == 0
+        res.append(ctx.download(k))
+    for f in (0, 1):
+        a = P.interior(info, res[1][f])[..., 0]
+        b = P.interior(info, res[0][f])[..., 0]
+        l1 = np.abs(a - b).mean() / np.abs(b).mean()
+        assert l1 <= 1e-4, (f, l1)
