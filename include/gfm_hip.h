@@ -117,6 +117,9 @@ int gfm_extrapolate(gfm_ctx* ctx, int N);     /* extrapolate_vector, extrapolate
                                                  on grids holding primitives */
 int gfm_ls_reinit(gfm_ctx* ctx, int iters);   /* reinitialise phi now */
 int gfm_get_bbox(gfm_ctx* ctx, int inst, int* bbox1, int* bbox2);   /* realcells1/2 */
+/* set realcells1/2 ({imin, imax, jmin, jmax}, half-open) for the next
+ * gfm_fs_update (FS_base::pure_fluid_update's BBrange argument) */
+int gfm_set_bbox(gfm_ctx* ctx, int inst, const int* bbox1, const int* bbox2);
 
 /* whole steps: the loop body above, nsteps times.  gfm_step launches
  * eagerly; gfm_step_graph captures one step as a hipGraph and replays it. */
@@ -133,6 +136,10 @@ int gfm_status(gfm_ctx* ctx, int inst, unsigned* flags);
 /* fluid masses after the last gfm_compute_dt: sim_twofluid::fluid_mass
  * (sim_twofluid.cpp:1803-1827) for fluid 1 (phi<0) and fluid 2 (phi>0) */
 int gfm_mass_sums(gfm_ctx* ctx, int inst, double* out2);
+/* (t_n, mass1, mass2) recorded by the dt reduction of step n for
+ * n = 0 .. n-1 (sim_twofluid::output_mass_error, sim_twofluid.cpp:298-331);
+ * capacity min(max_steps + 1, 2^20) per two-fluid instance */
+int gfm_mass_history(gfm_ctx* ctx, int inst, int n, double* out3);
 /* number of interfacial cells and dependency rounds of the last ghost fill */
 int gfm_iface_info(gfm_ctx* ctx, int inst, int* ncells, int* rounds);
 

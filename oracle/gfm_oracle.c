@@ -23,6 +23,9 @@
 #include "gfm_oracle.h"
 
 #include <math.h>
+#ifdef OR_DEBUG
+#include <stdio.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -214,6 +217,10 @@ int or_exact_rs_star(double gL, double pinfL, double gR, double pinfR,
         if (chg <= 1e-12 * ref) { conv = 1; break; }
     }
     if (!conv || p != p) st |= OR_ST_RS_FAIL;
+#ifdef OR_DEBUG
+    if (st) fprintf(stderr, "rs fail: L(%.17g %.17g %.17g %g %g) R(%.17g %.17g %.17g %g %g) p=%.17g it=%d ppv=%.17g\n",
+                    WL[0], WL[1], WL[2], gL, pinfL, WR[0], WR[1], WR[2], gR, pinfR, p, it, ppv);
+#endif
     double dL, dR;
     double fL = rs_f(&L, p, &dL);
     double fR = rs_f(&R, p, &dR);

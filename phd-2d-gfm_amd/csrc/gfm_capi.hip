@@ -83,12 +83,13 @@ struct gfm_ctx {
     bool twofluid() const { return m.sim_type == GFM_SIM_TWOFLUID; }
     int nfluid() const { return twofluid() ? 2 : 1; }
     size_t es() const { return mixed() ? 4 : 8; }
+    int narrow() const { return twofluid() && m.gfm == GFM_ORIGINAL ? 1 : 0; }
 };
 
 namespace {
 
 template <typename R>
-void dispatch_dt(gfm_ctx* c, int q) { gfm::launch_dt<R>(c->L, c->twofluid() ? 1 : 0, q); }
+void dispatch_dt(gfm_ctx* c, int q) { gfm::launch_dt<R>(c->L, c->twofluid() ? 1 : 0, q, c->narrow()); }
 
 // N Jacobi iterations of the extrapolation.  Source: conserved Q[.][qin]
 // (from_q) or primitives P[.][pstart]; destination: conserved Q[.][qout]
@@ -153,14 +154,14 @@ template <typename R>
 void step_launches(gfm_ctx* c, int cur) {
     const bool muscl = c->m.fs == GFM_FS_MUSCL;
     if (c->twofluid()) {
-        gfm::launch_dt<R>(c->L, 1, 0);
+        gfm::launch_dt<R>(c->L, 1, 0, c->narrow());
         ghost_chain<R>(c);
-        gfm::launch_fs<R>(c->L, muscl, 1, 0);
+        gfm::launch_fs<R>(c->L, muscl, 1, 0, 0);
         gfm::launch_ls<R>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
         gfm::launch_bc<R>(c->L, 0, 2, 1);
     } else {
-        gfm::launch_dt<R>(c->L, 0, cur);
-        gfm::launch_fs<R>(c->L, muscl, cur, 1 - cur);
+        gfm::launch_dt<R>(c->L, 0, cur, 0);
+        gfm::launch_fs<R>(c->L, muscl, cur, 1 - cur, 1);
         gfm::launch_bc<R>(c->L, 1 - cur, 1, 1);
     }
 }
@@ -367,7 +368,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.gam[1] = c->twofluid() ? h.eos.gamma2 : h.eos.gamma1;
         d.pinf[1] = c->twofluid() ? h.eos.pinf2 : h.eos.pinf1;
         d.cfl = h.cfl; d.T = h.T; d.max_steps = h.max_steps;
-        d.use_bbox = c->twofluid() && c->m.gfm == GFM_ORIGINAL;
+        d.use_bbox = 1;   // boxes are reset to the full domain each step unless OGFM narrows them
         const int tix = (s.Nx + 63) / 64, tiy = (s.Ny + 3) / 4;
         d.ntile_dt = tix * tiy;
         d.icap = c->twofluid() ? std::min(s.Nx * s.Ny, 49152) : 1;
@@ -390,6 +391,9 @@ int gfm_ctx_commit(gfm_ctx* c) {
             oside = take((size_t)d.icap * 8 * sizeof(double));
         }
         const size_t opart = take((size_t)d.ntile_dt * 4 * sizeof(double));
+        d.hist_cap = c->twofluid() ? (int)std::min<long long>((long long)h.max_steps + 1, 1 << 20) : 0;
+        if (d.hist_cap < 0) d.hist_cap = 0;
+        const size_t ohist = take((size_t)d.hist_cap * 3 * sizeof(double) + 8);
         const size_t osd = take(gfm::SD_N * sizeof(double));
         const size_t osi = take(gfm::SI_N * sizeof(int));
         hipError_t e = hipMalloc(&h.mem, off);
@@ -407,6 +411,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
             HC(hipMemsetAsync(d.imap, 0xff, P * sizeof(int), c->stream));   // -1 everywhere
         }
         d.partials = (double*)(base + opart);
+        d.mass_hist = (double*)(base + ohist);
         d.sd = (double*)(base + osd);
         d.si = (int*)(base + osi);
         h.d = d;
@@ -455,6 +460,10 @@ int gfm_ctx_commit(gfm_ctx* c) {
     for (auto& h : c->inst) {
         int si[gfm::SI_N] = {0};
         si[gfm::SI_ACTIVE] = 1;
+        const int g = h.info.numGC;
+        const int full[4] = {g, g + h.info.Ny, g, g + h.info.Nx};
+        for (int f = 0; f < 2; ++f)
+            for (int q = 0; q < 4; ++q) si[gfm::SI_BBOX + 4 * f + q] = full[q];
         HC(hipMemcpyAsync(h.d.si, si, sizeof(si), hipMemcpyHostToDevice, c->stream));
     }
     HC(hipStreamSynchronize(c->stream));
@@ -567,6 +576,17 @@ int gfm_iface_info(gfm_ctx* c, int inst, int* ncells, int* rounds) {
     return GFM_OK;
 }
 
+int gfm_set_bbox(gfm_ctx* c, int inst, const int* b1, const int* b2) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    if (!b1) return fail_arg("bbox1 is NULL");
+    int v[8];
+    for (int k = 0; k < 4; ++k) { v[k] = b1[k]; v[4 + k] = b2 ? b2[k] : b1[k]; }
+    HC(hipMemcpyAsync(c->inst[inst].d.si + gfm::SI_BBOX, v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return GFM_OK;
+}
+
 int gfm_get_bbox(gfm_ctx* c, int inst, int* b1, int* b2) {
     int r = check_inst(c, inst);
     if (r) return r;
@@ -574,18 +594,24 @@ int gfm_get_bbox(gfm_ctx* c, int inst, int* b1, int* b2) {
     if ((r = read_si(c, inst, si))) return r;
     const gfm_sim_info& s = c->inst[inst].info;
     const int g = s.numGC;
-    int full[4] = {g, g + s.Ny, g, g + s.Nx};
     for (int f = 0; f < 2; ++f) {
         int* b = f == 0 ? b1 : b2;
         if (!b) continue;
-        if (c->twofluid() && c->m.gfm == GFM_ORIGINAL) {
-            const int* v = si + gfm::SI_BBOX + 4 * f;
-            b[0] = std::max(v[0], g); b[1] = std::min(v[1], g + s.Ny);
-            b[2] = std::max(v[2], g); b[3] = std::min(v[3], g + s.Nx);
-        } else {
-            std::memcpy(b, full, sizeof(full));
-        }
+        const int* v = si + gfm::SI_BBOX + 4 * f;
+        b[0] = std::max(v[0], g); b[1] = std::min(v[1], g + s.Ny);
+        b[2] = std::max(v[2], g); b[3] = std::min(v[3], g + s.Nx);
     }
+    return GFM_OK;
+}
+
+int gfm_mass_history(gfm_ctx* c, int inst, int n, double* out3) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    const Inst& d = c->inst[inst].d;
+    if (n < 0 || n > d.hist_cap || (n > 0 && !out3)) return fail_arg("history length beyond capacity");
+    if (n == 0) return GFM_OK;
+    HC(hipMemcpyAsync(out3, d.mass_hist, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
     return GFM_OK;
 }
 
@@ -617,8 +643,8 @@ int gfm_set_ghost_states(gfm_ctx* c) {
     int r = ready(c);
     if (r) return r;
     if (!c->twofluid()) return fail_arg("ghost states need a two-fluid context");
-    if (c->mixed()) { gfm::launch_reset<float>(c->L); ghost_chain<float>(c); }
-    else { gfm::launch_reset<double>(c->L); ghost_chain<double>(c); }
+    if (c->mixed()) { gfm::launch_reset<float>(c->L, c->narrow()); ghost_chain<float>(c); }
+    else { gfm::launch_reset<double>(c->L, c->narrow()); ghost_chain<double>(c); }
     if ((r = post_launch(c))) return r;
     // keep the phase-by-phase state in buffer 0 (but not the OGFM boxes,
     // which live in the scalars)
@@ -630,8 +656,8 @@ int gfm_fs_update(gfm_ctx* c) {
     if (r) return r;
     const bool muscl = c->m.fs == GFM_FS_MUSCL;
     const int q = c->twofluid() ? 0 : c->cur;
-    if (c->mixed()) gfm::launch_fs<float>(c->L, muscl, q, 1 - q);
-    else gfm::launch_fs<double>(c->L, muscl, q, 1 - q);
+    if (c->mixed()) gfm::launch_fs<float>(c->L, muscl, q, 1 - q, 0);
+    else gfm::launch_fs<double>(c->L, muscl, q, 1 - q, 0);
     if ((r = post_launch(c))) return r;
     return copy_q(c, 1 - q, q);
 }
@@ -671,8 +697,8 @@ int gfm_extrapolate(gfm_ctx* c, int N) {
     if (N <= 0) return GFM_OK;
     int q = 0;
     for (int k = 0; k < N; ++k) {
-        if (c->mixed()) { if (k == 0) gfm::launch_reset<float>(c->L); gfm::launch_extrap<float>(c->L, 16, q, 1 - q); }
-        else { if (k == 0) gfm::launch_reset<double>(c->L); gfm::launch_extrap<double>(c->L, 16, q, 1 - q); }
+        if (c->mixed()) { if (k == 0) gfm::launch_reset<float>(c->L, c->narrow()); gfm::launch_extrap<float>(c->L, 16, q, 1 - q); }
+        else { if (k == 0) gfm::launch_reset<double>(c->L, c->narrow()); gfm::launch_extrap<double>(c->L, 16, q, 1 - q); }
         q = 1 - q;
     }
     if ((r = post_launch(c))) return r;
@@ -727,33 +753,33 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
         HC(hipEventRecord(ev[0], c->stream));
         if (c->twofluid()) {
             if (c->mixed()) {
-                gfm::launch_dt<float>(c->L, 1, 0);
+                gfm::launch_dt<float>(c->L, 1, 0, c->narrow());
                 HC(hipEventRecord(ev[1], c->stream));
                 ghost_chain<float>(c);
                 HC(hipEventRecord(ev[2], c->stream));
-                gfm::launch_fs<float>(c->L, muscl, 1, 0);
+                gfm::launch_fs<float>(c->L, muscl, 1, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));
                 gfm::launch_ls<float>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
                 HC(hipEventRecord(ev[4], c->stream));
                 gfm::launch_bc<float>(c->L, 0, 2, 1);
             } else {
-                gfm::launch_dt<double>(c->L, 1, 0);
+                gfm::launch_dt<double>(c->L, 1, 0, c->narrow());
                 HC(hipEventRecord(ev[1], c->stream));
                 ghost_chain<double>(c);
                 HC(hipEventRecord(ev[2], c->stream));
-                gfm::launch_fs<double>(c->L, muscl, 1, 0);
+                gfm::launch_fs<double>(c->L, muscl, 1, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));
                 gfm::launch_ls<double>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
                 HC(hipEventRecord(ev[4], c->stream));
                 gfm::launch_bc<double>(c->L, 0, 2, 1);
             }
         } else {
-            if (c->mixed()) gfm::launch_dt<float>(c->L, 0, cur);
-            else gfm::launch_dt<double>(c->L, 0, cur);
+            if (c->mixed()) gfm::launch_dt<float>(c->L, 0, cur, 0);
+            else gfm::launch_dt<double>(c->L, 0, cur, 0);
             HC(hipEventRecord(ev[1], c->stream));
             HC(hipEventRecord(ev[2], c->stream));
-            if (c->mixed()) gfm::launch_fs<float>(c->L, muscl, cur, 1 - cur);
-            else gfm::launch_fs<double>(c->L, muscl, cur, 1 - cur);
+            if (c->mixed()) gfm::launch_fs<float>(c->L, muscl, cur, 1 - cur, 1);
+            else gfm::launch_fs<double>(c->L, muscl, cur, 1 - cur, 1);
             HC(hipEventRecord(ev[3], c->stream));
             HC(hipEventRecord(ev[4], c->stream));
             if (c->mixed()) gfm::launch_bc<float>(c->L, 1 - cur, 1, 1);

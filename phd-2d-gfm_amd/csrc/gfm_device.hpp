@@ -45,6 +45,8 @@ struct Inst {
     double* partials;          // 4 doubles per dt tile
     double* sd;                // SD_* scalars
     int* si;                   // SI_* scalars
+    double* mass_hist;         // (t, mass1, mass2) per step, hist_cap entries
+    int hist_cap;
 };
 
 // HOUC first-derivative stencil for positive velocity: offsets o0..o0+n-1

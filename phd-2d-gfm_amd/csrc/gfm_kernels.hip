@@ -36,6 +36,17 @@ __device__ __forceinline__ void flag(const Inst& I, unsigned bits) {
     if (bits) atomicOr((unsigned*)&I.si[SI_STATUS], bits);
 }
 
+// update boxes (realcells1/2): the full domain, or empty when GFM_original is
+// about to narrow them to the bounding box of each fluid's real cells
+// (GFM_original.cpp:47-55)
+__device__ __forceinline__ void reset_boxes(const Inst& I, int narrow) {
+    for (int f = 0; f < 2; ++f) {
+        int* b = I.si + SI_BBOX + 4 * f;
+        if (narrow) { b[0] = I.g + I.Ny; b[1] = I.g; b[2] = I.g + I.Nx; b[3] = I.g; }
+        else { b[0] = I.g; b[1] = I.g + I.Ny; b[2] = I.g; b[3] = I.g + I.Nx; }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // A14 + A16: CFL dt (sim_twofluid.cpp:108-157, sim_onefluid.cpp:68-91) and
 // fluid masses (sim_twofluid.cpp:1803-1827).  Block partials, then the last
@@ -45,7 +56,7 @@ __device__ __forceinline__ void flag(const Inst& I, unsigned bits) {
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, const int4* __restrict__ work,
-                                            int twofluid, int q) {
+                                            int twofluid, int q, int narrow_boxes) {
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
     const int nstep = I.si[SI_NSTEP];
@@ -54,7 +65,7 @@ __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, cons
     __shared__ double red[4][256];
     __shared__ int last;
     double mu = 0.0, mv = 0.0, m1 = 0.0, m2 = 0.0;
-    if (active) {
+    {
         const int i = I.g + w.y * 4 + (threadIdx.x >> 6);
         const int j = I.g + w.z * 64 + (threadIdx.x & 63);
         if (i < I.g + I.Ny && j < I.g + I.Nx) {
@@ -105,30 +116,47 @@ __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, cons
     __syncthreads();
     if (!last) return;
     __threadfence();
-    // last block of this instance: ordered reduction of the partials
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    if (threadIdx.x == 0) {
+    // last block of this instance: each thread folds a fixed strided subset
+    // of the partials in order, then a fixed tree (deterministic)
+    {
         const volatile double* pp = I.partials;
-        for (int k = 0; k < I.ntile_dt; ++k) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int k = threadIdx.x; k < I.ntile_dt; k += 256) {
             a0 = fmax(a0, pp[4 * k]);
             a1 = fmax(a1, pp[4 * k + 1]);
             a2 += pp[4 * k + 2];
             a3 += pp[4 * k + 3];
         }
+        red[0][threadIdx.x] = a0; red[1][threadIdx.x] = a1; red[2][threadIdx.x] = a2; red[3][threadIdx.x] = a3;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+                red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+                red[2][threadIdx.x] += red[2][threadIdx.x + s];
+                red[3][threadIdx.x] += red[3][threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        const double a0 = red[0][0], a1 = red[1][0], a2 = red[2][0], a3 = red[3][0];
         I.si[SI_DTCOUNT] = 0;
         I.si[SI_ACTIVE] = active ? 1 : 0;
         I.si[SI_NIFACE] = 0;
-        I.si[SI_BBOX + 0] = I.g + I.Ny; I.si[SI_BBOX + 1] = I.g;
-        I.si[SI_BBOX + 2] = I.g + I.Nx; I.si[SI_BBOX + 3] = I.g;
-        I.si[SI_BBOX + 4] = I.g + I.Ny; I.si[SI_BBOX + 5] = I.g;
-        I.si[SI_BBOX + 6] = I.g + I.Nx; I.si[SI_BBOX + 7] = I.g;
+        reset_boxes(I, narrow_boxes);
         if (active) {
             const double cfl = nstep < 5 ? fmin(I.cfl, 0.2) : I.cfl;   // :66
             double dt = cfl * fmin(I.dx / a0, I.dy / a1);
             if (t + dt > I.T) dt = I.T - t;
             I.sd[SD_DT] = dt;
-            I.sd[SD_MASS1] = a2;
-            I.sd[SD_MASS2] = a3;
+        }
+        I.sd[SD_MASS1] = a2;
+        I.sd[SD_MASS2] = a3;
+        // fluid masses of the state at t_n (output_mass_error, sim_twofluid.cpp:298-331)
+        if (twofluid && nstep < I.hist_cap) {
+            double* h = I.mass_hist + 3 * (long long)nstep;
+            h[0] = t; h[1] = a2; h[2] = a3;
         }
     }
 }
@@ -158,15 +186,26 @@ __device__ __forceinline__ T minmod1(T dL, T dR) {
 
 template <typename R, bool MUSCL>
 __global__ __launch_bounds__(256) void k_fs(const Inst* __restrict__ insts, const int4* __restrict__ work,
-                                            int qin, int qout) {
+                                            int qin, int qout, int copy_inactive) {
     typedef R T;
     __shared__ T sW[4][FWY][FWX];
     __shared__ T sE[MUSCL ? 16 : 1][FEY][FEX];
     __shared__ T sF[MUSCL ? 1 : 4 * NFACE];   // Godunov fluxes; MUSCL aliases sE
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!I.si[SI_ACTIVE]) return;
     const int f = w.w;
+    if (!I.si[SI_ACTIVE]) {
+        // a finished instance keeps its state; the one-fluid ping-pong still
+        // needs it in the output buffer
+        if (copy_inactive) {
+            const int ci = I.g + w.y * FTY + (threadIdx.x / FTX), cj = I.g + w.z * FTX + (threadIdx.x % FTX);
+            if (ci < I.g + I.Ny && cj < I.g + I.Nx) {
+                const long long c = cidx(I, ci, cj);
+                st4<R>(I.Q[f][qout], I, c, ld4<R>(I.Q[f][qin], I, c));
+            }
+        }
+        return;
+    }
     const T g = (T)I.gam[f], pinf = (T)I.pinf[f];
     const double dtd = I.sd[SD_DT];
     const T dtodx = (T)(dtd / I.dx), dtody = (T)(dtd / I.dy);
@@ -914,7 +953,7 @@ __global__ __launch_bounds__(256) void k_bc(const Inst* __restrict__ insts, cons
                                             int q, int nfluid, int advance) {
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!I.si[SI_ACTIVE]) return;
+    const bool active = I.si[SI_ACTIVE] != 0;
     int i, j;
     if (frame_cell(I, w.y, w.z + threadIdx.x, i, j)) {
         bool fv, fu;
@@ -926,7 +965,7 @@ __global__ __launch_bounds__(256) void k_bc(const Inst* __restrict__ insts, cons
             st4<R>(I.Q[f][q], I, cidx(I, i, j), U);
         }
     }
-    if (advance && w.y == 0 && w.z == 0 && threadIdx.x == 0) {
+    if (active && advance && w.y == 0 && w.z == 0 && threadIdx.x == 0) {
         I.sd[SD_T] += I.sd[SD_DT];
         I.si[SI_NSTEP] += 1;
     }
@@ -966,16 +1005,13 @@ __global__ void k_mixed_rs(int n, double gL, double pL, double gR, double pR, co
 }
 
 // per-step counters for phase-by-phase use (the step path resets them in k_dt)
-__global__ void k_reset(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+__global__ void k_reset(const Inst* __restrict__ insts, const int4* __restrict__ work, int narrow_boxes) {
     const Inst& I = insts[work[blockIdx.x].x];
     if (threadIdx.x != 0) return;
     I.si[SI_ACTIVE] = 1;
     I.si[SI_NIFACE] = 0;
     I.si[SI_DTCOUNT] = 0;
-    I.si[SI_BBOX + 0] = I.g + I.Ny; I.si[SI_BBOX + 1] = I.g;
-    I.si[SI_BBOX + 2] = I.g + I.Nx; I.si[SI_BBOX + 3] = I.g;
-    I.si[SI_BBOX + 4] = I.g + I.Ny; I.si[SI_BBOX + 5] = I.g;
-    I.si[SI_BBOX + 6] = I.g + I.Nx; I.si[SI_BBOX + 7] = I.g;
+    reset_boxes(I, narrow_boxes);
 }
 
 // ---------------------------------------------------------------------------
@@ -984,14 +1020,19 @@ __global__ void k_reset(const Inst* __restrict__ insts, const int4* __restrict__
 #define GRID(n) dim3((unsigned)(n))
 
 template <typename R>
-void launch_dt(const LaunchCtx& L, int twofluid, int q) {
-    hipLaunchKernelGGL((k_dt<R>), GRID(L.n_int), dim3(256), 0, L.stream, L.insts, L.w_int, twofluid, q);
+void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes) {
+    hipLaunchKernelGGL((k_dt<R>), GRID(L.n_int), dim3(256), 0, L.stream, L.insts, L.w_int, twofluid, q,
+                       narrow_boxes);
 }
 
 template <typename R>
-void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout) {
-    if (muscl) hipLaunchKernelGGL((k_fs<R, true>), GRID(L.n_fs), dim3(256), 0, L.stream, L.insts, L.w_fs, qin, qout);
-    else hipLaunchKernelGGL((k_fs<R, false>), GRID(L.n_fs), dim3(256), 0, L.stream, L.insts, L.w_fs, qin, qout);
+void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive) {
+    if (muscl)
+        hipLaunchKernelGGL((k_fs<R, true>), GRID(L.n_fs), dim3(256), 0, L.stream, L.insts, L.w_fs, qin, qout,
+                           copy_inactive);
+    else
+        hipLaunchKernelGGL((k_fs<R, false>), GRID(L.n_fs), dim3(256), 0, L.stream, L.insts, L.w_fs, qin, qout,
+                           copy_inactive);
 }
 
 template <typename R>
@@ -1060,8 +1101,8 @@ void launch_reinit_now(const LaunchCtx& L, int iters) {
 }
 
 template <typename R>
-void launch_reset(const LaunchCtx& L) {
-    hipLaunchKernelGGL(k_reset, GRID(L.n_inst), dim3(64), 0, L.stream, L.insts, L.w_inst);
+void launch_reset(const LaunchCtx& L, int narrow_boxes) {
+    hipLaunchKernelGGL(k_reset, GRID(L.n_inst), dim3(64), 0, L.stream, L.insts, L.w_inst, narrow_boxes);
 }
 
 template <typename R>
@@ -1080,15 +1121,15 @@ void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, doub
 }
 
 #define INST(R)                                                               \
-    template void launch_dt<R>(const LaunchCtx&, int, int);                   \
-    template void launch_fs<R>(const LaunchCtx&, bool, int, int);             \
+    template void launch_dt<R>(const LaunchCtx&, int, int, int);                   \
+    template void launch_fs<R>(const LaunchCtx&, bool, int, int, int);             \
     template void launch_extrap<R>(const LaunchCtx&, int, int, int);          \
     template void launch_iface<R>(const LaunchCtx&, int, int);                \
     template void launch_mgfm_final<R>(const LaunchCtx&, int, int);           \
     template void launch_ls<R>(const LaunchCtx&, const Houc&, int, int);      \
     template void launch_reinit_now<R>(const LaunchCtx&, int);                \
     template void launch_bc<R>(const LaunchCtx&, int, int, int);               \
-    template void launch_reset<R>(const LaunchCtx&);
+    template void launch_reset<R>(const LaunchCtx&, int);
 INST(double)
 INST(float)
 

@@ -19,15 +19,15 @@ struct LaunchCtx {
 
 struct Houc;
 
-template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q);
-template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout);
+template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes);
+template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive);
 template <typename R> void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout);
 template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf);
 template <typename R> void launch_mgfm_final(const LaunchCtx& L, int qin, int qout);
 template <typename R> void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters);
 template <typename R> void launch_reinit_now(const LaunchCtx& L, int iters);
 template <typename R> void launch_bc(const LaunchCtx& L, int q, int nfluid, int advance);
-template <typename R> void launch_reset(const LaunchCtx& L);
+template <typename R> void launch_reset(const LaunchCtx& L, int narrow_boxes);
 void launch_pure_rs(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR,
                     double* F, unsigned* st);
 void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, double pR, const double* UL,

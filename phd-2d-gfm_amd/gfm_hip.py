@@ -28,8 +28,8 @@ EXPORTS = [
     "gfm_upload_state", "gfm_download_state", "gfm_upload_vfield", "gfm_download_vfield",
     "gfm_set_time", "gfm_get_time", "gfm_set_dt",
     "gfm_compute_dt", "gfm_set_ghost_states", "gfm_fs_update", "gfm_ls_update", "gfm_apply_bcs",
-    "gfm_extrapolate", "gfm_ls_reinit", "gfm_get_bbox",
-    "gfm_step", "gfm_step_graph", "gfm_sync", "gfm_profile_steps", "gfm_status", "gfm_mass_sums", "gfm_iface_info",
+    "gfm_extrapolate", "gfm_ls_reinit", "gfm_get_bbox", "gfm_set_bbox",
+    "gfm_step", "gfm_step_graph", "gfm_sync", "gfm_profile_steps", "gfm_status", "gfm_mass_sums", "gfm_mass_history", "gfm_iface_info",
     "gfm_pure_rs_solve", "gfm_mixed_rs_solve",
     "gfm_ic_onefluid", "gfm_ic_bubble", "gfm_ic_rmi", "gfm_ic_planar", "gfm_shock_state",
     "gfm_last_error",
@@ -92,9 +92,11 @@ def load():
         "gfm_ls_update": [vp], "gfm_apply_bcs": [vp],
         "gfm_extrapolate": [vp, C.c_int], "gfm_ls_reinit": [vp, C.c_int],
         "gfm_get_bbox": [vp, C.c_int, _I, _I],
+        "gfm_set_bbox": [vp, C.c_int, _I, _I],
         "gfm_step": [vp, C.c_int], "gfm_step_graph": [vp, C.c_int], "gfm_sync": [vp],
         "gfm_profile_steps": [vp, C.c_int, _D],
         "gfm_status": [vp, C.c_int, _U], "gfm_mass_sums": [vp, C.c_int, _D],
+        "gfm_mass_history": [vp, C.c_int, C.c_int, _D],
         "gfm_iface_info": [vp, C.c_int, _I, _I],
         "gfm_pure_rs_solve": [C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _D, _U],
         "gfm_mixed_rs_solve": [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _D, _D, _D, _U],
@@ -246,6 +248,11 @@ class Context:
     def mass(self, k):
         out = np.zeros(2)
         _check(load().gfm_mass_sums(self.h, k, _dp(out)), "gfm_mass_sums")
+        return out
+
+    def mass_history(self, k, n):
+        out = np.zeros((n, 3))
+        _check(load().gfm_mass_history(self.h, k, n, _dp(out)), "gfm_mass_history")
         return out
 
     def iface_info(self, k):

@@ -1,0 +1,79 @@
+"""The settings-file driver gfm2d (main.cpp / sim_base flow of the reference)
+run end to end on the GPU, its output files checked against the oracle.
+The output format prints 6 significant digits (as the reference's ofstream
+default does), so the bar is 2e-5 relative per printed field."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GFM2D = os.path.join(os.path.dirname(HERE), "phd-2d-gfm_amd", "gfm2d")
+
+
+def settings(tmp_path, **kv):
+    p = tmp_path / "settingsfile.txt"
+    kv.setdefault("outputpath", str(tmp_path) + "/")
+    p.write_text("".join("%s %s\n" % (k, v) for k, v in kv.items()))
+    return str(p)
+
+
+def read_dat(path):
+    rows = [l.split() for l in open(path) if l.strip()]
+    return np.array(rows, dtype=np.float64)
+
+
+def test_gfm2d_onefluid_circular_explosion(tmp_path, oracle):
+    f = settings(tmp_path, Nx=101, Ny=101, CFL=0.7, sim_type="onefluid", FS="MUSCL", test_case="circularexplosion")
+    out = subprocess.run([GFM2D, f], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0
+    files = [p for p in os.listdir(tmp_path) if p.endswith("-prims.dat")]
+    assert len(files) == 1
+    nsteps = int(files[0].split("-")[-2])
+    got = read_dat(os.path.join(tmp_path, files[0]))
+    G = oracle.Grid(101, 101, 2, 0.0, 0.0, 2 / 101, 2 / 101, 0, 0, 0, 0)
+    H, W = 105, 105
+    grid = np.zeros((H, W, 4))
+    oracle.lib().or_ic_onefluid(G, 1.4, 0.0, 100, oracle._p(grid))
+    s = np.zeros(1, dtype=np.int32)
+    t = np.zeros(1)
+    assert oracle.lib().or_onefluid_run(G, 1.4, 0.0, 0, 0.7, 0.25, 100000, oracle._p(grid), oracle._p(s),
+                                        oracle._p(t)) == 0
+    assert nsteps == s[0]
+    rho = grid[..., 0].reshape(-1)
+    assert np.allclose(got[:, 2], rho, rtol=2e-5, atol=0)
+    p = (0.4 * (grid[..., 3] - 0.5 * (grid[..., 1] ** 2 + grid[..., 2] ** 2) / grid[..., 0])).reshape(-1)
+    assert np.allclose(got[:, 6], p, rtol=2e-5, atol=1e-9)
+
+
+def test_gfm2d_helium_bubble(tmp_path, oracle):
+    """the reference's nondimensional shocked_helium_bubble case at 130x36, 40 steps"""
+    f = settings(tmp_path, Nx=130, Ny=36, CFL=0.4, sim_type="twofluid", FS="MUSCL",
+                 test_case="shocked_helium_bubble", ITM="LS5", GFM="riemannGFM", steps=40)
+    out = subprocess.run([GFM2D, f], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0 and "status 0" in out.stdout
+    prims = [p for p in os.listdir(tmp_path) if p.endswith("-prims.dat")][0]
+    got = read_dat(os.path.join(tmp_path, prims))
+    G = oracle.Grid(130, 36, 8, 0.0, 0.0, 325.0 / 130, 89.0 / 36, 0, 0, 1, 1)
+    E = oracle.Eos(1.667, 0.0, 1.4, 0.0)
+    H, W = 52, 146
+    g1, g2, phi = np.zeros((H, W, 4)), np.zeros((H, W, 4)), np.zeros((H, W))
+    oracle.lib().or_ic_bubble(G, E, 175.0, 44.5, 25.0, oracle._p(oracle.arr(0.138, 0, 0, 1)),
+                              oracle._p(oracle.arr(1, 0, 0, 1)), oracle._p(oracle.arr(1.3764, -0.394, 0, 1.5698)),
+                              225.0, oracle._p(g1), oracle._p(g2), oracle._p(phi))
+    run = oracle.Run(2, 0, 5, 5, 10, 0.4, 280.0, 40)
+    s = np.zeros(1, dtype=np.int32)
+    t = np.zeros(1)
+    assert oracle.lib().or_twofluid_run(G, E, run, oracle._p(g1), oracle._p(g2), oracle._p(phi), oracle._p(s),
+                                        oracle._p(t)) == 0
+    rho = np.where(phi < 0, g1[..., 0], g2[..., 0])[2:H - 2, 2:W - 2].reshape(-1)
+    assert np.allclose(got[:, 2], rho, rtol=2e-5, atol=0)
+    mass = read_dat(os.path.join(tmp_path, [p for p in os.listdir(tmp_path) if p.endswith("masschange.dat")][0]))
+    assert mass.shape == (41, 3) and mass[0, 1] == 0.0 and mass[0, 2] == 0.0
+    m1 = oracle.lib().or_fluid_mass(G, -1.0, oracle._p(g1), oracle._p(phi))
+    assert abs(mass[-1, 0] - t[0]) <= 1e-5 * t[0]
