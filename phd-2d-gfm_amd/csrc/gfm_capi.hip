@@ -351,7 +351,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
     if (c->committed) return GFM_OK;
     if (c->inst.empty()) return fail_arg("no instances");
     const size_t es = c->es();
-    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst;
+    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich;
     std::vector<Inst> descs;
     for (size_t k = 0; k < c->inst.size(); ++k) {
         InstHost& h = c->inst[k];
@@ -371,7 +371,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.use_bbox = 1;   // boxes are reset to the full domain each step unless OGFM narrows them
         const int tix = (s.Nx + 63) / 64, tiy = (s.Ny + 3) / 4;
         d.ntile_dt = tix * tiy;
-        d.icap = c->twofluid() ? std::min(s.Nx * s.Ny, 49152) : 1;
+        d.icap = c->twofluid() ? std::min(std::min(s.Nx * s.Ny, 16 * (s.Nx + s.Ny) + 1024), 49152) : 1;
         // layout of the single allocation
         const size_t P = (size_t)d.plane;
         const int nf = c->nfluid();
@@ -380,7 +380,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         size_t oQ[2][2] = {{0, 0}, {0, 0}}, oP[2][2] = {{0, 0}, {0, 0}};
         for (int f = 0; f < nf; ++f)
             for (int b = 0; b < 2; ++b) oQ[f][b] = take(4 * P * es);
-        size_t ophi = 0, oA = 0, oB = 0, ovel = 0, oimap = 0, oilist = 0, oside = 0;
+        size_t ophi = 0, oA = 0, oB = 0, ovel = 0, oimap = 0, oilist = 0, oside = 0, orec = 0, odone = 0;
         if (c->twofluid()) {
             for (int f = 0; f < 2; ++f)
                 for (int b = 0; b < 2; ++b) oP[f][b] = take(4 * P * es);
@@ -389,6 +389,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
             oimap = take(P * sizeof(int));
             oilist = take((size_t)d.icap * sizeof(int));
             oside = take((size_t)d.icap * 8 * sizeof(double));
+            orec = take((size_t)d.icap * gfm::IFREC_BYTES);
+            odone = take((size_t)d.icap * sizeof(int));
         }
         const size_t opart = take((size_t)d.ntile_dt * 4 * sizeof(double));
         d.hist_cap = c->twofluid() ? (int)std::min<long long>((long long)h.max_steps + 1, 1 << 20) : 0;
@@ -408,6 +410,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         if (c->twofluid()) {
             d.phi = base + ophi; d.phiA = base + oA; d.phiB = base + oB; d.vel = base + ovel;
             d.imap = (int*)(base + oimap); d.ilist = (int*)(base + oilist); d.side = (double*)(base + oside);
+            d.irec = base + orec; d.idone = (int*)(base + odone);
             HC(hipMemsetAsync(d.imap, 0xff, P * sizeof(int), c->stream));   // -1 everywhere
         }
         d.partials = (double*)(base + opart);
@@ -432,12 +435,15 @@ int gfm_ctx_commit(gfm_ctx* c) {
         for (int st = 0; st < 4; ++st)
             for (int o = 0; o < strip_n[st]; o += 256) w_frame.push_back(make_int4(ki, st, o, 0));
         w_inst.push_back(make_int4(ki, 0, 0, 0));
+        if (c->twofluid())
+            for (int o = 0; o < d.icap; o += 256) w_ich.push_back(make_int4(ki, o, 0, 0));
     }
     xcd_order(w_full, 0);
     xcd_order(w_fs, 0);
     HC(hipMalloc((void**)&c->d_insts, descs.size() * sizeof(Inst)));
     HC(hipMemcpyAsync(c->d_insts, descs.data(), descs.size() * sizeof(Inst), hipMemcpyHostToDevice, c->stream));
-    const size_t nw = w_full.size() + w_int.size() + w_fs.size() + w_frame.size() + w_inst.size();
+    if (w_ich.empty()) w_ich.push_back(make_int4(0, 1 << 30, 0, 0));
+    const size_t nw = w_full.size() + w_int.size() + w_fs.size() + w_frame.size() + w_inst.size() + w_ich.size();
     HC(hipMalloc((void**)&c->d_work, nw * sizeof(int4)));
     std::vector<int4> all;
     all.reserve(nw);
@@ -446,6 +452,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
     all.insert(all.end(), w_fs.begin(), w_fs.end());
     all.insert(all.end(), w_frame.begin(), w_frame.end());
     all.insert(all.end(), w_inst.begin(), w_inst.end());
+    all.insert(all.end(), w_ich.begin(), w_ich.end());
     HC(hipMemcpyAsync(c->d_work, all.data(), nw * sizeof(int4), hipMemcpyHostToDevice, c->stream));
     gfm::LaunchCtx& L = c->L;
     L.stream = c->stream;
@@ -455,7 +462,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
     L.w_int = c->d_work + o; L.n_int = (int)w_int.size(); o += w_int.size();
     L.w_fs = c->d_work + o; L.n_fs = (int)w_fs.size(); o += w_fs.size();
     L.w_frame = c->d_work + o; L.n_frame = (int)w_frame.size(); o += w_frame.size();
-    L.w_inst = c->d_work + o; L.n_inst = (int)w_inst.size();
+    L.w_inst = c->d_work + o; L.n_inst = (int)w_inst.size(); o += w_inst.size();
+    L.w_ich = c->d_work + o; L.n_ich = (int)w_ich.size();
     // initial scalars: t = 0, step 0, active
     for (auto& h : c->inst) {
         int si[gfm::SI_N] = {0};

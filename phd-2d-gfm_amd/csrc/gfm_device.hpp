@@ -47,6 +47,8 @@ struct Inst {
     int* si;                   // SI_* scalars
     double* mass_hist;         // (t, mass1, mass2) per step, hist_cap entries
     int hist_cap;
+    void* irec;                // interfacial records, icap entries
+    int* idone;                // 1 when an entry's star state is final after the parallel pass
 };
 
 // HOUC first-derivative stencil for positive velocity: offsets o0..o0+n-1
@@ -155,7 +157,9 @@ __device__ __forceinline__ double rs_f(const RSSide& s, double p, double& fd) {
 struct Star { double p, u, rl, rr; };
 
 // star state; returns status bits (ST_RS on non-convergence / vacuum / NaN)
-__device__ __forceinline__ unsigned rs_star(const RSSide& L, const RSSide& R, Star& S) {
+// p0: Newton starting pressure, used when it lies above -min(pinf); otherwise
+// the PVRS estimate (Toro 4.47).  The converged root does not depend on it.
+__device__ __forceinline__ unsigned rs_star(const RSSide& L, const RSSide& R, Star& S, double p0 = -1e300) {
     if (L.rho == R.rho && L.u == R.u && L.p == R.p && L.g == R.g && L.pinf == R.pinf) {
         S.p = L.p; S.u = L.u; S.rl = L.rho; S.rr = R.rho;
         return 0;
@@ -167,6 +171,7 @@ __device__ __forceinline__ unsigned rs_star(const RSSide& L, const RSSide& R, St
     double ppv = 0.5 * (L.p + R.p) - 0.125 * du * (L.rho + R.rho) * (L.c + R.c);
     double pfloor = plow + 1e-6 * (0.5 * (L.p + R.p) - plow);
     double p = ppv > pfloor ? ppv : pfloor;
+    if (p0 > pfloor) p = p0;
     bool conv = false;
     for (int it = 0; it < 60; ++it) {
         double dL, dR;

@@ -56,14 +56,13 @@ __device__ __forceinline__ void reset_boxes(const Inst& I, int narrow) {
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, const int4* __restrict__ work,
-                                            int twofluid, int q, int narrow_boxes) {
+                                            int twofluid, int q) {
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
     const int nstep = I.si[SI_NSTEP];
     const double t = I.sd[SD_T];
     const bool active = (t < I.T) && (nstep < I.max_steps);
     __shared__ double red[4][256];
-    __shared__ int last;
     double mu = 0.0, mv = 0.0, m1 = 0.0, m2 = 0.0;
     {
         const int i = I.g + w.y * 4 + (threadIdx.x >> 6);
@@ -109,55 +108,56 @@ __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, cons
     if (threadIdx.x == 0) {
         double* pp = I.partials + 4 * (long long)w.w;
         pp[0] = red[0][0]; pp[1] = red[1][0]; pp[2] = red[2][0]; pp[3] = red[3][0];
-        __threadfence();
-        int prev = atomicAdd(&I.si[SI_DTCOUNT], 1);
-        last = (prev == I.ntile_dt - 1);
     }
+}
+
+// second pass, one workgroup per instance: fixed-order reduction of the
+// block partials (deterministic), dt with the CFL ramp and the end-time clip
+// (sim_twofluid.cpp:66, :152-156), fluid masses, per-step counter resets
+// work: one entry per instance
+__global__ __launch_bounds__(256) void k_dt_final(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                  int twofluid, int narrow_boxes) {
+    const Inst& I = insts[work[blockIdx.x].x];
+    const int nstep = I.si[SI_NSTEP];
+    const double t = I.sd[SD_T];
+    const bool active = (t < I.T) && (nstep < I.max_steps);
+    __shared__ double red[4][256];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int k = threadIdx.x; k < I.ntile_dt; k += 256) {
+        const double* pp = I.partials + 4 * (long long)k;
+        a0 = fmax(a0, pp[0]);
+        a1 = fmax(a1, pp[1]);
+        a2 += pp[2];
+        a3 += pp[3];
+    }
+    red[0][threadIdx.x] = a0; red[1][threadIdx.x] = a1; red[2][threadIdx.x] = a2; red[3][threadIdx.x] = a3;
     __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // last block of this instance: each thread folds a fixed strided subset
-    // of the partials in order, then a fixed tree (deterministic)
-    {
-        const volatile double* pp = I.partials;
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        for (int k = threadIdx.x; k < I.ntile_dt; k += 256) {
-            a0 = fmax(a0, pp[4 * k]);
-            a1 = fmax(a1, pp[4 * k + 1]);
-            a2 += pp[4 * k + 2];
-            a3 += pp[4 * k + 3];
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+            red[2][threadIdx.x] += red[2][threadIdx.x + s];
+            red[3][threadIdx.x] += red[3][threadIdx.x + s];
         }
-        red[0][threadIdx.x] = a0; red[1][threadIdx.x] = a1; red[2][threadIdx.x] = a2; red[3][threadIdx.x] = a3;
         __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) {
-                red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
-                red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
-                red[2][threadIdx.x] += red[2][threadIdx.x + s];
-                red[3][threadIdx.x] += red[3][threadIdx.x + s];
-            }
-            __syncthreads();
-        }
     }
-    if (threadIdx.x == 0) {
-        const double a0 = red[0][0], a1 = red[1][0], a2 = red[2][0], a3 = red[3][0];
-        I.si[SI_DTCOUNT] = 0;
-        I.si[SI_ACTIVE] = active ? 1 : 0;
-        I.si[SI_NIFACE] = 0;
-        reset_boxes(I, narrow_boxes);
-        if (active) {
-            const double cfl = nstep < 5 ? fmin(I.cfl, 0.2) : I.cfl;   // :66
-            double dt = cfl * fmin(I.dx / a0, I.dy / a1);
-            if (t + dt > I.T) dt = I.T - t;
-            I.sd[SD_DT] = dt;
-        }
-        I.sd[SD_MASS1] = a2;
-        I.sd[SD_MASS2] = a3;
-        // fluid masses of the state at t_n (output_mass_error, sim_twofluid.cpp:298-331)
-        if (twofluid && nstep < I.hist_cap) {
-            double* h = I.mass_hist + 3 * (long long)nstep;
-            h[0] = t; h[1] = a2; h[2] = a3;
-        }
+    if (threadIdx.x != 0) return;
+    const double m0 = red[0][0], m1 = red[1][0], m2 = red[2][0], m3 = red[3][0];
+    I.si[SI_ACTIVE] = active ? 1 : 0;
+    I.si[SI_NIFACE] = 0;
+    reset_boxes(I, narrow_boxes);
+    if (active) {
+        const double cfl = nstep < 5 ? fmin(I.cfl, 0.2) : I.cfl;
+        double dt = cfl * fmin(I.dx / m0, I.dy / m1);
+        if (t + dt > I.T) dt = I.T - t;
+        I.sd[SD_DT] = dt;
+    }
+    I.sd[SD_MASS1] = m2;
+    I.sd[SD_MASS2] = m3;
+    // fluid masses of the state at t_n (output_mass_error, sim_twofluid.cpp:298-331)
+    if (twofluid && nstep < I.hist_cap) {
+        double* h = I.mass_hist + 3 * (long long)nstep;
+        h[0] = t; h[1] = m2; h[2] = m3;
     }
 }
 
@@ -525,10 +525,20 @@ __global__ __launch_bounds__(256) void k_iface_mark(const Inst* __restrict__ ins
     if (!I.si[SI_ACTIVE]) return;
     const int i = I.g + w.y * 4 + (threadIdx.x >> 6);
     const int j = I.g + w.z * 64 + (threadIdx.x & 63);
-    if (i >= I.g + I.Ny || j >= I.g + I.Nx) return;
+    const bool in = i < I.g + I.Ny && j < I.g + I.Nx;
+    const bool iface = in && is_iface<R>(I, i, j);
+    // one atomic per wavefront: ballot, popcount, lane prefix
+    const unsigned long long mask = __ballot(iface);
+    const int lane = threadIdx.x & 63;
+    const int cnt = __popcll(mask);
+    const int before = __popcll(mask & ((1ull << lane) - 1ull));
+    int base = 0;
+    if (cnt > 0 && lane == __ffsll((unsigned long long)mask) - 1) base = atomicAdd(&I.si[SI_NIFACE], cnt);
+    base = __shfl(base, __ffsll((unsigned long long)mask) - 1);
+    if (!in) return;
     int k = -1;
-    if (is_iface<R>(I, i, j)) {
-        k = atomicAdd(&I.si[SI_NIFACE], 1);
+    if (iface) {
+        k = base + before;
         if (k < I.icap) I.ilist[k] = i * I.W + j;
         else { k = -1; flag(I, ST_IOVF); }
     }
@@ -595,41 +605,10 @@ __device__ __forceinline__ int dep_of(const Inst& I, int ii, int jj, int f, int 
     return fr == f ? k : -1;
 }
 
-template <typename R, int VAR>
-__device__ __forceinline__ void read_prims(const Inst& I, int f, int ii, int jj, int lin, int pbuf, double* W) {
-    const long long c = cidx(I, ii, jj);
-    if (VAR != V_MODIFIED) {
-        const int d = dep_of<R>(I, ii, jj, f, lin);
-        if (d >= 0) {
-            const double* s = I.side + 8 * (long long)d;
-            W[0] = s[0]; W[1] = s[1]; W[2] = s[2]; W[3] = s[3];
-            return;
-        }
-    }
-    if (VAR == V_REAL) {
-        const V4<R> U = ld4<R>(I.Q[f][0], I, c);
-        const V4<R> P = cons2prims<R>((R)I.gam[f], (R)I.pinf[f], U);
-        W[0] = P.v[0]; W[1] = P.v[1]; W[2] = P.v[2]; W[3] = P.v[3];
-    } else {
-        const V4<R> P = ld4<R>(I.P[f][pbuf], I, c);
-        W[0] = P.v[0]; W[1] = P.v[1]; W[2] = P.v[2]; W[3] = P.v[3];
-    }
-}
-
-template <typename R, int VAR>
-__device__ __forceinline__ void bilinear_read(const Inst& I, const Probe& p, int f, int lin, int pbuf, double* out) {
-    double a[4], b[4], c[4], d[4];
-    read_prims<R, VAR>(I, f, p.i0, p.j0, lin, pbuf, a);
-    read_prims<R, VAR>(I, f, p.i0, p.j0 + 1, lin, pbuf, b);
-    read_prims<R, VAR>(I, f, p.i0 + 1, p.j0, lin, pbuf, c);
-    read_prims<R, VAR>(I, f, p.i0 + 1, p.j0 + 1, lin, pbuf, d);
-    for (int k = 0; k < 4; ++k)
-        out[k] = (1.0 - p.ty) * ((1.0 - p.tx) * a[k] + p.tx * b[k]) + p.ty * ((1.0 - p.tx) * c[k] + p.tx * d[k]);
-}
-
-// shared normal-frame mixed RP (e.g. GFM_riemann.cpp:76-121)
+// shared normal-frame mixed RP (e.g. GFM_riemann.cpp:76-121); p0 is the
+// Newton starting pressure (see rs_star)
 __device__ __forceinline__ unsigned iface_mrp(const Inst& I, const double* ip1, const double* ip2, double nx, double ny,
-                                              Star& S, double* t1, double* t2) {
+                                              Star& S, double* t1, double* t2, double p0) {
     const double u1n = ip1[1] * nx + ip1[2] * ny;
     const double u2n = ip2[1] * nx + ip2[2] * ny;
     t1[0] = ip1[1] - u1n * nx; t1[1] = ip1[2] - u1n * ny;
@@ -648,13 +627,173 @@ __device__ __forceinline__ unsigned iface_mrp(const Inst& I, const double* ip1, 
     RSSide L, R;
     rs_side(L, I.gam[0], I.pinf[0], Lw.v[0], Lw.v[1], Lw.v[3]);
     rs_side(R, I.gam[1], I.pinf[1], Rw.v[0], Rw.v[1], Rw.v[3]);
-    st |= rs_star(L, R, S);
+    st |= rs_star(L, R, S, p0);
     return st;
 }
 
+// original (unmodified) primitives of fluid f at a cell: the conversion of
+// the conserved grid for GFM_real, the extrapolated primitives otherwise
 template <typename R, int VAR>
-__global__ __launch_bounds__(512) void k_iface_solve(const Inst* __restrict__ insts, const int4* __restrict__ work,
-                                                     int pbuf) {
+__device__ __forceinline__ void orig_prims(const Inst& I, int f, int ii, int jj, int pbuf, double* W) {
+    const long long c = cidx(I, ii, jj);
+    V4<R> P;
+    if (VAR == V_REAL) P = cons2prims<R>((R)I.gam[f], (R)I.pinf[f], ld4<R>(I.Q[f][0], I, c));
+    else P = ld4<R>(I.P[f][pbuf], I, c);
+    W[0] = P.v[0]; W[1] = P.v[1]; W[2] = P.v[2]; W[3] = P.v[3];
+}
+
+// per interfacial entry: the normal, the corner values of both probes (index
+// 0: fluid 1 at -n, 1: fluid 2 at +n; GFM_real uses corner 0 with weight 1)
+// and which corners are earlier interfacial cells whose star state replaces
+// the stored value
+struct IfRec {
+    double nx, ny, lsval, pspec, preal;
+    double tx[2], ty[2];
+    double cv[2][4][4];
+    int dep[2][4];
+};
+static_assert(sizeof(IfRec) <= IFREC_BYTES, "interfacial record larger than its slot");
+
+__device__ __forceinline__ void bilerp(const double (*cv)[4], double tx, double ty, double* out) {
+    for (int k = 0; k < 4; ++k)
+        out[k] = (1.0 - ty) * ((1.0 - tx) * cv[0][k] + tx * cv[1][k]) + ty * ((1.0 - tx) * cv[2][k] + tx * cv[3][k]);
+}
+
+// star state of one interfacial cell from its probe states; writes the
+// side-buffer entry (GFM_riemann.cpp:97-145, GFM_real.cpp:152-205,
+// GFM_modified.cpp:113-196)
+template <int VAR>
+__device__ __forceinline__ unsigned iface_finish(const Inst& I, const IfRec& r, const double* ip1, const double* ip2,
+                                                 double p0, double* out, double* pstar) {
+    Star S;
+    double t1[2], t2[2];
+    unsigned st = iface_mrp(I, ip1, ip2, r.nx, r.ny, S, t1, t2, p0);
+    if (VAR == V_MODIFIED) {
+        // no-slip: the real fluid keeps its state except the density, the
+        // ghost takes the star state with the real fluid's tangential velocity
+        const double* tr = r.lsval <= 0.0 ? t1 : t2;
+        const int fr = r.lsval <= 0.0 ? 0 : 1;
+        const double rs_real = fr == 0 ? S.rl : S.rr;
+        out[0] = rs_real * pow((r.preal + I.pinf[fr]) / (S.p + I.pinf[fr]), 1.0 / I.gam[fr]);
+        out[1] = fr == 0 ? S.rr : S.rl;
+        out[2] = S.u * r.nx + tr[0];
+        out[3] = S.u * r.ny + tr[1];
+        out[4] = S.p;
+    } else if (r.lsval <= 0.0) {
+        out[0] = S.rl; out[1] = S.u * r.nx + t1[0]; out[2] = S.u * r.ny + t1[1]; out[3] = S.p;
+    } else {
+        out[0] = S.rr; out[1] = S.u * r.nx + t2[0]; out[2] = S.u * r.ny + t2[1]; out[3] = S.p;
+    }
+    if (S.p != S.p) st |= ST_RS;
+    *pstar = S.p;
+    return st;
+}
+
+// probe states from a record; dependent corners read the side buffer
+template <int VAR>
+__device__ __forceinline__ void probe_values(const Inst& I, const IfRec& r, bool use_side, double* ip1, double* ip2) {
+    double cv[2][4][4];
+    for (int p = 0; p < 2; ++p)
+        for (int q = 0; q < 4; ++q) {
+            const int d = r.dep[p][q];
+            const double* src = (use_side && d >= 0) ? I.side + 8 * (long long)d : r.cv[p][q];
+            for (int k = 0; k < 4; ++k) cv[p][q][k] = src[k];
+        }
+    if (VAR == V_REAL) {
+        for (int k = 0; k < 4; ++k) { ip1[k] = cv[0][0][k]; ip2[k] = cv[1][0][k]; }
+    } else {
+        bilerp(cv[0], r.tx[0], r.ty[0], ip1);
+        bilerp(cv[1], r.tx[1], r.ty[1], ip2);
+    }
+}
+
+// Pass 1, parallel over interfacial entries: build the records; entries
+// that read no earlier interfacial cell are final here, the others get a
+// speculative p* (from the unmodified states) as their Newton start.
+// work: {inst, first entry of a 256-entry chunk, 0, 0}
+template <typename R, int VAR>
+__global__ __launch_bounds__(256) void k_iface_prep(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                    int pbuf) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE]) return;
+    const int n = min(I.si[SI_NIFACE], I.icap);
+    const int k = w.y + threadIdx.x;
+    if (k >= n) return;
+    const int lin = I.ilist[k];
+    const int i = lin / I.W, j = lin % I.W;
+    IfRec r;
+    r.lsval = (double)ld1<R>(I.phi, cidx(I, i, j));
+    unit_normal<R>(I, i, j, r.nx, r.ny);
+    r.preal = 0.0;
+    for (int p = 0; p < 2; ++p) {
+        r.tx[p] = r.ty[p] = 0.0;
+        for (int q = 0; q < 4; ++q) r.dep[p][q] = -1;
+    }
+    unsigned st = 0;
+    bool has_dep = false;
+    if (VAR == V_REAL) {
+        // GFM_real.cpp:77-128: neighbour across the interface whose normal is
+        // closest in angle (first minimum in x-major order)
+        double mind = 1e100;
+        int io = -1000, jo = -1000;
+        for (int xi = -1; xi <= 1; ++xi)
+            for (int yi = -1; yi <= 1; ++yi) {
+                const double o = (double)ld1<R>(I.phi, cidx(I, i + yi, j + xi));
+                if (r.lsval * o <= 0.0) {
+                    double ox, oy;
+                    unit_normal<R>(I, i + yi, j + xi, ox, oy);
+                    double dp = r.nx * ox + r.ny * oy;
+                    dp = fmin(fmax(dp, -1.0), 1.0);
+                    const double ad = acos(dp);
+                    if (ad < mind) { io = i + yi; jo = j + xi; mind = ad; }
+                }
+            }
+        if (io == -1000) { io = i; jo = j; st |= ST_UNPHYS; }
+        const int fown = r.lsval <= 0.0 ? 0 : 1, fo = 1 - fown;
+        orig_prims<R, VAR>(I, fown, i, j, pbuf, r.cv[fown][0]);
+        orig_prims<R, VAR>(I, fo, io, jo, pbuf, r.cv[fo][0]);
+        r.dep[fo][0] = dep_of<R>(I, io, jo, fo, lin);
+        has_dep = r.dep[fo][0] >= 0;
+    } else {
+        Probe pr[2];
+        probes<R>(I, i, j, pr[1], pr[0]);
+        for (int p = 0; p < 2; ++p) {
+            r.tx[p] = pr[p].tx;
+            r.ty[p] = pr[p].ty;
+            for (int q = 0; q < 4; ++q) {
+                const int ii = pr[p].i0 + (q >> 1), jj = pr[p].j0 + (q & 1);
+                orig_prims<R, VAR>(I, p, ii, jj, pbuf, r.cv[p][q]);
+                if (VAR == V_RIEMANN) {
+                    r.dep[p][q] = dep_of<R>(I, ii, jj, p, lin);
+                    has_dep |= r.dep[p][q] >= 0;
+                }
+            }
+        }
+        if (VAR == V_MODIFIED) {
+            const int fr = r.lsval <= 0.0 ? 0 : 1;
+            r.preal = (double)ld4<R>(I.P[fr][pbuf], I, cidx(I, i, j)).v[3];
+        }
+    }
+    double ip1[4], ip2[4], out[8], ps;
+    probe_values<VAR>(I, r, false, ip1, ip2);
+    st |= iface_finish<VAR>(I, r, ip1, ip2, -1e300, out, &ps);
+    r.pspec = ps;
+    IfRec* recs = (IfRec*)I.irec;
+    recs[k] = r;
+    I.idone[k] = has_dep ? 0 : 1;
+    if (!has_dep) {
+        double* o = I.side + 8 * (long long)k;
+        for (int q = 0; q < 5; ++q) o[q] = out[q];
+    }
+    flag(I, st);
+}
+
+// Pass 2, one workgroup per instance: dependency rounds separated by
+// barriers; an entry is computed once every earlier interfacial cell it
+// reads has its star state.
+template <int VAR>
+__global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ insts, const int4* __restrict__ work) {
     __shared__ unsigned char fl[ICAP_LDS];
     __shared__ int pending[2];
     const int4 w = work[blockIdx.x];
@@ -662,7 +801,8 @@ __global__ __launch_bounds__(512) void k_iface_solve(const Inst* __restrict__ in
     if (!I.si[SI_ACTIVE]) return;
     const int n = min(I.si[SI_NIFACE], I.icap);
     if (n > ICAP_LDS) { if (threadIdx.x == 0) flag(I, ST_IOVF); return; }
-    for (int k = threadIdx.x; k < n; k += blockDim.x) fl[k] = 0;
+    const IfRec* recs = (const IfRec*)I.irec;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) fl[k] = (unsigned char)I.idone[k];
     unsigned st = 0;
     int round = 0;
     for (;; ++round) {
@@ -670,81 +810,19 @@ __global__ __launch_bounds__(512) void k_iface_solve(const Inst* __restrict__ in
         __syncthreads();
         for (int k = threadIdx.x; k < n; k += blockDim.x) {
             if (fl[k]) continue;
-            const int lin = I.ilist[k];
-            const int i = lin / I.W, j = lin % I.W;
-            const double lsval = (double)ld1<R>(I.phi, cidx(I, i, j));
-            double nx, ny;
-            unit_normal<R>(I, i, j, nx, ny);
-            double ip1[4], ip2[4];
-            if (VAR == V_REAL) {
-                // GFM_real.cpp:77-128: neighbour across the interface whose
-                // normal is closest in angle (first minimum in x-major order)
-                double mind = 1e100;
-                int io = -1000, jo = -1000;
-                for (int xi = -1; xi <= 1; ++xi)
-                    for (int yi = -1; yi <= 1; ++yi) {
-                        const double o = (double)ld1<R>(I.phi, cidx(I, i + yi, j + xi));
-                        if (lsval * o <= 0.0) {
-                            double ox, oy;
-                            unit_normal<R>(I, i + yi, j + xi, ox, oy);
-                            double dp = nx * ox + ny * oy;
-                            dp = fmin(fmax(dp, -1.0), 1.0);
-                            const double ad = acos(dp);
-                            if (ad < mind) { io = i + yi; jo = j + xi; mind = ad; }
-                        }
-                    }
-                if (io == -1000) { fl[k] = 2; st |= ST_UNPHYS; continue; }
-                const int fo = lsval <= 0.0 ? 1 : 0;   // fluid read at the opposite cell
-                const int d = dep_of<R>(I, io, jo, fo, lin);
-                if (d >= 0 && fl[d] != 1) { atomicAdd(&pending[round & 1], 1); continue; }
-                if (lsval <= 0.0) {
-                    read_prims<R, VAR>(I, 0, i, j, lin, pbuf, ip1);
-                    read_prims<R, VAR>(I, 1, io, jo, lin, pbuf, ip2);
-                } else {
-                    read_prims<R, VAR>(I, 1, i, j, lin, pbuf, ip2);
-                    read_prims<R, VAR>(I, 0, io, jo, lin, pbuf, ip1);
+            bool ready = true;
+            for (int p = 0; p < 2; ++p)
+                for (int q = 0; q < 4; ++q) {
+                    const int d = recs[k].dep[p][q];
+                    if (d >= 0 && fl[d] != 1) ready = false;
                 }
-            } else {
-                Probe p2, p1;
-                probes<R>(I, i, j, p2, p1);
-                if (VAR == V_RIEMANN) {
-                    bool ready = true;
-                    for (int a = 0; a < 2 && ready; ++a)
-                        for (int b = 0; b < 2 && ready; ++b) {
-                            int d = dep_of<R>(I, p2.i0 + a, p2.j0 + b, 1, lin);
-                            if (d >= 0 && fl[d] != 1) ready = false;
-                            d = dep_of<R>(I, p1.i0 + a, p1.j0 + b, 0, lin);
-                            if (d >= 0 && fl[d] != 1) ready = false;
-                        }
-                    if (!ready) { atomicAdd(&pending[round & 1], 1); continue; }
-                }
-                bilinear_read<R, VAR>(I, p2, 1, lin, pbuf, ip2);
-                bilinear_read<R, VAR>(I, p1, 0, lin, pbuf, ip1);
-            }
-            Star S;
-            double t1[2], t2[2];
-            st |= iface_mrp(I, ip1, ip2, nx, ny, S, t1, t2);
-            double* out = I.side + 8 * (long long)k;
-            const long long c = cidx(I, i, j);
-            if (VAR == V_MODIFIED) {
-                // GFM_modified.cpp:149-191 (no-slip): real fluid keeps its
-                // state except the density, the ghost takes the star state
-                const double* tr = lsval <= 0.0 ? t1 : t2;
-                const double usx = S.u * nx + tr[0], usy = S.u * ny + tr[1];
-                const int fr = lsval <= 0.0 ? 0 : 1;
-                const double preal = (double)ld4<R>(I.P[fr][pbuf], I, c).v[3];
-                const double rs_real = fr == 0 ? S.rl : S.rr;
-                out[0] = rs_real * pow((preal + I.pinf[fr]) / (S.p + I.pinf[fr]), 1.0 / I.gam[fr]);
-                out[1] = fr == 0 ? S.rr : S.rl;
-                out[2] = usx;
-                out[3] = usy;
-                out[4] = S.p;
-            } else if (lsval <= 0.0) {
-                out[0] = S.rl; out[1] = S.u * nx + t1[0]; out[2] = S.u * ny + t1[1]; out[3] = S.p;
-            } else {
-                out[0] = S.rr; out[1] = S.u * nx + t2[0]; out[2] = S.u * ny + t2[1]; out[3] = S.p;
-            }
-            if (S.p != S.p) st |= ST_RS;
+            if (!ready) { atomicAdd(&pending[round & 1], 1); continue; }
+            const IfRec& r = recs[k];
+            double ip1[4], ip2[4], out[8], ps;
+            probe_values<VAR>(I, r, true, ip1, ip2);
+            st |= iface_finish<VAR>(I, r, ip1, ip2, r.pspec, out, &ps);
+            double* o = I.side + 8 * (long long)k;
+            for (int q = 0; q < 4; ++q) o[q] = out[q];
             fl[k] = 2;
         }
         __syncthreads();
@@ -1021,8 +1099,8 @@ __global__ void k_reset(const Inst* __restrict__ insts, const int4* __restrict__
 
 template <typename R>
 void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes) {
-    hipLaunchKernelGGL((k_dt<R>), GRID(L.n_int), dim3(256), 0, L.stream, L.insts, L.w_int, twofluid, q,
-                       narrow_boxes);
+    hipLaunchKernelGGL((k_dt<R>), GRID(L.n_int), dim3(256), 0, L.stream, L.insts, L.w_int, twofluid, q);
+    hipLaunchKernelGGL(k_dt_final, GRID(L.n_inst), dim3(256), 0, L.stream, L.insts, L.w_inst, twofluid, narrow_boxes);
 }
 
 template <typename R>
@@ -1064,13 +1142,15 @@ void launch_iface(const LaunchCtx& L, int variant, int pbuf) {
     hipLaunchKernelGGL((k_iface_mark<R>), GRID(L.n_int), dim3(256), 0, L.stream, L.insts, L.w_int);
     switch (variant) {
         case V_REAL:
-            hipLaunchKernelGGL((k_iface_solve<R, V_REAL>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst, pbuf);
+            hipLaunchKernelGGL((k_iface_prep<R, V_REAL>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich, pbuf);
+            hipLaunchKernelGGL((k_iface_rounds<V_REAL>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst);
             break;
         case V_RIEMANN:
-            hipLaunchKernelGGL((k_iface_solve<R, V_RIEMANN>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst, pbuf);
+            hipLaunchKernelGGL((k_iface_prep<R, V_RIEMANN>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich, pbuf);
+            hipLaunchKernelGGL((k_iface_rounds<V_RIEMANN>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst);
             break;
-        case V_MODIFIED:
-            hipLaunchKernelGGL((k_iface_solve<R, V_MODIFIED>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst, pbuf);
+        case V_MODIFIED:   // no dependencies: the parallel pass is final
+            hipLaunchKernelGGL((k_iface_prep<R, V_MODIFIED>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich, pbuf);
             break;
         default: break;
     }

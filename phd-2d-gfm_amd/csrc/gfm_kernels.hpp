@@ -15,9 +15,11 @@ struct LaunchCtx {
     const int4* w_fs;    int n_fs;      // 32x8 interior tiles x fluids (.w = fluid)
     const int4* w_frame; int n_frame;   // 256-cell chunks of the ghost frame
     const int4* w_inst;  int n_inst;    // one entry per instance
+    const int4* w_ich;   int n_ich;     // 256-entry chunks of the interfacial lists
 };
 
 struct Houc;
+constexpr int IFREC_BYTES = 384;   // >= sizeof(IfRec), checked in the kernels
 
 template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes);
 template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive);
