@@ -164,7 +164,7 @@ def main():
         "status": sorted(set(status)),
     }
     if prec == G.MIXED and not a.no_fp64:
-        nfp = min(a.steps or 300, 300)
+        nfp = steps
         ctx64, st64, cells64 = build(cases, G.FP64, nfp + a.warmup)
         el64 = timed_run(ctx64, st64, nfp, a.warmup)
         out["fp64_plain"] = {"value": cells64 * nfp / el64, "steps": nfp,

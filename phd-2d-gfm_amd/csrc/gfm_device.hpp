@@ -154,12 +154,41 @@ __device__ __forceinline__ double rs_f(const RSSide& s, double p, double& fd) {
     return 2.0 * s.c / (s.g - 1.0) * (pw - 1.0);
 }
 
+// Starting pressure (after Toro 9.5.2): PVRS when the pressures differ by less
+// than a factor 2; otherwise the two-rarefaction estimate (exact for two
+// rarefactions of one gas) when PVRS is below both, the two-shock estimate
+// when not.  Barred pressures p + pinf throughout.
+__device__ __forceinline__ double rs_guess(const RSSide& L, const RSSide& R, double pfloor) {
+    const double du = R.u - L.u;
+    const double ppv = 0.5 * (L.p + R.p) - 0.125 * du * (L.rho + R.rho) * (L.c + R.c);
+    const double pmin = fmin(L.p, R.p);
+    const double bmin = fmin(L.pbar, R.pbar), bmax = fmax(L.pbar, R.pbar);
+    double p;
+    if (bmax < 2.0 * bmin) {   // weak waves: PVRS is already close
+        p = ppv;
+    } else if (ppv < pmin && L.g == R.g && L.pinf == R.pinf) {
+        const double z = (L.g - 1.0) / (2.0 * L.g);
+        const double num = L.c + R.c - 0.5 * (L.g - 1.0) * du;
+        const double den = L.c / pow(L.pbar, z) + R.c / pow(R.pbar, z);
+        p = (num > 0.0) ? pow(num / den, 1.0 / z) - L.pinf : pfloor;
+    } else {
+        const double ps = fmax(pfloor, ppv);
+        const double gL = sqrt(2.0 / ((L.g + 1.0) * L.rho) / (ps + L.pinf + (L.g - 1.0) / (L.g + 1.0) * L.pbar));
+        const double gR = sqrt(2.0 / ((R.g + 1.0) * R.rho) / (ps + R.pinf + (R.g - 1.0) / (R.g + 1.0) * R.pbar));
+        p = (gL * L.p + gR * R.p - du) / (gL + gR);
+    }
+    return p > pfloor ? p : pfloor;
+}
+
 struct Star { double p, u, rl, rr; };
 
 // star state; returns status bits (ST_RS on non-convergence / vacuum / NaN)
 // p0: Newton starting pressure, used when it lies above -min(pinf); otherwise
-// the PVRS estimate (Toro 4.47).  The converged root does not depend on it.
-__device__ __forceinline__ unsigned rs_star(const RSSide& L, const RSSide& R, Star& S, double p0 = -1e300) {
+// the adaptive estimate.  tol: relative pressure change that ends the Newton
+// iteration: 1e-6 (Toro's TOLPRE; ~1e-12 after the quadratic last step) for
+// the face fluxes, 1e-12 for the interfacial star states.
+__device__ __forceinline__ unsigned rs_star(const RSSide& L, const RSSide& R, Star& S, double p0 = -1e300,
+                                            double tol = 1e-6) {
     if (L.rho == R.rho && L.u == R.u && L.p == R.p && L.g == R.g && L.pinf == R.pinf) {
         S.p = L.p; S.u = L.u; S.rl = L.rho; S.rr = R.rho;
         return 0;
@@ -168,10 +197,8 @@ __device__ __forceinline__ unsigned rs_star(const RSSide& L, const RSSide& R, St
     double du = R.u - L.u;
     double plow = -fmin(L.pinf, R.pinf);
     if (2.0 * L.c / (L.g - 1.0) + 2.0 * R.c / (R.g - 1.0) <= du) st |= ST_RS;
-    double ppv = 0.5 * (L.p + R.p) - 0.125 * du * (L.rho + R.rho) * (L.c + R.c);
-    double pfloor = plow + 1e-6 * (0.5 * (L.p + R.p) - plow);
-    double p = ppv > pfloor ? ppv : pfloor;
-    if (p0 > pfloor) p = p0;
+    const double pfloor = plow + 1e-6 * (0.5 * (L.p + R.p) - plow);
+    double p = p0 > pfloor ? p0 : rs_guess(L, R, pfloor);
     bool conv = false;
     for (int it = 0; it < 60; ++it) {
         double dL, dR;
@@ -182,7 +209,7 @@ __device__ __forceinline__ unsigned rs_star(const RSSide& L, const RSSide& R, St
         double chg = fabs(pn - p);
         double ref = 0.5 * (pn + p) - plow;
         p = pn;
-        if (chg <= 1e-12 * ref) { conv = true; break; }
+        if (chg <= tol * ref) { conv = true; break; }
     }
     if (!conv || p != p) st |= ST_RS;
     double dL, dR;
@@ -251,38 +278,97 @@ __device__ __forceinline__ void rs_sample(const RSSide& L, const RSSide& R, cons
     }
 }
 
+// sampling at x/t = 0 in the storage precision: FP64 for the plain path,
+// FP32 in mixed precision (the star state itself comes from the FP64 Newton)
+template <typename T>
+__device__ __forceinline__ void rs_sample_t(const RSSide& L, const RSSide& R, const Star& st, T& orho, T& ou, T& op) {
+    if (sizeof(T) == sizeof(double)) {
+        double r, u, p;
+        rs_sample(L, R, st, 0.0, r, u, p);
+        orho = (T)r; ou = (T)u; op = (T)p;
+        return;
+    }
+    const bool left = 0.0 <= st.u;
+    RSSide s;
+    s.g = left ? L.g : R.g; s.pinf = left ? L.pinf : R.pinf; s.rho = left ? L.rho : R.rho;
+    s.u = left ? L.u : R.u; s.p = left ? L.p : R.p; s.c = left ? L.c : R.c; s.pbar = left ? L.pbar : R.pbar;
+    const T sg = left ? T(1) : T(-1);   // mirror the right side onto the left formulas
+    const T g = (T)s.g, pinf = (T)s.pinf, rho = (T)s.rho, c = (T)s.c, pbar = (T)s.pbar;
+    const T uk = sg * (T)s.u, us = sg * (T)st.u, ps = (T)st.p;
+    const T rs = left ? (T)st.rl : (T)st.rr;
+    if (st.p > s.p) {
+        const T Sk = uk - c * sqrt((g + T(1)) / (T(2) * g) * (ps + pinf) / pbar + (g - T(1)) / (T(2) * g));
+        if (T(0) <= Sk) { orho = rho; ou = (T)s.u; op = (T)s.p; }
+        else { orho = rs; ou = (T)st.u; op = ps; }
+        return;
+    }
+    const T SH = uk - c;
+    if (T(0) <= SH) { orho = rho; ou = (T)s.u; op = (T)s.p; return; }
+    const T cm = c * pow((ps + pinf) / pbar, (g - T(1)) / (T(2) * g));
+    const T ST = us - cm;
+    if (T(0) > ST) { orho = rs; ou = (T)st.u; op = ps; return; }
+    const T cf = T(2) / (g + T(1)) * (c + T(0.5) * (g - T(1)) * uk);
+    orho = rho * pow(cf / c, T(2) / (g - T(1)));
+    ou = sg * (T(2) / (g + T(1)) * (c + T(0.5) * (g - T(1)) * uk));
+    op = pbar * pow(cf / c, T(2) * g / (g - T(1))) - pinf;
+}
+
+// flux from the sampled solution (pure_RS_exact.cpp:45-71): tangential
+// velocity upwinded on u*, y-faces swap the momentum components
+template <typename T>
+__device__ __forceinline__ T tang(bool px, const V4<T>& W) { return px ? W.v[2] : W.v[1]; }
+template <typename T>
+__device__ __forceinline__ T norm_u(bool px, const V4<T>& W) { return px ? W.v[1] : W.v[2]; }
+
+template <typename T>
+__device__ __forceinline__ V4<T> rs_flux_out(bool px, T g, T pinf, T r, T u, T p, double ustar, const V4<T>& WL,
+                                             const V4<T>& WR) {
+    V4<T> soln;
+    soln.v[0] = r;
+    soln.v[1] = u;
+    const T tr = tang(px, WR), tl = tang(px, WL);
+    soln.v[2] = ustar <= 0.0 ? tr : tl;
+    soln.v[3] = p;
+    V4<T> F = flux_prims(g, pinf, soln);
+    if (!px) { T tmp = F.v[1]; F.v[1] = F.v[2]; F.v[2] = tmp; }
+    return F;
+}
+
+// identical normal states (rho, u_n, p): the exact solution is the state
+// itself, bit for bit what the Newton iteration below returns
+template <typename T>
+__device__ __forceinline__ bool rs_trivial(bool px, const V4<T>& WL, const V4<T>& WR) {
+    return WL.v[0] == WR.v[0] && norm_u(px, WL) == norm_u(px, WR) && WL.v[3] == WR.v[3];
+}
+
+template <typename T>
+__device__ __forceinline__ V4<T> rs_flux_trivial(bool px, T g, T pinf, const V4<T>& WL, const V4<T>& WR) {
+    const T un = norm_u(px, WL);
+    return rs_flux_out<T>(px, g, pinf, WL.v[0], un, WL.v[3], (double)un, WL, WR);
+}
+
+// the general case: FP64 Newton for the star state, sampling at x/t = 0
+template <typename T>
+__device__ __forceinline__ V4<T> rs_flux_full(bool px, T g, T pinf, const V4<T>& WL, const V4<T>& WR, unsigned& st) {
+    RSSide L, R;
+    rs_side(L, (double)g, (double)pinf, (double)WL.v[0], (double)norm_u(px, WL), (double)WL.v[3]);
+    rs_side(R, (double)g, (double)pinf, (double)WR.v[0], (double)norm_u(px, WR), (double)WR.v[3]);
+    Star S;
+    st |= rs_star(L, R, S);
+    T r, u, p;
+    rs_sample_t<T>(L, R, S, r, u, p);
+    return rs_flux_out<T>(px, g, pinf, r, u, p, S.u, WL, WR);
+}
+
 // pure_RS_exact::solve_RP (pure_RS_exact.cpp:9-73): conserved L/R -> flux
 template <typename T>
 __device__ __forceinline__ V4<T> pure_rs_flux(bool px, T g, T pinf, const V4<T>& UL, const V4<T>& UR,
                                               unsigned& st) {
     if (!phys_state(UL) || !phys_state(UR)) st |= ST_UNPHYS;
-    V4<T> WL = cons2prims(g, pinf, UL);
-    V4<T> WR = cons2prims(g, pinf, UR);
-    const int n = px ? 1 : 2, t = px ? 2 : 1;
-    V4<T> soln;
-    if (WL.v[0] == WR.v[0] && WL.v[n] == WR.v[n] && WL.v[3] == WR.v[3]) {
-        // identical normal states: the exact solution is the state itself;
-        // the tangential component follows the sign of u* = u_n
-        soln.v[0] = WL.v[0];
-        soln.v[1] = WL.v[n];
-        soln.v[2] = WL.v[n] <= T(0) ? WR.v[t] : WL.v[t];
-        soln.v[3] = WL.v[3];
-    } else {
-        RSSide L, R;
-        rs_side(L, (double)g, (double)pinf, (double)WL.v[0], (double)WL.v[n], (double)WL.v[3]);
-        rs_side(R, (double)g, (double)pinf, (double)WR.v[0], (double)WR.v[n], (double)WR.v[3]);
-        Star S;
-        st |= rs_star(L, R, S);
-        double r, u, p;
-        rs_sample(L, R, S, 0.0, r, u, p);
-        soln.v[0] = (T)r;
-        soln.v[1] = (T)u;
-        soln.v[2] = S.u <= 0.0 ? WR.v[t] : WL.v[t];
-        soln.v[3] = (T)p;
-    }
-    V4<T> F = flux_prims(g, pinf, soln);
-    if (!px) { T tmp = F.v[1]; F.v[1] = F.v[2]; F.v[2] = tmp; }
-    return F;
+    const V4<T> WL = cons2prims(g, pinf, UL);
+    const V4<T> WR = cons2prims(g, pinf, UR);
+    if (rs_trivial(px, WL, WR)) return rs_flux_trivial<T>(px, g, pinf, WL, WR);
+    return rs_flux_full<T>(px, g, pinf, WL, WR, st);
 }
 
 // ---------------------------------------------------------------------------

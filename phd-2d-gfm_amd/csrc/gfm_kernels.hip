@@ -184,13 +184,73 @@ __device__ __forceinline__ T minmod1(T dL, T dR) {
     return dR > T(0) ? fmax(T(0), fmin(dL, dR)) : fmin(T(0), fmax(dL, dR));
 }
 
+// Riemann-solver input states of face c of a flux tile (x-faces first,
+// then y-faces); false if no updated cell of the tile needs this face.
+// MUSCL: evolved boundary-extrapolated values with the face-level fallback
+// (FS_MUSCL.cpp:276-285) through W->U->W; Godunov: the stored states.
+template <typename R, bool MUSCL, typename T>
+__device__ __forceinline__ bool face_input(const Inst& I, int c, int i0, int j0, int lo_i, int lim_i, int lo_j,
+                                           int lim_j, T g, T pinf, const T (*sW)[FWY][FWX],
+                                           const T (*sE)[FEY][FEX], const void* Qi, V4<T>& WL, V4<T>& WR,
+                                           bool& px, unsigned& st) {
+    int la, lb, ra, rb, ci_lo, cj_lo, ci_hi, cj_hi;
+    if (c < NFX) {
+        const int rr = c / (FTX + 1), fc = c % (FTX + 1);
+        px = true;
+        la = rr + 2; lb = fc + 1; ra = rr + 2; rb = fc + 2;
+        ci_lo = i0 + rr; cj_lo = j0 + fc - 1; ci_hi = ci_lo; cj_hi = j0 + fc;
+    } else {
+        const int cc = c - NFX;
+        const int rr = cc / FTX, fc = cc % FTX;
+        px = false;
+        la = rr + 1; lb = fc + 2; ra = rr + 2; rb = fc + 2;
+        ci_lo = i0 + rr - 1; cj_lo = j0 + fc; ci_hi = i0 + rr; cj_hi = cj_lo;
+    }
+    const bool lo_upd = ci_lo >= lo_i && ci_lo < lim_i && cj_lo >= lo_j && cj_lo < lim_j;
+    const bool hi_upd = ci_hi >= lo_i && ci_hi < lim_i && cj_hi >= lo_j && cj_hi < lim_j;
+    if (!lo_upd && !hi_upd) return false;
+    V4<T> UL, UR;
+    if (MUSCL) {
+        V4<T> SL, SR;
+        bool phys = true;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            SL.v[k] = px ? sE[4 + k][la - 1][lb - 1] : sE[12 + k][la - 1][lb - 1];   // evolved R / T of lower cell
+            SR.v[k] = px ? sE[k][ra - 1][rb - 1] : sE[8 + k][ra - 1][rb - 1];        // evolved L / B of upper cell
+        }
+        phys = phys_prims(g, pinf, SR) && phys_prims(g, pinf, SL);
+        if (!phys) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { SL.v[k] = sW[k][la][lb]; SR.v[k] = sW[k][ra][rb]; }
+        }
+        UL = prims2cons(g, pinf, SL);
+        UR = prims2cons(g, pinf, SR);
+    } else {
+        const V4<R> a = ld4<R>(Qi, I, cidx(I, ci_lo, cj_lo));
+        const V4<R> b = ld4<R>(Qi, I, cidx(I, ci_hi, cj_hi));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { UL.v[k] = (T)a.v[k]; UR.v[k] = (T)b.v[k]; }
+    }
+    if (!phys_state(UL) || !phys_state(UR)) st |= ST_UNPHYS;   // pure_RS_exact.cpp:18-19
+    WL = cons2prims(g, pinf, UL);
+    WR = cons2prims(g, pinf, UR);
+    return true;
+}
+
+// FP32 storage: 4 workgroups per CU (128 VGPRs) measured fastest; FP64 is
+// insensitive to the cap
+template <typename R>
+struct FsMinBlocks { static constexpr int value = sizeof(R) == 4 ? 4 : 1; };
+
 template <typename R, bool MUSCL>
-__global__ __launch_bounds__(256) void k_fs(const Inst* __restrict__ insts, const int4* __restrict__ work,
+__global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* __restrict__ insts, const int4* __restrict__ work,
                                             int qin, int qout, int copy_inactive) {
     typedef R T;
     __shared__ T sW[4][FWY][FWX];
     __shared__ T sE[MUSCL ? 16 : 1][FEY][FEX];
-    __shared__ T sF[MUSCL ? 1 : 4 * NFACE];   // Godunov fluxes; MUSCL aliases sE
+    __shared__ T sFl[4][NFACE];
+    __shared__ short sHard[NFACE];
+    __shared__ int nHard;
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
     const int f = w.w;
@@ -220,6 +280,7 @@ __global__ __launch_bounds__(256) void k_fs(const Inst* __restrict__ insts, cons
     void* Qo = I.Q[f][qout];
     const bool tile_active = !(i0 + FTY <= bb0 || i0 >= bb1 || j0 + FTX <= bb2 || j0 >= bb3);
     const int tid = threadIdx.x;
+    if (tid == 0) nHard = 0;
     const int ci = i0 + (tid / FTX), cj = j0 + (tid % FTX);
     if (!tile_active) {
         if (ci < I.g + I.Ny && cj < I.g + I.Nx) {
@@ -277,64 +338,39 @@ __global__ __launch_bounds__(256) void k_fs(const Inst* __restrict__ insts, cons
         }
         __syncthreads();
     }
-    // C
+    // C1: every needed face gets its Riemann-solver input states (the
+    // conserved round trip of FS_MUSCL.cpp:287-288 included); identical normal
+    // states take their exact flux now, the others go to a compact list so
+    // that C2 runs the Newton iteration on full wavefronts
     const int lim_i = min(bb1, i0 + FTY), lim_j = min(bb3, j0 + FTX);
+    const int lo_i = max(bb0, i0), lo_j = max(bb2, j0);
     unsigned st = 0;
-    V4<T> fl[3];
-    int fid[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        const int c = tid + 256 * r;
-        fid[r] = -1;
-        if (c >= NFACE) continue;
+    for (int c = tid; c < NFACE; c += 256) {
+        V4<T> WL, WR;
         bool px;
-        int la, lb, ra, rb;   // sW coordinates of the left/lower and right/upper cell
-        int ci_lo, cj_lo, ci_hi, cj_hi;
-        if (c < NFX) {
-            const int rr = c / (FTX + 1), fc = c % (FTX + 1);
-            px = true;
-            la = rr + 2; lb = fc + 1; ra = rr + 2; rb = fc + 2;
-            ci_lo = i0 + rr; cj_lo = j0 + fc - 1; ci_hi = ci_lo; cj_hi = j0 + fc;
-        } else {
-            const int cc = c - NFX;
-            const int rr = cc / FTX, fc = cc % FTX;
-            px = false;
-            la = rr + 1; lb = fc + 2; ra = rr + 2; rb = fc + 2;
-            ci_lo = i0 + rr - 1; cj_lo = j0 + fc; ci_hi = i0 + rr; cj_hi = cj_lo;
-        }
-        // needed only if one side is an updated cell of this tile
-        const bool lo_upd = ci_lo >= max(bb0, i0) && ci_lo < lim_i && cj_lo >= max(bb2, j0) && cj_lo < lim_j;
-        const bool hi_upd = ci_hi >= max(bb0, i0) && ci_hi < lim_i && cj_hi >= max(bb2, j0) && cj_hi < lim_j;
-        if (!lo_upd && !hi_upd) continue;
-        V4<T> SL, SR, WL, WR;
+        if (!face_input<R, MUSCL>(I, c, i0, j0, lo_i, lim_i, lo_j, lim_j, g, pinf, sW, sE, Qi, WL, WR, px, st)) continue;
+        if (rs_trivial(px, WL, WR)) {
+            const V4<T> F = rs_flux_trivial<T>(px, g, pinf, WL, WR);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) { WL.v[k] = sW[k][la][lb]; WR.v[k] = sW[k][ra][rb]; }
-        if (MUSCL) {
-            const int lea = la - 1, leb = lb - 1, rea = ra - 1, reb = rb - 1;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                SL.v[k] = px ? sE[4 + k][lea][leb] : sE[12 + k][lea][leb];   // evolved R / T of lower cell
-                SR.v[k] = px ? sE[k][rea][reb] : sE[8 + k][rea][reb];        // evolved L / B of upper cell
-            }
-            if (!phys_prims(g, pinf, SR) || !phys_prims(g, pinf, SL)) { SR = WR; SL = WL; }   // :276-285
-            fl[r] = pure_rs_flux<T>(px, g, pinf, prims2cons(g, pinf, SL), prims2cons(g, pinf, SR), st);
+            for (int k = 0; k < 4; ++k) sFl[k][c] = F.v[k];
         } else {
-            // Godunov passes the stored conserved states (FS_godunov.cpp:34-35)
-            const V4<R> UL = ld4<R>(Qi, I, cidx(I, ci_lo, cj_lo));
-            const V4<R> UR = ld4<R>(Qi, I, cidx(I, ci_hi, cj_hi));
-            fl[r] = pure_rs_flux<T>(px, g, pinf, UL, UR, st);
+            sHard[atomicAdd(&nHard, 1)] = (short)c;
         }
-        fid[r] = c;
+    }
+    __syncthreads();
+    // C2: the exact Riemann solves of the listed faces
+    const int nh = nHard;
+    for (int h = tid; h < nh; h += 256) {
+        const int c = sHard[h];
+        V4<T> WL, WR;
+        bool px;
+        unsigned dummy = 0;
+        face_input<R, MUSCL>(I, c, i0, j0, lo_i, lim_i, lo_j, lim_j, g, pinf, sW, sE, Qi, WL, WR, px, dummy);
+        const V4<T> F = rs_flux_full<T>(px, g, pinf, WL, WR, st);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sFl[k][c] = F.v[k];
     }
     flag(I, st);
-    __syncthreads();
-    T* sFl = MUSCL ? &sE[0][0][0] : &sF[0];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-        if (fid[r] >= 0) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) sFl[k * NFACE + fid[r]] = fl[r].v[k];
-        }
     __syncthreads();
     // D
     if (ci < I.g + I.Ny && cj < I.g + I.Nx) {
@@ -347,11 +383,10 @@ __global__ __launch_bounds__(256) void k_fs(const Inst* __restrict__ insts, cons
             const int fyb = NFX + rr * FTX + fc, fyt = fyb + FTX;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const T* Fk = sFl + k * NFACE;
                 T u = (T)U.v[k];
-                u = u + (dtodx * Fk[fxl] + dtody * Fk[fyb]);   // :290
-                u = u - dtodx * Fk[fxr];                        // :291 of the next column
-                u = u - dtody * Fk[fyt];                        // :292 of the next row
+                u = u + (dtodx * sFl[k][fxl] + dtody * sFl[k][fyb]);   // :290
+                u = u - dtodx * sFl[k][fxr];                             // :291 of the next column
+                u = u - dtody * sFl[k][fyt];                             // :292 of the next row
                 U.v[k] = (R)u;
             }
         }
@@ -627,7 +662,7 @@ __device__ __forceinline__ unsigned iface_mrp(const Inst& I, const double* ip1, 
     RSSide L, R;
     rs_side(L, I.gam[0], I.pinf[0], Lw.v[0], Lw.v[1], Lw.v[3]);
     rs_side(R, I.gam[1], I.pinf[1], Rw.v[0], Rw.v[1], Rw.v[3]);
-    st |= rs_star(L, R, S, p0);
+    st |= rs_star(L, R, S, p0, 1e-12);
     return st;
 }
 
@@ -1077,7 +1112,7 @@ __global__ void k_mixed_rs(int n, double gL, double pL, double gR, double pR, co
     rs_side(a, gL, pL, WL.v[0], WL.v[1], WL.v[3]);
     rs_side(b, gR, pR, WR.v[0], WR.v[1], WR.v[3]);
     Star S;
-    st |= rs_star(a, b, S);
+    st |= rs_star(a, b, S, -1e300, 1e-12);
     out[4 * t] = S.p; out[4 * t + 1] = S.u; out[4 * t + 2] = S.rl; out[4 * t + 3] = S.rr;
     if (st) atomicOr(status, st);
 }

@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgfm_hip.so")
+# GFM_HIP_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("GFM_HIP_LIB", os.path.join(HERE, "libgfm_hip.so"))
 
 BC_TRANSMISSIVE, BC_REFLECTIVE, BC_PERIODIC = 0, 1, 2
 SIM_ONEFLUID, SIM_TWOFLUID = 0, 1
