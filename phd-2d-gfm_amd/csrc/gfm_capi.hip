@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -79,6 +80,7 @@ struct gfm_ctx {
     gfm::Houc houc{};
     int cur = 0;                       // current conserved buffer (one-fluid step path)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    bool tb = false;                   // temporally blocked extrapolation (no periodic instance)
     bool mixed() const { return m.precision == GFM_MIXED; }
     bool twofluid() const { return m.sim_type == GFM_SIM_TWOFLUID; }
     int nfluid() const { return twofluid() ? 2 : 1; }
@@ -98,6 +100,15 @@ void dispatch_dt(gfm_ctx* c, int q) { gfm::launch_dt<R>(c->L, c->twofluid() ? 1 
 template <typename R>
 int extrap_chain(gfm_ctx* c, int N, bool from_q, int qin, int pstart, bool inject_first, bool to_q, int qout,
                  int last_extra) {
+    if (c->tb && N >= 2) {
+        // one launch for all N iterations; it writes only the last one's
+        // destination, which is the same buffer as in the loop below
+        int cur_p = pstart;
+        for (int k = 0; k < N && !(k == N - 1 && to_q); ++k) cur_p = (k == 0 && from_q) ? 0 : 1 - cur_p;
+        const int min = (from_q ? 1 : 0) | (inject_first ? 2 : 0);
+        const int mout = to_q ? (4 | last_extra) : 0;
+        if (gfm::launch_extrap_tb<R>(c->L, N, min, mout, from_q ? qin : pstart, to_q ? qout : cur_p)) return cur_p;
+    }
     int cur_p = pstart;
     for (int k = 0; k < N; ++k) {
         int mode = 0, bin, bout;
@@ -272,7 +283,7 @@ static int rs_common(int n, const double* UL, const double* UR, double* out, uns
     unsigned st = 0;
     HC(hipMemcpy(out, dO, bytes, hipMemcpyDeviceToHost));
     HC(hipMemcpy(&st, dS, sizeof(unsigned), hipMemcpyDeviceToHost));
-    hipFree(dL); hipFree(dR); hipFree(dO); hipFree(dS);
+    (void)hipFree(dL); (void)hipFree(dR); (void)hipFree(dO); (void)hipFree(dS);
     if (status) *status = st;
     return GFM_OK;
 }
@@ -304,14 +315,15 @@ int gfm_ctx_create(const gfm_methods* methods, gfm_ctx** out) {
 
 int gfm_ctx_destroy(gfm_ctx* c) {
     if (!c) return GFM_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    // teardown: errors here leave nothing to recover, so they are not reported
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& g : c->gexec)
-        if (g) hipGraphExecDestroy(g);
+        if (g) (void)hipGraphExecDestroy(g);
     for (auto& h : c->inst)
-        if (h.mem) hipFree(h.mem);
-    if (c->d_insts) hipFree(c->d_insts);
-    if (c->d_work) hipFree(c->d_work);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (h.mem) (void)hipFree(h.mem);
+    if (c->d_insts) (void)hipFree(c->d_insts);
+    if (c->d_work) (void)hipFree(c->d_work);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GFM_OK;
 }
@@ -351,8 +363,9 @@ int gfm_ctx_commit(gfm_ctx* c) {
     if (c->committed) return GFM_OK;
     if (c->inst.empty()) return fail_arg("no instances");
     const size_t es = c->es();
-    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich;
+    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich, w_tb;
     std::vector<Inst> descs;
+    bool periodic = false;
     for (size_t k = 0; k < c->inst.size(); ++k) {
         InstHost& h = c->inst[k];
         const gfm_sim_info& s = h.info;
@@ -367,6 +380,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.gam[0] = h.eos.gamma1; d.pinf[0] = h.eos.pinf1;
         d.gam[1] = c->twofluid() ? h.eos.gamma2 : h.eos.gamma1;
         d.pinf[1] = c->twofluid() ? h.eos.pinf2 : h.eos.pinf1;
+        for (int f = 0; f < 2; ++f) gfm::rs_fluid(d.rsf[f], d.gam[f], d.pinf[f]);
         d.cfl = h.cfl; d.T = h.T; d.max_steps = h.max_steps;
         d.use_bbox = 1;   // boxes are reset to the full domain each step unless OGFM narrows them
         const int tix = (s.Nx + 63) / 64, tiy = (s.Ny + 3) / 4;
@@ -436,14 +450,22 @@ int gfm_ctx_commit(gfm_ctx* c) {
             for (int o = 0; o < strip_n[st]; o += 256) w_frame.push_back(make_int4(ki, st, o, 0));
         w_inst.push_back(make_int4(ki, 0, 0, 0));
         if (c->twofluid())
+            for (int ty = 0; ty < (s.Ny + 15) / 16; ++ty)
+                for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_tb.push_back(make_int4(ki, ty, tx, 0));
+        if (s.BC_L == GFM_BC_PERIODIC || s.BC_B == GFM_BC_PERIODIC) periodic = true;
+        if (c->twofluid())
             for (int o = 0; o < d.icap; o += 256) w_ich.push_back(make_int4(ki, o, 0, 0));
     }
     xcd_order(w_full, 0);
     xcd_order(w_fs, 0);
+    xcd_order(w_tb, 0);
+    const char* no_tb = std::getenv("GFM_NO_TEMPORAL_BLOCKING");
+    c->tb = c->twofluid() && !periodic && !(no_tb && no_tb[0] == '1');
     HC(hipMalloc((void**)&c->d_insts, descs.size() * sizeof(Inst)));
     HC(hipMemcpyAsync(c->d_insts, descs.data(), descs.size() * sizeof(Inst), hipMemcpyHostToDevice, c->stream));
     if (w_ich.empty()) w_ich.push_back(make_int4(0, 1 << 30, 0, 0));
-    const size_t nw = w_full.size() + w_int.size() + w_fs.size() + w_frame.size() + w_inst.size() + w_ich.size();
+    const size_t nw =
+        w_full.size() + w_int.size() + w_fs.size() + w_frame.size() + w_inst.size() + w_ich.size() + w_tb.size();
     HC(hipMalloc((void**)&c->d_work, nw * sizeof(int4)));
     std::vector<int4> all;
     all.reserve(nw);
@@ -453,6 +475,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
     all.insert(all.end(), w_frame.begin(), w_frame.end());
     all.insert(all.end(), w_inst.begin(), w_inst.end());
     all.insert(all.end(), w_ich.begin(), w_ich.end());
+    all.insert(all.end(), w_tb.begin(), w_tb.end());
     HC(hipMemcpyAsync(c->d_work, all.data(), nw * sizeof(int4), hipMemcpyHostToDevice, c->stream));
     gfm::LaunchCtx& L = c->L;
     L.stream = c->stream;
@@ -463,7 +486,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
     L.w_fs = c->d_work + o; L.n_fs = (int)w_fs.size(); o += w_fs.size();
     L.w_frame = c->d_work + o; L.n_frame = (int)w_frame.size(); o += w_frame.size();
     L.w_inst = c->d_work + o; L.n_inst = (int)w_inst.size(); o += w_inst.size();
-    L.w_ich = c->d_work + o; L.n_ich = (int)w_ich.size();
+    L.w_ich = c->d_work + o; L.n_ich = (int)w_ich.size(); o += w_ich.size();
+    L.w_tb = c->d_work + o; L.n_tb = (int)w_tb.size();
     // initial scalars: t = 0, step 0, active
     for (auto& h : c->inst) {
         int si[gfm::SI_N] = {0};
@@ -737,7 +761,7 @@ int gfm_step_graph(gfm_ctx* c, int nsteps) {
         else step_launches<double>(c, k);
         HC(hipStreamEndCapture(c->stream, &g));
         hipError_t e = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         if (e != hipSuccess) return fail("hipGraphInstantiate", e);
     }
     for (int s = 0; s < nsteps; ++s) {

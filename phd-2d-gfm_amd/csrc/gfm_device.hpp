@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "gfm_fastmath.hpp"
+
 namespace gfm {
 
 enum { BC_T = 0, BC_R = 1, BC_P = 2 };
@@ -20,6 +22,29 @@ enum {
 };
 enum { ST_UNPHYS = 1, ST_RS = 2, ST_IOVF = 4, ST_IROUNDS = 8 };
 
+// Per-fluid constants of the mixed-precision face Riemann solver, set on the
+// host at commit: z = (g-1)/(2g), g6 = (g-1)/(g+1), a2 = 2/(g+1), ig = 1/g,
+// k2 = 2/(g-1), bc[k-1] = C(z, k) for k = 1..9.
+struct RSFluid {
+    double g, pinf, z, g6, a2, ig, k2;
+    double bc[9];
+};
+
+__host__ __device__ inline void rs_fluid(RSFluid& F, double g, double pinf) {
+    F.g = g; F.pinf = pinf;
+    F.z = (g - 1.0) / (2.0 * g);
+    F.g6 = (g - 1.0) / (g + 1.0);
+    F.a2 = 2.0 / (g + 1.0);
+    F.ig = 1.0 / g;
+    F.k2 = 2.0 / (g - 1.0);
+    double c = F.z;
+    F.bc[0] = c;
+    for (int k = 2; k <= 9; ++k) {
+        c = c * (F.z - (double)(k - 1)) / (double)k;
+        F.bc[k - 1] = c;
+    }
+}
+
 // One simulation.  Plane arrays are row-major with row pitch `pitch`
 // elements; a 4-component field is 4 consecutive planes of `plane` elements.
 struct Inst {
@@ -28,6 +53,7 @@ struct Inst {
     double x0, y0, dx, dy;
     int bc[4];                 // L, R, B, T
     double gam[2], pinf[2];
+    RSFluid rsf[2];
     double cfl, T;
     int max_steps;
     int ntile_dt;
@@ -62,10 +88,19 @@ __device__ __forceinline__ long long cidx(const Inst& I, int i, int j) {
 // stiffened-gas EOS (closed forms; the reference takes them from PhD-Common
 // stiffened_gas_eos.hpp, used at euler_misc.hpp:46,59,77,156)
 // ---------------------------------------------------------------------------
+// Conversions round every operation on its own (no FMA contraction), as the
+// reference's compiled C++ does: each kernel that converts the same state then
+// gets the same bits, whatever the surrounding code lets the compiler fuse.
 template <typename T>
-__device__ __forceinline__ T eos_p(T g, T pinf, T e, T rho) { return (g - T(1)) * rho * e - g * pinf; }
+__device__ __forceinline__ T eos_p(T g, T pinf, T e, T rho) {
+#pragma clang fp contract(off)
+    return (g - T(1)) * rho * e - g * pinf;
+}
 template <typename T>
-__device__ __forceinline__ T eos_e(T g, T pinf, T p, T rho) { return (p + g * pinf) / ((g - T(1)) * rho); }
+__device__ __forceinline__ T eos_e(T g, T pinf, T p, T rho) {
+#pragma clang fp contract(off)
+    return (p + g * pinf) / ((g - T(1)) * rho);
+}
 template <typename T>
 __device__ __forceinline__ T eos_c(T g, T pinf, T p, T rho) { return sqrt(g * (p + pinf) / rho); }
 
@@ -75,6 +110,7 @@ struct V4 { T v[4]; };
 // euler_misc.hpp:52-67
 template <typename T>
 __device__ __forceinline__ V4<T> cons2prims(T g, T pinf, const V4<T>& U) {
+#pragma clang fp contract(off)
     V4<T> W;
     T rho = U.v[0];
     T u = U.v[1] / rho;
@@ -87,6 +123,7 @@ __device__ __forceinline__ V4<T> cons2prims(T g, T pinf, const V4<T>& U) {
 // euler_misc.hpp:89-108
 template <typename T>
 __device__ __forceinline__ V4<T> prims2cons(T g, T pinf, const V4<T>& W) {
+#pragma clang fp contract(off)
     V4<T> U;
     T rho = W.v[0], u = W.v[1], v = W.v[2];
     T e = eos_e(g, pinf, W.v[3], rho);
@@ -278,38 +315,135 @@ __device__ __forceinline__ void rs_sample(const RSSide& L, const RSSide& R, cons
     }
 }
 
-// sampling at x/t = 0 in the storage precision: FP64 for the plain path,
-// FP32 in mixed precision (the star state itself comes from the FP64 Newton)
-template <typename T>
-__device__ __forceinline__ void rs_sample_t(const RSSide& L, const RSSide& R, const Star& st, T& orho, T& ou, T& op) {
-    if (sizeof(T) == sizeof(double)) {
-        double r, u, p;
-        rs_sample(L, R, st, 0.0, r, u, p);
-        orho = (T)r; ou = (T)u; op = (T)p;
-        return;
+// ---------------------------------------------------------------------------
+// Mixed-precision face solver (FP32 storage).  The star pressure and velocity
+// come from rs_star's Newton iteration, in FP64, on the same pressure
+// functions (Toro 4.6-4.7).  What differs is the evaluation: the fluid's
+// constants are computed once per workgroup, (pbar / pbarK)^z - 1 is summed
+// as its binomial series when |pbar / pbarK - 1| < 1/64 (no cancellation)
+// and from pow_fast otherwise, the rarefaction derivative uses
+// z k r^(z-1) / pbarK = (c / g) r^z / pbar, and the star densities are left to
+// the FP32 sampling, which needs at most one of them.
+// ---------------------------------------------------------------------------
+// one side: rho, u_n, p, pbar = p + pinf, 1 / pbar, 1 / rho, c; the
+// shock and rarefaction constants A = a2 / rho, B = g6 pbar, k = k2 c and
+// c / g are formed where used
+struct RSSideM { double rho, u, p, pbar, ipb, irho, c; };
+
+__device__ __forceinline__ void rs_side_m(RSSideM& s, const RSFluid& F, double rho, double u, double p) {
+    s.rho = rho; s.u = u; s.p = p;
+    s.pbar = p + F.pinf;
+    s.ipb = 1.0 / s.pbar;
+    s.irho = 1.0 / rho;
+    s.c = sqrt(F.g * s.pbar * s.irho);
+}
+
+__device__ __forceinline__ double rs_f_m(const RSFluid& F, const RSSideM& s, double p, double& fd) {
+    const double pbar = p + F.pinf;
+    if (p > s.p) {   // shock (Toro 4.6)
+        const double t = 1.0 / (pbar + F.g6 * s.pbar);
+        const double q = sqrt(F.a2 * s.irho * t);
+        fd = q * (1.0 - 0.5 * (p - s.p) * t);
+        return (p - s.p) * q;
     }
+    // rarefaction (Toro 4.7): f = k ((pbar/pbarK)^z - 1)
+    const double eps = (p - s.p) * s.ipb;
+    double w;
+    if (fabs(eps) < 0.015625) {
+        double q = F.bc[8];
+#pragma unroll
+        for (int k = 7; k >= 0; --k) q = fma(q, eps, F.bc[k]);
+        w = q * eps;
+    } else {
+        w = pow_fast(pbar * s.ipb, F.z) - 1.0;
+    }
+    fd = s.c * F.ig * (1.0 + w) / pbar;
+    return F.k2 * s.c * w;
+}
+
+struct StarM { double p, u; };
+
+// rs_star's iteration (same guess, tolerance, floor and status bits) on the
+// mixed-precision evaluation
+__device__ __forceinline__ unsigned rs_star_m(const RSFluid& F, const RSSideM& L, const RSSideM& R, StarM& S) {
+    if (L.rho == R.rho && L.u == R.u && L.p == R.p) {
+        S.p = L.p; S.u = L.u;
+        return 0;
+    }
+    unsigned st = 0;
+    const double du = R.u - L.u;
+    const double plow = -F.pinf;
+    if (F.k2 * (L.c + R.c) <= du) st |= ST_RS;
+    const double pfloor = plow + 1e-6 * (0.5 * (L.p + R.p) - plow);
+    double p;
+    {
+        const double ppv = 0.5 * (L.p + R.p) - 0.125 * du * (L.rho + R.rho) * (L.c + R.c);
+        const double bmin = fmin(L.pbar, R.pbar), bmax = fmax(L.pbar, R.pbar);
+        if (bmax < 2.0 * bmin) {
+            p = ppv;
+        } else if (ppv < fmin(L.p, R.p)) {   // two rarefactions: exact for one gas
+            const double num = L.c + R.c - 0.5 * (F.g - 1.0) * du;
+            const double den = L.c / pow_fast(L.pbar, F.z) + R.c / pow_fast(R.pbar, F.z);
+            p = (num > 0.0) ? pow_fast(num / den, 1.0 / F.z) - F.pinf : pfloor;
+        } else {                             // two shocks
+            const double ps = fmax(pfloor, ppv);
+            const double gL = sqrt(F.a2 * L.irho / (ps + F.pinf + F.g6 * L.pbar));
+            const double gR = sqrt(F.a2 * R.irho / (ps + F.pinf + F.g6 * R.pbar));
+            p = (gL * L.p + gR * R.p - du) / (gL + gR);
+        }
+        p = p > pfloor ? p : pfloor;
+    }
+    bool conv = false;
+    for (int it = 0; it < 60; ++it) {
+        double dL, dR;
+        const double fL = rs_f_m(F, L, p, dL);
+        const double fR = rs_f_m(F, R, p, dR);
+        double pn = p - (fL + fR + du) / (dL + dR);
+        if (pn <= plow) pn = 0.5 * (p + plow);
+        const double chg = fabs(pn - p);
+        const double ref = 0.5 * (pn + p) - plow;
+        p = pn;
+        if (chg <= 1e-6 * ref) { conv = true; break; }
+    }
+    if (!conv || p != p) st |= ST_RS;
+    double dL, dR;
+    const double fL = rs_f_m(F, L, p, dL);
+    const double fR = rs_f_m(F, R, p, dR);
+    S.p = p;
+    S.u = 0.5 * (L.u + R.u) + 0.5 * (fR - fL);
+    return st;
+}
+
+// Toro 4.5 sampling at x/t = 0 in FP32, the side mirrored onto the left-hand
+// formulas; the star density of the sampled side is computed here, in FP32
+__device__ __forceinline__ void rs_sample_m(const RSFluid& F, const RSSideM& L, const RSSideM& R, const StarM& st,
+                                            float& orho, float& ou, float& op) {
+    typedef float T;
     const bool left = 0.0 <= st.u;
-    RSSide s;
-    s.g = left ? L.g : R.g; s.pinf = left ? L.pinf : R.pinf; s.rho = left ? L.rho : R.rho;
-    s.u = left ? L.u : R.u; s.p = left ? L.p : R.p; s.c = left ? L.c : R.c; s.pbar = left ? L.pbar : R.pbar;
-    const T sg = left ? T(1) : T(-1);   // mirror the right side onto the left formulas
-    const T g = (T)s.g, pinf = (T)s.pinf, rho = (T)s.rho, c = (T)s.c, pbar = (T)s.pbar;
-    const T uk = sg * (T)s.u, us = sg * (T)st.u, ps = (T)st.p;
-    const T rs = left ? (T)st.rl : (T)st.rr;
-    if (st.p > s.p) {
-        const T Sk = uk - c * sqrt((g + T(1)) / (T(2) * g) * (ps + pinf) / pbar + (g - T(1)) / (T(2) * g));
-        if (T(0) <= Sk) { orho = rho; ou = (T)s.u; op = (T)s.p; }
-        else { orho = rs; ou = (T)st.u; op = ps; }
+    const double srho = left ? L.rho : R.rho, su = left ? L.u : R.u, sp = left ? L.p : R.p;
+    const double sc = left ? L.c : R.c, spbar = left ? L.pbar : R.pbar;
+    const T sg = left ? T(1) : T(-1);
+    const T g = (T)F.g, pinf = (T)F.pinf, rho = (T)srho, c = (T)sc, pbar = (T)spbar;
+    const T uk = sg * (T)su, us = sg * (T)st.u, ps = (T)st.p;
+    const T ratio = (ps + pinf) / pbar;
+    if (st.p > sp) {
+        const T Sk = uk - c * sqrt((g + T(1)) / (T(2) * g) * ratio + (g - T(1)) / (T(2) * g));
+        if (T(0) <= Sk) { orho = rho; ou = (T)su; op = (T)sp; }
+        else {
+            const T g6 = (T)F.g6;
+            orho = rho * (ratio + g6) / (g6 * ratio + T(1));
+            ou = (T)st.u; op = ps;
+        }
         return;
     }
     const T SH = uk - c;
-    if (T(0) <= SH) { orho = rho; ou = (T)s.u; op = (T)s.p; return; }
-    const T cm = c * pow((ps + pinf) / pbar, (g - T(1)) / (T(2) * g));
+    if (T(0) <= SH) { orho = rho; ou = (T)su; op = (T)sp; return; }
+    const T cm = c * pow(ratio, (g - T(1)) / (T(2) * g));
     const T ST = us - cm;
-    if (T(0) > ST) { orho = rs; ou = (T)st.u; op = ps; return; }
+    if (T(0) > ST) { orho = rho * pow(ratio, T(1) / g); ou = (T)st.u; op = ps; return; }
     const T cf = T(2) / (g + T(1)) * (c + T(0.5) * (g - T(1)) * uk);
     orho = rho * pow(cf / c, T(2) / (g - T(1)));
-    ou = sg * (T(2) / (g + T(1)) * (c + T(0.5) * (g - T(1)) * uk));
+    ou = sg * cf;
     op = pbar * pow(cf / c, T(2) * g / (g - T(1))) - pinf;
 }
 
@@ -347,17 +481,30 @@ __device__ __forceinline__ V4<T> rs_flux_trivial(bool px, T g, T pinf, const V4<
     return rs_flux_out<T>(px, g, pinf, WL.v[0], un, WL.v[3], (double)un, WL, WR);
 }
 
-// the general case: FP64 Newton for the star state, sampling at x/t = 0
-template <typename T>
-__device__ __forceinline__ V4<T> rs_flux_full(bool px, T g, T pinf, const V4<T>& WL, const V4<T>& WR, unsigned& st) {
+// the general case, FP64 storage: rs_star, then Toro's sampling at x/t = 0
+__device__ __forceinline__ V4<double> rs_flux_full(bool px, double g, double pinf, const V4<double>& WL,
+                                                   const V4<double>& WR, unsigned& st) {
     RSSide L, R;
-    rs_side(L, (double)g, (double)pinf, (double)WL.v[0], (double)norm_u(px, WL), (double)WL.v[3]);
-    rs_side(R, (double)g, (double)pinf, (double)WR.v[0], (double)norm_u(px, WR), (double)WR.v[3]);
+    rs_side(L, g, pinf, WL.v[0], norm_u(px, WL), WL.v[3]);
+    rs_side(R, g, pinf, WR.v[0], norm_u(px, WR), WR.v[3]);
     Star S;
     st |= rs_star(L, R, S);
-    T r, u, p;
-    rs_sample_t<T>(L, R, S, r, u, p);
-    return rs_flux_out<T>(px, g, pinf, r, u, p, S.u, WL, WR);
+    double r, u, p;
+    rs_sample(L, R, S, 0.0, r, u, p);
+    return rs_flux_out<double>(px, g, pinf, r, u, p, S.u, WL, WR);
+}
+
+// FP32 storage: the mixed-precision solver above
+__device__ __forceinline__ V4<float> rs_flux_full_m(const RSFluid& F, bool px, float g, float pinf,
+                                                    const V4<float>& WL, const V4<float>& WR, unsigned& st) {
+    RSSideM L, R;
+    rs_side_m(L, F, (double)WL.v[0], (double)norm_u(px, WL), (double)WL.v[3]);
+    rs_side_m(R, F, (double)WR.v[0], (double)norm_u(px, WR), (double)WR.v[3]);
+    StarM S;
+    st |= rs_star_m(F, L, R, S);
+    float r, u, p;
+    rs_sample_m(F, L, R, S, r, u, p);
+    return rs_flux_out<float>(px, g, pinf, r, u, p, S.u, WL, WR);
 }
 
 // pure_RS_exact::solve_RP (pure_RS_exact.cpp:9-73): conserved L/R -> flux
@@ -368,7 +515,7 @@ __device__ __forceinline__ V4<T> pure_rs_flux(bool px, T g, T pinf, const V4<T>&
     const V4<T> WL = cons2prims(g, pinf, UL);
     const V4<T> WR = cons2prims(g, pinf, UR);
     if (rs_trivial(px, WL, WR)) return rs_flux_trivial<T>(px, g, pinf, WL, WR);
-    return rs_flux_full<T>(px, g, pinf, WL, WR, st);
+    return rs_flux_full(px, g, pinf, WL, WR, st);
 }
 
 // ---------------------------------------------------------------------------

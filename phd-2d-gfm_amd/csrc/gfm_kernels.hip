@@ -360,13 +360,16 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
     __syncthreads();
     // C2: the exact Riemann solves of the listed faces
     const int nh = nHard;
+    const RSFluid& rf = I.rsf[f];
     for (int h = tid; h < nh; h += 256) {
         const int c = sHard[h];
         V4<T> WL, WR;
         bool px;
         unsigned dummy = 0;
         face_input<R, MUSCL>(I, c, i0, j0, lo_i, lim_i, lo_j, lim_j, g, pinf, sW, sE, Qi, WL, WR, px, dummy);
-        const V4<T> F = rs_flux_full<T>(px, g, pinf, WL, WR, st);
+        V4<T> F;
+        if constexpr (sizeof(T) == 4) F = rs_flux_full_m(rf, px, g, pinf, WL, WR, st);
+        else F = rs_flux_full(px, g, pinf, WL, WR, st);
 #pragma unroll
         for (int k = 0; k < 4; ++k) sFl[k][c] = F.v[k];
     }
@@ -411,6 +414,7 @@ enum { MF_FIRST = 1, MF_INJECT = 2, MF_LAST = 4, MF_OGFM = 8, MF_RAW = 16 };
 
 template <typename R>
 __device__ __forceinline__ void unit_normal(const Inst& I, int i, int j, double& nx, double& ny) {
+#pragma clang fp contract(off)
     nx = ((double)ld1<R>(I.phi, cidx(I, i, j + 1)) - (double)ld1<R>(I.phi, cidx(I, i, j - 1))) / (2.0 * I.dx);
     ny = ((double)ld1<R>(I.phi, cidx(I, i + 1, j)) - (double)ld1<R>(I.phi, cidx(I, i - 1, j))) / (2.0 * I.dy);
     const double m = sqrt(nx * nx + ny * ny);
@@ -437,6 +441,100 @@ __device__ __forceinline__ V4<R> ex_load(const Inst& I, int f, int i, int j, con
     return W;
 }
 
+// upwind neighbour offsets of extrapolate_vector.cpp:74-121: the fluid-2
+// ghost takes information along -n, the fluid-1 ghost along +n
+template <typename R>
+__device__ __forceinline__ void extrap_dirs(int fg, R nx, R ny, int& dj, int& di) {
+    dj = ((fg == 1) == (nx > R(0))) ? 1 : -1;
+    di = ((fg == 1) == (ny > R(0))) ? 1 : -1;
+}
+
+// one Jacobi update of the ghost fluid's primitives from the cell (c0) and
+// its upwind x (xo) and y (yo) neighbours, pseudo-time step h/2
+template <typename R>
+__device__ __forceinline__ V4<R> extrap_combine(int fg, R nx, R ny, R ax, R ay, const V4<R>& c0, const V4<R>& xo,
+                                                const V4<R>& yo) {
+#pragma clang fp contract(off)
+    V4<R> Wg;
+    if (fg == 1) {   // fluid 2 ghost (:74-97)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const R dxv = nx > R(0) ? xo.v[k] - c0.v[k] : c0.v[k] - xo.v[k];
+            const R dyv = ny > R(0) ? yo.v[k] - c0.v[k] : c0.v[k] - yo.v[k];
+            Wg.v[k] = c0.v[k] + ax * nx * dxv + ay * ny * dyv;
+        }
+    } else {         // fluid 1 ghost (:98-121)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const R dxv = nx > R(0) ? c0.v[k] - xo.v[k] : xo.v[k] - c0.v[k];
+            const R dyv = ny > R(0) ? c0.v[k] - yo.v[k] : yo.v[k] - c0.v[k];
+            Wg.v[k] = c0.v[k] - ax * nx * dxv - ay * ny * dyv;
+        }
+    }
+    return Wg;
+}
+
+// GFM_original.cpp:65-116: the ghost fluid takes the real fluid's velocity
+// and pressure, its density by isentropic extrapolation; false if unphysical
+template <typename R>
+__device__ __forceinline__ bool ogfm_ghost(const Inst& I, int fg, const V4<R>& Wr, V4<R>& Wg) {
+#pragma clang fp contract(off)
+    const double gg = I.gam[fg], pg = I.pinf[fg];
+    const double rn = (double)Wg.v[0] * pow(((double)Wr.v[3] + pg) / ((double)Wg.v[3] + pg), 1.0 / gg);
+    Wg.v[0] = (R)rn; Wg.v[1] = Wr.v[1]; Wg.v[2] = Wr.v[2]; Wg.v[3] = Wr.v[3];
+    return rn >= 0.0;
+}
+
+// one cell's output in the form the mode asks for, before any boundary flip:
+// (a, b) = fluid 1 and 2 (conserved for MF_LAST, else primitives) and the
+// velocity of the fluid real in the cell.  A ghost cell's output is its
+// source's with the momentum / velocity components negated (negation commutes
+// exactly with W->U); every ghost image of a cell is written from the one
+// conversion, so no two conversions of equal data can round differently.
+template <typename R>
+struct ExOut { V4<R> a, b; R vu, vv; };
+
+template <typename R, int MODE>
+__device__ __forceinline__ ExOut<R> extrap_out(const Inst& I, int fr, const V4<R>& Wr, const V4<R>& Wg) {
+    ExOut<R> o;
+    V4<R> A, B;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        A.v[k] = fr == 0 ? Wr.v[k] : Wg.v[k];
+        B.v[k] = fr == 0 ? Wg.v[k] : Wr.v[k];
+    }
+    if (MODE & MF_LAST) {
+        o.a = prims2cons<R>((R)I.gam[0], (R)I.pinf[0], A);
+        o.b = prims2cons<R>((R)I.gam[1], (R)I.pinf[1], B);
+    } else {
+        o.a = A;
+        o.b = B;
+    }
+    o.vu = Wr.v[1];
+    o.vv = Wr.v[2];
+    return o;
+}
+
+template <typename R, int MODE>
+__device__ __forceinline__ void extrap_put(const Inst& I, int i, int j, int bout, ExOut<R> o, bool fu, bool fv) {
+    if (fu) { o.a.v[1] = -o.a.v[1]; o.b.v[1] = -o.b.v[1]; o.vu = -o.vu; }
+    if (fv) { o.a.v[2] = -o.a.v[2]; o.b.v[2] = -o.b.v[2]; o.vv = -o.vv; }
+    const long long c = cidx(I, i, j);
+    if (MODE & MF_LAST) {
+        st4<R>(I.Q[0][bout], I, c, o.a);
+        st4<R>(I.Q[1][bout], I, c, o.b);
+        R* vel = (R*)I.vel;
+        vel[c] = o.vu;
+        vel[c + I.plane] = o.vv;
+    } else if (MODE & MF_RAW) {
+        st4<R>(I.Q[0][bout], I, c, o.a);
+        st4<R>(I.Q[1][bout], I, c, o.b);
+    } else {
+        st4<R>(I.P[0][bout], I, c, o.a);
+        st4<R>(I.P[1][bout], I, c, o.b);
+    }
+}
+
 template <typename R, int MODE>
 __device__ __forceinline__ void extrap_cell(const Inst& I, int i, int j, int bin, int bout, int* sbb) {
     bool fv, fu;
@@ -454,66 +552,20 @@ __device__ __forceinline__ void extrap_cell(const Inst& I, int i, int j, int bin
     const R pdt = R(0.5 * fmin(I.dx, I.dy));
     const R ax = pdt / (R)I.dx, ay = pdt / (R)I.dy;
     V4<R> Wr = ex_load<R, MODE>(I, fr, si, sj, in_r);
-    V4<R> Wg;
-    {
-        const V4<R> c0 = ex_load<R, MODE>(I, fg, si, sj, in_g);
-        if (fg == 1) {   // fluid 2 ghost: information along -n (:74-97)
-            const V4<R> xo = ex_load<R, MODE>(I, fg, si, nx > R(0) ? sj + 1 : sj - 1, in_g);
-            const V4<R> yo = ex_load<R, MODE>(I, fg, ny > R(0) ? si + 1 : si - 1, sj, in_g);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const R dxv = nx > R(0) ? xo.v[k] - c0.v[k] : c0.v[k] - xo.v[k];
-                const R dyv = ny > R(0) ? yo.v[k] - c0.v[k] : c0.v[k] - yo.v[k];
-                Wg.v[k] = c0.v[k] + ax * nx * dxv + ay * ny * dyv;
-            }
-        } else {         // fluid 1 ghost: information along +n (:98-121)
-            const V4<R> xo = ex_load<R, MODE>(I, fg, si, nx > R(0) ? sj - 1 : sj + 1, in_g);
-            const V4<R> yo = ex_load<R, MODE>(I, fg, ny > R(0) ? si - 1 : si + 1, sj, in_g);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const R dxv = nx > R(0) ? c0.v[k] - xo.v[k] : xo.v[k] - c0.v[k];
-                const R dyv = ny > R(0) ? c0.v[k] - yo.v[k] : yo.v[k] - c0.v[k];
-                Wg.v[k] = c0.v[k] - ax * nx * dxv - ay * ny * dyv;
-            }
-        }
-        if (!(Wg.v[0] >= R(0))) flag(I, ST_UNPHYS);
-    }
+    int dj, di;
+    extrap_dirs<R>(fg, nx, ny, dj, di);
+    V4<R> Wg = extrap_combine<R>(fg, nx, ny, ax, ay, ex_load<R, MODE>(I, fg, si, sj, in_g),
+                                 ex_load<R, MODE>(I, fg, si, sj + dj, in_g),
+                                 ex_load<R, MODE>(I, fg, si + di, sj, in_g));
+    if (!(Wg.v[0] >= R(0))) flag(I, ST_UNPHYS);
     const bool own = (si == i && sj == j);
     if ((MODE & MF_OGFM) && own) {
-        // GFM_original.cpp:65-116: the ghost fluid takes the real fluid's
-        // velocity and pressure, density by isentropic extrapolation
-        const double gg = I.gam[fg], pg = I.pinf[fg];
-        const double rn = (double)Wg.v[0] * pow(((double)Wr.v[3] + pg) / ((double)Wg.v[3] + pg), 1.0 / gg);
-        if (!(rn >= 0.0)) flag(I, ST_UNPHYS);
-        Wg.v[0] = (R)rn; Wg.v[1] = Wr.v[1]; Wg.v[2] = Wr.v[2]; Wg.v[3] = Wr.v[3];
+        if (!ogfm_ghost<R>(I, fg, Wr, Wg)) flag(I, ST_UNPHYS);
         int* bb = sbb + 4 * fr;
         atomicMin(&bb[0], i - 2); atomicMax(&bb[1], i + 2);
         atomicMin(&bb[2], j - 2); atomicMax(&bb[3], j + 2);
     }
-    if (fu) { Wr.v[1] = -Wr.v[1]; Wg.v[1] = -Wg.v[1]; }
-    if (fv) { Wr.v[2] = -Wr.v[2]; Wg.v[2] = -Wg.v[2]; }
-    V4<R> W1, W2;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        W1.v[k] = fr == 0 ? Wr.v[k] : Wg.v[k];
-        W2.v[k] = fr == 0 ? Wg.v[k] : Wr.v[k];
-    }
-    const long long c = cidx(I, i, j);
-    if (MODE & MF_LAST) {
-        // W->U of both grids and the stored velocity field of the fluid real
-        // in this cell (GFM_riemann.cpp:156-187)
-        st4<R>(I.Q[0][bout], I, c, prims2cons<R>((R)I.gam[0], (R)I.pinf[0], W1));
-        st4<R>(I.Q[1][bout], I, c, prims2cons<R>((R)I.gam[1], (R)I.pinf[1], W2));
-        R* vel = (R*)I.vel;
-        vel[c] = Wr.v[1];
-        vel[c + I.plane] = Wr.v[2];
-    } else if (MODE & MF_RAW) {
-        st4<R>(I.Q[0][bout], I, c, W1);
-        st4<R>(I.Q[1][bout], I, c, W2);
-    } else {
-        st4<R>(I.P[0][bout], I, c, W1);
-        st4<R>(I.P[1][bout], I, c, W2);
-    }
+    extrap_put<R, MODE>(I, i, j, bout, extrap_out<R, MODE>(I, fr, Wr, Wg), fu, fv);
 }
 
 template <typename R, int MODE>
@@ -531,6 +583,190 @@ __global__ __launch_bounds__(256) void k_extrap(const Inst* __restrict__ insts, 
     const int j = w.z * 64 + (threadIdx.x & 63);
     if (i < I.H && j < I.W) extrap_cell<R, MODE>(I, i, j, bin, bout, sbb);
     if (MODE & MF_OGFM) {
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            const int v = sbb[threadIdx.x];
+            int* dst = I.si + SI_BBOX + threadIdx.x;
+            if (threadIdx.x & 1) { if (v != INT_MIN) atomicMax(dst, v); }
+            else if (v != INT_MAX) atomicMin(dst, v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// NIT Jacobi iterations of k_extrap in one launch (temporal blocking).  A
+// workgroup owns a TBX x TBY tile of interior cells and keeps the tile plus an
+// NIT-cell halo of both fluids' primitives in LDS.  Iteration t is exact on
+// the region shrunk by t cells, so after NIT iterations the tile holds what
+// NIT launches of k_extrap produce, bit for bit (same arithmetic, same order).
+// Ghost cells: k_extrap gives a ghost cell its BC source cell's new value,
+// reflected.  With mirror boundaries the stencil reads only the first ghost
+// layer, whose source is the adjacent interior cell, so that cell refreshes
+// it after each iteration; the first iteration reads the arrays as they are,
+// ghost frame included.  The last iteration writes every owned cell and each
+// ghost cell whose source it is.  Periodic contexts use k_extrap.
+// MIN: read-side mode bits of the first iteration (MF_FIRST, MF_INJECT);
+// MOUT: write-side bits of the last (MF_LAST, MF_OGFM; none = primitives to
+// P[.][bout]).
+// work: {inst, tile_y, tile_x, 0}, TBX x TBY interior tiles
+// ---------------------------------------------------------------------------
+constexpr int TBX = 64, TBY = 16, TB_THREADS = 512;
+
+template <typename R, int NC>
+__device__ __forceinline__ V4<R> tb_get(const R* s, int f, int idx) {
+    V4<R> W;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) W.v[q] = s[(4 * f + q) * NC + idx];
+    return W;
+}
+
+template <typename R, int NC>
+__device__ __forceinline__ void tb_put(R* s, int f, int idx, const V4<R>& W) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[(4 * f + q) * NC + idx] = W.v[q];
+}
+
+// new ghost-fluid primitives of LDS cell idx (extrap_cell's arithmetic)
+template <typename R, int NC, int RX>
+__device__ __forceinline__ V4<R> tb_update(const R* s, int idx, int fg, R nx, R ny, R ax, R ay) {
+    int dj, di;
+    extrap_dirs<R>(fg, nx, ny, dj, di);
+    return extrap_combine<R>(fg, nx, ny, ax, ay, tb_get<R, NC>(s, fg, idx), tb_get<R, NC>(s, fg, idx + dj),
+                             tb_get<R, NC>(s, fg, idx + di * RX));
+}
+
+template <typename R, int NIT, int MIN, int MOUT>
+__global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict__ insts,
+                                                          const int4* __restrict__ work, int bin, int bout) {
+    constexpr int RX = TBX + 2 * NIT, RY = TBY + 2 * NIT, NC = RX * RY;
+    constexpr int SLOTS = (NC + TB_THREADS - 1) / TB_THREADS;
+    constexpr int LM = MIN & (MF_FIRST | MF_INJECT);
+    __shared__ R s[8 * NC];   // s[(4 f + q) NC + cell]: primitive q of fluid f
+    __shared__ int sbb[8];
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE]) return;
+    const int g = I.g;
+    const int ri = g + w.y * TBY - NIT, rj = g + w.z * TBX - NIT;   // region origin
+    if (MOUT & MF_OGFM) {
+        if (threadIdx.x < 8) sbb[threadIdx.x] = (threadIdx.x & 1) ? INT_MIN : INT_MAX;
+    }
+    const void* in0 = (MIN & MF_FIRST) ? I.Q[0][bin] : I.P[0][bin];
+    const void* in1 = (MIN & MF_FIRST) ? I.Q[1][bin] : I.P[1][bin];
+    // per cell: unit normal and bits (1: interior with its stencil inside the
+    // region, 2: fluid 2 real, 4: owned)
+    R nxs[SLOTS], nys[SLOTS];
+    int meta[SLOTS];
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+        const int idx = threadIdx.x + k * TB_THREADS;
+        nxs[k] = R(0); nys[k] = R(0); meta[k] = 0;
+        if (idx < NC) {
+            const int a = idx / RX, b = idx - (idx / RX) * RX;
+            const int i = ri + a, j = rj + b;
+            V4<R> W1 = {{R(0), R(0), R(0), R(0)}}, W2 = W1;
+            if (i >= 0 && i < I.H && j >= 0 && j < I.W) {
+                W1 = ex_load<R, LM>(I, 0, i, j, in0);
+                W2 = ex_load<R, LM>(I, 1, i, j, in1);
+                if (interior(I, i, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1) {
+                    double nxd, nyd;
+                    unit_normal<R>(I, i, j, nxd, nyd);
+                    nxs[k] = (R)nxd; nys[k] = (R)nyd;
+                    const bool own = a >= NIT && a < NIT + TBY && b >= NIT && b < NIT + TBX;
+                    meta[k] = 1 | (ld1<R>(I.phi, cidx(I, i, j)) <= R(0) ? 0 : 2) | (own ? 4 : 0);
+                }
+            }
+            tb_put<R, NC>(s, 0, idx, W1);
+            tb_put<R, NC>(s, 1, idx, W2);
+        }
+    }
+    __syncthreads();
+    const R pdt = R(0.5 * fmin(I.dx, I.dy));
+    const R ax = pdt / (R)I.dx, ay = pdt / (R)I.dy;
+    const bool rfb = I.bc[2] == BC_R, rft = I.bc[3] == BC_R, rfx = I.bc[0] == BC_R;
+    const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
+    bool bad = false;
+#pragma unroll 1
+    for (int t = 1; t < NIT; ++t) {
+        V4<R> nv[SLOTS];
+        int act = 0;
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) {
+            const int idx = threadIdx.x + k * TB_THREADS;
+            const int a = idx / RX, b = idx - (idx / RX) * RX;
+            if ((meta[k] & 1) && a >= t && a < RY - t && b >= t && b < RX - t) {
+                act |= 1 << k;
+                nv[k] = tb_update<R, NC, RX>(s, idx, (meta[k] & 2) ? 0 : 1, nxs[k], nys[k], ax, ay);
+                if ((meta[k] & 4) && !(nv[k].v[0] >= R(0))) bad = true;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) {
+            if (!(act & (1 << k))) continue;
+            const int idx = threadIdx.x + k * TB_THREADS;
+            const int a = idx / RX, b = idx - (idx / RX) * RX;
+            const int i = ri + a, j = rj + b;
+            tb_put<R, NC>(s, (meta[k] & 2) ? 0 : 1, idx, nv[k]);
+            if (i == ib || i == it || j == jl || j == jr) {
+                // first ghost layer: this cell's new state, reflected
+                const V4<R> W1 = tb_get<R, NC>(s, 0, idx), W2 = tb_get<R, NC>(s, 1, idx);
+                V4<R> Wv1 = W1, Wv2 = W2, Wu1 = W1, Wu2 = W2;
+                Wv1.v[2] = -Wv1.v[2]; Wv2.v[2] = -Wv2.v[2];
+                Wu1.v[1] = -Wu1.v[1]; Wu2.v[1] = -Wu2.v[1];
+                if (i == ib) { tb_put<R, NC>(s, 0, idx - RX, rfb ? Wv1 : W1); tb_put<R, NC>(s, 1, idx - RX, rfb ? Wv2 : W2); }
+                if (i == it) { tb_put<R, NC>(s, 0, idx + RX, rft ? Wv1 : W1); tb_put<R, NC>(s, 1, idx + RX, rft ? Wv2 : W2); }
+                if (j == jl) { tb_put<R, NC>(s, 0, idx - 1, rfx ? Wu1 : W1); tb_put<R, NC>(s, 1, idx - 1, rfx ? Wu2 : W2); }
+                if (j == jr) { tb_put<R, NC>(s, 0, idx + 1, rfx ? Wu1 : W1); tb_put<R, NC>(s, 1, idx + 1, rfx ? Wu2 : W2); }
+            }
+        }
+        __syncthreads();
+    }
+    // last iteration: owned cells, written with the ghost cells they source
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+        if (!(meta[k] & 4)) continue;
+        const int idx = threadIdx.x + k * TB_THREADS;
+        const int a = idx / RX, b = idx - (idx / RX) * RX;
+        const int i = ri + a, j = rj + b;
+        const int fr = (meta[k] & 2) ? 1 : 0, fg = 1 - fr;
+        const V4<R> Wg = tb_update<R, NC, RX>(s, idx, fg, nxs[k], nys[k], ax, ay);
+        if (!(Wg.v[0] >= R(0))) bad = true;
+        const V4<R> Wr = tb_get<R, NC>(s, fr, idx);
+        V4<R> Wo = Wg;
+        if (MOUT & MF_OGFM) {
+            if (!ogfm_ghost<R>(I, fg, Wr, Wo)) bad = true;
+            int* bb = sbb + 4 * fr;
+            atomicMin(&bb[0], i - 2); atomicMax(&bb[1], i + 2);
+            atomicMin(&bb[2], j - 2); atomicMax(&bb[3], j + 2);
+        }
+        const ExOut<R> o = extrap_out<R, MOUT>(I, fr, Wr, Wo);
+        extrap_put<R, MOUT>(I, i, j, bout, o, false, false);
+        if (i < 2 * g || i >= I.Ny || j < 2 * g || j >= I.Nx) {
+            // ghost cells sourced here (inverse of src_row / src_col): rows
+            // {i, 2g-1-i, 2(Ny+g)-1-i} x columns {j, 2g-1-j, 2(Nx+g)-1-j}
+            // where those are ghost cells.  They take the plain extrapolated
+            // ghost fluid (GFM_original's rule applies to interior cells).
+            ExOut<R> oa = o;
+            if (MOUT & MF_OGFM) {
+                const V4<R> Ug = (MOUT & MF_LAST) ? prims2cons<R>((R)I.gam[fg], (R)I.pinf[fg], Wg) : Wg;
+                if (fr == 0) oa.b = Ug; else oa.a = Ug;
+            }
+#pragma unroll 1
+            for (int m = 1; m < 9; ++m) {
+                const int pr = m / 3, pc = m - 3 * (m / 3);
+                int r = i, cc = j;
+                bool fv = false, fu = false;
+                if (pr == 1) { if (i >= 2 * g) continue; r = 2 * g - 1 - i; fv = rfb; }
+                if (pr == 2) { if (i < I.Ny) continue; r = 2 * (I.Ny + g) - 1 - i; fv = rft; }
+                if (pc == 1) { if (j >= 2 * g) continue; cc = 2 * g - 1 - j; fu = rfx; }
+                if (pc == 2) { if (j < I.Nx) continue; cc = 2 * (I.Nx + g) - 1 - j; fu = rfx; }
+                extrap_put<R, MOUT & MF_LAST>(I, r, cc, bout, oa, fu, fv);
+            }
+        }
+    }
+    if (bad) flag(I, ST_UNPHYS);
+    if (MOUT & MF_OGFM) {
         __syncthreads();
         if (threadIdx.x < 8) {
             const int v = sbb[threadIdx.x];
@@ -1172,6 +1408,24 @@ void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout) {
 #undef EXC
 }
 
+// NIT iterations of launch_extrap in one launch; false when the combination
+// has no temporally blocked instance (the caller then launches per iteration)
+template <typename R>
+bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout) {
+#define EXT(N, MI, MO)                                                                                     \
+    if (nit == N && min == (MI) && mout == (MO)) {                                                         \
+        hipLaunchKernelGGL((k_extrap_tb<R, N, MI, MO>), GRID(L.n_tb), dim3(TB_THREADS), 0, L.stream, L.insts, \
+                           L.w_tb, bin, bout);                                                             \
+        return true;                                                                                       \
+    }
+    EXT(6, MF_FIRST, MF_LAST | MF_OGFM)           // GFM_original
+    EXT(6, MF_FIRST | MF_INJECT, MF_LAST)         // GFM_real
+    EXT(2, MF_FIRST, 0)                           // GFM_riemann / GFM_modified, before the star states
+    EXT(6, MF_INJECT, MF_LAST)                    // GFM_riemann, after the star states
+#undef EXT
+    return false;
+}
+
 template <typename R>
 void launch_iface(const LaunchCtx& L, int variant, int pbuf) {
     hipLaunchKernelGGL((k_iface_mark<R>), GRID(L.n_int), dim3(256), 0, L.stream, L.insts, L.w_int);
@@ -1239,6 +1493,7 @@ void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, doub
     template void launch_dt<R>(const LaunchCtx&, int, int, int);                   \
     template void launch_fs<R>(const LaunchCtx&, bool, int, int, int);             \
     template void launch_extrap<R>(const LaunchCtx&, int, int, int);          \
+    template bool launch_extrap_tb<R>(const LaunchCtx&, int, int, int, int, int); \
     template void launch_iface<R>(const LaunchCtx&, int, int);                \
     template void launch_mgfm_final<R>(const LaunchCtx&, int, int);           \
     template void launch_ls<R>(const LaunchCtx&, const Houc&, int, int);      \

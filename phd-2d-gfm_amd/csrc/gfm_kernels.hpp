@@ -16,6 +16,7 @@ struct LaunchCtx {
     const int4* w_frame; int n_frame;   // 256-cell chunks of the ghost frame
     const int4* w_inst;  int n_inst;    // one entry per instance
     const int4* w_ich;   int n_ich;     // 256-entry chunks of the interfacial lists
+    const int4* w_tb;    int n_tb;      // 64x16 interior tiles (temporally blocked extrapolation)
 };
 
 struct Houc;
@@ -24,6 +25,7 @@ constexpr int IFREC_BYTES = 384;   // >= sizeof(IfRec), checked in the kernels
 template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes);
 template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive);
 template <typename R> void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout);
+template <typename R> bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout);
 template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf);
 template <typename R> void launch_mgfm_final(const LaunchCtx& L, int qin, int qout);
 template <typename R> void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters);
