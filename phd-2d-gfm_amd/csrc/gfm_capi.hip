@@ -461,6 +461,10 @@ int gfm_ctx_commit(gfm_ctx* c) {
     xcd_order(w_tb, 0);
     const char* no_tb = std::getenv("GFM_NO_TEMPORAL_BLOCKING");
     c->tb = c->twofluid() && !periodic && !(no_tb && no_tb[0] == '1');
+    // GFM_NO_TEMPORAL_BLOCKING=1 / GFM_IFACE_PLAIN_ROUNDS=1 select the simpler
+    // schedules that the tests compare the default ones against
+    const char* plain = std::getenv("GFM_IFACE_PLAIN_ROUNDS");
+    const int iface_plain = (plain && plain[0] == '1') ? 1 : 0;
     HC(hipMalloc((void**)&c->d_insts, descs.size() * sizeof(Inst)));
     HC(hipMemcpyAsync(c->d_insts, descs.data(), descs.size() * sizeof(Inst), hipMemcpyHostToDevice, c->stream));
     if (w_ich.empty()) w_ich.push_back(make_int4(0, 1 << 30, 0, 0));
@@ -488,6 +492,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
     L.w_inst = c->d_work + o; L.n_inst = (int)w_inst.size(); o += w_inst.size();
     L.w_ich = c->d_work + o; L.n_ich = (int)w_ich.size(); o += w_ich.size();
     L.w_tb = c->d_work + o; L.n_tb = (int)w_tb.size();
+    L.iface_plain = iface_plain;
     // initial scalars: t = 0, step 0, active
     for (auto& h : c->inst) {
         int si[gfm::SI_N] = {0};

@@ -1060,51 +1060,143 @@ __global__ __launch_bounds__(256) void k_iface_prep(const Inst* __restrict__ ins
     flag(I, st);
 }
 
-// Pass 2, one workgroup per instance: dependency rounds separated by
-// barriers; an entry is computed once every earlier interfacial cell it
-// reads has its star state.
+// star state of pending entry k once every earlier interfacial cell it reads
+// has its own (the side buffer holds those)
 template <int VAR>
-__global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ insts, const int4* __restrict__ work) {
-    __shared__ unsigned char fl[ICAP_LDS];
-    __shared__ int pending[2];
+__device__ __forceinline__ unsigned iface_solve_pending(const Inst& I, const IfRec* recs, int k) {
+    const IfRec& r = recs[k];
+    double ip1[4], ip2[4], out[8], ps;
+    probe_values<VAR>(I, r, true, ip1, ip2);
+    const unsigned st = iface_finish<VAR>(I, r, ip1, ip2, r.pspec, out, &ps);
+    double* o = I.side + 8 * (long long)k;
+    for (int q = 0; q < 4; ++q) o[q] = out[q];
+    return st;
+}
+
+// Pass 2, one workgroup per instance.  The entries left pending by pass 1
+// are ordered by dependency level (level 0: final after pass 1; otherwise
+// 1 + the highest level among the entries it reads), found by integer-only
+// rounds over an LDS list; then each level is solved in parallel, one
+// barrier per level.  Instances with more than IPEND_LDS pending entries or
+// deeper than 254 levels (and every instance when `plain` is set) use plain
+// rounds: solve whatever is ready, repeat.  Both give the same bits.
+constexpr int IPEND_LDS = 8192;
+
+template <int VAR>
+__global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                      int plain) {
+    __shared__ unsigned char lev[ICAP_LDS];   // 0 final, 1..254 level, 255 unknown
+    __shared__ int plist[IPEND_LDS];          // pending entries
+    __shared__ int slist[IPEND_LDS];          // pending entries sorted by level
+    __shared__ int cnt[257];
+    __shared__ int npend, overflow;
+    __shared__ int nunk[2];   // per round parity: reset for the next round while this one's is read
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
     if (!I.si[SI_ACTIVE]) return;
     const int n = min(I.si[SI_NIFACE], I.icap);
     if (n > ICAP_LDS) { if (threadIdx.x == 0) flag(I, ST_IOVF); return; }
     const IfRec* recs = (const IfRec*)I.irec;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) fl[k] = (unsigned char)I.idone[k];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (tid == 0) { npend = 0; overflow = 0; }
+    for (int k = tid; k < 257; k += nt) cnt[k] = 0;
+    __syncthreads();
+    for (int k = tid; k < n; k += nt) {
+        const bool done = I.idone[k] != 0;
+        lev[k] = done ? 0 : 255;
+        if (!done) {
+            const int p = atomicAdd(&npend, 1);
+            if (p < IPEND_LDS) plist[p] = k;
+        }
+    }
+    __syncthreads();
     unsigned st = 0;
-    int round = 0;
-    for (;; ++round) {
-        if (threadIdx.x == 0) pending[round & 1] = 0;
+    int rounds = 0;
+    const int np = npend;
+    if (np <= IPEND_LDS && !plain) {
+        // levels
+        for (;; ++rounds) {
+            if (tid == 0) nunk[rounds & 1] = 0;
+            __syncthreads();
+            for (int p = tid; p < np; p += nt) {
+                const int k = plist[p];
+                if (lev[k] != 255) continue;
+                int m = 0;
+                bool ok = true;
+                for (int a = 0; a < 2; ++a)
+                    for (int q = 0; q < 4; ++q) {
+                        const int d = recs[k].dep[a][q];
+                        if (d < 0) continue;
+                        const int l = lev[d];
+                        if (l == 255) ok = false;
+                        else m = max(m, l);
+                    }
+                if (!ok) { atomicAdd(&nunk[rounds & 1], 1); continue; }
+                if (m + 1 > 254) { overflow = 1; continue; }
+                lev[k] = (unsigned char)(m + 1);
+            }
+            __syncthreads();
+            if (nunk[rounds & 1] == 0 || overflow) break;
+        }
+    } else {
+        overflow = 1;
+    }
+    __syncthreads();
+    if (!overflow) {
+        // counting sort by level, then one parallel step per level
+        for (int p = tid; p < np; p += nt) atomicAdd(&cnt[lev[plist[p]]], 1);
         __syncthreads();
-        for (int k = threadIdx.x; k < n; k += blockDim.x) {
-            if (fl[k]) continue;
-            bool ready = true;
-            for (int p = 0; p < 2; ++p)
-                for (int q = 0; q < 4; ++q) {
-                    const int d = recs[k].dep[p][q];
-                    if (d >= 0 && fl[d] != 1) ready = false;
-                }
-            if (!ready) { atomicAdd(&pending[round & 1], 1); continue; }
-            const IfRec& r = recs[k];
-            double ip1[4], ip2[4], out[8], ps;
-            probe_values<VAR>(I, r, true, ip1, ip2);
-            st |= iface_finish<VAR>(I, r, ip1, ip2, r.pspec, out, &ps);
-            double* o = I.side + 8 * (long long)k;
-            for (int q = 0; q < 4; ++q) o[q] = out[q];
-            fl[k] = 2;
+        if (tid == 0) {
+            int acc = 0;
+            for (int l = 0; l < 256; ++l) { const int c = cnt[l]; cnt[l] = acc; acc += c; }
+            cnt[256] = acc;
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < n; k += blockDim.x)
-            if (fl[k] == 2) fl[k] = 1;
+        int maxl = 0;
+        for (int l = 255; l > 0; --l)
+            if (cnt[l + 1] > cnt[l]) { maxl = l; break; }
         __syncthreads();
-        if (pending[round & 1] == 0) break;
-        if (round > n + 1) { if (threadIdx.x == 0) flag(I, ST_IROUNDS); break; }
+        for (int p = tid; p < np; p += nt) {
+            const int k = plist[p];
+            slist[atomicAdd(&cnt[lev[k]], 1)] = k;
+        }
+        __syncthreads();
+        // cnt[l] now holds the end of level l's run, which starts at cnt[l - 1]
+        for (int l = 1; l <= maxl; ++l) {
+            const int b0 = cnt[l - 1], b1 = cnt[l];
+            for (int p = b0 + tid; p < b1; p += nt) st |= iface_solve_pending<VAR>(I, recs, slist[p]);
+            __syncthreads();
+        }
+        rounds += maxl;
+    } else {
+        // plain rounds over all entries (lev: 0 final, 1 solved this round, 255 pending)
+        for (int k = tid; k < n; k += nt) lev[k] = I.idone[k] ? 0 : 255;
+        __syncthreads();
+        for (rounds = 0;; ++rounds) {
+            if (tid == 0) nunk[rounds & 1] = 0;
+            __syncthreads();
+            for (int k = tid; k < n; k += nt) {
+                if (lev[k] != 255) continue;
+                bool ready = true;
+                for (int a = 0; a < 2; ++a)
+                    for (int q = 0; q < 4; ++q) {
+                        const int d = recs[k].dep[a][q];
+                        if (d >= 0 && lev[d] != 0) ready = false;
+                    }
+                if (!ready) { atomicAdd(&nunk[rounds & 1], 1); continue; }
+                st |= iface_solve_pending<VAR>(I, recs, k);
+                lev[k] = 1;
+            }
+            __syncthreads();
+            for (int k = tid; k < n; k += nt)
+                if (lev[k] == 1) lev[k] = 0;
+            __syncthreads();
+            if (nunk[rounds & 1] == 0) break;
+            if (rounds > n + 1) { if (tid == 0) flag(I, ST_IROUNDS); break; }
+        }
     }
     flag(I, st);
-    if (threadIdx.x == 0) I.si[SI_ROUNDS] = round + 1;
+    if (tid == 0) I.si[SI_ROUNDS] = rounds + 1;
 }
 
 // GFM_modified write-back (GFM_modified.cpp:37-47, :230-239): the grids are
@@ -1432,11 +1524,13 @@ void launch_iface(const LaunchCtx& L, int variant, int pbuf) {
     switch (variant) {
         case V_REAL:
             hipLaunchKernelGGL((k_iface_prep<R, V_REAL>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich, pbuf);
-            hipLaunchKernelGGL((k_iface_rounds<V_REAL>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst);
+            hipLaunchKernelGGL((k_iface_rounds<V_REAL>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst,
+                               L.iface_plain);
             break;
         case V_RIEMANN:
             hipLaunchKernelGGL((k_iface_prep<R, V_RIEMANN>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich, pbuf);
-            hipLaunchKernelGGL((k_iface_rounds<V_RIEMANN>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst);
+            hipLaunchKernelGGL((k_iface_rounds<V_RIEMANN>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst,
+                               L.iface_plain);
             break;
         case V_MODIFIED:   // no dependencies: the parallel pass is final
             hipLaunchKernelGGL((k_iface_prep<R, V_MODIFIED>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich, pbuf);

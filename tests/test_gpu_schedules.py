@@ -1,13 +1,14 @@
-"""Temporally blocked ghost extrapolation against one launch per Jacobi
-iteration.
+"""The fast schedules against the simple ones, bit for bit.
 
-The blocked kernel runs all iterations of a ghost fill in one launch, with
-the tile and its halo in LDS; it must reproduce the per-iteration launches
-bit for bit (same arithmetic in the same order), ghost frame included.
-Covered: all four GFM variants, FP64 and mixed precision, each boundary
-combination the blocked path takes (reflective and transmissive on every
-side; the right side reflects with the left BC as in the reference),
-partial tiles, several instances per context, and full steps.
+1. Temporally blocked ghost extrapolation (all Jacobi iterations of a ghost
+   fill in one launch, tile and halo in LDS) against one launch per
+   iteration, ghost frame included.  Covered: all four GFM variants, FP64 and
+   mixed precision, each boundary combination the blocked path takes
+   (reflective and transmissive on every side; the right side reflects with
+   the left BC as in the reference), partial tiles, several instances per
+   context, and full steps.
+2. The interfacial solve ordered by dependency level against plain rounds
+   (solve whatever is ready, repeat), for the variants with dependencies.
 """
 import os
 
@@ -24,9 +25,10 @@ T, R = G.BC_TRANSMISSIVE, G.BC_REFLECTIVE
 BCS = [(T, T, R, R), (R, R, T, T), (R, T, R, T), (T, R, T, R)]
 
 
-def make_ctx(cases, variant, precision, blocked):
-    old = os.environ.get("GFM_NO_TEMPORAL_BLOCKING")
-    os.environ["GFM_NO_TEMPORAL_BLOCKING"] = "0" if blocked else "1"
+def make_ctx(cases, variant, precision, blocked, plain_rounds=False):
+    env = {"GFM_NO_TEMPORAL_BLOCKING": "0" if blocked else "1", "GFM_IFACE_PLAIN_ROUNDS": "1" if plain_rounds else "0"}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         ctx = G.Context(G.methods(sim_type=G.SIM_TWOFLUID, fs=G.FS_MUSCL, gfm=variant, ls_order=5,
                                   reinit_every=5, reinit_iters=10, precision=precision))
@@ -34,10 +36,11 @@ def make_ctx(cases, variant, precision, blocked):
             ctx.add(info, eos, 0.6, 1e300, 1 << 30)
         ctx.commit()
     finally:
-        if old is None:
-            del os.environ["GFM_NO_TEMPORAL_BLOCKING"]
-        else:
-            os.environ["GFM_NO_TEMPORAL_BLOCKING"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
     for k, (_, _, st) in enumerate(cases):
         ctx.upload(k, *st)
     return ctx
@@ -99,3 +102,19 @@ def test_steps_blocked_matches_per_iteration(variant, precision):
     compare(ca, cb, len(cases))
     for k in range(len(cases)):
         assert ca.get_time(k) == cb.get_time(k)
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
+@pytest.mark.parametrize("variant", [G.GFM_REAL, G.GFM_RIEMANN])
+def test_iface_levels_match_plain_rounds(variant, precision):
+    print("CASE iface levels variant=%d precision=%d" % (variant, precision), flush=True)
+    cases = [perturbed_bubble(300, 84, BCS[0], 5, R=0.03), perturbed_bubble(128, 40, BCS[1], 6)]
+    ca = make_ctx(cases, variant, precision, True)
+    cb = make_ctx(cases, variant, precision, True, plain_rounds=True)
+    ca.step_graph(8)
+    cb.step_graph(8)
+    ca.sync()
+    cb.sync()
+    compare(ca, cb, len(cases))
+    for k in range(len(cases)):
+        assert ca.iface_info(k)[0] > 0
