@@ -363,7 +363,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
     if (c->committed) return GFM_OK;
     if (c->inst.empty()) return fail_arg("no instances");
     const size_t es = c->es();
-    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich, w_tb;
+    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich, w_tb, w_rt;
     std::vector<Inst> descs;
     bool periodic = false;
     for (size_t k = 0; k < c->inst.size(); ++k) {
@@ -452,6 +452,9 @@ int gfm_ctx_commit(gfm_ctx* c) {
         if (c->twofluid())
             for (int ty = 0; ty < (s.Ny + 15) / 16; ++ty)
                 for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_tb.push_back(make_int4(ki, ty, tx, 0));
+        if (c->twofluid())
+            for (int ty = 0; ty < (s.Ny + 31) / 32; ++ty)
+                for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_rt.push_back(make_int4(ki, ty, tx, 0));
         if (s.BC_L == GFM_BC_PERIODIC || s.BC_B == GFM_BC_PERIODIC) periodic = true;
         if (c->twofluid())
             for (int o = 0; o < d.icap; o += 256) w_ich.push_back(make_int4(ki, o, 0, 0));
@@ -459,6 +462,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
     xcd_order(w_full, 0);
     xcd_order(w_fs, 0);
     xcd_order(w_tb, 0);
+    xcd_order(w_rt, 0);
     const char* no_tb = std::getenv("GFM_NO_TEMPORAL_BLOCKING");
     c->tb = c->twofluid() && !periodic && !(no_tb && no_tb[0] == '1');
     // GFM_NO_TEMPORAL_BLOCKING=1 / GFM_IFACE_PLAIN_ROUNDS=1 select the simpler
@@ -469,7 +473,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
     HC(hipMemcpyAsync(c->d_insts, descs.data(), descs.size() * sizeof(Inst), hipMemcpyHostToDevice, c->stream));
     if (w_ich.empty()) w_ich.push_back(make_int4(0, 1 << 30, 0, 0));
     const size_t nw =
-        w_full.size() + w_int.size() + w_fs.size() + w_frame.size() + w_inst.size() + w_ich.size() + w_tb.size();
+        w_full.size() + w_int.size() + w_fs.size() + w_frame.size() + w_inst.size() + w_ich.size() + w_tb.size() +
+        w_rt.size();
     HC(hipMalloc((void**)&c->d_work, nw * sizeof(int4)));
     std::vector<int4> all;
     all.reserve(nw);
@@ -480,6 +485,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
     all.insert(all.end(), w_inst.begin(), w_inst.end());
     all.insert(all.end(), w_ich.begin(), w_ich.end());
     all.insert(all.end(), w_tb.begin(), w_tb.end());
+    all.insert(all.end(), w_rt.begin(), w_rt.end());
     HC(hipMemcpyAsync(c->d_work, all.data(), nw * sizeof(int4), hipMemcpyHostToDevice, c->stream));
     gfm::LaunchCtx& L = c->L;
     L.stream = c->stream;
@@ -491,7 +497,9 @@ int gfm_ctx_commit(gfm_ctx* c) {
     L.w_frame = c->d_work + o; L.n_frame = (int)w_frame.size(); o += w_frame.size();
     L.w_inst = c->d_work + o; L.n_inst = (int)w_inst.size(); o += w_inst.size();
     L.w_ich = c->d_work + o; L.n_ich = (int)w_ich.size(); o += w_ich.size();
-    L.w_tb = c->d_work + o; L.n_tb = (int)w_tb.size();
+    L.w_tb = c->d_work + o; L.n_tb = (int)w_tb.size(); o += w_tb.size();
+    L.w_rt = c->d_work + o; L.n_rt = (int)w_rt.size();
+    L.tb = c->tb ? 1 : 0;
     L.iface_plain = iface_plain;
     // initial scalars: t = 0, step 0, active
     for (auto& h : c->inst) {

@@ -24,10 +24,13 @@ enum { ST_UNPHYS = 1, ST_RS = 2, ST_IOVF = 4, ST_IROUNDS = 8 };
 
 // Per-fluid constants of the mixed-precision face Riemann solver, set on the
 // host at commit: z = (g-1)/(2g), g6 = (g-1)/(g+1), a2 = 2/(g+1), ig = 1/g,
-// k2 = 2/(g-1), bc[k-1] = C(z, k) for k = 1..9.
+// k2 = 2/(g-1), bc[k-1] = C(z, k) for k = 1..9; FP32 copies for the
+// sampling: sk = (g+1)/(2g), z, 1/g, 2/(g+1), 2/(g-1), 2g/(g-1), g6,
+// g pinf, 1/(g-1).
 struct RSFluid {
     double g, pinf, z, g6, a2, ig, k2;
     double bc[9];
+    float fsk, fz, fig, fa2, fk2, fk3, fg6, fgpinf, figm1;
 };
 
 __host__ __device__ inline void rs_fluid(RSFluid& F, double g, double pinf) {
@@ -43,6 +46,16 @@ __host__ __device__ inline void rs_fluid(RSFluid& F, double g, double pinf) {
         c = c * (F.z - (double)(k - 1)) / (double)k;
         F.bc[k - 1] = c;
     }
+    F.fsk = (float)((g + 1.0) / (2.0 * g));
+    F.fz = (float)F.z;
+    F.fig = (float)F.ig;
+    F.fa2 = (float)F.a2;
+    F.fk2 = (float)F.k2;
+    F.fk3 = (float)(2.0 * g / (g - 1.0));
+    F.fg6 = (float)F.g6;
+    const float gf = (float)g, pf = (float)pinf;
+    F.fgpinf = gf * pf;
+    F.figm1 = 1.0f / (gf - 1.0f);
 }
 
 // One simulation.  Plane arrays are row-major with row pitch `pitch`
@@ -162,6 +175,59 @@ __device__ __forceinline__ V4<T> flux_prims(T g, T pinf, const V4<T>& W) {
     return F;
 }
 
+
+// ---------------------------------------------------------------------------
+// FP32 storage (mixed precision): the conversions and checks above with one
+// reciprocal per state in place of a division per component, rho e =
+// (p + g pinf) / (g - 1) from a per-fluid reciprocal, and the positivity
+// tests in division-free form: for rho > 0, e >= 0 iff p + g pinf >= 0 (W)
+// and iff 2 E rho >= |m|^2 (U); both are false for rho <= 0 or NaN.
+// ---------------------------------------------------------------------------
+struct FluidF { float g, pinf, gpinf, igm1; };
+
+__device__ __forceinline__ FluidF fluid_f(const RSFluid& R) {
+    FluidF F;
+    F.g = (float)R.g; F.pinf = (float)R.pinf;
+    F.gpinf = R.fgpinf;
+    F.igm1 = R.figm1;
+    return F;
+}
+
+__device__ __forceinline__ V4<float> cons2prims_f(const FluidF& F, const V4<float>& U) {
+    V4<float> W;
+    const float rho = U.v[0], ir = 1.0f / rho;
+    const float u = U.v[1] * ir, v = U.v[2] * ir;
+    W.v[0] = rho; W.v[1] = u; W.v[2] = v;
+    W.v[3] = (F.g - 1.0f) * (U.v[3] - 0.5f * (U.v[1] * u + U.v[2] * v)) - F.gpinf;
+    return W;
+}
+
+__device__ __forceinline__ V4<float> prims2cons_f(const FluidF& F, const V4<float>& W) {
+    V4<float> U;
+    const float rho = W.v[0], u = W.v[1], v = W.v[2];
+    U.v[0] = rho; U.v[1] = rho * u; U.v[2] = rho * v;
+    U.v[3] = (W.v[3] + F.gpinf) * F.igm1 + 0.5f * rho * (u * u + v * v);
+    return U;
+}
+
+__device__ __forceinline__ bool phys_prims_f(const FluidF& F, const V4<float>& W) {
+    return W.v[0] > 0.0f && W.v[3] + F.gpinf >= 0.0f;
+}
+
+__device__ __forceinline__ bool phys_state_f(const V4<float>& U) {
+    return U.v[0] > 0.0f && 2.0f * U.v[3] * U.v[0] >= U.v[1] * U.v[1] + U.v[2] * U.v[2];
+}
+
+__device__ __forceinline__ V4<float> flux_prims_f(const FluidF& F, const V4<float>& W) {
+    V4<float> Fl;
+    const float rho = W.v[0], u = W.v[1], v = W.v[2], p = W.v[3];
+    const float E = (p + F.gpinf) * F.igm1 + 0.5f * rho * (u * u + v * v);
+    Fl.v[0] = rho * u;
+    Fl.v[1] = rho * u * u + p;
+    Fl.v[2] = rho * u * v;
+    Fl.v[3] = u * (E + p);
+    return Fl;
+}
 // ---------------------------------------------------------------------------
 // Exact Riemann solver for two stiffened gases (Toro ch. 4 with p -> p+pinf
 // per side; the reference links it from exact_rs_stiffenedgas, call sites
@@ -421,34 +487,32 @@ __device__ __forceinline__ void rs_sample_m(const RSFluid& F, const RSSideM& L, 
     typedef float T;
     const bool left = 0.0 <= st.u;
     const double srho = left ? L.rho : R.rho, su = left ? L.u : R.u, sp = left ? L.p : R.p;
-    const double sc = left ? L.c : R.c, spbar = left ? L.pbar : R.pbar;
+    const double sc = left ? L.c : R.c, sipb = left ? L.ipb : R.ipb;
     const T sg = left ? T(1) : T(-1);
-    const T g = (T)F.g, pinf = (T)F.pinf, rho = (T)srho, c = (T)sc, pbar = (T)spbar;
+    const T pinf = (T)F.pinf, rho = (T)srho, c = (T)sc;
     const T uk = sg * (T)su, us = sg * (T)st.u, ps = (T)st.p;
-    const T ratio = (ps + pinf) / pbar;
+    const T ratio = (ps + pinf) * (T)sipb;
     if (st.p > sp) {
-        const T Sk = uk - c * sqrt((g + T(1)) / (T(2) * g) * ratio + (g - T(1)) / (T(2) * g));
+        const T Sk = uk - c * sqrt(F.fsk * ratio + F.fz);
         if (T(0) <= Sk) { orho = rho; ou = (T)su; op = (T)sp; }
         else {
-            const T g6 = (T)F.g6;
-            orho = rho * (ratio + g6) / (g6 * ratio + T(1));
+            orho = rho * (ratio + F.fg6) / (F.fg6 * ratio + T(1));
             ou = (T)st.u; op = ps;
         }
         return;
     }
     const T SH = uk - c;
     if (T(0) <= SH) { orho = rho; ou = (T)su; op = (T)sp; return; }
-    const T cm = c * pow(ratio, (g - T(1)) / (T(2) * g));
+    const T cm = c * pow(ratio, F.fz);
     const T ST = us - cm;
-    if (T(0) > ST) { orho = rho * pow(ratio, T(1) / g); ou = (T)st.u; op = ps; return; }
-    const T cf = T(2) / (g + T(1)) * (c + T(0.5) * (g - T(1)) * uk);
-    orho = rho * pow(cf / c, T(2) / (g - T(1)));
+    if (T(0) > ST) { orho = rho * pow(ratio, F.fig); ou = (T)st.u; op = ps; return; }
+    const T cf = F.fa2 * (c + T(0.5) * ((T)F.g - T(1)) * uk);
+    const T q = cf / c;
+    orho = rho * pow(q, F.fk2);
     ou = sg * cf;
-    op = pbar * pow(cf / c, T(2) * g / (g - T(1))) - pinf;
+    op = (T)(left ? L.pbar : R.pbar) * pow(q, F.fk3) - pinf;
 }
 
-// flux from the sampled solution (pure_RS_exact.cpp:45-71): tangential
-// velocity upwinded on u*, y-faces swap the momentum components
 template <typename T>
 __device__ __forceinline__ T tang(bool px, const V4<T>& W) { return px ? W.v[2] : W.v[1]; }
 template <typename T>
@@ -481,6 +545,26 @@ __device__ __forceinline__ V4<T> rs_flux_trivial(bool px, T g, T pinf, const V4<
     return rs_flux_out<T>(px, g, pinf, WL.v[0], un, WL.v[3], (double)un, WL, WR);
 }
 
+// FP32 storage: the same flux with flux_prims_f
+__device__ __forceinline__ V4<float> rs_flux_out_f(bool px, const FluidF& Ff, float r, float u, float p, double ustar,
+                                                   const V4<float>& WL, const V4<float>& WR) {
+    V4<float> soln;
+    soln.v[0] = r;
+    soln.v[1] = u;
+    const float tr = tang(px, WR), tl = tang(px, WL);
+    soln.v[2] = ustar <= 0.0 ? tr : tl;
+    soln.v[3] = p;
+    V4<float> F = flux_prims_f(Ff, soln);
+    if (!px) { float tmp = F.v[1]; F.v[1] = F.v[2]; F.v[2] = tmp; }
+    return F;
+}
+
+__device__ __forceinline__ V4<float> rs_flux_trivial_f(bool px, const FluidF& Ff, const V4<float>& WL,
+                                                       const V4<float>& WR) {
+    const float un = norm_u(px, WL);
+    return rs_flux_out_f(px, Ff, WL.v[0], un, WL.v[3], (double)un, WL, WR);
+}
+
 // the general case, FP64 storage: rs_star, then Toro's sampling at x/t = 0
 __device__ __forceinline__ V4<double> rs_flux_full(bool px, double g, double pinf, const V4<double>& WL,
                                                    const V4<double>& WR, unsigned& st) {
@@ -495,8 +579,8 @@ __device__ __forceinline__ V4<double> rs_flux_full(bool px, double g, double pin
 }
 
 // FP32 storage: the mixed-precision solver above
-__device__ __forceinline__ V4<float> rs_flux_full_m(const RSFluid& F, bool px, float g, float pinf,
-                                                    const V4<float>& WL, const V4<float>& WR, unsigned& st) {
+__device__ __forceinline__ V4<float> rs_flux_full_m(const RSFluid& F, const FluidF& Ff, bool px, const V4<float>& WL,
+                                                    const V4<float>& WR, unsigned& st) {
     RSSideM L, R;
     rs_side_m(L, F, (double)WL.v[0], (double)norm_u(px, WL), (double)WL.v[3]);
     rs_side_m(R, F, (double)WR.v[0], (double)norm_u(px, WR), (double)WR.v[3]);
@@ -504,7 +588,7 @@ __device__ __forceinline__ V4<float> rs_flux_full_m(const RSFluid& F, bool px, f
     st |= rs_star_m(F, L, R, S);
     float r, u, p;
     rs_sample_m(F, L, R, S, r, u, p);
-    return rs_flux_out<float>(px, g, pinf, r, u, p, S.u, WL, WR);
+    return rs_flux_out_f(px, Ff, r, u, p, S.u, WL, WR);
 }
 
 // pure_RS_exact::solve_RP (pure_RS_exact.cpp:9-73): conserved L/R -> flux

@@ -190,7 +190,7 @@ __device__ __forceinline__ T minmod1(T dL, T dR) {
 // (FS_MUSCL.cpp:276-285) through W->U->W; Godunov: the stored states.
 template <typename R, bool MUSCL, typename T>
 __device__ __forceinline__ bool face_input(const Inst& I, int c, int i0, int j0, int lo_i, int lim_i, int lo_j,
-                                           int lim_j, T g, T pinf, const T (*sW)[FWY][FWX],
+                                           int lim_j, T g, T pinf, const FluidF& Ff, const T (*sW)[FWY][FWX],
                                            const T (*sE)[FEY][FEX], const void* Qi, V4<T>& WL, V4<T>& WR,
                                            bool& px, unsigned& st) {
     int la, lb, ra, rb, ci_lo, cj_lo, ci_hi, cj_hi;
@@ -209,6 +209,33 @@ __device__ __forceinline__ bool face_input(const Inst& I, int c, int i0, int j0,
     const bool lo_upd = ci_lo >= lo_i && ci_lo < lim_i && cj_lo >= lo_j && cj_lo < lim_j;
     const bool hi_upd = ci_hi >= lo_i && ci_hi < lim_i && cj_hi >= lo_j && cj_hi < lim_j;
     if (!lo_upd && !hi_upd) return false;
+    if constexpr (sizeof(T) == 4) {
+        // FP32 storage: the evolved values (or, if unphysical, the cell
+        // values) go to the solver as primitives, without the W->U->W round
+        // trip, which only adds rounding
+        if (MUSCL) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                WL.v[k] = px ? sE[4 + k][la - 1][lb - 1] : sE[12 + k][la - 1][lb - 1];
+                WR.v[k] = px ? sE[k][ra - 1][rb - 1] : sE[8 + k][ra - 1][rb - 1];
+            }
+            if (!phys_prims_f(Ff, WR) || !phys_prims_f(Ff, WL)) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) { WL.v[k] = sW[k][la][lb]; WR.v[k] = sW[k][ra][rb]; }
+                if (!phys_prims_f(Ff, WR) || !phys_prims_f(Ff, WL)) st |= ST_UNPHYS;
+            }
+        } else {
+            const V4<R> a = ld4<R>(Qi, I, cidx(I, ci_lo, cj_lo));
+            const V4<R> b = ld4<R>(Qi, I, cidx(I, ci_hi, cj_hi));
+            V4<float> UL, UR;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { UL.v[k] = (float)a.v[k]; UR.v[k] = (float)b.v[k]; }
+            if (!phys_state_f(UL) || !phys_state_f(UR)) st |= ST_UNPHYS;
+            WL = cons2prims_f(Ff, UL);
+            WR = cons2prims_f(Ff, UR);
+        }
+        return true;
+    }
     V4<T> UL, UR;
     if (MUSCL) {
         V4<T> SL, SR;
@@ -267,6 +294,7 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
         return;
     }
     const T g = (T)I.gam[f], pinf = (T)I.pinf[f];
+    const FluidF Ff = fluid_f(I.rsf[f]);   // FP32 constants (mixed precision only)
     const double dtd = I.sd[SD_DT];
     const T dtodx = (T)(dtd / I.dx), dtody = (T)(dtd / I.dy);
     const int i0 = I.g + w.y * FTY, j0 = I.g + w.z * FTX;
@@ -294,7 +322,9 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
         const int a = c / FWX, b = c % FWX;
         const int i = min(max(i0 - 2 + a, 0), I.H - 1);
         const int j = min(max(j0 - 2 + b, 0), I.W - 1);
-        V4<T> W = cons2prims(g, pinf, ld4<R>(Qi, I, cidx(I, i, j)));
+        V4<T> W;
+        if constexpr (sizeof(T) == 4) W = cons2prims_f(Ff, ld4<R>(Qi, I, cidx(I, i, j)));
+        else W = cons2prims(g, pinf, ld4<R>(Qi, I, cidx(I, i, j)));
         sW[0][a][b] = W.v[0]; sW[1][a][b] = W.v[1]; sW[2][a][b] = W.v[2]; sW[3][a][b] = W.v[3];
     }
     __syncthreads();
@@ -313,13 +343,21 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
                 BL.v[k] = wc - T(0.5) * sx; BR.v[k] = wc + T(0.5) * sx;
                 BB.v[k] = wc - T(0.5) * sy; BT.v[k] = wc + T(0.5) * sy;
             }
-            if (!phys_prims(g, pinf, BL) || !phys_prims(g, pinf, BR)) { BL = Wc; BR = Wc; }
-            if (!phys_prims(g, pinf, BB) || !phys_prims(g, pinf, BT)) { BB = Wc; BT = Wc; }
-            // Hancock increment hx*A(W)(BL-BR) + hy*B(W)(BB-BT) (euler_misc.hpp:149-180)
+            T rc2, ir;
             const T rho = Wc.v[0], u = Wc.v[1], v = Wc.v[2];
-            const T cs = eos_c(g, pinf, Wc.v[3], rho);
-            const T rc2 = rho * cs * cs;
-            const T ir = T(1) / rho;
+            if constexpr (sizeof(T) == 4) {
+                if (!phys_prims_f(Ff, BL) || !phys_prims_f(Ff, BR)) { BL = Wc; BR = Wc; }
+                if (!phys_prims_f(Ff, BB) || !phys_prims_f(Ff, BT)) { BB = Wc; BT = Wc; }
+                ir = T(1) / rho;
+                rc2 = g * (Wc.v[3] + pinf);   // rho c^2
+            } else {
+                if (!phys_prims(g, pinf, BL) || !phys_prims(g, pinf, BR)) { BL = Wc; BR = Wc; }
+                if (!phys_prims(g, pinf, BB) || !phys_prims(g, pinf, BT)) { BB = Wc; BT = Wc; }
+                const T cs = eos_c(g, pinf, Wc.v[3], rho);
+                rc2 = rho * cs * cs;
+                ir = T(1) / rho;
+            }
+            // Hancock increment hx*A(W)(BL-BR) + hy*B(W)(BB-BT) (euler_misc.hpp:149-180)
             const T x0 = BL.v[0] - BR.v[0], x1 = BL.v[1] - BR.v[1], x2 = BL.v[2] - BR.v[2], x3 = BL.v[3] - BR.v[3];
             const T y0 = BB.v[0] - BT.v[0], y1 = BB.v[1] - BT.v[1], y2 = BB.v[2] - BT.v[2], y3 = BB.v[3] - BT.v[3];
             T d[4];
@@ -348,9 +386,11 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
     for (int c = tid; c < NFACE; c += 256) {
         V4<T> WL, WR;
         bool px;
-        if (!face_input<R, MUSCL>(I, c, i0, j0, lo_i, lim_i, lo_j, lim_j, g, pinf, sW, sE, Qi, WL, WR, px, st)) continue;
+        if (!face_input<R, MUSCL>(I, c, i0, j0, lo_i, lim_i, lo_j, lim_j, g, pinf, Ff, sW, sE, Qi, WL, WR, px, st)) continue;
         if (rs_trivial(px, WL, WR)) {
-            const V4<T> F = rs_flux_trivial<T>(px, g, pinf, WL, WR);
+            V4<T> F;
+            if constexpr (sizeof(T) == 4) F = rs_flux_trivial_f(px, Ff, WL, WR);
+            else F = rs_flux_trivial<T>(px, g, pinf, WL, WR);
 #pragma unroll
             for (int k = 0; k < 4; ++k) sFl[k][c] = F.v[k];
         } else {
@@ -366,9 +406,9 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
         V4<T> WL, WR;
         bool px;
         unsigned dummy = 0;
-        face_input<R, MUSCL>(I, c, i0, j0, lo_i, lim_i, lo_j, lim_j, g, pinf, sW, sE, Qi, WL, WR, px, dummy);
+        face_input<R, MUSCL>(I, c, i0, j0, lo_i, lim_i, lo_j, lim_j, g, pinf, Ff, sW, sE, Qi, WL, WR, px, dummy);
         V4<T> F;
-        if constexpr (sizeof(T) == 4) F = rs_flux_full_m(rf, px, g, pinf, WL, WR, st);
+        if constexpr (sizeof(T) == 4) F = rs_flux_full_m(rf, Ff, px, WL, WR, st);
         else F = rs_flux_full(px, g, pinf, WL, WR, st);
 #pragma unroll
         for (int k = 0; k < 4; ++k) sFl[k][c] = F.v[k];
@@ -380,7 +420,10 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
         const long long c = cidx(I, ci, cj);
         V4<R> U = ld4<R>(Qi, I, c);
         if (ci >= bb0 && ci < bb1 && cj >= bb2 && cj < bb3) {
-            if (!phys_state(U)) flag(I, ST_UNPHYS);   // FS_MUSCL.cpp:167
+            bool ok;
+            if constexpr (sizeof(T) == 4) ok = phys_state_f(U);
+            else ok = phys_state(U);
+            if (!ok) flag(I, ST_UNPHYS);   // FS_MUSCL.cpp:167
             const int rr = tid / FTX, fc = tid % FTX;
             const int fxl = rr * (FTX + 1) + fc, fxr = fxl + 1;
             const int fyb = NFX + rr * FTX + fc, fyt = fyb + FTX;
@@ -1301,9 +1344,67 @@ __global__ __launch_bounds__(256) void k_ls_copy(const Inst* __restrict__ insts,
 }
 
 // Sussman-Smereka-Osher reinitialisation, first-order Godunov, Russo-Smereka
-// subcell fix near the interface; phi0 frozen in A.  Iteration k of K reads
-// (k==0 ? A : previous output) and writes phi when K-1-k is even, else B.
-// gate: 0 = always (explicit reinit), otherwise only on reinit steps.
+// subcell fix near the interface; phi0 frozen in A.  Every operation rounds
+// on its own so that the per-iteration and the blocked kernels agree bit for
+// bit.  reinit_setup: the per-cell constants from phi0 at the cell and its
+// four neighbours (mode 1: interface cell, 2: phi0 > 0, 4: phi0 < 0;
+// cst: phi0 / |grad phi0| at interface cells, the smoothed sign otherwise).
+template <typename R>
+__device__ __forceinline__ void reinit_setup(R f0v, R f0e, R f0w, R f0n, R f0s, double dx, double dy, double h,
+                                             int& mode, R& cst) {
+#pragma clang fp contract(off)
+    const bool s0 = f0v <= R(0);
+    const bool ifc = ((f0e <= R(0)) != s0) || ((f0w <= R(0)) != s0) || ((f0n <= R(0)) != s0) || ((f0s <= R(0)) != s0);
+    mode = (ifc ? 1 : 0) | (f0v > R(0) ? 2 : 0) | (f0v < R(0) ? 4 : 0);
+    if (ifc) {
+        const R gx = (f0e - f0w) / R(2.0 * dx), gy = (f0n - f0s) / R(2.0 * dy);
+        R gm = sqrt(gx * gx + gy * gy);
+        gm = fmax(gm, fabs(f0e - f0v) / R(dx));
+        gm = fmax(gm, fabs(f0v - f0w) / R(dx));
+        gm = fmax(gm, fabs(f0n - f0v) / R(dy));
+        gm = fmax(gm, fabs(f0v - f0s) / R(dy));
+        gm = fmax(gm, R(1e-12));
+        cst = f0v / gm;
+    } else {
+        cst = f0v / sqrt(f0v * f0v + R(h * h));
+    }
+}
+
+// one pseudo-time step of the cell value f from its neighbours' values
+template <typename R>
+__device__ __forceinline__ R reinit_update(int mode, R cst, R f, R fw, R fe, R fs, R fn, double dx, double dy,
+                                           double h) {
+#pragma clang fp contract(off)
+    const double dtau = 0.5 * h;
+    if (mode & 1) {
+        const R sg = (mode & 2) ? R(1) : ((mode & 4) ? R(-1) : R(0));
+        return f - R(dtau / h) * (sg * fabs(f) - cst);
+    }
+    const R a = (f - fw) / R(dx);
+    const R b = (fe - f) / R(dx);
+    const R c = (f - fs) / R(dy);
+    const R d = (fn - f) / R(dy);
+    R gsq;
+    if (mode & 2) {
+        const R ap = fmax(a, R(0)), bm = fmin(b, R(0)), cp = fmax(c, R(0)), dm = fmin(d, R(0));
+        gsq = fmax(ap * ap, bm * bm) + fmax(cp * cp, dm * dm);
+    } else {
+        const R am = fmin(a, R(0)), bp = fmax(b, R(0)), cm = fmin(c, R(0)), dp = fmax(d, R(0));
+        gsq = fmax(am * am, bp * bp) + fmax(cm * cm, dp * dp);
+    }
+    return f - R(dtau) * cst * (sqrt(gsq) - R(1));
+}
+
+template <typename R>
+__device__ __forceinline__ void reinit_cell_setup(const Inst& I, const R* P0, int si, int sj, int& mode, R& cst) {
+    const double h = fmin(I.dx, I.dy);
+    reinit_setup<R>(P0[cidx(I, si, sj)], P0[cidx(I, si, sj + 1)], P0[cidx(I, si, sj - 1)], P0[cidx(I, si + 1, sj)],
+                    P0[cidx(I, si - 1, sj)], I.dx, I.dy, h, mode, cst);
+}
+
+// Iteration k of K reads (k==0 ? A : previous output) and writes phi when
+// K-1-k is even, else B.  gate: 0 = always (explicit reinit), otherwise only
+// on reinit steps.
 template <typename R>
 __global__ __launch_bounds__(256) void k_ls_reinit(const Inst* __restrict__ insts, const int4* __restrict__ work,
                                                    int k, int K, int gate_every, int force) {
@@ -1319,43 +1420,128 @@ __global__ __launch_bounds__(256) void k_ls_reinit(const Inst* __restrict__ inst
     const bool out_phi = ((K - 1 - k) % 2) == 0;
     const R* in = k == 0 ? P0 : (const R*)(out_phi ? I.phiB : I.phi);
     R* out = (R*)(out_phi ? I.phi : I.phiB);
-    const double dx = I.dx, dy = I.dy, h = fmin(dx, dy), dtau = 0.5 * h;
-    const long long q = cidx(I, si, sj);
-    const R f0v = P0[q];
-    const R f0e = P0[cidx(I, si, sj + 1)], f0w = P0[cidx(I, si, sj - 1)];
-    const R f0n = P0[cidx(I, si + 1, sj)], f0s = P0[cidx(I, si - 1, sj)];
-    const bool s0 = f0v <= R(0);
-    const bool ifc = ((f0e <= R(0)) != s0) || ((f0w <= R(0)) != s0) || ((f0n <= R(0)) != s0) || ((f0s <= R(0)) != s0);
-    const R f = in[q];
-    R res;
-    if (ifc) {
-        const R gx = (f0e - f0w) / R(2.0 * dx), gy = (f0n - f0s) / R(2.0 * dy);
-        R gm = sqrt(gx * gx + gy * gy);
-        gm = fmax(gm, fabs(f0e - f0v) / R(dx));
-        gm = fmax(gm, fabs(f0v - f0w) / R(dx));
-        gm = fmax(gm, fabs(f0n - f0v) / R(dy));
-        gm = fmax(gm, fabs(f0v - f0s) / R(dy));
-        gm = fmax(gm, R(1e-12));
-        const R D = f0v / gm;
-        const R sg = f0v > R(0) ? R(1) : (f0v < R(0) ? R(-1) : R(0));
-        res = f - R(dtau / h) * (sg * fabs(f) - D);
-    } else {
-        const R a = (f - in[cidx(I, si, sj - 1)]) / R(dx);
-        const R b = (in[cidx(I, si, sj + 1)] - f) / R(dx);
-        const R c = (f - in[cidx(I, si - 1, sj)]) / R(dy);
-        const R d = (in[cidx(I, si + 1, sj)] - f) / R(dy);
-        R gsq;
-        if (f0v > R(0)) {
-            const R ap = fmax(a, R(0)), bm = fmin(b, R(0)), cp = fmax(c, R(0)), dm = fmin(d, R(0));
-            gsq = fmax(ap * ap, bm * bm) + fmax(cp * cp, dm * dm);
-        } else {
-            const R am = fmin(a, R(0)), bp = fmax(b, R(0)), cm = fmin(c, R(0)), dp = fmax(d, R(0));
-            gsq = fmax(am * am, bp * bp) + fmax(cm * cm, dp * dp);
-        }
-        const R S = f0v / sqrt(f0v * f0v + R(h * h));
-        res = f - R(dtau) * S * (sqrt(gsq) - R(1));
+    int mode;
+    R cst;
+    reinit_cell_setup<R>(I, P0, si, sj, mode, cst);
+    R fw = R(0), fe = R(0), fs = R(0), fn = R(0);
+    if (!(mode & 1)) {
+        fw = in[cidx(I, si, sj - 1)]; fe = in[cidx(I, si, sj + 1)];
+        fs = in[cidx(I, si - 1, sj)]; fn = in[cidx(I, si + 1, sj)];
     }
-    out[cidx(I, i, j)] = res;
+    out[cidx(I, i, j)] = reinit_update<R>(mode, cst, in[cidx(I, si, sj)], fw, fe, fs, fn, I.dx, I.dy, fmin(I.dx, I.dy));
+}
+
+// ---------------------------------------------------------------------------
+// Up to RT_MAXIT iterations of k_ls_reinit in one launch (temporal blocking,
+// as k_extrap_tb): a RTX x RTY tile of interior cells plus an RT_MAXIT-cell
+// halo of the iterate in LDS; iteration t is exact on the region shrunk by t
+// cells; the first ghost layer (the only one the stencil reads) is refreshed
+// from its interior source after each iteration; the last iteration writes
+// the owned cells and the ghost cells they source.  Mirror BCs only.
+// in_sel / out_sel: 0 = A, 1 = phi, 2 = B.
+// work: {inst, tile_y, tile_x, 0}, RTX x RTY interior tiles
+// ---------------------------------------------------------------------------
+constexpr int RTX = 64, RTY = 32, RT_THREADS = 1024, RT_MAXIT = 10;
+
+template <typename R>
+__device__ __forceinline__ R* ls_buf(const Inst& I, int sel) {
+    return (R*)(sel == 0 ? I.phiA : (sel == 1 ? I.phi : I.phiB));
+}
+
+template <typename R>
+__global__ __launch_bounds__(RT_THREADS) void k_ls_reinit_tb(const Inst* __restrict__ insts,
+                                                             const int4* __restrict__ work, int nit, int in_sel,
+                                                             int out_sel, int gate_every, int force) {
+    constexpr int RX = RTX + 2 * RT_MAXIT, RY = RTY + 2 * RT_MAXIT, NC = RX * RY;
+    constexpr int SLOTS = (NC + RT_THREADS - 1) / RT_THREADS;
+    __shared__ R s[NC];
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!force && (!I.si[SI_ACTIVE] || !reinit_step(I, gate_every))) return;
+    const int g = I.g;
+    const int ri = g + w.y * RTY - RT_MAXIT, rj = g + w.z * RTX - RT_MAXIT;
+    const R* P0 = (const R*)I.phiA;
+    const R* in = ls_buf<R>(I, in_sel);
+    R* out = ls_buf<R>(I, out_sel);
+    const double h = fmin(I.dx, I.dy);
+    // per cell: reinit_setup's mode and constant; bit 8: interior with its
+    // stencil inside the region, bit 9: owned
+    R cs[SLOTS];
+    int meta[SLOTS];
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+        const int idx = threadIdx.x + k * RT_THREADS;
+        cs[k] = R(0); meta[k] = 0;
+        if (idx < NC) {
+            const int a = idx / RX, b = idx - (idx / RX) * RX;
+            const int i = ri + a, j = rj + b;
+            R v = R(0);
+            if (i >= 0 && i < I.H && j >= 0 && j < I.W) {
+                v = in[cidx(I, i, j)];
+                if (interior(I, i, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1) {
+                    int mode;
+                    reinit_cell_setup<R>(I, P0, i, j, mode, cs[k]);
+                    const bool own = a >= RT_MAXIT && a < RT_MAXIT + RTY && b >= RT_MAXIT && b < RT_MAXIT + RTX;
+                    meta[k] = mode | 256 | (own ? 512 : 0);
+                }
+            }
+            s[idx] = v;
+        }
+    }
+    __syncthreads();
+    const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
+    for (int t = 1; t <= nit; ++t) {
+        R nv[SLOTS];
+        int act = 0;
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) {
+            const int idx = threadIdx.x + k * RT_THREADS;
+            const int a = idx / RX, b = idx - (idx / RX) * RX;
+            if ((meta[k] & 256) && a >= t && a < RY - t && b >= t && b < RX - t) {
+                act |= 1 << k;
+                nv[k] = reinit_update<R>(meta[k] & 7, cs[k], s[idx], s[idx - 1], s[idx + 1], s[idx - RX], s[idx + RX],
+                                         I.dx, I.dy, h);
+            }
+        }
+        if (t == nit) {
+            // owned cells and their ghost images (inverse of src_row / src_col)
+#pragma unroll
+            for (int k = 0; k < SLOTS; ++k) {
+                if (!(meta[k] & 512)) continue;
+                const int idx = threadIdx.x + k * RT_THREADS;
+                const int a = idx / RX, b = idx - (idx / RX) * RX;
+                const int i = ri + a, j = rj + b;
+                out[cidx(I, i, j)] = nv[k];
+                if (i < 2 * g || i >= I.Ny || j < 2 * g || j >= I.Nx) {
+#pragma unroll 1
+                    for (int m = 1; m < 9; ++m) {
+                        const int pr = m / 3, pc = m - 3 * (m / 3);
+                        int r = i, cc = j;
+                        if (pr == 1) { if (i >= 2 * g) continue; r = 2 * g - 1 - i; }
+                        if (pr == 2) { if (i < I.Ny) continue; r = 2 * (I.Ny + g) - 1 - i; }
+                        if (pc == 1) { if (j >= 2 * g) continue; cc = 2 * g - 1 - j; }
+                        if (pc == 2) { if (j < I.Nx) continue; cc = 2 * (I.Nx + g) - 1 - j; }
+                        out[cidx(I, r, cc)] = nv[k];
+                    }
+                }
+            }
+            break;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) {
+            if (!(act & (1 << k))) continue;
+            const int idx = threadIdx.x + k * RT_THREADS;
+            const int a = idx / RX, b = idx - (idx / RX) * RX;
+            const int i = ri + a, j = rj + b;
+            s[idx] = nv[k];
+            if (i == ib) s[idx - RX] = nv[k];
+            if (i == it) s[idx + RX] = nv[k];
+            if (j == jl) s[idx - 1] = nv[k];
+            if (j == jr) s[idx + 1] = nv[k];
+        }
+        __syncthreads();
+    }
 }
 
 // copy phi -> A (frozen phi0 for an explicit reinitialisation)
@@ -1544,23 +1730,41 @@ void launch_mgfm_final(const LaunchCtx& L, int qin, int qout) {
     hipLaunchKernelGGL((k_mgfm_final<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, qin, qout);
 }
 
+// K reinitialisation iterations from A into phi: blocked launches of up to
+// RT_MAXIT iterations each, or one launch per iteration (periodic contexts)
+template <typename R>
+void launch_reinit_chain(const LaunchCtx& L, int K, int every, int force) {
+    if (L.tb) {
+        const int m = (K + RT_MAXIT - 1) / RT_MAXIT;
+        int in = 0, done = 0;
+        for (int c = 0; c < m; ++c) {
+            const int nit = (K - done + (m - c) - 1) / (m - c);   // even split
+            const int out = ((m - 1 - c) % 2 == 0) ? 1 : 2;
+            hipLaunchKernelGGL((k_ls_reinit_tb<R>), GRID(L.n_rt), dim3(RT_THREADS), 0, L.stream, L.insts, L.w_rt, nit,
+                               in, out, every, force);
+            in = out;
+            done += nit;
+        }
+        return;
+    }
+    for (int k = 0; k < K; ++k)
+        hipLaunchKernelGGL((k_ls_reinit<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, k, K, every,
+                           force);
+}
+
 template <typename R>
 void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters) {
     hipLaunchKernelGGL((k_ls_stage<R, 1>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
     hipLaunchKernelGGL((k_ls_stage<R, 2>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
     hipLaunchKernelGGL((k_ls_stage<R, 3>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
     hipLaunchKernelGGL((k_ls_copy<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, every);
-    if (every > 0)
-        for (int k = 0; k < iters; ++k)
-            hipLaunchKernelGGL((k_ls_reinit<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, k, iters,
-                               every, 0);
+    if (every > 0) launch_reinit_chain<R>(L, iters, every, 0);
 }
 
 template <typename R>
 void launch_reinit_now(const LaunchCtx& L, int iters) {
     hipLaunchKernelGGL((k_phi_to_A<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full);
-    for (int k = 0; k < iters; ++k)
-        hipLaunchKernelGGL((k_ls_reinit<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, k, iters, 0, 1);
+    launch_reinit_chain<R>(L, iters, 0, 1);
 }
 
 template <typename R>

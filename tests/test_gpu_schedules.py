@@ -118,3 +118,25 @@ def test_iface_levels_match_plain_rounds(variant, precision):
     compare(ca, cb, len(cases))
     for k in range(len(cases)):
         assert ca.iface_info(k)[0] > 0
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
+@pytest.mark.parametrize("iters", [1, 7, 23])
+def test_reinit_blocked_matches_per_iteration(iters, precision):
+    """explicit reinitialisation of a distorted level set (1, 7 and 23
+    iterations: one partial block, one block, three blocks)"""
+    print("CASE reinit iters=%d precision=%d" % (iters, precision), flush=True)
+    cases = []
+    for k, bcs in enumerate(BCS[:2]):
+        info, eos, (g1, g2, phi) = perturbed_bubble(150 + 17 * k, 44 + 5 * k, bcs, 7 + k)
+        H, W = phi.shape
+        yy, xx = np.mgrid[0:H, 0:W]
+        phi = np.ascontiguousarray(phi * (1.0 + 0.3 * np.sin(xx / 9.0) * np.cos(yy / 7.0)))
+        cases.append((info, eos, (g1, g2, phi)))
+    ca = make_ctx(cases, G.GFM_RIEMANN, precision, True)
+    cb = make_ctx(cases, G.GFM_RIEMANN, precision, False)
+    for c in (ca, cb):
+        c.ls_reinit(iters)
+        c.sync()
+    for k in range(len(cases)):
+        assert_same(ca.download(k)[2], cb.download(k)[2], "instance %d phi" % k)
