@@ -697,7 +697,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
     const void* in0 = (MIN & MF_FIRST) ? I.Q[0][bin] : I.P[0][bin];
     const void* in1 = (MIN & MF_FIRST) ? I.Q[1][bin] : I.P[1][bin];
     // per cell: unit normal and bits (1: interior with its stencil inside the
-    // region, 2: fluid 2 real, 4: owned)
+    // region, 2: fluid 2 real, 4: owned, 8 and up: distance to the region edge)
     R nxs[SLOTS], nys[SLOTS];
     int meta[SLOTS];
 #pragma unroll
@@ -716,7 +716,8 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
                     unit_normal<R>(I, i, j, nxd, nyd);
                     nxs[k] = (R)nxd; nys[k] = (R)nyd;
                     const bool own = a >= NIT && a < NIT + TBY && b >= NIT && b < NIT + TBX;
-                    meta[k] = 1 | (ld1<R>(I.phi, cidx(I, i, j)) <= R(0) ? 0 : 2) | (own ? 4 : 0);
+                    const int dist = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
+                    meta[k] = 1 | (ld1<R>(I.phi, cidx(I, i, j)) <= R(0) ? 0 : 2) | (own ? 4 : 0) | (dist << 3);
                 }
             }
             tb_put<R, NC>(s, 0, idx, W1);
@@ -736,8 +737,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
             const int idx = threadIdx.x + k * TB_THREADS;
-            const int a = idx / RX, b = idx - (idx / RX) * RX;
-            if ((meta[k] & 1) && a >= t && a < RY - t && b >= t && b < RX - t) {
+            if ((meta[k] & 1) && (meta[k] >> 3) >= t) {
                 act |= 1 << k;
                 nv[k] = tb_update<R, NC, RX>(s, idx, (meta[k] & 2) ? 0 : 1, nxs[k], nys[k], ax, ay);
                 if ((meta[k] & 4) && !(nv[k].v[0] >= R(0))) bad = true;
@@ -1288,17 +1288,36 @@ __global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ ins
 //   stage 3: A = 1/3 phi + 2/3 (B + dt L(B))
 // work: full-array 64x4 tiles
 // ---------------------------------------------------------------------------
+// -(u d/dx + v d/dy) psi by HOUC upwinding; psi points at the cell, sx / sy
+// are the strides of a column / row step.  Every operation rounds on its own
+// (fixed order, no contraction) so that each kernel using it gets the same
+// bits.
+template <typename R>
+__device__ __forceinline__ R houc_rhs(const Houc& H, int n, const R* psi, long long sx, long long sy, R u, R v,
+                                      double dx, double dy) {
+#pragma clang fp contract(off)
+    R px = R(0), py = R(0);
+    if (u > R(0)) { for (int m = 0; m < n; ++m) px += (R)H.c[m] * psi[(H.o0 + m) * sx]; }
+    else { for (int m = 0; m < n; ++m) px -= (R)H.c[m] * psi[(-H.o0 - m) * sx]; }
+    if (v > R(0)) { for (int m = 0; m < n; ++m) py += (R)H.c[m] * psi[(H.o0 + m) * sy]; }
+    else { for (int m = 0; m < n; ++m) py -= (R)H.c[m] * psi[(-H.o0 - m) * sy]; }
+    return -(u * (px / (R)dx) + v * (py / (R)dy));
+}
+
 template <typename R>
 __device__ __forceinline__ R ls_rhs(const Inst& I, const Houc& H, const R* psi, int i, int j) {
     const long long c = cidx(I, i, j);
     const R* vel = (const R*)I.vel;
-    const R u = vel[c], v = vel[c + I.plane];
-    R px = R(0), py = R(0);
-    if (u > R(0)) { for (int m = 0; m < H.n; ++m) px += (R)H.c[m] * psi[cidx(I, i, j + H.o0 + m)]; }
-    else { for (int m = 0; m < H.n; ++m) px -= (R)H.c[m] * psi[cidx(I, i, j - H.o0 - m)]; }
-    if (v > R(0)) { for (int m = 0; m < H.n; ++m) py += (R)H.c[m] * psi[cidx(I, i + H.o0 + m, j)]; }
-    else { for (int m = 0; m < H.n; ++m) py -= (R)H.c[m] * psi[cidx(I, i - H.o0 - m, j)]; }
-    return -(u * (px / (R)I.dx) + v * (py / (R)I.dy));
+    return houc_rhs<R>(H, H.n, psi + c, 1, I.pitch, vel[c], vel[c + I.plane], I.dx, I.dy);
+}
+
+// the TVD-RK3 combinations of the three stages
+template <typename R, int STAGE>
+__device__ __forceinline__ R rk3_combine(R phi0, R prev, R dt, R rhs) {
+#pragma clang fp contract(off)
+    if (STAGE == 1) return phi0 + dt * rhs;
+    if (STAGE == 2) return R(0.75) * phi0 + R(0.25) * (prev + dt * rhs);
+    return R(1.0 / 3.0) * phi0 + R(2.0 / 3.0) * (prev + dt * rhs);
 }
 
 template <typename R, int STAGE>
@@ -1318,11 +1337,112 @@ __global__ __launch_bounds__(256) void k_ls_stage(const Inst* __restrict__ insts
     const R* B = (const R*)I.phiB;
     const long long cs = cidx(I, si, sj);
     R val;
-    if (STAGE == 1) val = phi[cs] + dt * ls_rhs<R>(I, H, phi, si, sj);
-    else if (STAGE == 2) val = R(0.75) * phi[cs] + R(0.25) * (A[cs] + dt * ls_rhs<R>(I, H, A, si, sj));
-    else val = R(1.0 / 3.0) * phi[cs] + R(2.0 / 3.0) * (B[cs] + dt * ls_rhs<R>(I, H, B, si, sj));
+    if (STAGE == 1) val = rk3_combine<R, 1>(phi[cs], R(0), dt, ls_rhs<R>(I, H, phi, si, sj));
+    else if (STAGE == 2) val = rk3_combine<R, 2>(phi[cs], A[cs], dt, ls_rhs<R>(I, H, A, si, sj));
+    else val = rk3_combine<R, 3>(phi[cs], B[cs], dt, ls_rhs<R>(I, H, B, si, sj));
     R* out = (R*)(STAGE == 2 ? I.phiB : I.phiA);
     out[cidx(I, i, j)] = val;
+}
+
+// ---------------------------------------------------------------------------
+// The three TVD-RK3 stages in one launch (temporal blocking, as k_extrap_tb):
+// a LTX x LTY tile plus a 3 r halo (r = HOUC stencil radius) in LDS, stage t
+// exact on the region shrunk by t r cells, the r ghost layers the stencil
+// reads refreshed from their mirror sources after stages 1 and 2, the result
+// written to A for the owned cells and the ghost cells they source.  Same
+// arithmetic as k_ls_stage, bit for bit.  Mirror BCs only.
+// work: the RTX x RTY interior tiles of the reinitialisation
+// ---------------------------------------------------------------------------
+constexpr int LTX = 64, LTY = 32;
+
+template <typename R, int ORDER>
+__global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                    Houc H) {
+    constexpr int N = ORDER + 1, RAD = N / 2, HALO = 3 * RAD;
+    constexpr int RX = LTX + 2 * HALO, RY = LTY + 2 * HALO, NC = RX * RY;
+    constexpr int SLOTS = (NC + 1023) / 1024;
+    __shared__ R s0[NC], s1[NC];
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE]) return;
+    const int g = I.g;
+    const int ri = g + w.y * LTY - HALO, rj = g + w.z * LTX - HALO;
+    const R* phi = (const R*)I.phi;
+    const R* vel = (const R*)I.vel;
+    const R dt = (R)I.sd[SD_DT];
+    // per cell: phi, velocity; meta bits 0-7: distance to the region edge,
+    // 8: interior, 9: owned
+    R p0[SLOTS], us[SLOTS], vs[SLOTS];
+    int meta[SLOTS];
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+        const int idx = threadIdx.x + k * 1024;
+        p0[k] = R(0); us[k] = R(0); vs[k] = R(0); meta[k] = 0;
+        if (idx < NC) {
+            const int a = idx / RX, b = idx - (idx / RX) * RX;
+            const int i = ri + a, j = rj + b;
+            R val = R(0);
+            if (i >= 0 && i < I.H && j >= 0 && j < I.W) {
+                const long long c = cidx(I, i, j);
+                val = phi[c];
+                if (interior(I, i, j)) {
+                    us[k] = vel[c];
+                    vs[k] = vel[c + I.plane];
+                    const int d = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
+                    const bool own = a >= HALO && a < HALO + LTY && b >= HALO && b < HALO + LTX;
+                    meta[k] = d | 256 | (own ? 512 : 0);
+                }
+            }
+            p0[k] = val;
+            s0[idx] = val;
+        }
+    }
+    __syncthreads();
+    const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
+#pragma unroll
+    for (int st = 1; st <= 2; ++st) {
+        const R* in = st == 1 ? s0 : s1;
+        R* out = st == 1 ? s1 : s0;
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) {
+            if (!(meta[k] & 256) || (meta[k] & 255) < st * RAD) continue;
+            const int idx = threadIdx.x + k * 1024;
+            const R rhs = houc_rhs<R>(H, N, in + idx, 1, RX, us[k], vs[k], I.dx, I.dy);
+            const R nv = st == 1 ? rk3_combine<R, 1>(p0[k], R(0), dt, rhs) : rk3_combine<R, 2>(p0[k], in[idx], dt, rhs);
+            out[idx] = nv;
+            // mirror ghosts within the stencil radius of the boundary
+            const int a = idx / RX, b = idx - (idx / RX) * RX;
+            const int i = ri + a, j = rj + b;
+            if (i - ib < RAD) { const int ga = a - 2 * (i - ib) - 1; if (ga >= 0) out[ga * RX + b] = nv; }
+            if (it - i < RAD) { const int ga = a + 2 * (it - i) + 1; if (ga < RY) out[ga * RX + b] = nv; }
+            if (j - jl < RAD) { const int gb = b - 2 * (j - jl) - 1; if (gb >= 0) out[a * RX + gb] = nv; }
+            if (jr - j < RAD) { const int gb = b + 2 * (jr - j) + 1; if (gb < RX) out[a * RX + gb] = nv; }
+        }
+        __syncthreads();
+    }
+    R* A = (R*)I.phiA;
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+        if (!(meta[k] & 512)) continue;
+        const int idx = threadIdx.x + k * 1024;
+        const R rhs = houc_rhs<R>(H, N, s0 + idx, 1, RX, us[k], vs[k], I.dx, I.dy);
+        const R nv = rk3_combine<R, 3>(p0[k], s0[idx], dt, rhs);
+        const int a = idx / RX, b = idx - (idx / RX) * RX;
+        const int i = ri + a, j = rj + b;
+        A[cidx(I, i, j)] = nv;
+        if (i < 2 * g || i >= I.Ny || j < 2 * g || j >= I.Nx) {
+#pragma unroll 1
+            for (int m = 1; m < 9; ++m) {
+                const int pr = m / 3, pc = m - 3 * (m / 3);
+                int r = i, cc = j;
+                if (pr == 1) { if (i >= 2 * g) continue; r = 2 * g - 1 - i; }
+                if (pr == 2) { if (i < I.Ny) continue; r = 2 * (I.Ny + g) - 1 - i; }
+                if (pc == 1) { if (j >= 2 * g) continue; cc = 2 * g - 1 - j; }
+                if (pc == 2) { if (j < I.Nx) continue; cc = 2 * (I.Nx + g) - 1 - j; }
+                A[cidx(I, r, cc)] = nv;
+            }
+        }
+    }
 }
 
 __device__ __forceinline__ bool reinit_step(const Inst& I, int every) {
@@ -1370,29 +1490,40 @@ __device__ __forceinline__ void reinit_setup(R f0v, R f0e, R f0w, R f0n, R f0s, 
     }
 }
 
-// one pseudo-time step of the cell value f from its neighbours' values
+// one pseudo-time step of the cell value f from its neighbours' values; the
+// one-sided differences are squared before the 1/dx^2, 1/dy^2 scaling
+// (max((a/dx)^2, (b/dx)^2) = max(a^2, b^2) / dx^2)
 template <typename R>
-__device__ __forceinline__ R reinit_update(int mode, R cst, R f, R fw, R fe, R fs, R fn, double dx, double dy,
-                                           double h) {
+__device__ __forceinline__ R reinit_update(int mode, R cst, R f, R fw, R fe, R fs, R fn, R idx2, R idy2, R dtau) {
 #pragma clang fp contract(off)
-    const double dtau = 0.5 * h;
     if (mode & 1) {
         const R sg = (mode & 2) ? R(1) : ((mode & 4) ? R(-1) : R(0));
-        return f - R(dtau / h) * (sg * fabs(f) - cst);
+        return f - R(0.5) * (sg * fabs(f) - cst);
     }
-    const R a = (f - fw) / R(dx);
-    const R b = (fe - f) / R(dx);
-    const R c = (f - fs) / R(dy);
-    const R d = (fn - f) / R(dy);
-    R gsq;
+    const R a = f - fw, b = fe - f, c = f - fs, d = fn - f;
+    R gx, gy;
     if (mode & 2) {
         const R ap = fmax(a, R(0)), bm = fmin(b, R(0)), cp = fmax(c, R(0)), dm = fmin(d, R(0));
-        gsq = fmax(ap * ap, bm * bm) + fmax(cp * cp, dm * dm);
+        gx = fmax(ap * ap, bm * bm);
+        gy = fmax(cp * cp, dm * dm);
     } else {
         const R am = fmin(a, R(0)), bp = fmax(b, R(0)), cm = fmin(c, R(0)), dp = fmax(d, R(0));
-        gsq = fmax(am * am, bp * bp) + fmax(cm * cm, dp * dp);
+        gx = fmax(am * am, bp * bp);
+        gy = fmax(cm * cm, dp * dp);
     }
-    return f - R(dtau) * cst * (sqrt(gsq) - R(1));
+    return f - dtau * cst * (sqrt(gx * idx2 + gy * idy2) - R(1));
+}
+
+// the constants reinit_update takes: 1/dx^2, 1/dy^2, dtau = h/2
+template <typename R>
+struct ReinitK { R idx2, idy2, dtau; };
+template <typename R>
+__device__ __forceinline__ ReinitK<R> reinit_k(const Inst& I) {
+    ReinitK<R> k;
+    k.idx2 = R(1.0 / (I.dx * I.dx));
+    k.idy2 = R(1.0 / (I.dy * I.dy));
+    k.dtau = R(0.5 * fmin(I.dx, I.dy));
+    return k;
 }
 
 template <typename R>
@@ -1428,7 +1559,8 @@ __global__ __launch_bounds__(256) void k_ls_reinit(const Inst* __restrict__ inst
         fw = in[cidx(I, si, sj - 1)]; fe = in[cidx(I, si, sj + 1)];
         fs = in[cidx(I, si - 1, sj)]; fn = in[cidx(I, si + 1, sj)];
     }
-    out[cidx(I, i, j)] = reinit_update<R>(mode, cst, in[cidx(I, si, sj)], fw, fe, fs, fn, I.dx, I.dy, fmin(I.dx, I.dy));
+    const ReinitK<R> rk = reinit_k<R>(I);
+    out[cidx(I, i, j)] = reinit_update<R>(mode, cst, in[cidx(I, si, sj)], fw, fe, fs, fn, rk.idx2, rk.idy2, rk.dtau);
 }
 
 // ---------------------------------------------------------------------------
@@ -1451,21 +1583,45 @@ __device__ __forceinline__ R* ls_buf(const Inst& I, int sel) {
 template <typename R>
 __global__ __launch_bounds__(RT_THREADS) void k_ls_reinit_tb(const Inst* __restrict__ insts,
                                                              const int4* __restrict__ work, int nit, int in_sel,
-                                                             int out_sel, int gate_every, int force) {
+                                                             int out_sel, int gate_every, int force, int copy_idle) {
     constexpr int RX = RTX + 2 * RT_MAXIT, RY = RTY + 2 * RT_MAXIT, NC = RX * RY;
     constexpr int SLOTS = (NC + RT_THREADS - 1) / RT_THREADS;
     __shared__ R s[NC];
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!force && (!I.si[SI_ACTIVE] || !reinit_step(I, gate_every))) return;
     const int g = I.g;
+    if (!force) {
+        if (!I.si[SI_ACTIVE]) return;
+        if (!reinit_step(I, gate_every)) {
+            // a step without reinitialisation: phi = A (k_ls_copy's work)
+            if (copy_idle) {
+                const R* A = (const R*)I.phiA;
+                R* phi = (R*)I.phi;
+                for (int q = threadIdx.x; q < RTX * RTY; q += RT_THREADS) {
+                    const int i = g + w.y * RTY + q / RTX, j = g + w.z * RTX + q % RTX;
+                    if (!interior(I, i, j)) continue;
+                    for (int m = 0; m < 9; ++m) {
+                        const int pr = m / 3, pc = m - 3 * (m / 3);
+                        int r = i, cc = j;
+                        if (pr == 1) { if (i >= 2 * g) continue; r = 2 * g - 1 - i; }
+                        if (pr == 2) { if (i < I.Ny) continue; r = 2 * (I.Ny + g) - 1 - i; }
+                        if (pc == 1) { if (j >= 2 * g) continue; cc = 2 * g - 1 - j; }
+                        if (pc == 2) { if (j < I.Nx) continue; cc = 2 * (I.Nx + g) - 1 - j; }
+                        const long long c = cidx(I, r, cc);
+                        phi[c] = A[c];
+                    }
+                }
+            }
+            return;
+        }
+    }
     const int ri = g + w.y * RTY - RT_MAXIT, rj = g + w.z * RTX - RT_MAXIT;
     const R* P0 = (const R*)I.phiA;
     const R* in = ls_buf<R>(I, in_sel);
     R* out = ls_buf<R>(I, out_sel);
-    const double h = fmin(I.dx, I.dy);
     // per cell: reinit_setup's mode and constant; bit 8: interior with its
-    // stencil inside the region, bit 9: owned
+    // stencil inside the region, bit 9: owned, bits 10+: distance to the
+    // region edge
     R cs[SLOTS];
     int meta[SLOTS];
 #pragma unroll
@@ -1482,7 +1638,8 @@ __global__ __launch_bounds__(RT_THREADS) void k_ls_reinit_tb(const Inst* __restr
                     int mode;
                     reinit_cell_setup<R>(I, P0, i, j, mode, cs[k]);
                     const bool own = a >= RT_MAXIT && a < RT_MAXIT + RTY && b >= RT_MAXIT && b < RT_MAXIT + RTX;
-                    meta[k] = mode | 256 | (own ? 512 : 0);
+                    const int dist = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
+                    meta[k] = mode | 256 | (own ? 512 : 0) | (dist << 10);
                 }
             }
             s[idx] = v;
@@ -1490,17 +1647,17 @@ __global__ __launch_bounds__(RT_THREADS) void k_ls_reinit_tb(const Inst* __restr
     }
     __syncthreads();
     const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
+    const ReinitK<R> rk = reinit_k<R>(I);
     for (int t = 1; t <= nit; ++t) {
         R nv[SLOTS];
         int act = 0;
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
             const int idx = threadIdx.x + k * RT_THREADS;
-            const int a = idx / RX, b = idx - (idx / RX) * RX;
-            if ((meta[k] & 256) && a >= t && a < RY - t && b >= t && b < RX - t) {
+            if ((meta[k] & 256) && (meta[k] >> 10) >= t) {
                 act |= 1 << k;
                 nv[k] = reinit_update<R>(meta[k] & 7, cs[k], s[idx], s[idx - 1], s[idx + 1], s[idx - RX], s[idx + RX],
-                                         I.dx, I.dy, h);
+                                         rk.idx2, rk.idy2, rk.dtau);
             }
         }
         if (t == nit) {
@@ -1733,7 +1890,7 @@ void launch_mgfm_final(const LaunchCtx& L, int qin, int qout) {
 // K reinitialisation iterations from A into phi: blocked launches of up to
 // RT_MAXIT iterations each, or one launch per iteration (periodic contexts)
 template <typename R>
-void launch_reinit_chain(const LaunchCtx& L, int K, int every, int force) {
+void launch_reinit_chain(const LaunchCtx& L, int K, int every, int force, int copy_idle) {
     if (L.tb) {
         const int m = (K + RT_MAXIT - 1) / RT_MAXIT;
         int in = 0, done = 0;
@@ -1741,7 +1898,7 @@ void launch_reinit_chain(const LaunchCtx& L, int K, int every, int force) {
             const int nit = (K - done + (m - c) - 1) / (m - c);   // even split
             const int out = ((m - 1 - c) % 2 == 0) ? 1 : 2;
             hipLaunchKernelGGL((k_ls_reinit_tb<R>), GRID(L.n_rt), dim3(RT_THREADS), 0, L.stream, L.insts, L.w_rt, nit,
-                               in, out, every, force);
+                               in, out, every, force, c == 0 ? copy_idle : 0);
             in = out;
             done += nit;
         }
@@ -1754,17 +1911,33 @@ void launch_reinit_chain(const LaunchCtx& L, int K, int every, int force) {
 
 template <typename R>
 void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters) {
+    if (iters <= 0) every = 0;   // no reinitialisation iterations: phi = A every step
+    if (L.tb) {
+        static_assert(LTX == RTX && LTY == RTY, "the fused level-set kernel uses the reinitialisation tiles");
+#define RK3(O) hipLaunchKernelGGL((k_ls_rk3_tb<R, O>), GRID(L.n_rt), dim3(1024), 0, L.stream, L.insts, L.w_rt, H)
+        switch (H.n) {
+            case 2: RK3(1); break;
+            case 4: RK3(3); break;
+            case 6: RK3(5); break;
+            case 8: RK3(7); break;
+            default: RK3(9); break;
+        }
+#undef RK3
+        if (every > 0) launch_reinit_chain<R>(L, iters, every, 0, 1);
+        else hipLaunchKernelGGL((k_ls_copy<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, every);
+        return;
+    }
     hipLaunchKernelGGL((k_ls_stage<R, 1>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
     hipLaunchKernelGGL((k_ls_stage<R, 2>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
     hipLaunchKernelGGL((k_ls_stage<R, 3>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
     hipLaunchKernelGGL((k_ls_copy<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, every);
-    if (every > 0) launch_reinit_chain<R>(L, iters, every, 0);
+    if (every > 0) launch_reinit_chain<R>(L, iters, every, 0, 0);
 }
 
 template <typename R>
 void launch_reinit_now(const LaunchCtx& L, int iters) {
     hipLaunchKernelGGL((k_phi_to_A<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full);
-    launch_reinit_chain<R>(L, iters, 0, 1);
+    launch_reinit_chain<R>(L, iters, 0, 1, 0);
 }
 
 template <typename R>
