@@ -21,7 +21,9 @@ namespace gfm {
 //  exp t = 2^n exp(r), n = rint(t / ln2), |r| <= ln2/2, Taylor to r^13
 //          (truncation < 5e-18 relative).
 // Relative error about (2 + |y log x|) 1e-16.  x <= 0 or NaN gives NaN.
-GFM_HD inline double pow_fast(double x, double y) {
+// Not inlined: its two dozen FP64 coefficients would otherwise be hoisted
+// into registers of the calling kernel's loop.
+GFM_HD inline __attribute__((noinline)) double pow_fast(double x, double y) {
 #pragma clang fp contract(off)
     if (!(x > 0.0)) return x == 0.0 ? 0.0 : NAN;
     int e;
