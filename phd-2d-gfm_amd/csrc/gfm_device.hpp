@@ -181,7 +181,9 @@ __device__ __forceinline__ V4<T> flux_prims(T g, T pinf, const V4<T>& W) {
 // reciprocal per state in place of a division per component, rho e =
 // (p + g pinf) / (g - 1) from a per-fluid reciprocal, and the positivity
 // tests in division-free form: for rho > 0, e >= 0 iff p + g pinf >= 0 (W)
-// and iff 2 E rho >= |m|^2 (U); both are false for rho <= 0 or NaN.
+// and iff 2 E rho >= |m|^2 (U); both are false for rho <= 0 or NaN.  No FMA
+// contraction, so that every kernel converting the same state gets the same
+// bits.
 // ---------------------------------------------------------------------------
 struct FluidF { float g, pinf, gpinf, igm1; };
 
@@ -194,6 +196,7 @@ __device__ __forceinline__ FluidF fluid_f(const RSFluid& R) {
 }
 
 __device__ __forceinline__ V4<float> cons2prims_f(const FluidF& F, const V4<float>& U) {
+#pragma clang fp contract(off)
     V4<float> W;
     const float rho = U.v[0], ir = 1.0f / rho;
     const float u = U.v[1] * ir, v = U.v[2] * ir;
@@ -203,6 +206,7 @@ __device__ __forceinline__ V4<float> cons2prims_f(const FluidF& F, const V4<floa
 }
 
 __device__ __forceinline__ V4<float> prims2cons_f(const FluidF& F, const V4<float>& W) {
+#pragma clang fp contract(off)
     V4<float> U;
     const float rho = W.v[0], u = W.v[1], v = W.v[2];
     U.v[0] = rho; U.v[1] = rho * u; U.v[2] = rho * v;
@@ -219,6 +223,7 @@ __device__ __forceinline__ bool phys_state_f(const V4<float>& U) {
 }
 
 __device__ __forceinline__ V4<float> flux_prims_f(const FluidF& F, const V4<float>& W) {
+#pragma clang fp contract(off)
     V4<float> Fl;
     const float rho = W.v[0], u = W.v[1], v = W.v[2], p = W.v[3];
     const float E = (p + F.gpinf) * F.igm1 + 0.5f * rho * (u * u + v * v);

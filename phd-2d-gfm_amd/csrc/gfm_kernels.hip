@@ -465,12 +465,35 @@ __device__ __forceinline__ void unit_normal(const Inst& I, int i, int j, double&
     else { nx /= m; ny /= m; }
 }
 
+// unit normal of phi in the storage precision: FP64 as unit_normal; FP32
+// (mixed precision) from one reciprocal square root
+template <typename R>
+__device__ __forceinline__ void unit_normal_r(const Inst& I, int i, int j, R& nx, R& ny) {
+    if constexpr (sizeof(R) == 8) {
+        double a, b;
+        unit_normal<R>(I, i, j, a, b);
+        nx = a; ny = b;
+    } else {
+#pragma clang fp contract(off)
+        const float* phi = (const float*)I.phi;
+        const float gx = (phi[cidx(I, i, j + 1)] - phi[cidx(I, i, j - 1)]) * (float)(0.5 / I.dx);
+        const float gy = (phi[cidx(I, i + 1, j)] - phi[cidx(I, i - 1, j)]) * (float)(0.5 / I.dy);
+        const float m2 = gx * gx + gy * gy;
+        if (m2 < 1e-24f) { nx = 1.0f; ny = 0.0f; }
+        else { const float r = rsqrtf(m2); nx = gx * r; ny = gy * r; }
+    }
+}
+
 template <typename R, int MODE>
 __device__ __forceinline__ V4<R> ex_load(const Inst& I, int f, int i, int j, const void* src) {
     const long long c = cidx(I, i, j);
     V4<R> W;
-    if (MODE & MF_FIRST) W = cons2prims<R>((R)I.gam[f], (R)I.pinf[f], ld4<R>(src, I, c));
-    else W = ld4<R>(src, I, c);
+    if (MODE & MF_FIRST) {
+        if constexpr (sizeof(R) == 4) W = cons2prims_f(fluid_f(I.rsf[f]), ld4<R>(src, I, c));
+        else W = cons2prims<R>((R)I.gam[f], (R)I.pinf[f], ld4<R>(src, I, c));
+    } else {
+        W = ld4<R>(src, I, c);
+    }
     if (MODE & MF_INJECT) {
         const int k = I.imap[c];
         if (k >= 0) {
@@ -493,28 +516,28 @@ __device__ __forceinline__ void extrap_dirs(int fg, R nx, R ny, int& dj, int& di
 }
 
 // one Jacobi update of the ghost fluid's primitives from the cell (c0) and
-// its upwind x (xo) and y (yo) neighbours, pseudo-time step h/2
+// its upwind x (xo) and y (yo) neighbours, pseudo-time step h/2:
+//   fluid 2 ghost (:74-97):  c0 + ax nx dx+ + ay ny dy+, d+ = +-(xo - c0)
+//   fluid 1 ghost (:98-121): c0 - ax nx dx- - ay ny dy-, d- = -+(xo - c0)
+// with the neighbours taken upwind as extrap_dirs does; both are exactly
+// c0 + wx (xo - c0) + wy (yo - c0), wx = ax |nx|, wy = ay |ny| (IEEE
+// negation is exact), which is what is evaluated
 template <typename R>
-__device__ __forceinline__ V4<R> extrap_combine(int fg, R nx, R ny, R ax, R ay, const V4<R>& c0, const V4<R>& xo,
-                                                const V4<R>& yo) {
+__device__ __forceinline__ V4<R> extrap_combine(R wx, R wy, const V4<R>& c0, const V4<R>& xo, const V4<R>& yo) {
 #pragma clang fp contract(off)
     V4<R> Wg;
-    if (fg == 1) {   // fluid 2 ghost (:74-97)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const R dxv = nx > R(0) ? xo.v[k] - c0.v[k] : c0.v[k] - xo.v[k];
-            const R dyv = ny > R(0) ? yo.v[k] - c0.v[k] : c0.v[k] - yo.v[k];
-            Wg.v[k] = c0.v[k] + ax * nx * dxv + ay * ny * dyv;
-        }
-    } else {         // fluid 1 ghost (:98-121)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const R dxv = nx > R(0) ? c0.v[k] - xo.v[k] : xo.v[k] - c0.v[k];
-            const R dyv = ny > R(0) ? c0.v[k] - yo.v[k] : yo.v[k] - c0.v[k];
-            Wg.v[k] = c0.v[k] - ax * nx * dxv - ay * ny * dyv;
-        }
-    }
+    for (int k = 0; k < 4; ++k) Wg.v[k] = (c0.v[k] + wx * (xo.v[k] - c0.v[k])) + wy * (yo.v[k] - c0.v[k]);
     return Wg;
+}
+
+template <typename R>
+__device__ __forceinline__ void extrap_weights(const Inst& I, R nx, R ny, R& wx, R& wy) {
+#pragma clang fp contract(off)
+    const R pdt = R(0.5 * fmin(I.dx, I.dy));
+    const R ax = pdt / (R)I.dx, ay = pdt / (R)I.dy;
+    wx = ax * fabs(nx);
+    wy = ay * fabs(ny);
 }
 
 // GFM_original.cpp:65-116: the ghost fluid takes the real fluid's velocity
@@ -537,6 +560,13 @@ __device__ __forceinline__ bool ogfm_ghost(const Inst& I, int fg, const V4<R>& W
 template <typename R>
 struct ExOut { V4<R> a, b; R vu, vv; };
 
+// W -> U of fluid f in the storage precision's form
+template <typename R>
+__device__ __forceinline__ V4<R> to_cons(const Inst& I, int f, const V4<R>& W) {
+    if constexpr (sizeof(R) == 4) return prims2cons_f(fluid_f(I.rsf[f]), W);
+    else return prims2cons<R>((R)I.gam[f], (R)I.pinf[f], W);
+}
+
 template <typename R, int MODE>
 __device__ __forceinline__ ExOut<R> extrap_out(const Inst& I, int fr, const V4<R>& Wr, const V4<R>& Wg) {
     ExOut<R> o;
@@ -547,8 +577,8 @@ __device__ __forceinline__ ExOut<R> extrap_out(const Inst& I, int fr, const V4<R
         B.v[k] = fr == 0 ? Wg.v[k] : Wr.v[k];
     }
     if (MODE & MF_LAST) {
-        o.a = prims2cons<R>((R)I.gam[0], (R)I.pinf[0], A);
-        o.b = prims2cons<R>((R)I.gam[1], (R)I.pinf[1], B);
+        o.a = to_cons<R>(I, 0, A);
+        o.b = to_cons<R>(I, 1, B);
     } else {
         o.a = A;
         o.b = B;
@@ -586,18 +616,17 @@ __device__ __forceinline__ void extrap_cell(const Inst& I, int i, int j, int bin
     const bool qsrc = (MODE & (MF_FIRST | MF_RAW)) != 0;
     const long long cs = cidx(I, si, sj);
     const R lsval = ld1<R>(I.phi, cs);
-    double nxd, nyd;
-    unit_normal<R>(I, si, sj, nxd, nyd);
-    const R nx = (R)nxd, ny = (R)nyd;
+    R nx, ny;
+    unit_normal_r<R>(I, si, sj, nx, ny);
     const int fr = lsval <= R(0) ? 0 : 1, fg = 1 - fr;
     const void* in_r = qsrc ? I.Q[fr][bin] : I.P[fr][bin];
     const void* in_g = qsrc ? I.Q[fg][bin] : I.P[fg][bin];
-    const R pdt = R(0.5 * fmin(I.dx, I.dy));
-    const R ax = pdt / (R)I.dx, ay = pdt / (R)I.dy;
+    R wx, wy;
+    extrap_weights<R>(I, nx, ny, wx, wy);
     V4<R> Wr = ex_load<R, MODE>(I, fr, si, sj, in_r);
     int dj, di;
     extrap_dirs<R>(fg, nx, ny, dj, di);
-    V4<R> Wg = extrap_combine<R>(fg, nx, ny, ax, ay, ex_load<R, MODE>(I, fg, si, sj, in_g),
+    V4<R> Wg = extrap_combine<R>(wx, wy, ex_load<R, MODE>(I, fg, si, sj, in_g),
                                  ex_load<R, MODE>(I, fg, si, sj + dj, in_g),
                                  ex_load<R, MODE>(I, fg, si + di, sj, in_g));
     if (!(Wg.v[0] >= R(0))) flag(I, ST_UNPHYS);
@@ -669,13 +698,15 @@ __device__ __forceinline__ void tb_put(R* s, int f, int idx, const V4<R>& W) {
     for (int q = 0; q < 4; ++q) s[(4 * f + q) * NC + idx] = W.v[q];
 }
 
-// new ghost-fluid primitives of LDS cell idx (extrap_cell's arithmetic)
-template <typename R, int NC, int RX>
-__device__ __forceinline__ V4<R> tb_update(const R* s, int idx, int fg, R nx, R ny, R ax, R ay) {
-    int dj, di;
-    extrap_dirs<R>(fg, nx, ny, dj, di);
-    return extrap_combine<R>(fg, nx, ny, ax, ay, tb_get<R, NC>(s, fg, idx), tb_get<R, NC>(s, fg, idx + dj),
-                             tb_get<R, NC>(s, fg, idx + di * RX));
+// new ghost-fluid primitives of an LDS cell (extrap_cell's arithmetic): oc,
+// ox, oy index component 0 of the ghost fluid at the cell and its upwind x
+// and y neighbours
+template <typename R, int NC>
+__device__ __forceinline__ V4<R> tb_update(const R* s, int oc, int ox, int oy, R wx, R wy) {
+    V4<R> c0, xo, yo;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { c0.v[q] = s[oc + q * NC]; xo.v[q] = s[ox + q * NC]; yo.v[q] = s[oy + q * NC]; }
+    return extrap_combine<R>(wx, wy, c0, xo, yo);
 }
 
 template <typename R, int NIT, int MIN, int MOUT>
@@ -696,14 +727,18 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
     }
     const void* in0 = (MIN & MF_FIRST) ? I.Q[0][bin] : I.P[0][bin];
     const void* in1 = (MIN & MF_FIRST) ? I.Q[1][bin] : I.P[1][bin];
-    // per cell: unit normal and bits (1: interior with its stencil inside the
-    // region, 2: fluid 2 real, 4: owned, 8 and up: distance to the region edge)
-    R nxs[SLOTS], nys[SLOTS];
+    // per cell: the update weights, the LDS offsets of the ghost fluid at the
+    // cell and its upwind neighbours, and bits (1: interior with its stencil
+    // inside the region, 2: fluid 2 real, 4: owned, 8 and up: distance to
+    // the region edge)
+    R wxs[SLOTS], wys[SLOTS];
+    int oc[SLOTS], ox[SLOTS], oy[SLOTS];
     int meta[SLOTS];
 #pragma unroll
     for (int k = 0; k < SLOTS; ++k) {
         const int idx = threadIdx.x + k * TB_THREADS;
-        nxs[k] = R(0); nys[k] = R(0); meta[k] = 0;
+        wxs[k] = R(0); wys[k] = R(0); meta[k] = 0;
+        oc[k] = ox[k] = oy[k] = idx;
         if (idx < NC) {
             const int a = idx / RX, b = idx - (idx / RX) * RX;
             const int i = ri + a, j = rj + b;
@@ -712,12 +747,18 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
                 W1 = ex_load<R, LM>(I, 0, i, j, in0);
                 W2 = ex_load<R, LM>(I, 1, i, j, in1);
                 if (interior(I, i, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1) {
-                    double nxd, nyd;
-                    unit_normal<R>(I, i, j, nxd, nyd);
-                    nxs[k] = (R)nxd; nys[k] = (R)nyd;
+                    R nx, ny;
+                    unit_normal_r<R>(I, i, j, nx, ny);
+                    extrap_weights<R>(I, nx, ny, wxs[k], wys[k]);
                     const bool own = a >= NIT && a < NIT + TBY && b >= NIT && b < NIT + TBX;
                     const int dist = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
-                    meta[k] = 1 | (ld1<R>(I.phi, cidx(I, i, j)) <= R(0) ? 0 : 2) | (own ? 4 : 0) | (dist << 3);
+                    const int fr = ld1<R>(I.phi, cidx(I, i, j)) <= R(0) ? 0 : 1;
+                    meta[k] = 1 | (fr ? 2 : 0) | (own ? 4 : 0) | (dist << 3);
+                    int dj, di;
+                    extrap_dirs<R>(1 - fr, nx, ny, dj, di);
+                    oc[k] = 4 * (1 - fr) * NC + idx;
+                    ox[k] = oc[k] + dj;
+                    oy[k] = oc[k] + di * RX;
                 }
             }
             tb_put<R, NC>(s, 0, idx, W1);
@@ -725,8 +766,6 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
         }
     }
     __syncthreads();
-    const R pdt = R(0.5 * fmin(I.dx, I.dy));
-    const R ax = pdt / (R)I.dx, ay = pdt / (R)I.dy;
     const bool rfb = I.bc[2] == BC_R, rft = I.bc[3] == BC_R, rfx = I.bc[0] == BC_R;
     const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
     bool bad = false;
@@ -739,7 +778,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
             const int idx = threadIdx.x + k * TB_THREADS;
             if ((meta[k] & 1) && (meta[k] >> 3) >= t) {
                 act |= 1 << k;
-                nv[k] = tb_update<R, NC, RX>(s, idx, (meta[k] & 2) ? 0 : 1, nxs[k], nys[k], ax, ay);
+                nv[k] = tb_update<R, NC>(s, oc[k], ox[k], oy[k], wxs[k], wys[k]);
                 if ((meta[k] & 4) && !(nv[k].v[0] >= R(0))) bad = true;
             }
         }
@@ -750,7 +789,8 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
             const int idx = threadIdx.x + k * TB_THREADS;
             const int a = idx / RX, b = idx - (idx / RX) * RX;
             const int i = ri + a, j = rj + b;
-            tb_put<R, NC>(s, (meta[k] & 2) ? 0 : 1, idx, nv[k]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s[oc[k] + q * NC] = nv[k].v[q];
             if (i == ib || i == it || j == jl || j == jr) {
                 // first ghost layer: this cell's new state, reflected
                 const V4<R> W1 = tb_get<R, NC>(s, 0, idx), W2 = tb_get<R, NC>(s, 1, idx);
@@ -773,7 +813,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
         const int a = idx / RX, b = idx - (idx / RX) * RX;
         const int i = ri + a, j = rj + b;
         const int fr = (meta[k] & 2) ? 1 : 0, fg = 1 - fr;
-        const V4<R> Wg = tb_update<R, NC, RX>(s, idx, fg, nxs[k], nys[k], ax, ay);
+        const V4<R> Wg = tb_update<R, NC>(s, oc[k], ox[k], oy[k], wxs[k], wys[k]);
         if (!(Wg.v[0] >= R(0))) bad = true;
         const V4<R> Wr = tb_get<R, NC>(s, fr, idx);
         V4<R> Wo = Wg;
@@ -792,7 +832,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
             // ghost fluid (GFM_original's rule applies to interior cells).
             ExOut<R> oa = o;
             if (MOUT & MF_OGFM) {
-                const V4<R> Ug = (MOUT & MF_LAST) ? prims2cons<R>((R)I.gam[fg], (R)I.pinf[fg], Wg) : Wg;
+                const V4<R> Ug = (MOUT & MF_LAST) ? to_cons<R>(I, fg, Wg) : Wg;
                 if (fr == 0) oa.b = Ug; else oa.a = Ug;
             }
 #pragma unroll 1
