@@ -60,14 +60,15 @@ def load_all(ctx, states):
         ctx.set_time(k, 0.0, 0)
 
 
-def timed_run(ctx, states, steps, warmup):
+def timed_run(ctx, states, steps, warmup, eager=False):
+    step = ctx.step if eager else ctx.step_graph
     load_all(ctx, states)
-    ctx.step_graph(warmup)
+    step(warmup)
     ctx.sync()
     load_all(ctx, states)
     ctx.sync()
     t0 = time.perf_counter()
-    ctx.step_graph(steps)
+    step(steps)
     ctx.sync()
     return time.perf_counter() - t0
 
@@ -105,7 +106,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=None,
+                    help="steps of the per-phase HIP-event pass (default: the timed step count)")
+    ap.add_argument("--eager", action="store_true", help="launch kernels directly instead of one hipGraph per step")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one MI355X: the per-timestep path runs on one GPU here "
@@ -129,14 +132,16 @@ def main():
         name = "shocked bubble 4096x1152, GFM_riemann, mixed FP32/FP64"
 
     ctx, states, cells = build(cases, prec, steps + a.warmup)
-    el = timed_run(ctx, states, steps, a.warmup)
+    el = timed_run(ctx, states, steps, a.warmup, a.eager)
     status = [ctx.status(k) for k in range(len(cases))]
     value = cells * steps / el
 
-    # per-phase HIP-event timing on the same workload from the initial state
+    # per-phase HIP-event timing: the same workload replayed from the initial
+    # state with events around each phase (eager launches)
+    nprof = a.profile_steps or steps
     load_all(ctx, states)
-    ms = ctx.profile_steps(a.profile_steps)
-    fs_ms = ms[2] / a.profile_steps
+    ms = ctx.profile_steps(nprof)
+    fs_ms = ms[2] / nprof
     es = 8 if prec == G.FP64 else 4
     fs_bytes = 2 * cells * 8 * es
     achieved = fs_bytes / (fs_ms * 1e-3) / 1e9
@@ -160,13 +165,14 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_fs (MUSCL-Hancock, both fluids)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
-        "phase_ms_per_step": {n: v / a.profile_steps for n, v in zip(("dt", "ghost", "fs", "ls", "bc"), ms)},
+        "phase_ms_per_step": {n: v / nprof for n, v in zip(("dt", "ghost", "fs", "ls", "bc"), ms)},
+        "phase_steps": nprof,
         "status": sorted(set(status)),
     }
     if prec == G.MIXED and not a.no_fp64:
         nfp = steps
         ctx64, st64, cells64 = build(cases, G.FP64, nfp + a.warmup)
-        el64 = timed_run(ctx64, st64, nfp, a.warmup)
+        el64 = timed_run(ctx64, st64, nfp, a.warmup, a.eager)
         out["fp64_plain"] = {"value": cells64 * nfp / el64, "steps": nfp,
                              "ratio_mixed_over_fp64": value / (cells64 * nfp / el64)}
         ctx64.close()

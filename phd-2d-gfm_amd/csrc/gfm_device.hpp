@@ -247,6 +247,15 @@ __device__ __forceinline__ void rs_side(RSSide& s, double g, double pinf, double
     s.c = sqrt(g * s.pbar / rho);
 }
 
+// x^y: the C library's pow, or pow_fast (FP64, a few ulp) on the
+// mixed-precision path
+template <bool FAST>
+__device__ __forceinline__ double rpow(double x, double y) {
+    if constexpr (FAST) return pow_fast(x, y);
+    else return pow(x, y);
+}
+
+template <bool FAST = false>
 __device__ __forceinline__ double rs_f(const RSSide& s, double p, double& fd) {
     double pbar = p + s.pinf;
     if (p > s.p) {
@@ -257,7 +266,7 @@ __device__ __forceinline__ double rs_f(const RSSide& s, double p, double& fd) {
         return (p - s.p) * q;
     }
     double r = pbar / s.pbar;
-    double pw = pow(r, (s.g - 1.0) / (2.0 * s.g));
+    double pw = rpow<FAST>(r, (s.g - 1.0) / (2.0 * s.g));
     fd = pw / (s.rho * s.c * r);
     return 2.0 * s.c / (s.g - 1.0) * (pw - 1.0);
 }
@@ -266,6 +275,7 @@ __device__ __forceinline__ double rs_f(const RSSide& s, double p, double& fd) {
 // than a factor 2; otherwise the two-rarefaction estimate (exact for two
 // rarefactions of one gas) when PVRS is below both, the two-shock estimate
 // when not.  Barred pressures p + pinf throughout.
+template <bool FAST = false>
 __device__ __forceinline__ double rs_guess(const RSSide& L, const RSSide& R, double pfloor) {
     const double du = R.u - L.u;
     const double ppv = 0.5 * (L.p + R.p) - 0.125 * du * (L.rho + R.rho) * (L.c + R.c);
@@ -277,8 +287,8 @@ __device__ __forceinline__ double rs_guess(const RSSide& L, const RSSide& R, dou
     } else if (ppv < pmin && L.g == R.g && L.pinf == R.pinf) {
         const double z = (L.g - 1.0) / (2.0 * L.g);
         const double num = L.c + R.c - 0.5 * (L.g - 1.0) * du;
-        const double den = L.c / pow(L.pbar, z) + R.c / pow(R.pbar, z);
-        p = (num > 0.0) ? pow(num / den, 1.0 / z) - L.pinf : pfloor;
+        const double den = L.c / rpow<FAST>(L.pbar, z) + R.c / rpow<FAST>(R.pbar, z);
+        p = (num > 0.0) ? rpow<FAST>(num / den, 1.0 / z) - L.pinf : pfloor;
     } else {
         const double ps = fmax(pfloor, ppv);
         const double gL = sqrt(2.0 / ((L.g + 1.0) * L.rho) / (ps + L.pinf + (L.g - 1.0) / (L.g + 1.0) * L.pbar));
@@ -295,6 +305,7 @@ struct Star { double p, u, rl, rr; };
 // the adaptive estimate.  tol: relative pressure change that ends the Newton
 // iteration: 1e-6 (Toro's TOLPRE; ~1e-12 after the quadratic last step) for
 // the face fluxes, 1e-12 for the interfacial star states.
+template <bool FAST = false>
 __device__ __forceinline__ unsigned rs_star(const RSSide& L, const RSSide& R, Star& S, double p0 = -1e300,
                                             double tol = 1e-6) {
     if (L.rho == R.rho && L.u == R.u && L.p == R.p && L.g == R.g && L.pinf == R.pinf) {
@@ -306,12 +317,12 @@ __device__ __forceinline__ unsigned rs_star(const RSSide& L, const RSSide& R, St
     double plow = -fmin(L.pinf, R.pinf);
     if (2.0 * L.c / (L.g - 1.0) + 2.0 * R.c / (R.g - 1.0) <= du) st |= ST_RS;
     const double pfloor = plow + 1e-6 * (0.5 * (L.p + R.p) - plow);
-    double p = p0 > pfloor ? p0 : rs_guess(L, R, pfloor);
+    double p = p0 > pfloor ? p0 : rs_guess<FAST>(L, R, pfloor);
     bool conv = false;
     for (int it = 0; it < 60; ++it) {
         double dL, dR;
-        double fL = rs_f(L, p, dL);
-        double fR = rs_f(R, p, dR);
+        double fL = rs_f<FAST>(L, p, dL);
+        double fR = rs_f<FAST>(R, p, dR);
         double pn = p - (fL + fR + du) / (dL + dR);
         if (pn <= plow) pn = 0.5 * (p + plow);
         double chg = fabs(pn - p);
@@ -321,19 +332,19 @@ __device__ __forceinline__ unsigned rs_star(const RSSide& L, const RSSide& R, St
     }
     if (!conv || p != p) st |= ST_RS;
     double dL, dR;
-    double fL = rs_f(L, p, dL);
-    double fR = rs_f(R, p, dR);
+    double fL = rs_f<FAST>(L, p, dL);
+    double fR = rs_f<FAST>(R, p, dR);
     S.p = p;
     S.u = 0.5 * (L.u + R.u) + 0.5 * (fR - fL);
     {
         double ratio = (p + L.pinf) / L.pbar;
         if (p > L.p) { double g6 = (L.g - 1.0) / (L.g + 1.0); S.rl = L.rho * (ratio + g6) / (g6 * ratio + 1.0); }
-        else S.rl = L.rho * pow(ratio, 1.0 / L.g);
+        else S.rl = L.rho * rpow<FAST>(ratio, 1.0 / L.g);
     }
     {
         double ratio = (p + R.pinf) / R.pbar;
         if (p > R.p) { double g6 = (R.g - 1.0) / (R.g + 1.0); S.rr = R.rho * (ratio + g6) / (g6 * ratio + 1.0); }
-        else S.rr = R.rho * pow(ratio, 1.0 / R.g);
+        else S.rr = R.rho * rpow<FAST>(ratio, 1.0 / R.g);
     }
     return st;
 }

@@ -961,6 +961,8 @@ __device__ __forceinline__ int dep_of(const Inst& I, int ii, int jj, int f, int 
 
 // shared normal-frame mixed RP (e.g. GFM_riemann.cpp:76-121); p0 is the
 // Newton starting pressure (see rs_star)
+// FAST: pow_fast in place of pow (mixed-precision contexts)
+template <bool FAST>
 __device__ __forceinline__ unsigned iface_mrp(const Inst& I, const double* ip1, const double* ip2, double nx, double ny,
                                               Star& S, double* t1, double* t2, double p0) {
     const double u1n = ip1[1] * nx + ip1[2] * ny;
@@ -981,7 +983,7 @@ __device__ __forceinline__ unsigned iface_mrp(const Inst& I, const double* ip1, 
     RSSide L, R;
     rs_side(L, I.gam[0], I.pinf[0], Lw.v[0], Lw.v[1], Lw.v[3]);
     rs_side(R, I.gam[1], I.pinf[1], Rw.v[0], Rw.v[1], Rw.v[3]);
-    st |= rs_star(L, R, S, p0, 1e-12);
+    st |= rs_star<FAST>(L, R, S, p0, 1e-12);
     return st;
 }
 
@@ -1016,19 +1018,19 @@ __device__ __forceinline__ void bilerp(const double (*cv)[4], double tx, double 
 // star state of one interfacial cell from its probe states; writes the
 // side-buffer entry (GFM_riemann.cpp:97-145, GFM_real.cpp:152-205,
 // GFM_modified.cpp:113-196)
-template <int VAR>
+template <int VAR, bool FAST>
 __device__ __forceinline__ unsigned iface_finish(const Inst& I, const IfRec& r, const double* ip1, const double* ip2,
                                                  double p0, double* out, double* pstar) {
     Star S;
     double t1[2], t2[2];
-    unsigned st = iface_mrp(I, ip1, ip2, r.nx, r.ny, S, t1, t2, p0);
+    unsigned st = iface_mrp<FAST>(I, ip1, ip2, r.nx, r.ny, S, t1, t2, p0);
     if (VAR == V_MODIFIED) {
         // no-slip: the real fluid keeps its state except the density, the
         // ghost takes the star state with the real fluid's tangential velocity
         const double* tr = r.lsval <= 0.0 ? t1 : t2;
         const int fr = r.lsval <= 0.0 ? 0 : 1;
         const double rs_real = fr == 0 ? S.rl : S.rr;
-        out[0] = rs_real * pow((r.preal + I.pinf[fr]) / (S.p + I.pinf[fr]), 1.0 / I.gam[fr]);
+        out[0] = rs_real * rpow<FAST>((r.preal + I.pinf[fr]) / (S.p + I.pinf[fr]), 1.0 / I.gam[fr]);
         out[1] = fr == 0 ? S.rr : S.rl;
         out[2] = S.u * r.nx + tr[0];
         out[3] = S.u * r.ny + tr[1];
@@ -1131,7 +1133,7 @@ __global__ __launch_bounds__(256) void k_iface_prep(const Inst* __restrict__ ins
     }
     double ip1[4], ip2[4], out[8], ps;
     probe_values<VAR>(I, r, false, ip1, ip2);
-    st |= iface_finish<VAR>(I, r, ip1, ip2, -1e300, out, &ps);
+    st |= iface_finish<VAR, (sizeof(R) == 4)>(I, r, ip1, ip2, -1e300, out, &ps);
     r.pspec = ps;
     IfRec* recs = (IfRec*)I.irec;
     recs[k] = r;
@@ -1145,12 +1147,12 @@ __global__ __launch_bounds__(256) void k_iface_prep(const Inst* __restrict__ ins
 
 // star state of pending entry k once every earlier interfacial cell it reads
 // has its own (the side buffer holds those)
-template <int VAR>
+template <int VAR, bool FAST>
 __device__ __forceinline__ unsigned iface_solve_pending(const Inst& I, const IfRec* recs, int k) {
     const IfRec& r = recs[k];
     double ip1[4], ip2[4], out[8], ps;
     probe_values<VAR>(I, r, true, ip1, ip2);
-    const unsigned st = iface_finish<VAR>(I, r, ip1, ip2, r.pspec, out, &ps);
+    const unsigned st = iface_finish<VAR, FAST>(I, r, ip1, ip2, r.pspec, out, &ps);
     double* o = I.side + 8 * (long long)k;
     for (int q = 0; q < 4; ++q) o[q] = out[q];
     return st;
@@ -1163,11 +1165,12 @@ __device__ __forceinline__ unsigned iface_solve_pending(const Inst& I, const IfR
 // barrier per level.  Instances with more than IPEND_LDS pending entries or
 // deeper than 254 levels (and every instance when `plain` is set) use plain
 // rounds: solve whatever is ready, repeat.  Both give the same bits.
-constexpr int IPEND_LDS = 8192;
+constexpr int IPEND_LDS = 12288;
 
-template <int VAR>
+template <typename R, int VAR>
 __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ insts, const int4* __restrict__ work,
                                                       int plain) {
+    constexpr bool FAST = sizeof(R) == 4;
     __shared__ unsigned char lev[ICAP_LDS];   // 0 final, 1..254 level, 255 unknown
     __shared__ int plist[IPEND_LDS];          // pending entries
     __shared__ int slist[IPEND_LDS];          // pending entries sorted by level
@@ -1247,7 +1250,7 @@ __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ i
         // cnt[l] now holds the end of level l's run, which starts at cnt[l - 1]
         for (int l = 1; l <= maxl; ++l) {
             const int b0 = cnt[l - 1], b1 = cnt[l];
-            for (int p = b0 + tid; p < b1; p += nt) st |= iface_solve_pending<VAR>(I, recs, slist[p]);
+            for (int p = b0 + tid; p < b1; p += nt) st |= iface_solve_pending<VAR, FAST>(I, recs, slist[p]);
             __syncthreads();
         }
         rounds += maxl;
@@ -1267,7 +1270,7 @@ __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ i
                         if (d >= 0 && lev[d] != 0) ready = false;
                     }
                 if (!ready) { atomicAdd(&nunk[rounds & 1], 1); continue; }
-                st |= iface_solve_pending<VAR>(I, recs, k);
+                st |= iface_solve_pending<VAR, FAST>(I, recs, k);
                 lev[k] = 1;
             }
             __syncthreads();
@@ -1907,12 +1910,12 @@ void launch_iface(const LaunchCtx& L, int variant, int pbuf) {
     switch (variant) {
         case V_REAL:
             hipLaunchKernelGGL((k_iface_prep<R, V_REAL>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich, pbuf);
-            hipLaunchKernelGGL((k_iface_rounds<V_REAL>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst,
+            hipLaunchKernelGGL((k_iface_rounds<R, V_REAL>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst,
                                L.iface_plain);
             break;
         case V_RIEMANN:
             hipLaunchKernelGGL((k_iface_prep<R, V_RIEMANN>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich, pbuf);
-            hipLaunchKernelGGL((k_iface_rounds<V_RIEMANN>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst,
+            hipLaunchKernelGGL((k_iface_rounds<R, V_RIEMANN>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts, L.w_inst,
                                L.iface_plain);
             break;
         case V_MODIFIED:   // no dependencies: the parallel pass is final
