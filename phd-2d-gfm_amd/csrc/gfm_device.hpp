@@ -198,7 +198,7 @@ __device__ __forceinline__ FluidF fluid_f(const RSFluid& R) {
 __device__ __forceinline__ V4<float> cons2prims_f(const FluidF& F, const V4<float>& U) {
 #pragma clang fp contract(off)
     V4<float> W;
-    const float rho = U.v[0], ir = 1.0f / rho;
+    const float rho = U.v[0], ir = __builtin_amdgcn_rcpf(rho);   // 1 ulp
     const float u = U.v[1] * ir, v = U.v[2] * ir;
     W.v[0] = rho; W.v[1] = u; W.v[2] = v;
     W.v[3] = (F.g - 1.0f) * (U.v[3] - 0.5f * (U.v[1] * u + U.v[2] * v)) - F.gpinf;
@@ -412,19 +412,40 @@ __device__ __forceinline__ void rs_sample(const RSSide& L, const RSSide& R, cons
 // c / g are formed where used
 struct RSSideM { double rho, u, p, pbar, ipb, irho, c; };
 
+// FP64 reciprocal and square root for the mixed-precision Newton iteration:
+// the hardware estimates refined by two Newton-Raphson steps (a few ulp from
+// the correctly rounded results, against one ulp-exact but several times
+// longer IEEE sequences); zero, infinite and NaN arguments take the IEEE
+// operation
+__device__ __forceinline__ double rcp_nr(double x) {
+    if (!(fabs(x) < INFINITY) || x == 0.0) return 1.0 / x;
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
+__device__ __forceinline__ double sqrt_nr(double x) {
+    if (!(x > 0.0) || !(x < INFINITY)) return sqrt(x);
+    double y = __builtin_amdgcn_rsq(x);                // ~ 1/sqrt(x)
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    const double s = x * y;                            // ~ sqrt(x)
+    return fma(fma(-s, s, x), 0.5 * y, s);             // one Newton step on s
+}
+
 __device__ __forceinline__ void rs_side_m(RSSideM& s, const RSFluid& F, double rho, double u, double p) {
     s.rho = rho; s.u = u; s.p = p;
     s.pbar = p + F.pinf;
-    s.ipb = 1.0 / s.pbar;
-    s.irho = 1.0 / rho;
-    s.c = sqrt(F.g * s.pbar * s.irho);
+    s.ipb = rcp_nr(s.pbar);
+    s.irho = rcp_nr(rho);
+    s.c = sqrt_nr(F.g * s.pbar * s.irho);
 }
 
 __device__ __forceinline__ double rs_f_m(const RSFluid& F, const RSSideM& s, double p, double& fd) {
     const double pbar = p + F.pinf;
     if (p > s.p) {   // shock (Toro 4.6)
-        const double t = 1.0 / (pbar + F.g6 * s.pbar);
-        const double q = sqrt(F.a2 * s.irho * t);
+        const double t = rcp_nr(pbar + F.g6 * s.pbar);
+        const double q = sqrt_nr(F.a2 * s.irho * t);
         fd = q * (1.0 - 0.5 * (p - s.p) * t);
         return (p - s.p) * q;
     }
@@ -439,7 +460,7 @@ __device__ __forceinline__ double rs_f_m(const RSFluid& F, const RSSideM& s, dou
     } else {
         w = pow_fast(pbar * s.ipb, F.z) - 1.0;
     }
-    fd = s.c * F.ig * (1.0 + w) / pbar;
+    fd = s.c * F.ig * (1.0 + w) * rcp_nr(pbar);
     return F.k2 * s.c * w;
 }
 
@@ -476,23 +497,23 @@ __device__ __forceinline__ unsigned rs_star_m(const RSFluid& F, const RSSideM& L
         p = p > pfloor ? p : pfloor;
     }
     bool conv = false;
+    double fL = 0.0, fR = 0.0, dL = 0.0, dR = 0.0, dp = 0.0;
     for (int it = 0; it < 60; ++it) {
-        double dL, dR;
-        const double fL = rs_f_m(F, L, p, dL);
-        const double fR = rs_f_m(F, R, p, dR);
-        double pn = p - (fL + fR + du) / (dL + dR);
+        fL = rs_f_m(F, L, p, dL);
+        fR = rs_f_m(F, R, p, dR);
+        double pn = p - (fL + fR + du) * rcp_nr(dL + dR);
         if (pn <= plow) pn = 0.5 * (p + plow);
         const double chg = fabs(pn - p);
         const double ref = 0.5 * (pn + p) - plow;
+        dp = pn - p;
         p = pn;
         if (chg <= 1e-6 * ref) { conv = true; break; }
     }
     if (!conv || p != p) st |= ST_RS;
-    double dL, dR;
-    const double fL = rs_f_m(F, L, p, dL);
-    const double fR = rs_f_m(F, R, p, dR);
+    // u* from the pressure functions at the converged p, to first order in
+    // the last (relative size < 1e-6) Newton step: error ~1e-12 relative
     S.p = p;
-    S.u = 0.5 * (L.u + R.u) + 0.5 * (fR - fL);
+    S.u = 0.5 * (L.u + R.u) + 0.5 * ((fR + dR * dp) - (fL + dL * dp));
     return st;
 }
 

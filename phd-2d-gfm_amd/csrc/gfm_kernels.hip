@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
             if constexpr (sizeof(T) == 4) {
                 if (!phys_prims_f(Ff, BL) || !phys_prims_f(Ff, BR)) { BL = Wc; BR = Wc; }
                 if (!phys_prims_f(Ff, BB) || !phys_prims_f(Ff, BT)) { BB = Wc; BT = Wc; }
-                ir = T(1) / rho;
+                ir = __builtin_amdgcn_rcpf(rho);   // 1 ulp
                 rc2 = g * (Wc.v[3] + pinf);   // rho c^2
             } else {
                 if (!phys_prims(g, pinf, BL) || !phys_prims(g, pinf, BR)) { BL = Wc; BR = Wc; }
