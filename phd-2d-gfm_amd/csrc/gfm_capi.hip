@@ -363,7 +363,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
     if (c->committed) return GFM_OK;
     if (c->inst.empty()) return fail_arg("no instances");
     const size_t es = c->es();
-    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich, w_tb, w_rt;
+    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich, w_tb, w_rt, w_seg;
     std::vector<Inst> descs;
     bool periodic = false;
     for (size_t k = 0; k < c->inst.size(); ++k) {
@@ -394,7 +394,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
         size_t oQ[2][2] = {{0, 0}, {0, 0}}, oP[2][2] = {{0, 0}, {0, 0}};
         for (int f = 0; f < nf; ++f)
             for (int b = 0; b < 2; ++b) oQ[f][b] = take(4 * P * es);
-        size_t ophi = 0, oA = 0, oB = 0, ovel = 0, oimap = 0, oilist = 0, oside = 0, orec = 0, odone = 0;
+        size_t ophi = 0, oA = 0, oB = 0, ovel = 0, oimap = 0, oilist = 0, oside = 0, orec = 0, odone = 0, oseg = 0;
+        d.nseg = c->twofluid() ? s.Ny * ((s.Nx + 255) / 256) : 0;
         if (c->twofluid()) {
             for (int f = 0; f < 2; ++f)
                 for (int b = 0; b < 2; ++b) oP[f][b] = take(4 * P * es);
@@ -405,6 +406,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
             oside = take((size_t)d.icap * 8 * sizeof(double));
             orec = take((size_t)d.icap * gfm::IFREC_BYTES);
             odone = take((size_t)d.icap * sizeof(int));
+            oseg = take((size_t)d.nseg * sizeof(int));
         }
         const size_t opart = take((size_t)d.ntile_dt * 4 * sizeof(double));
         d.hist_cap = c->twofluid() ? (int)std::min<long long>((long long)h.max_steps + 1, 1 << 20) : 0;
@@ -424,7 +426,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         if (c->twofluid()) {
             d.phi = base + ophi; d.phiA = base + oA; d.phiB = base + oB; d.vel = base + ovel;
             d.imap = (int*)(base + oimap); d.ilist = (int*)(base + oilist); d.side = (double*)(base + oside);
-            d.irec = base + orec; d.idone = (int*)(base + odone);
+            d.irec = base + orec; d.idone = (int*)(base + odone); d.iseg = (int*)(base + oseg);
             HC(hipMemsetAsync(d.imap, 0xff, P * sizeof(int), c->stream));   // -1 everywhere
         }
         d.partials = (double*)(base + opart);
@@ -452,6 +454,11 @@ int gfm_ctx_commit(gfm_ctx* c) {
         if (c->twofluid())
             for (int ty = 0; ty < (s.Ny + 15) / 16; ++ty)
                 for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_tb.push_back(make_int4(ki, ty, tx, 0));
+        if (c->twofluid()) {
+            const int nsx = (s.Nx + 255) / 256;
+            for (int r = 0; r < s.Ny; ++r)
+                for (int sx = 0; sx < nsx; ++sx) w_seg.push_back(make_int4(ki, r, sx, r * nsx + sx));
+        }
         if (c->twofluid())
             for (int ty = 0; ty < (s.Ny + 31) / 32; ++ty)
                 for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_rt.push_back(make_int4(ki, ty, tx, 0));
@@ -474,7 +481,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
     if (w_ich.empty()) w_ich.push_back(make_int4(0, 1 << 30, 0, 0));
     const size_t nw =
         w_full.size() + w_int.size() + w_fs.size() + w_frame.size() + w_inst.size() + w_ich.size() + w_tb.size() +
-        w_rt.size();
+        w_rt.size() + w_seg.size();
     HC(hipMalloc((void**)&c->d_work, nw * sizeof(int4)));
     std::vector<int4> all;
     all.reserve(nw);
@@ -486,6 +493,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
     all.insert(all.end(), w_ich.begin(), w_ich.end());
     all.insert(all.end(), w_tb.begin(), w_tb.end());
     all.insert(all.end(), w_rt.begin(), w_rt.end());
+    all.insert(all.end(), w_seg.begin(), w_seg.end());
     HC(hipMemcpyAsync(c->d_work, all.data(), nw * sizeof(int4), hipMemcpyHostToDevice, c->stream));
     gfm::LaunchCtx& L = c->L;
     L.stream = c->stream;
@@ -498,7 +506,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
     L.w_inst = c->d_work + o; L.n_inst = (int)w_inst.size(); o += w_inst.size();
     L.w_ich = c->d_work + o; L.n_ich = (int)w_ich.size(); o += w_ich.size();
     L.w_tb = c->d_work + o; L.n_tb = (int)w_tb.size(); o += w_tb.size();
-    L.w_rt = c->d_work + o; L.n_rt = (int)w_rt.size();
+    L.w_rt = c->d_work + o; L.n_rt = (int)w_rt.size(); o += w_rt.size();
+    L.w_seg = c->d_work + o; L.n_seg = (int)w_seg.size();
     L.tb = c->tb ? 1 : 0;
     L.iface_plain = iface_plain;
     // initial scalars: t = 0, step 0, active

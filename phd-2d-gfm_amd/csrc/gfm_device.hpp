@@ -88,6 +88,8 @@ struct Inst {
     int hist_cap;
     void* irec;                // interfacial records, icap entries
     int* idone;                // 1 when an entry's star state is final after the parallel pass
+    int* iseg;                 // interfacial cells per 256-cell row segment, then their offsets
+    int nseg;                  // row segments (Ny x ceil(Nx / 256))
 };
 
 // HOUC first-derivative stencil for positive velocity: offsets o0..o0+n-1
@@ -514,6 +516,70 @@ __device__ __forceinline__ unsigned rs_star_m(const RSFluid& F, const RSSideM& L
     // the last (relative size < 1e-6) Newton step: error ~1e-12 relative
     S.p = p;
     S.u = 0.5 * (L.u + R.u) + 0.5 * ((fR + dR * dp) - (fL + dL * dp));
+    return st;
+}
+
+// Two-fluid form for the interfacial problems of the mixed-precision path:
+// per-side constants, Newton start p0 when it lies above the floor (else
+// rs_guess's choice), tolerance tol, and the star densities (Toro 4.5.1,
+// 4.5.2: shock rho (r + g6) / (g6 r + 1), rarefaction rho r^(1/g), r =
+// (p* + pinf) / (p + pinf)).  Only the evaluation differs from rs_star.
+__device__ __forceinline__ unsigned rs_star_m2(const RSFluid& FL, const RSFluid& FR, const RSSideM& L,
+                                               const RSSideM& R, Star& S, double p0, double tol) {
+    const bool same = FL.g == FR.g && FL.pinf == FR.pinf;
+    if (same && L.rho == R.rho && L.u == R.u && L.p == R.p) {
+        S.p = L.p; S.u = L.u; S.rl = L.rho; S.rr = R.rho;
+        return 0;
+    }
+    unsigned st = 0;
+    const double du = R.u - L.u;
+    const double plow = -fmin(FL.pinf, FR.pinf);
+    if (FL.k2 * L.c + FR.k2 * R.c <= du) st |= ST_RS;
+    const double pfloor = plow + 1e-6 * (0.5 * (L.p + R.p) - plow);
+    double p;
+    if (p0 > pfloor) {
+        p = p0;
+    } else {
+        const double ppv = 0.5 * (L.p + R.p) - 0.125 * du * (L.rho + R.rho) * (L.c + R.c);
+        const double bmin = fmin(L.pbar, R.pbar), bmax = fmax(L.pbar, R.pbar);
+        if (bmax < 2.0 * bmin) {
+            p = ppv;
+        } else if (ppv < fmin(L.p, R.p) && same) {
+            const double num = L.c + R.c - 0.5 * (FL.g - 1.0) * du;
+            const double den = L.c / pow_fast(L.pbar, FL.z) + R.c / pow_fast(R.pbar, FL.z);
+            p = (num > 0.0) ? pow_fast(num / den, 1.0 / FL.z) - FL.pinf : pfloor;
+        } else {
+            const double ps = fmax(pfloor, ppv);
+            const double gL = sqrt(FL.a2 * L.irho / (ps + FL.pinf + FL.g6 * L.pbar));
+            const double gR = sqrt(FR.a2 * R.irho / (ps + FR.pinf + FR.g6 * R.pbar));
+            p = (gL * L.p + gR * R.p - du) / (gL + gR);
+        }
+        p = p > pfloor ? p : pfloor;
+    }
+    bool conv = false;
+    double fL = 0.0, fR = 0.0, dL = 0.0, dR = 0.0, dp = 0.0;
+    for (int it = 0; it < 60; ++it) {
+        fL = rs_f_m(FL, L, p, dL);
+        fR = rs_f_m(FR, R, p, dR);
+        double pn = p - (fL + fR + du) * rcp_nr(dL + dR);
+        if (pn <= plow) pn = 0.5 * (p + plow);
+        const double chg = fabs(pn - p);
+        const double ref = 0.5 * (pn + p) - plow;
+        dp = pn - p;
+        p = pn;
+        if (chg <= tol * ref) { conv = true; break; }
+    }
+    if (!conv || p != p) st |= ST_RS;
+    S.p = p;
+    S.u = 0.5 * (L.u + R.u) + 0.5 * ((fR + dR * dp) - (fL + dL * dp));
+    {
+        const double r = (p + FL.pinf) * L.ipb;
+        S.rl = p > L.p ? L.rho * (r + FL.g6) * rcp_nr(FL.g6 * r + 1.0) : L.rho * pow_fast(r, FL.ig);
+    }
+    {
+        const double r = (p + FR.pinf) * R.ipb;
+        S.rr = p > R.p ? R.rho * (r + FR.g6) * rcp_nr(FR.g6 * r + 1.0) : R.rho * pow_fast(r, FR.ig);
+    }
     return st;
 }
 

@@ -872,29 +872,89 @@ __device__ __forceinline__ bool is_iface(const Inst& I, int i, int j) {
            ((ld1<R>(I.phi, cidx(I, i + 1, j)) <= R(0)) != s) || ((ld1<R>(I.phi, cidx(I, i - 1, j)) <= R(0)) != s);
 }
 
+// The list is built in row-major cell order (the order of the reference's
+// i, j loops), so that every interfacial cell an entry depends on has a
+// smaller entry index: per 256-cell row segment a count (k_iface_count),
+// an exclusive scan of the counts per instance (k_iface_scan), then each
+// segment writes its cells in column order from its offset (k_iface_mark).
+// work (count, mark): {inst, interior row, segment column, segment index}
+constexpr int SEGW = 256;
+
+// interfacial cells of one segment, as a per-wave mask and the rank of this
+// thread's cell among them
 template <typename R>
-__global__ __launch_bounds__(256) void k_iface_mark(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+__device__ __forceinline__ bool seg_iface(const Inst& I, const int4& w, int& i, int& j) {
+    i = I.g + w.y;
+    j = I.g + w.z * SEGW + (int)threadIdx.x;
+    return j < I.g + I.Nx && is_iface<R>(I, i, j);
+}
+
+template <typename R>
+__global__ __launch_bounds__(SEGW) void k_iface_count(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+    __shared__ int wc[SEGW / 64];
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
     if (!I.si[SI_ACTIVE]) return;
-    const int i = I.g + w.y * 4 + (threadIdx.x >> 6);
-    const int j = I.g + w.z * 64 + (threadIdx.x & 63);
-    const bool in = i < I.g + I.Ny && j < I.g + I.Nx;
-    const bool iface = in && is_iface<R>(I, i, j);
-    // one atomic per wavefront: ballot, popcount, lane prefix
-    const unsigned long long mask = __ballot(iface);
-    const int lane = threadIdx.x & 63;
-    const int cnt = __popcll(mask);
-    const int before = __popcll(mask & ((1ull << lane) - 1ull));
-    int base = 0;
-    if (cnt > 0 && lane == __ffsll((unsigned long long)mask) - 1) base = atomicAdd(&I.si[SI_NIFACE], cnt);
-    base = __shfl(base, __ffsll((unsigned long long)mask) - 1);
-    if (!in) return;
-    int k = -1;
-    if (iface) {
-        k = base + before;
+    int i, j;
+    const bool f = seg_iface<R>(I, w, i, j);
+    const unsigned long long mask = __ballot(f);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int c = 0;
+        for (int q = 0; q < SEGW / 64; ++q) c += wc[q];
+        I.iseg[w.w] = c;
+    }
+}
+
+// one workgroup per instance: exclusive scan of the segment counts in place;
+// the total is the number of interfacial cells
+__global__ __launch_bounds__(1024) void k_iface_scan(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+    __shared__ int part[1024];
+    const Inst& I = insts[work[blockIdx.x].x];
+    if (!I.si[SI_ACTIVE]) return;
+    const int n = I.nseg, tid = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b0 = min(n, tid * per), b1 = min(n, b0 + per);
+    int sum = 0;
+    for (int q = b0; q < b1; ++q) sum += I.iseg[q];
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {   // inclusive Hillis-Steele scan
+        const int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int acc = part[tid] - sum;
+    for (int q = b0; q < b1; ++q) {
+        const int c = I.iseg[q];
+        I.iseg[q] = acc;
+        acc += c;
+    }
+    if (tid == 1023) I.si[SI_NIFACE] = part[1023];
+}
+
+template <typename R>
+__global__ __launch_bounds__(SEGW) void k_iface_mark(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+    __shared__ int wc[SEGW / 64];
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!I.si[SI_ACTIVE]) return;
+    int i, j;
+    const bool f = seg_iface<R>(I, w, i, j);
+    const unsigned long long mask = __ballot(f);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) wc[wv] = __popcll(mask);
+    __syncthreads();
+    int k = I.iseg[w.w] + __popcll(mask & ((1ull << lane) - 1ull));
+    for (int q = 0; q < wv; ++q) k += wc[q];
+    if (j >= I.g + I.Nx) return;
+    if (f) {
         if (k < I.icap) I.ilist[k] = i * I.W + j;
         else { k = -1; flag(I, ST_IOVF); }
+    } else {
+        k = -1;
     }
     I.imap[cidx(I, i, j)] = k;
 }
@@ -974,6 +1034,21 @@ __device__ __forceinline__ unsigned iface_mrp(const Inst& I, const double* ip1, 
     V4<double> Lp, Rp;
     Lp.v[0] = ip1[0]; Lp.v[1] = u1n; Lp.v[2] = 0.0; Lp.v[3] = fmax(1e-6, ip1[3]);
     Rp.v[0] = ip2[0]; Rp.v[1] = u2n; Rp.v[2] = 0.0; Rp.v[3] = fmax(1e-6, ip2[3]);
+    if constexpr (FAST) {
+        // mixed precision: the floored primitives go to the solver directly
+        // (the round trip only adds rounding); the same positivity test in
+        // division-free form
+        unsigned st = 0;
+        const RSFluid& F1 = I.rsf[0];
+        const RSFluid& F2 = I.rsf[1];
+        if (!(Lp.v[0] > 0.0 && Lp.v[3] + F1.g * F1.pinf >= 0.0) || !(Rp.v[0] > 0.0 && Rp.v[3] + F2.g * F2.pinf >= 0.0))
+            st |= ST_UNPHYS;
+        RSSideM L, R;
+        rs_side_m(L, F1, Lp.v[0], Lp.v[1], Lp.v[3]);
+        rs_side_m(R, F2, Rp.v[0], Rp.v[1], Rp.v[3]);
+        st |= rs_star_m2(F1, F2, L, R, S, p0, 1e-12);
+        return st;
+    }
     const V4<double> Ls = prims2cons(I.gam[0], I.pinf[0], Lp);
     const V4<double> Rs = prims2cons(I.gam[1], I.pinf[1], Rp);
     unsigned st = 0;
@@ -1148,8 +1223,7 @@ __global__ __launch_bounds__(256) void k_iface_prep(const Inst* __restrict__ ins
 // star state of pending entry k once every earlier interfacial cell it reads
 // has its own (the side buffer holds those)
 template <int VAR, bool FAST>
-__device__ __forceinline__ unsigned iface_solve_pending(const Inst& I, const IfRec* recs, int k) {
-    const IfRec& r = recs[k];
+__device__ __forceinline__ unsigned iface_solve_pending(const Inst& I, const IfRec& r, int k) {
     double ip1[4], ip2[4], out[8], ps;
     probe_values<VAR>(I, r, true, ip1, ip2);
     const unsigned st = iface_finish<VAR, FAST>(I, r, ip1, ip2, r.pspec, out, &ps);
@@ -1158,25 +1232,24 @@ __device__ __forceinline__ unsigned iface_solve_pending(const Inst& I, const IfR
     return st;
 }
 
-// Pass 2, one workgroup per instance.  The entries left pending by pass 1
-// are ordered by dependency level (level 0: final after pass 1; otherwise
-// 1 + the highest level among the entries it reads), found by integer-only
-// rounds over an LDS list; then each level is solved in parallel, one
-// barrier per level.  Instances with more than IPEND_LDS pending entries or
-// deeper than 254 levels (and every instance when `plain` is set) use plain
-// rounds: solve whatever is ready, repeat.  Both give the same bits.
-constexpr int IPEND_LDS = 12288;
+// Pass 2, one workgroup per instance.  Entries are in row-major cell order
+// and every entry an entry depends on comes earlier.  Each lane holds one
+// entry, solves it as soon as the entries it reads are final (LDS flags,
+// acquire / release fences for the side buffer) and then claims the next
+// unclaimed entry.  Entries are claimed in increasing order, so the earliest
+// unfinished entry is always held by a lane and has its inputs final: the
+// pass cannot stall; a spin limit guards it anyway (ST_IROUNDS).  With `plain`
+// set it runs plain rounds instead: solve whatever is ready, barrier,
+// repeat.  Both give the same bits.
+constexpr unsigned IFACE_SPIN_LIMIT = 1u << 24;
 
 template <typename R, int VAR>
 __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ insts, const int4* __restrict__ work,
                                                       int plain) {
     constexpr bool FAST = sizeof(R) == 4;
-    __shared__ unsigned char lev[ICAP_LDS];   // 0 final, 1..254 level, 255 unknown
-    __shared__ int plist[IPEND_LDS];          // pending entries
-    __shared__ int slist[IPEND_LDS];          // pending entries sorted by level
-    __shared__ int cnt[257];
-    __shared__ int npend, overflow;
-    __shared__ int nunk[2];   // per round parity: reset for the next round while this one's is read
+    __shared__ unsigned char fin[ICAP_LDS];   // 1: star state final
+    __shared__ int nunk[2];                   // per round parity (plain rounds)
+    __shared__ int nclaim;                    // next unclaimed entry
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
     if (!I.si[SI_ACTIVE]) return;
@@ -1184,98 +1257,68 @@ __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ i
     if (n > ICAP_LDS) { if (threadIdx.x == 0) flag(I, ST_IOVF); return; }
     const IfRec* recs = (const IfRec*)I.irec;
     const int tid = threadIdx.x, nt = blockDim.x;
-    if (tid == 0) { npend = 0; overflow = 0; }
-    for (int k = tid; k < 257; k += nt) cnt[k] = 0;
-    __syncthreads();
-    for (int k = tid; k < n; k += nt) {
-        const bool done = I.idone[k] != 0;
-        lev[k] = done ? 0 : 255;
-        if (!done) {
-            const int p = atomicAdd(&npend, 1);
-            if (p < IPEND_LDS) plist[p] = k;
-        }
-    }
+    for (int k = tid; k < n; k += nt) fin[k] = I.idone[k] ? 1 : 0;
+    if (tid == 0) nclaim = nt;
     __syncthreads();
     unsigned st = 0;
     int rounds = 0;
-    const int np = npend;
-    if (np <= IPEND_LDS && !plain) {
-        // levels
-        for (;; ++rounds) {
-            if (tid == 0) nunk[rounds & 1] = 0;
-            __syncthreads();
-            for (int p = tid; p < np; p += nt) {
-                const int k = plist[p];
-                if (lev[k] != 255) continue;
-                int m = 0;
-                bool ok = true;
-                for (int a = 0; a < 2; ++a)
-                    for (int q = 0; q < 4; ++q) {
-                        const int d = recs[k].dep[a][q];
-                        if (d < 0) continue;
-                        const int l = lev[d];
-                        if (l == 255) ok = false;
-                        else m = max(m, l);
-                    }
-                if (!ok) { atomicAdd(&nunk[rounds & 1], 1); continue; }
-                if (m + 1 > 254) { overflow = 1; continue; }
-                lev[k] = (unsigned char)(m + 1);
+    if (!plain) {
+        // every lane holds one entry at a time and claims the next unclaimed
+        // one (in increasing order) once its entry is final
+        unsigned spins = 0;
+        int k = tid;
+        bool have = k < n;
+        int dep[8];
+        auto claim = [&]() {
+            while (have && fin[k]) {   // final after pass 1: take the next
+                k = atomicAdd(&nclaim, 1);
+                have = k < n;
             }
-            __syncthreads();
-            if (nunk[rounds & 1] == 0 || overflow) break;
+            if (have) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) dep[q] = recs[k].dep[q >> 2][q & 3];
+            }
+        };
+        claim();
+        while (true) {
+            if (have) {
+                bool ready = true;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (dep[q] >= 0 && __atomic_load_n(&fin[dep[q]], __ATOMIC_RELAXED) == 0) ready = false;
+                if (ready) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    st |= iface_solve_pending<VAR, FAST>(I, recs[k], k);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __atomic_store_n(&fin[k], (unsigned char)1, __ATOMIC_RELAXED);
+                    k = atomicAdd(&nclaim, 1);
+                    have = k < n;
+                    claim();
+                }
+            }
+            if (__all(!have)) break;
+            if (++spins > IFACE_SPIN_LIMIT) { st |= ST_IROUNDS; break; }
         }
+        rounds = (int)min(spins, 1u << 30);
     } else {
-        overflow = 1;
-    }
-    __syncthreads();
-    if (!overflow) {
-        // counting sort by level, then one parallel step per level
-        for (int p = tid; p < np; p += nt) atomicAdd(&cnt[lev[plist[p]]], 1);
-        __syncthreads();
-        if (tid == 0) {
-            int acc = 0;
-            for (int l = 0; l < 256; ++l) { const int c = cnt[l]; cnt[l] = acc; acc += c; }
-            cnt[256] = acc;
-        }
-        __syncthreads();
-        int maxl = 0;
-        for (int l = 255; l > 0; --l)
-            if (cnt[l + 1] > cnt[l]) { maxl = l; break; }
-        __syncthreads();
-        for (int p = tid; p < np; p += nt) {
-            const int k = plist[p];
-            slist[atomicAdd(&cnt[lev[k]], 1)] = k;
-        }
-        __syncthreads();
-        // cnt[l] now holds the end of level l's run, which starts at cnt[l - 1]
-        for (int l = 1; l <= maxl; ++l) {
-            const int b0 = cnt[l - 1], b1 = cnt[l];
-            for (int p = b0 + tid; p < b1; p += nt) st |= iface_solve_pending<VAR, FAST>(I, recs, slist[p]);
-            __syncthreads();
-        }
-        rounds += maxl;
-    } else {
-        // plain rounds over all entries (lev: 0 final, 1 solved this round, 255 pending)
-        for (int k = tid; k < n; k += nt) lev[k] = I.idone[k] ? 0 : 255;
-        __syncthreads();
         for (rounds = 0;; ++rounds) {
             if (tid == 0) nunk[rounds & 1] = 0;
             __syncthreads();
             for (int k = tid; k < n; k += nt) {
-                if (lev[k] != 255) continue;
+                if (fin[k]) continue;
                 bool ready = true;
                 for (int a = 0; a < 2; ++a)
                     for (int q = 0; q < 4; ++q) {
                         const int d = recs[k].dep[a][q];
-                        if (d >= 0 && lev[d] != 0) ready = false;
+                        if (d >= 0 && fin[d] != 1) ready = false;
                     }
                 if (!ready) { atomicAdd(&nunk[rounds & 1], 1); continue; }
-                st |= iface_solve_pending<VAR, FAST>(I, recs, k);
-                lev[k] = 1;
+                st |= iface_solve_pending<VAR, FAST>(I, recs[k], k);
+                fin[k] = 2;
             }
             __syncthreads();
             for (int k = tid; k < n; k += nt)
-                if (lev[k] == 1) lev[k] = 0;
+                if (fin[k] == 2) fin[k] = 1;
             __syncthreads();
             if (nunk[rounds & 1] == 0) break;
             if (rounds > n + 1) { if (tid == 0) flag(I, ST_IROUNDS); break; }
@@ -1906,7 +1949,9 @@ bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, i
 
 template <typename R>
 void launch_iface(const LaunchCtx& L, int variant, int pbuf) {
-    hipLaunchKernelGGL((k_iface_mark<R>), GRID(L.n_int), dim3(256), 0, L.stream, L.insts, L.w_int);
+    hipLaunchKernelGGL((k_iface_count<R>), GRID(L.n_seg), dim3(SEGW), 0, L.stream, L.insts, L.w_seg);
+    hipLaunchKernelGGL(k_iface_scan, GRID(L.n_inst), dim3(1024), 0, L.stream, L.insts, L.w_inst);
+    hipLaunchKernelGGL((k_iface_mark<R>), GRID(L.n_seg), dim3(SEGW), 0, L.stream, L.insts, L.w_seg);
     switch (variant) {
         case V_REAL:
             hipLaunchKernelGGL((k_iface_prep<R, V_REAL>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich, pbuf);

@@ -18,6 +18,7 @@ struct LaunchCtx {
     const int4* w_ich;   int n_ich;     // 256-entry chunks of the interfacial lists
     const int4* w_tb;    int n_tb;      // 64x16 interior tiles (temporally blocked extrapolation)
     const int4* w_rt;    int n_rt;      // 64x32 interior tiles (temporally blocked reinitialisation)
+    const int4* w_seg;   int n_seg;     // 256-cell interior row segments (interfacial list)
     int tb;                             // temporally blocked kernels (no periodic instance)
     int iface_plain;                    // interfacial solve by plain rounds instead of levels
 };
