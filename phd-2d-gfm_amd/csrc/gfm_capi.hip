@@ -91,7 +91,7 @@ struct gfm_ctx {
 namespace {
 
 template <typename R>
-void dispatch_dt(gfm_ctx* c, int q) { gfm::launch_dt<R>(c->L, c->twofluid() ? 1 : 0, q, c->narrow()); }
+void dispatch_dt(gfm_ctx* c, int q) { gfm::launch_dt<R>(c->L, c->twofluid() ? 1 : 0, q, c->narrow(), 0); }
 
 // N Jacobi iterations of the extrapolation.  Source: conserved Q[.][qin]
 // (from_q) or primitives P[.][pstart]; destination: conserved Q[.][qout]
@@ -133,25 +133,25 @@ int extrap_chain(gfm_ctx* c, int N, bool from_q, int qin, int pstart, bool injec
 
 // GFM_base::set_ghost_states for the context's variant: Q[.][0] -> Q[.][1]
 template <typename R>
-void ghost_chain(gfm_ctx* c) {
+void ghost_chain(gfm_ctx* c, int counted = 0) {
     switch (c->m.gfm) {
         case GFM_ORIGINAL:   // GFM_original.cpp:34-165
             extrap_chain<R>(c, 6, true, 0, 0, false, true, 1, 8);
             break;
         case GFM_REAL: {     // GFM_real.cpp:32-247
-            gfm::launch_iface<R>(c->L, gfm::V_REAL, 0);
+            gfm::launch_iface<R>(c->L, gfm::V_REAL, 0, counted);
             extrap_chain<R>(c, 6, true, 0, 0, true, true, 1, 0);
             break;
         }
         case GFM_RIEMANN: {  // GFM_riemann.cpp:33-187
             const int p = extrap_chain<R>(c, 2, true, 0, 0, false, false, 0, 0);
-            gfm::launch_iface<R>(c->L, gfm::V_RIEMANN, p);
+            gfm::launch_iface<R>(c->L, gfm::V_RIEMANN, p, counted);
             extrap_chain<R>(c, 6, false, 0, p, true, true, 1, 0);
             break;
         }
         case GFM_MODIFIED: { // GFM_modified.cpp:37-239
             const int p = extrap_chain<R>(c, 2, true, 0, 0, false, false, 0, 0);
-            gfm::launch_iface<R>(c->L, gfm::V_MODIFIED, p);
+            gfm::launch_iface<R>(c->L, gfm::V_MODIFIED, p, counted);
             gfm::launch_mgfm_final<R>(c->L, 0, 1);
             break;
         }
@@ -165,13 +165,15 @@ template <typename R>
 void step_launches(gfm_ctx* c, int cur) {
     const bool muscl = c->m.fs == GFM_FS_MUSCL;
     if (c->twofluid()) {
-        gfm::launch_dt<R>(c->L, 1, 0, c->narrow());
-        ghost_chain<R>(c);
+        // the dt pass also counts the interfacial cells when the variant lists them
+        const int count = c->m.gfm != GFM_ORIGINAL;
+        gfm::launch_dt<R>(c->L, 1, 0, c->narrow(), count);
+        ghost_chain<R>(c, count);
         gfm::launch_fs<R>(c->L, muscl, 1, 0, 0);
         gfm::launch_ls<R>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
         gfm::launch_bc<R>(c->L, 0, 2, 1);
     } else {
-        gfm::launch_dt<R>(c->L, 0, cur, 0);
+        gfm::launch_dt<R>(c->L, 0, cur, 0, 0);
         gfm::launch_fs<R>(c->L, muscl, cur, 1 - cur, 1);
         gfm::launch_bc<R>(c->L, 1 - cur, 1, 1);
     }
@@ -384,7 +386,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.cfl = h.cfl; d.T = h.T; d.max_steps = h.max_steps;
         d.use_bbox = 1;   // boxes are reset to the full domain each step unless OGFM narrows them
         const int tix = (s.Nx + 63) / 64, tiy = (s.Ny + 3) / 4;
-        d.ntile_dt = tix * tiy;
+        d.ntile_dt = s.Ny * ((s.Nx + 255) / 256);   // dt partials: one per 256-cell row segment
         d.icap = c->twofluid() ? std::min(std::min(s.Nx * s.Ny, 16 * (s.Nx + s.Ny) + 1024), 49152) : 1;
         // layout of the single allocation
         const size_t P = (size_t)d.plane;
@@ -395,7 +397,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         for (int f = 0; f < nf; ++f)
             for (int b = 0; b < 2; ++b) oQ[f][b] = take(4 * P * es);
         size_t ophi = 0, oA = 0, oB = 0, ovel = 0, oimap = 0, oilist = 0, oside = 0, orec = 0, odone = 0, oseg = 0;
-        d.nseg = c->twofluid() ? s.Ny * ((s.Nx + 255) / 256) : 0;
+        d.nseg = s.Ny * ((s.Nx + 255) / 256);
         if (c->twofluid()) {
             for (int f = 0; f < 2; ++f)
                 for (int b = 0; b < 2; ++b) oP[f][b] = take(4 * P * es);
@@ -406,7 +408,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
             oside = take((size_t)d.icap * 8 * sizeof(double));
             orec = take((size_t)d.icap * gfm::IFREC_BYTES);
             odone = take((size_t)d.icap * sizeof(int));
-            oseg = take((size_t)d.nseg * sizeof(int));
+            oseg = take((size_t)d.nseg * sizeof(int));   // interfacial counts per segment
         }
         const size_t opart = take((size_t)d.ntile_dt * 4 * sizeof(double));
         d.hist_cap = c->twofluid() ? (int)std::min<long long>((long long)h.max_steps + 1, 1 << 20) : 0;
@@ -454,7 +456,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         if (c->twofluid())
             for (int ty = 0; ty < (s.Ny + 15) / 16; ++ty)
                 for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_tb.push_back(make_int4(ki, ty, tx, 0));
-        if (c->twofluid()) {
+        {
             const int nsx = (s.Nx + 255) / 256;
             for (int r = 0; r < s.Ny; ++r)
                 for (int sx = 0; sx < nsx; ++sx) w_seg.push_back(make_int4(ki, r, sx, r * nsx + sx));
@@ -807,9 +809,9 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
         HC(hipEventRecord(ev[0], c->stream));
         if (c->twofluid()) {
             if (c->mixed()) {
-                gfm::launch_dt<float>(c->L, 1, 0, c->narrow());
+                gfm::launch_dt<float>(c->L, 1, 0, c->narrow(), c->m.gfm != GFM_ORIGINAL);
                 HC(hipEventRecord(ev[1], c->stream));
-                ghost_chain<float>(c);
+                ghost_chain<float>(c, c->m.gfm != GFM_ORIGINAL);
                 HC(hipEventRecord(ev[2], c->stream));
                 gfm::launch_fs<float>(c->L, muscl, 1, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));
@@ -817,9 +819,9 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
                 HC(hipEventRecord(ev[4], c->stream));
                 gfm::launch_bc<float>(c->L, 0, 2, 1);
             } else {
-                gfm::launch_dt<double>(c->L, 1, 0, c->narrow());
+                gfm::launch_dt<double>(c->L, 1, 0, c->narrow(), c->m.gfm != GFM_ORIGINAL);
                 HC(hipEventRecord(ev[1], c->stream));
-                ghost_chain<double>(c);
+                ghost_chain<double>(c, c->m.gfm != GFM_ORIGINAL);
                 HC(hipEventRecord(ev[2], c->stream));
                 gfm::launch_fs<double>(c->L, muscl, 1, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));
@@ -828,8 +830,8 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
                 gfm::launch_bc<double>(c->L, 0, 2, 1);
             }
         } else {
-            if (c->mixed()) gfm::launch_dt<float>(c->L, 0, cur, 0);
-            else gfm::launch_dt<double>(c->L, 0, cur, 0);
+            if (c->mixed()) gfm::launch_dt<float>(c->L, 0, cur, 0, 0);
+            else gfm::launch_dt<double>(c->L, 0, cur, 0, 0);
             HC(hipEventRecord(ev[1], c->stream));
             HC(hipEventRecord(ev[2], c->stream));
             if (c->mixed()) gfm::launch_fs<float>(c->L, muscl, cur, 1 - cur, 1);

@@ -36,6 +36,15 @@ __device__ __forceinline__ void flag(const Inst& I, unsigned bits) {
     if (bits) atomicOr((unsigned*)&I.si[SI_STATUS], bits);
 }
 
+// interfacial cell (external Levelset::is_interfacial_cell, restated as a
+// 4-neighbour change of side with the side taken as phi <= 0)
+template <typename R>
+__device__ __forceinline__ bool is_iface(const Inst& I, int i, int j) {
+    const bool s = ld1<R>(I.phi, cidx(I, i, j)) <= R(0);
+    return ((ld1<R>(I.phi, cidx(I, i, j + 1)) <= R(0)) != s) || ((ld1<R>(I.phi, cidx(I, i, j - 1)) <= R(0)) != s) ||
+           ((ld1<R>(I.phi, cidx(I, i + 1, j)) <= R(0)) != s) || ((ld1<R>(I.phi, cidx(I, i - 1, j)) <= R(0)) != s);
+}
+
 // update boxes (realcells1/2): the full domain, or empty when GFM_original is
 // about to narrow them to the bounding box of each fluid's real cells
 // (GFM_original.cpp:47-55)
@@ -52,22 +61,21 @@ __device__ __forceinline__ void reset_boxes(const Inst& I, int narrow) {
 // fluid masses (sim_twofluid.cpp:1803-1827).  Block partials, then the last
 // block of each instance reduces them in tile order (deterministic) and also
 // resets the per-step counters used later in the step.
-// work: {inst, tile_y, tile_x, local tile index}, 64x4 interior tiles.
+// work: {inst, interior row, segment column, segment index}, 256-cell row
+// segments.  With `count` it also counts each segment's interfacial cells
+// for the interfacial list (k_iface_count's work, from the same phi).
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, const int4* __restrict__ work,
-                                            int twofluid, int q) {
+                                            int twofluid, int q, int count) {
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    const int nstep = I.si[SI_NSTEP];
-    const double t = I.sd[SD_T];
-    const bool active = (t < I.T) && (nstep < I.max_steps);
-    __shared__ double red[4][256];
     double mu = 0.0, mv = 0.0, m1 = 0.0, m2 = 0.0;
+    bool iface = false;
     {
-        const int i = I.g + w.y * 4 + (threadIdx.x >> 6);
-        const int j = I.g + w.z * 64 + (threadIdx.x & 63);
-        if (i < I.g + I.Ny && j < I.g + I.Nx) {
+        const int i = I.g + w.y;
+        const int j = I.g + w.z * 256 + (int)threadIdx.x;
+        if (j < I.g + I.Nx) {
             const long long c = cidx(I, i, j);
             int f = 0;
             double phi = 0.0;
@@ -89,25 +97,38 @@ __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, cons
                 if (-phi > 0.0) m1 = (double)ld4<R>(I.Q[0][q], I, c).v[0] * dA;
                 if (phi > 0.0) m2 = (double)ld4<R>(I.Q[1][q], I, c).v[0] * dA;
             }
+            if (count) iface = is_iface<R>(I, i, j);
         }
     }
-    red[0][threadIdx.x] = mu;
-    red[1][threadIdx.x] = mv;
-    red[2][threadIdx.x] = m1;
-    red[3][threadIdx.x] = m2;
+    // fixed-order reductions: a butterfly within each wavefront, then the
+    // four wavefronts in order
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mu = fmax(mu, __shfl_xor(mu, off));
+        mv = fmax(mv, __shfl_xor(mv, off));
+        m1 += __shfl_xor(m1, off);
+        m2 += __shfl_xor(m2, off);
+    }
+    const unsigned long long mask = __ballot(iface);
+    __shared__ double red[4][4];
+    __shared__ int wc[4];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wv][0] = mu; red[wv][1] = mv; red[wv][2] = m1; red[wv][3] = m2;
+        wc[wv] = __popcll(mask);
+    }
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
-            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
-            red[2][threadIdx.x] += red[2][threadIdx.x + s];
-            red[3][threadIdx.x] += red[3][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
+        double a0 = red[0][0], a1 = red[0][1], a2 = red[0][2], a3 = red[0][3];
+        int nc = wc[0];
+        for (int k = 1; k < 4; ++k) {
+            a0 = fmax(a0, red[k][0]); a1 = fmax(a1, red[k][1]);
+            a2 += red[k][2]; a3 += red[k][3];
+            nc += wc[k];
+        }
         double* pp = I.partials + 4 * (long long)w.w;
-        pp[0] = red[0][0]; pp[1] = red[1][0]; pp[2] = red[2][0]; pp[3] = red[3][0];
+        pp[0] = a0; pp[1] = a1; pp[2] = a2; pp[3] = a3;
+        if (count) I.iseg[w.w] = nc;
     }
 }
 
@@ -465,10 +486,28 @@ __device__ __forceinline__ void unit_normal(const Inst& I, int i, int j, double&
     else { nx /= m; ny /= m; }
 }
 
+// per-instance constants of the extrapolation sweep, computed once per
+// thread: the pseudo-time step over dx and dy (ax, ay) and 1/(2dx), 1/(2dy)
+// for the FP32 normal
+template <typename R>
+struct ExK { R ax, ay; float i2dx, i2dy; };
+
+template <typename R>
+__device__ __forceinline__ ExK<R> ex_k(const Inst& I) {
+#pragma clang fp contract(off)
+    ExK<R> k;
+    const R pdt = R(0.5 * fmin(I.dx, I.dy));
+    k.ax = pdt / (R)I.dx;
+    k.ay = pdt / (R)I.dy;
+    k.i2dx = (float)(0.5 / I.dx);
+    k.i2dy = (float)(0.5 / I.dy);
+    return k;
+}
+
 // unit normal of phi in the storage precision: FP64 as unit_normal; FP32
 // (mixed precision) from one reciprocal square root
 template <typename R>
-__device__ __forceinline__ void unit_normal_r(const Inst& I, int i, int j, R& nx, R& ny) {
+__device__ __forceinline__ void unit_normal_r(const Inst& I, const ExK<R>& K, int i, int j, R& nx, R& ny) {
     if constexpr (sizeof(R) == 8) {
         double a, b;
         unit_normal<R>(I, i, j, a, b);
@@ -476,8 +515,8 @@ __device__ __forceinline__ void unit_normal_r(const Inst& I, int i, int j, R& nx
     } else {
 #pragma clang fp contract(off)
         const float* phi = (const float*)I.phi;
-        const float gx = (phi[cidx(I, i, j + 1)] - phi[cidx(I, i, j - 1)]) * (float)(0.5 / I.dx);
-        const float gy = (phi[cidx(I, i + 1, j)] - phi[cidx(I, i - 1, j)]) * (float)(0.5 / I.dy);
+        const float gx = (phi[cidx(I, i, j + 1)] - phi[cidx(I, i, j - 1)]) * K.i2dx;
+        const float gy = (phi[cidx(I, i + 1, j)] - phi[cidx(I, i - 1, j)]) * K.i2dy;
         const float m2 = gx * gx + gy * gy;
         if (m2 < 1e-24f) { nx = 1.0f; ny = 0.0f; }
         else { const float r = rsqrtf(m2); nx = gx * r; ny = gy * r; }
@@ -532,12 +571,10 @@ __device__ __forceinline__ V4<R> extrap_combine(R wx, R wy, const V4<R>& c0, con
 }
 
 template <typename R>
-__device__ __forceinline__ void extrap_weights(const Inst& I, R nx, R ny, R& wx, R& wy) {
+__device__ __forceinline__ void extrap_weights(const ExK<R>& K, R nx, R ny, R& wx, R& wy) {
 #pragma clang fp contract(off)
-    const R pdt = R(0.5 * fmin(I.dx, I.dy));
-    const R ax = pdt / (R)I.dx, ay = pdt / (R)I.dy;
-    wx = ax * fabs(nx);
-    wy = ay * fabs(ny);
+    wx = K.ax * fabs(nx);
+    wy = K.ay * fabs(ny);
 }
 
 // GFM_original.cpp:65-116: the ghost fluid takes the real fluid's velocity
@@ -617,12 +654,13 @@ __device__ __forceinline__ void extrap_cell(const Inst& I, int i, int j, int bin
     const long long cs = cidx(I, si, sj);
     const R lsval = ld1<R>(I.phi, cs);
     R nx, ny;
-    unit_normal_r<R>(I, si, sj, nx, ny);
+    const ExK<R> K = ex_k<R>(I);
+    unit_normal_r<R>(I, K, si, sj, nx, ny);
     const int fr = lsval <= R(0) ? 0 : 1, fg = 1 - fr;
     const void* in_r = qsrc ? I.Q[fr][bin] : I.P[fr][bin];
     const void* in_g = qsrc ? I.Q[fg][bin] : I.P[fg][bin];
     R wx, wy;
-    extrap_weights<R>(I, nx, ny, wx, wy);
+    extrap_weights<R>(K, nx, ny, wx, wy);
     V4<R> Wr = ex_load<R, MODE>(I, fr, si, sj, in_r);
     int dj, di;
     extrap_dirs<R>(fg, nx, ny, dj, di);
@@ -727,6 +765,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
     }
     const void* in0 = (MIN & MF_FIRST) ? I.Q[0][bin] : I.P[0][bin];
     const void* in1 = (MIN & MF_FIRST) ? I.Q[1][bin] : I.P[1][bin];
+    const ExK<R> K = ex_k<R>(I);
     // per cell: the update weights, the LDS offsets of the ghost fluid at the
     // cell and its upwind neighbours, and bits (1: interior with its stencil
     // inside the region, 2: fluid 2 real, 4: owned, 8 and up: distance to
@@ -748,8 +787,8 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
                 W2 = ex_load<R, LM>(I, 1, i, j, in1);
                 if (interior(I, i, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1) {
                     R nx, ny;
-                    unit_normal_r<R>(I, i, j, nx, ny);
-                    extrap_weights<R>(I, nx, ny, wxs[k], wys[k]);
+                    unit_normal_r<R>(I, K, i, j, nx, ny);
+                    extrap_weights<R>(K, nx, ny, wxs[k], wys[k]);
                     const bool own = a >= NIT && a < NIT + TBY && b >= NIT && b < NIT + TBX;
                     const int dist = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
                     const int fr = ld1<R>(I.phi, cidx(I, i, j)) <= R(0) ? 0 : 1;
@@ -865,12 +904,6 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
 // 4-neighbour change of side with the side taken as phi <= 0).
 // work: {inst, tile_y, tile_x, tile index}, 64x4 interior tiles
 // ---------------------------------------------------------------------------
-template <typename R>
-__device__ __forceinline__ bool is_iface(const Inst& I, int i, int j) {
-    const bool s = ld1<R>(I.phi, cidx(I, i, j)) <= R(0);
-    return ((ld1<R>(I.phi, cidx(I, i, j + 1)) <= R(0)) != s) || ((ld1<R>(I.phi, cidx(I, i, j - 1)) <= R(0)) != s) ||
-           ((ld1<R>(I.phi, cidx(I, i + 1, j)) <= R(0)) != s) || ((ld1<R>(I.phi, cidx(I, i - 1, j)) <= R(0)) != s);
-}
 
 // The list is built in row-major cell order (the order of the reference's
 // i, j loops), so that every interfacial cell an entry depends on has a
@@ -1378,23 +1411,24 @@ __global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ ins
 // are the strides of a column / row step.  Every operation rounds on its own
 // (fixed order, no contraction) so that each kernel using it gets the same
 // bits.
+// (idx, idy: 1/dx, 1/dy in the storage precision)
 template <typename R>
 __device__ __forceinline__ R houc_rhs(const Houc& H, int n, const R* psi, long long sx, long long sy, R u, R v,
-                                      double dx, double dy) {
+                                      R idx, R idy) {
 #pragma clang fp contract(off)
     R px = R(0), py = R(0);
     if (u > R(0)) { for (int m = 0; m < n; ++m) px += (R)H.c[m] * psi[(H.o0 + m) * sx]; }
     else { for (int m = 0; m < n; ++m) px -= (R)H.c[m] * psi[(-H.o0 - m) * sx]; }
     if (v > R(0)) { for (int m = 0; m < n; ++m) py += (R)H.c[m] * psi[(H.o0 + m) * sy]; }
     else { for (int m = 0; m < n; ++m) py -= (R)H.c[m] * psi[(-H.o0 - m) * sy]; }
-    return -(u * (px / (R)dx) + v * (py / (R)dy));
+    return -(u * (px * idx) + v * (py * idy));
 }
 
 template <typename R>
 __device__ __forceinline__ R ls_rhs(const Inst& I, const Houc& H, const R* psi, int i, int j) {
     const long long c = cidx(I, i, j);
     const R* vel = (const R*)I.vel;
-    return houc_rhs<R>(H, H.n, psi + c, 1, I.pitch, vel[c], vel[c + I.plane], I.dx, I.dy);
+    return houc_rhs<R>(H, H.n, psi + c, 1, I.pitch, vel[c], vel[c + I.plane], R(1.0 / I.dx), R(1.0 / I.dy));
 }
 
 // the TVD-RK3 combinations of the three stages
@@ -1456,6 +1490,7 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
     const R* phi = (const R*)I.phi;
     const R* vel = (const R*)I.vel;
     const R dt = (R)I.sd[SD_DT];
+    const R idxr = R(1.0 / I.dx), idyr = R(1.0 / I.dy);
     // per cell: phi, velocity; meta bits 0-7: distance to the region edge,
     // 8: interior, 9: owned
     R p0[SLOTS], us[SLOTS], vs[SLOTS];
@@ -1493,7 +1528,7 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
         for (int k = 0; k < SLOTS; ++k) {
             if (!(meta[k] & 256) || (meta[k] & 255) < st * RAD) continue;
             const int idx = threadIdx.x + k * 1024;
-            const R rhs = houc_rhs<R>(H, N, in + idx, 1, RX, us[k], vs[k], I.dx, I.dy);
+            const R rhs = houc_rhs<R>(H, N, in + idx, 1, RX, us[k], vs[k], idxr, idyr);
             const R nv = st == 1 ? rk3_combine<R, 1>(p0[k], R(0), dt, rhs) : rk3_combine<R, 2>(p0[k], in[idx], dt, rhs);
             out[idx] = nv;
             // mirror ghosts within the stencil radius of the boundary
@@ -1511,7 +1546,7 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
     for (int k = 0; k < SLOTS; ++k) {
         if (!(meta[k] & 512)) continue;
         const int idx = threadIdx.x + k * 1024;
-        const R rhs = houc_rhs<R>(H, N, s0 + idx, 1, RX, us[k], vs[k], I.dx, I.dy);
+        const R rhs = houc_rhs<R>(H, N, s0 + idx, 1, RX, us[k], vs[k], idxr, idyr);
         const R nv = rk3_combine<R, 3>(p0[k], s0[idx], dt, rhs);
         const int a = idx / RX, b = idx - (idx / RX) * RX;
         const int i = ri + a, j = rj + b;
@@ -1890,8 +1925,9 @@ __global__ void k_reset(const Inst* __restrict__ insts, const int4* __restrict__
 #define GRID(n) dim3((unsigned)(n))
 
 template <typename R>
-void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes) {
-    hipLaunchKernelGGL((k_dt<R>), GRID(L.n_int), dim3(256), 0, L.stream, L.insts, L.w_int, twofluid, q);
+void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes, int count_iface) {
+    hipLaunchKernelGGL((k_dt<R>), GRID(L.n_seg), dim3(256), 0, L.stream, L.insts, L.w_seg, twofluid, q,
+                       count_iface);
     hipLaunchKernelGGL(k_dt_final, GRID(L.n_inst), dim3(256), 0, L.stream, L.insts, L.w_inst, twofluid, narrow_boxes);
 }
 
@@ -1948,8 +1984,8 @@ bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, i
 }
 
 template <typename R>
-void launch_iface(const LaunchCtx& L, int variant, int pbuf) {
-    hipLaunchKernelGGL((k_iface_count<R>), GRID(L.n_seg), dim3(SEGW), 0, L.stream, L.insts, L.w_seg);
+void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted) {
+    if (!counted) hipLaunchKernelGGL((k_iface_count<R>), GRID(L.n_seg), dim3(SEGW), 0, L.stream, L.insts, L.w_seg);
     hipLaunchKernelGGL(k_iface_scan, GRID(L.n_inst), dim3(1024), 0, L.stream, L.insts, L.w_inst);
     hipLaunchKernelGGL((k_iface_mark<R>), GRID(L.n_seg), dim3(SEGW), 0, L.stream, L.insts, L.w_seg);
     switch (variant) {
@@ -2049,11 +2085,11 @@ void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, doub
 }
 
 #define INST(R)                                                               \
-    template void launch_dt<R>(const LaunchCtx&, int, int, int);                   \
+    template void launch_dt<R>(const LaunchCtx&, int, int, int, int);              \
     template void launch_fs<R>(const LaunchCtx&, bool, int, int, int);             \
     template void launch_extrap<R>(const LaunchCtx&, int, int, int);          \
     template bool launch_extrap_tb<R>(const LaunchCtx&, int, int, int, int, int); \
-    template void launch_iface<R>(const LaunchCtx&, int, int);                \
+    template void launch_iface<R>(const LaunchCtx&, int, int, int);           \
     template void launch_mgfm_final<R>(const LaunchCtx&, int, int);           \
     template void launch_ls<R>(const LaunchCtx&, const Houc&, int, int);      \
     template void launch_reinit_now<R>(const LaunchCtx&, int);                \

@@ -18,7 +18,7 @@ struct LaunchCtx {
     const int4* w_ich;   int n_ich;     // 256-entry chunks of the interfacial lists
     const int4* w_tb;    int n_tb;      // 64x16 interior tiles (temporally blocked extrapolation)
     const int4* w_rt;    int n_rt;      // 64x32 interior tiles (temporally blocked reinitialisation)
-    const int4* w_seg;   int n_seg;     // 256-cell interior row segments (interfacial list)
+    const int4* w_seg;   int n_seg;     // 256-cell interior row segments (dt, interfacial list)
     int tb;                             // temporally blocked kernels (no periodic instance)
     int iface_plain;                    // interfacial solve by plain rounds instead of levels
 };
@@ -26,11 +26,11 @@ struct LaunchCtx {
 struct Houc;
 constexpr int IFREC_BYTES = 384;   // >= sizeof(IfRec), checked in the kernels
 
-template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes);
+template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes, int count_iface);
 template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive);
 template <typename R> void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout);
 template <typename R> bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout);
-template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf);
+template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted);
 template <typename R> void launch_mgfm_final(const LaunchCtx& L, int qin, int qout);
 template <typename R> void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters);
 template <typename R> void launch_reinit_now(const LaunchCtx& L, int iters);
