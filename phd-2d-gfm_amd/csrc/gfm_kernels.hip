@@ -722,29 +722,33 @@ __global__ __launch_bounds__(256) void k_extrap(const Inst* __restrict__ insts, 
 // ---------------------------------------------------------------------------
 constexpr int TBX = 64, TBY = 16, TB_THREADS = 512;
 
-template <typename R, int NC>
-__device__ __forceinline__ V4<R> tb_get(const R* s, int f, int idx) {
+// LDS cell of the blocked kernels: one fluid's four primitives, aligned so
+// that a cell moves in one 16-byte (FP32) or two (FP64) LDS accesses
+template <typename R>
+struct alignas(4 * sizeof(R) > 16 ? 16 : 4 * sizeof(R)) V4A { R v[4]; };
+
+template <typename R>
+__device__ __forceinline__ V4<R> tb_get(const V4A<R>* s, int k) {
+    const V4A<R> a = s[k];
     V4<R> W;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) W.v[q] = s[(4 * f + q) * NC + idx];
+    for (int q = 0; q < 4; ++q) W.v[q] = a.v[q];
     return W;
 }
 
-template <typename R, int NC>
-__device__ __forceinline__ void tb_put(R* s, int f, int idx, const V4<R>& W) {
+template <typename R>
+__device__ __forceinline__ void tb_put(V4A<R>* s, int k, const V4<R>& W) {
+    V4A<R> a;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) s[(4 * f + q) * NC + idx] = W.v[q];
+    for (int q = 0; q < 4; ++q) a.v[q] = W.v[q];
+    s[k] = a;
 }
 
 // new ghost-fluid primitives of an LDS cell (extrap_cell's arithmetic): oc,
-// ox, oy index component 0 of the ghost fluid at the cell and its upwind x
-// and y neighbours
-template <typename R, int NC>
-__device__ __forceinline__ V4<R> tb_update(const R* s, int oc, int ox, int oy, R wx, R wy) {
-    V4<R> c0, xo, yo;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) { c0.v[q] = s[oc + q * NC]; xo.v[q] = s[ox + q * NC]; yo.v[q] = s[oy + q * NC]; }
-    return extrap_combine<R>(wx, wy, c0, xo, yo);
+// ox, oy index the ghost fluid at the cell and its upwind x and y neighbours
+template <typename R>
+__device__ __forceinline__ V4<R> tb_update(const V4A<R>* s, int oc, int ox, int oy, R wx, R wy) {
+    return extrap_combine<R>(wx, wy, tb_get<R>(s, oc), tb_get<R>(s, ox), tb_get<R>(s, oy));
 }
 
 template <typename R, int NIT, int MIN, int MOUT>
@@ -753,7 +757,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
     constexpr int RX = TBX + 2 * NIT, RY = TBY + 2 * NIT, NC = RX * RY;
     constexpr int SLOTS = (NC + TB_THREADS - 1) / TB_THREADS;
     constexpr int LM = MIN & (MF_FIRST | MF_INJECT);
-    __shared__ R s[8 * NC];   // s[(4 f + q) NC + cell]: primitive q of fluid f
+    __shared__ V4A<R> s[2 * NC];   // s[f NC + cell]: the primitives of fluid f
     __shared__ int sbb[8];
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
@@ -795,13 +799,13 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
                     meta[k] = 1 | (fr ? 2 : 0) | (own ? 4 : 0) | (dist << 3);
                     int dj, di;
                     extrap_dirs<R>(1 - fr, nx, ny, dj, di);
-                    oc[k] = 4 * (1 - fr) * NC + idx;
+                    oc[k] = (1 - fr) * NC + idx;
                     ox[k] = oc[k] + dj;
                     oy[k] = oc[k] + di * RX;
                 }
             }
-            tb_put<R, NC>(s, 0, idx, W1);
-            tb_put<R, NC>(s, 1, idx, W2);
+            tb_put<R>(s, idx, W1);
+            tb_put<R>(s, NC + idx, W2);
         }
     }
     __syncthreads();
@@ -817,7 +821,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
             const int idx = threadIdx.x + k * TB_THREADS;
             if ((meta[k] & 1) && (meta[k] >> 3) >= t) {
                 act |= 1 << k;
-                nv[k] = tb_update<R, NC>(s, oc[k], ox[k], oy[k], wxs[k], wys[k]);
+                nv[k] = tb_update<R>(s, oc[k], ox[k], oy[k], wxs[k], wys[k]);
                 if ((meta[k] & 4) && !(nv[k].v[0] >= R(0))) bad = true;
             }
         }
@@ -828,18 +832,17 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
             const int idx = threadIdx.x + k * TB_THREADS;
             const int a = idx / RX, b = idx - (idx / RX) * RX;
             const int i = ri + a, j = rj + b;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) s[oc[k] + q * NC] = nv[k].v[q];
+            tb_put<R>(s, oc[k], nv[k]);
             if (i == ib || i == it || j == jl || j == jr) {
                 // first ghost layer: this cell's new state, reflected
-                const V4<R> W1 = tb_get<R, NC>(s, 0, idx), W2 = tb_get<R, NC>(s, 1, idx);
+                const V4<R> W1 = tb_get<R>(s, idx), W2 = tb_get<R>(s, NC + idx);
                 V4<R> Wv1 = W1, Wv2 = W2, Wu1 = W1, Wu2 = W2;
                 Wv1.v[2] = -Wv1.v[2]; Wv2.v[2] = -Wv2.v[2];
                 Wu1.v[1] = -Wu1.v[1]; Wu2.v[1] = -Wu2.v[1];
-                if (i == ib) { tb_put<R, NC>(s, 0, idx - RX, rfb ? Wv1 : W1); tb_put<R, NC>(s, 1, idx - RX, rfb ? Wv2 : W2); }
-                if (i == it) { tb_put<R, NC>(s, 0, idx + RX, rft ? Wv1 : W1); tb_put<R, NC>(s, 1, idx + RX, rft ? Wv2 : W2); }
-                if (j == jl) { tb_put<R, NC>(s, 0, idx - 1, rfx ? Wu1 : W1); tb_put<R, NC>(s, 1, idx - 1, rfx ? Wu2 : W2); }
-                if (j == jr) { tb_put<R, NC>(s, 0, idx + 1, rfx ? Wu1 : W1); tb_put<R, NC>(s, 1, idx + 1, rfx ? Wu2 : W2); }
+                if (i == ib) { tb_put<R>(s, idx - RX, rfb ? Wv1 : W1); tb_put<R>(s, NC + idx - RX, rfb ? Wv2 : W2); }
+                if (i == it) { tb_put<R>(s, idx + RX, rft ? Wv1 : W1); tb_put<R>(s, NC + idx + RX, rft ? Wv2 : W2); }
+                if (j == jl) { tb_put<R>(s, idx - 1, rfx ? Wu1 : W1); tb_put<R>(s, NC + idx - 1, rfx ? Wu2 : W2); }
+                if (j == jr) { tb_put<R>(s, idx + 1, rfx ? Wu1 : W1); tb_put<R>(s, NC + idx + 1, rfx ? Wu2 : W2); }
             }
         }
         __syncthreads();
@@ -852,9 +855,9 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
         const int a = idx / RX, b = idx - (idx / RX) * RX;
         const int i = ri + a, j = rj + b;
         const int fr = (meta[k] & 2) ? 1 : 0, fg = 1 - fr;
-        const V4<R> Wg = tb_update<R, NC>(s, oc[k], ox[k], oy[k], wxs[k], wys[k]);
+        const V4<R> Wg = tb_update<R>(s, oc[k], ox[k], oy[k], wxs[k], wys[k]);
         if (!(Wg.v[0] >= R(0))) bad = true;
-        const V4<R> Wr = tb_get<R, NC>(s, fr, idx);
+        const V4<R> Wr = tb_get<R>(s, fr * NC + idx);
         V4<R> Wo = Wg;
         if (MOUT & MF_OGFM) {
             if (!ogfm_ghost<R>(I, fg, Wr, Wo)) bad = true;
