@@ -81,6 +81,17 @@ struct gfm_ctx {
     int cur = 0;                       // current conserved buffer (one-fluid step path)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     bool tb = false;                   // temporally blocked extrapolation (no periodic instance)
+    // two-fluid step path: the instances in groups, each group's step on its
+    // own stream (group 0 on `stream`), forked from and joined back to
+    // `stream`, so that the latency-bound phases of one group (the ordered
+    // interfacial solve: one workgroup per instance) overlap the
+    // throughput-bound phases of the others
+    std::vector<gfm::LaunchCtx> GL;
+    std::vector<hipStream_t> gstream;  // gstream[0] == stream
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_join, ev_stag;
+    hipGraphExec_t gexec_multi = nullptr;   // graph_steps two-fluid steps of all groups
+    int graph_steps = 0;
     bool mixed() const { return m.precision == GFM_MIXED; }
     bool twofluid() const { return m.sim_type == GFM_SIM_TWOFLUID; }
     int nfluid() const { return twofluid() ? 2 : 1; }
@@ -98,16 +109,16 @@ void dispatch_dt(gfm_ctx* c, int q) { gfm::launch_dt<R>(c->L, c->twofluid() ? 1 
 // with W->U and the velocity field (to_q) or the P ping-pong.  Returns the
 // P buffer holding the result when !to_q.
 template <typename R>
-int extrap_chain(gfm_ctx* c, int N, bool from_q, int qin, int pstart, bool inject_first, bool to_q, int qout,
-                 int last_extra) {
-    if (c->tb && N >= 2) {
+int extrap_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int N, bool from_q, int qin, int pstart, bool inject_first,
+                 bool to_q, int qout, int last_extra) {
+    if (L.tb && N >= 2) {
         // one launch for all N iterations; it writes only the last one's
         // destination, which is the same buffer as in the loop below
         int cur_p = pstart;
         for (int k = 0; k < N && !(k == N - 1 && to_q); ++k) cur_p = (k == 0 && from_q) ? 0 : 1 - cur_p;
         const int min = (from_q ? 1 : 0) | (inject_first ? 2 : 0);
         const int mout = to_q ? (4 | last_extra) : 0;
-        if (gfm::launch_extrap_tb<R>(c->L, N, min, mout, from_q ? qin : pstart, to_q ? qout : cur_p)) return cur_p;
+        if (gfm::launch_extrap_tb<R>(L, N, min, mout, from_q ? qin : pstart, to_q ? qout : cur_p)) return cur_p;
     }
     int cur_p = pstart;
     for (int k = 0; k < N; ++k) {
@@ -126,57 +137,102 @@ int extrap_chain(gfm_ctx* c, int N, bool from_q, int qin, int pstart, bool injec
             bout = (k == 0 && from_q) ? 0 : 1 - cur_p;
             cur_p = bout;
         }
-        gfm::launch_extrap<R>(c->L, mode, bin, bout);
+        gfm::launch_extrap<R>(L, mode, bin, bout);
     }
     return cur_p;
 }
 
-// GFM_base::set_ghost_states for the context's variant: Q[.][0] -> Q[.][1]
+// GFM_base::set_ghost_states for the context's variant: Q[.][0] -> Q[.][1].
+// `mark`, if given, is recorded on L's stream once the interfacial solve (or,
+// for GFM_original, the whole chain) is launched.
 template <typename R>
-void ghost_chain(gfm_ctx* c, int counted = 0) {
+int ghost_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int counted = 0, hipEvent_t mark = nullptr) {
     switch (c->m.gfm) {
         case GFM_ORIGINAL:   // GFM_original.cpp:34-165
-            extrap_chain<R>(c, 6, true, 0, 0, false, true, 1, 8);
+            extrap_chain<R>(c, L, 6, true, 0, 0, false, true, 1, 8);
+            if (mark) HC(hipEventRecord(mark, L.stream));
             break;
         case GFM_REAL: {     // GFM_real.cpp:32-247
-            gfm::launch_iface<R>(c->L, gfm::V_REAL, 0, counted);
-            extrap_chain<R>(c, 6, true, 0, 0, true, true, 1, 0);
+            gfm::launch_iface<R>(L, gfm::V_REAL, 0, counted);
+            if (mark) HC(hipEventRecord(mark, L.stream));
+            extrap_chain<R>(c, L, 6, true, 0, 0, true, true, 1, 0);
             break;
         }
         case GFM_RIEMANN: {  // GFM_riemann.cpp:33-187
-            const int p = extrap_chain<R>(c, 2, true, 0, 0, false, false, 0, 0);
-            gfm::launch_iface<R>(c->L, gfm::V_RIEMANN, p, counted);
-            extrap_chain<R>(c, 6, false, 0, p, true, true, 1, 0);
+            const int p = extrap_chain<R>(c, L, 2, true, 0, 0, false, false, 0, 0);
+            gfm::launch_iface<R>(L, gfm::V_RIEMANN, p, counted);
+            if (mark) HC(hipEventRecord(mark, L.stream));
+            extrap_chain<R>(c, L, 6, false, 0, p, true, true, 1, 0);
             break;
         }
         case GFM_MODIFIED: { // GFM_modified.cpp:37-239
-            const int p = extrap_chain<R>(c, 2, true, 0, 0, false, false, 0, 0);
-            gfm::launch_iface<R>(c->L, gfm::V_MODIFIED, p, counted);
-            gfm::launch_mgfm_final<R>(c->L, 0, 1);
+            const int p = extrap_chain<R>(c, L, 2, true, 0, 0, false, false, 0, 0);
+            gfm::launch_iface<R>(L, gfm::V_MODIFIED, p, counted);
+            if (mark) HC(hipEventRecord(mark, L.stream));
+            gfm::launch_mgfm_final<R>(L, 0, 1);
             break;
         }
         default: break;
     }
+    return GFM_OK;
 }
 
-// one step of sim_twofluid::run_sim (sim_twofluid.cpp:64-88) or
-// sim_onefluid::run_sim (sim_onefluid.cpp:40-54)
+// one two-fluid step of the instances of L (sim_twofluid.cpp:64-88)
 template <typename R>
-void step_launches(gfm_ctx* c, int cur) {
+int twofluid_step(gfm_ctx* c, const gfm::LaunchCtx& L, hipEvent_t mark) {
     const bool muscl = c->m.fs == GFM_FS_MUSCL;
-    if (c->twofluid()) {
-        // the dt pass also counts the interfacial cells when the variant lists them
-        const int count = c->m.gfm != GFM_ORIGINAL;
-        gfm::launch_dt<R>(c->L, 1, 0, c->narrow(), count);
-        ghost_chain<R>(c, count);
-        gfm::launch_fs<R>(c->L, muscl, 1, 0, 0);
-        gfm::launch_ls<R>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
-        gfm::launch_bc<R>(c->L, 0, 2, 1);
-    } else {
-        gfm::launch_dt<R>(c->L, 0, cur, 0, 0);
-        gfm::launch_fs<R>(c->L, muscl, cur, 1 - cur, 1);
-        gfm::launch_bc<R>(c->L, 1 - cur, 1, 1);
+    // the dt pass also counts the interfacial cells when the variant lists them
+    const int count = c->m.gfm != GFM_ORIGINAL;
+    gfm::launch_dt<R>(L, 1, 0, c->narrow(), count);
+    int r = ghost_chain<R>(c, L, count, mark);
+    if (r) return r;
+    gfm::launch_fs<R>(L, muscl, 1, 0, 0);
+    gfm::launch_ls<R>(L, c->houc, c->m.reinit_every, c->m.reinit_iters);
+    gfm::launch_bc<R>(L, 0, 2, 1);
+    return GFM_OK;
+}
+
+// nsteps two-fluid steps.  With several instance groups, each group's steps
+// go to its own stream with no synchronisation between the groups until all
+// nsteps are launched; for more than one step, group g starts once group g-1
+// has reached its first interfacial solve, so that the groups'
+// latency-bound phases fall at different times.  Instances are independent,
+// so the result is the same as with one stream.
+template <typename R>
+int twofluid_steps(gfm_ctx* c, int nsteps) {
+    const size_t ng = c->GL.size();
+    if (ng <= 1) {
+        for (int s = 0; s < nsteps; ++s) {
+            int r = twofluid_step<R>(c, c->L, nullptr);
+            if (r) return r;
+        }
+        return GFM_OK;
     }
+    const bool stagger = nsteps > 1;
+    HC(hipEventRecord(c->ev_fork, c->stream));
+    for (int s = 0; s < nsteps; ++s)
+        for (size_t g = 0; g < ng; ++g) {
+            if (s == 0 && g > 0) HC(hipStreamWaitEvent(c->gstream[g], stagger ? c->ev_stag[g - 1] : c->ev_fork, 0));
+            int r = twofluid_step<R>(c, c->GL[g], (stagger && s == 0 && g + 1 < ng) ? c->ev_stag[g] : nullptr);
+            if (r) return r;
+        }
+    for (size_t g = 1; g < ng; ++g) {
+        HC(hipEventRecord(c->ev_join[g], c->gstream[g]));
+        HC(hipStreamWaitEvent(c->stream, c->ev_join[g], 0));
+    }
+    return GFM_OK;
+}
+
+// nsteps steps of sim_twofluid::run_sim (sim_twofluid.cpp:64-88) or one step
+// of sim_onefluid::run_sim (sim_onefluid.cpp:40-54) on conserved buffer cur
+template <typename R>
+int step_launches(gfm_ctx* c, int cur, int nsteps = 1) {
+    if (c->twofluid()) return twofluid_steps<R>(c, nsteps);
+    const bool muscl = c->m.fs == GFM_FS_MUSCL;
+    gfm::launch_dt<R>(c->L, 0, cur, 0, 0);
+    gfm::launch_fs<R>(c->L, muscl, cur, 1 - cur, 1);
+    gfm::launch_bc<R>(c->L, 1 - cur, 1, 1);
+    return GFM_OK;
 }
 
 int copy_q(gfm_ctx* c, int from, int to) {
@@ -290,6 +346,105 @@ static int rs_common(int n, const double* UL, const double* UR, double* out, uns
     return GFM_OK;
 }
 
+namespace {
+
+// work tables of a set of instances, appended to `all`
+struct TableOffsets {
+    size_t o[9];
+    int n[9];
+};
+
+TableOffsets append_tables(const gfm_ctx* c, const std::vector<Inst>& descs, const std::vector<int>& members,
+                           std::vector<int4>& all) {
+    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich, w_tb, w_rt, w_seg;
+    const int nf = c->nfluid();
+    for (int ki : members) {
+        const Inst& d = descs[ki];
+        const gfm_sim_info& s = c->inst[ki].info;
+        const int fx = (d.W + 63) / 64, fy = (d.H + 3) / 4;
+        for (int ty = 0; ty < fy; ++ty)
+            for (int tx = 0; tx < fx; ++tx) w_full.push_back(make_int4(ki, ty, tx, 0));
+        const int tix = (s.Nx + 63) / 64, tiy = (s.Ny + 3) / 4;
+        int li = 0;
+        for (int ty = 0; ty < tiy; ++ty)
+            for (int tx = 0; tx < tix; ++tx) w_int.push_back(make_int4(ki, ty, tx, li++));
+        const int sx = (s.Nx + 31) / 32, sy = (s.Ny + 7) / 8;
+        for (int f = 0; f < nf; ++f)
+            for (int ty = 0; ty < sy; ++ty)
+                for (int tx = 0; tx < sx; ++tx) w_fs.push_back(make_int4(ki, ty, tx, f));
+        const int strip_n[4] = {d.g * d.W, d.g * d.W, d.g * d.Ny, d.g * d.Ny};
+        for (int st = 0; st < 4; ++st)
+            for (int o = 0; o < strip_n[st]; o += 256) w_frame.push_back(make_int4(ki, st, o, 0));
+        w_inst.push_back(make_int4(ki, 0, 0, 0));
+        if (c->twofluid())
+            for (int ty = 0; ty < (s.Ny + 15) / 16; ++ty)
+                for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_tb.push_back(make_int4(ki, ty, tx, 0));
+        {
+            const int nsx = (s.Nx + 255) / 256;
+            for (int r = 0; r < s.Ny; ++r)
+                for (int x = 0; x < nsx; ++x) w_seg.push_back(make_int4(ki, r, x, r * nsx + x));
+        }
+        if (c->twofluid())
+            for (int ty = 0; ty < (s.Ny + 31) / 32; ++ty)
+                for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_rt.push_back(make_int4(ki, ty, tx, 0));
+        if (c->twofluid())
+            for (int o = 0; o < d.icap; o += 256) w_ich.push_back(make_int4(ki, o, 0, 0));
+    }
+    xcd_order(w_full, 0);
+    xcd_order(w_fs, 0);
+    xcd_order(w_tb, 0);
+    xcd_order(w_rt, 0);
+    if (w_ich.empty()) w_ich.push_back(make_int4(members.empty() ? 0 : members[0], 1 << 30, 0, 0));
+    const std::vector<int4>* t[9] = {&w_full, &w_int, &w_fs, &w_frame, &w_inst, &w_ich, &w_tb, &w_rt, &w_seg};
+    TableOffsets r{};
+    for (int q = 0; q < 9; ++q) {
+        r.o[q] = all.size();
+        r.n[q] = (int)t[q]->size();
+        all.insert(all.end(), t[q]->begin(), t[q]->end());
+    }
+    return r;
+}
+
+gfm::LaunchCtx launch_ctx(const gfm_ctx* c, const TableOffsets& t, hipStream_t stream, int iface_plain) {
+    gfm::LaunchCtx L{};
+    L.stream = stream;
+    L.insts = c->d_insts;
+    int4* w = c->d_work;
+    L.w_full = w + t.o[0]; L.n_full = t.n[0];
+    L.w_int = w + t.o[1]; L.n_int = t.n[1];
+    L.w_fs = w + t.o[2]; L.n_fs = t.n[2];
+    L.w_frame = w + t.o[3]; L.n_frame = t.n[3];
+    L.w_inst = w + t.o[4]; L.n_inst = t.n[4];
+    L.w_ich = w + t.o[5]; L.n_ich = t.n[5];
+    L.w_tb = w + t.o[6]; L.n_tb = t.n[6];
+    L.w_rt = w + t.o[7]; L.n_rt = t.n[7];
+    L.w_seg = w + t.o[8]; L.n_seg = t.n[8];
+    L.tb = c->tb ? 1 : 0;
+    L.iface_plain = iface_plain;
+    return L;
+}
+
+}  // namespace
+
+namespace {
+
+// graph of `launch`'s work, captured on `stream`
+template <typename F>
+int capture(hipStream_t stream, hipGraphExec_t* out, F launch) {
+    hipGraph_t g = nullptr;
+    HC(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int r = launch();
+    hipError_t ec = hipStreamEndCapture(stream, &g);
+    if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+    if (ec != hipSuccess) return fail("hipStreamEndCapture", ec);
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail("hipGraphInstantiate", e);
+    return GFM_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* gfm_last_error(void) { return g_err.c_str(); }
@@ -321,10 +476,15 @@ int gfm_ctx_destroy(gfm_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& g : c->gexec)
         if (g) (void)hipGraphExecDestroy(g);
+    if (c->gexec_multi) (void)hipGraphExecDestroy(c->gexec_multi);
     for (auto& h : c->inst)
         if (h.mem) (void)hipFree(h.mem);
     if (c->d_insts) (void)hipFree(c->d_insts);
     if (c->d_work) (void)hipFree(c->d_work);
+    for (size_t g = 1; g < c->gstream.size(); ++g) (void)hipStreamDestroy(c->gstream[g]);
+    for (size_t g = 1; g < c->ev_join.size(); ++g) (void)hipEventDestroy(c->ev_join[g]);
+    for (auto ev : c->ev_stag) (void)hipEventDestroy(ev);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GFM_OK;
@@ -365,7 +525,6 @@ int gfm_ctx_commit(gfm_ctx* c) {
     if (c->committed) return GFM_OK;
     if (c->inst.empty()) return fail_arg("no instances");
     const size_t es = c->es();
-    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich, w_tb, w_rt, w_seg;
     std::vector<Inst> descs;
     bool periodic = false;
     for (size_t k = 0; k < c->inst.size(); ++k) {
@@ -385,7 +544,6 @@ int gfm_ctx_commit(gfm_ctx* c) {
         for (int f = 0; f < 2; ++f) gfm::rs_fluid(d.rsf[f], d.gam[f], d.pinf[f]);
         d.cfl = h.cfl; d.T = h.T; d.max_steps = h.max_steps;
         d.use_bbox = 1;   // boxes are reset to the full domain each step unless OGFM narrows them
-        const int tix = (s.Nx + 63) / 64, tiy = (s.Ny + 3) / 4;
         d.ntile_dt = s.Ny * ((s.Nx + 255) / 256);   // dt partials: one per 256-cell row segment
         d.icap = c->twofluid() ? std::min(std::min(s.Nx * s.Ny, 16 * (s.Nx + s.Ny) + 1024), 49152) : 1;
         // layout of the single allocation
@@ -437,41 +595,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.si = (int*)(base + osi);
         h.d = d;
         descs.push_back(d);
-        // work tables
-        const int ki = (int)k;
-        const int fx = (d.W + 63) / 64, fy = (d.H + 3) / 4;
-        for (int ty = 0; ty < fy; ++ty)
-            for (int tx = 0; tx < fx; ++tx) w_full.push_back(make_int4(ki, ty, tx, 0));
-        int li = 0;
-        for (int ty = 0; ty < tiy; ++ty)
-            for (int tx = 0; tx < tix; ++tx) w_int.push_back(make_int4(ki, ty, tx, li++));
-        const int sx = (s.Nx + 31) / 32, sy = (s.Ny + 7) / 8;
-        for (int f = 0; f < nf; ++f)
-            for (int ty = 0; ty < sy; ++ty)
-                for (int tx = 0; tx < sx; ++tx) w_fs.push_back(make_int4(ki, ty, tx, f));
-        const int strip_n[4] = {d.g * d.W, d.g * d.W, d.g * d.Ny, d.g * d.Ny};
-        for (int st = 0; st < 4; ++st)
-            for (int o = 0; o < strip_n[st]; o += 256) w_frame.push_back(make_int4(ki, st, o, 0));
-        w_inst.push_back(make_int4(ki, 0, 0, 0));
-        if (c->twofluid())
-            for (int ty = 0; ty < (s.Ny + 15) / 16; ++ty)
-                for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_tb.push_back(make_int4(ki, ty, tx, 0));
-        {
-            const int nsx = (s.Nx + 255) / 256;
-            for (int r = 0; r < s.Ny; ++r)
-                for (int sx = 0; sx < nsx; ++sx) w_seg.push_back(make_int4(ki, r, sx, r * nsx + sx));
-        }
-        if (c->twofluid())
-            for (int ty = 0; ty < (s.Ny + 31) / 32; ++ty)
-                for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_rt.push_back(make_int4(ki, ty, tx, 0));
         if (s.BC_L == GFM_BC_PERIODIC || s.BC_B == GFM_BC_PERIODIC) periodic = true;
-        if (c->twofluid())
-            for (int o = 0; o < d.icap; o += 256) w_ich.push_back(make_int4(ki, o, 0, 0));
     }
-    xcd_order(w_full, 0);
-    xcd_order(w_fs, 0);
-    xcd_order(w_tb, 0);
-    xcd_order(w_rt, 0);
     const char* no_tb = std::getenv("GFM_NO_TEMPORAL_BLOCKING");
     c->tb = c->twofluid() && !periodic && !(no_tb && no_tb[0] == '1');
     // GFM_NO_TEMPORAL_BLOCKING=1 / GFM_IFACE_PLAIN_ROUNDS=1 select the simpler
@@ -480,38 +605,64 @@ int gfm_ctx_commit(gfm_ctx* c) {
     const int iface_plain = (plain && plain[0] == '1') ? 1 : 0;
     HC(hipMalloc((void**)&c->d_insts, descs.size() * sizeof(Inst)));
     HC(hipMemcpyAsync(c->d_insts, descs.data(), descs.size() * sizeof(Inst), hipMemcpyHostToDevice, c->stream));
-    if (w_ich.empty()) w_ich.push_back(make_int4(0, 1 << 30, 0, 0));
-    const size_t nw =
-        w_full.size() + w_int.size() + w_fs.size() + w_frame.size() + w_inst.size() + w_ich.size() + w_tb.size() +
-        w_rt.size() + w_seg.size();
-    HC(hipMalloc((void**)&c->d_work, nw * sizeof(int4)));
+    // step groups: GFM_STEP_GROUPS (default 4) groups of instances with about
+    // equal cell counts, largest instances first
+    const int ninst = (int)c->inst.size();
+    int ng = 1;
+    if (c->twofluid()) {
+        const char* eg = std::getenv("GFM_STEP_GROUPS");
+        ng = eg ? std::atoi(eg) : 4;
+        ng = std::max(1, std::min(ng, std::min(ninst, 4)));
+    }
+    std::vector<std::vector<int>> members(ng);
+    {
+        std::vector<int> order(ninst);
+        for (int k = 0; k < ninst; ++k) order[k] = k;
+        auto cells = [&](int k) { return (long long)c->inst[k].info.Nx * c->inst[k].info.Ny; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
+        std::vector<long long> load(ng, 0);
+        for (int k : order) {
+            const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            members[g].push_back(k);
+            load[g] += cells(k);
+        }
+        for (auto& m : members) std::sort(m.begin(), m.end());
+    }
+    std::vector<int> everyone(ninst);
+    for (int k = 0; k < ninst; ++k) everyone[k] = k;
+    // device work tables: the whole context's, then each group's (if more than one)
     std::vector<int4> all;
-    all.reserve(nw);
-    all.insert(all.end(), w_full.begin(), w_full.end());
-    all.insert(all.end(), w_int.begin(), w_int.end());
-    all.insert(all.end(), w_fs.begin(), w_fs.end());
-    all.insert(all.end(), w_frame.begin(), w_frame.end());
-    all.insert(all.end(), w_inst.begin(), w_inst.end());
-    all.insert(all.end(), w_ich.begin(), w_ich.end());
-    all.insert(all.end(), w_tb.begin(), w_tb.end());
-    all.insert(all.end(), w_rt.begin(), w_rt.end());
-    all.insert(all.end(), w_seg.begin(), w_seg.end());
-    HC(hipMemcpyAsync(c->d_work, all.data(), nw * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-    gfm::LaunchCtx& L = c->L;
-    L.stream = c->stream;
-    L.insts = c->d_insts;
-    size_t o = 0;
-    L.w_full = c->d_work + o; L.n_full = (int)w_full.size(); o += w_full.size();
-    L.w_int = c->d_work + o; L.n_int = (int)w_int.size(); o += w_int.size();
-    L.w_fs = c->d_work + o; L.n_fs = (int)w_fs.size(); o += w_fs.size();
-    L.w_frame = c->d_work + o; L.n_frame = (int)w_frame.size(); o += w_frame.size();
-    L.w_inst = c->d_work + o; L.n_inst = (int)w_inst.size(); o += w_inst.size();
-    L.w_ich = c->d_work + o; L.n_ich = (int)w_ich.size(); o += w_ich.size();
-    L.w_tb = c->d_work + o; L.n_tb = (int)w_tb.size(); o += w_tb.size();
-    L.w_rt = c->d_work + o; L.n_rt = (int)w_rt.size(); o += w_rt.size();
-    L.w_seg = c->d_work + o; L.n_seg = (int)w_seg.size();
-    L.tb = c->tb ? 1 : 0;
-    L.iface_plain = iface_plain;
+    std::vector<TableOffsets> offs;
+    offs.push_back(append_tables(c, descs, everyone, all));
+    if (ng > 1)
+        for (int g = 0; g < ng; ++g) offs.push_back(append_tables(c, descs, members[g], all));
+    HC(hipMalloc((void**)&c->d_work, all.size() * sizeof(int4)));
+    HC(hipMemcpyAsync(c->d_work, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    c->L = launch_ctx(c, offs[0], c->stream, iface_plain);
+    {
+        // steps per graph of a grouped context (GFM_GRAPH_STEPS, default 100)
+        const char* gs = std::getenv("GFM_GRAPH_STEPS");
+        c->graph_steps = std::max(2, gs ? std::atoi(gs) : 100);
+    }
+    c->GL.clear();
+    c->gstream.assign(1, c->stream);
+    c->ev_join.assign(1, nullptr);
+    if (ng > 1) {
+        HC(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        c->ev_stag.assign(ng - 1, nullptr);
+        for (auto& ev : c->ev_stag) HC(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        for (int g = 0; g < ng; ++g) {
+            if (g > 0) {
+                hipStream_t st = nullptr;
+                hipEvent_t ev = nullptr;
+                HC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+                c->gstream.push_back(st);
+                HC(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                c->ev_join.push_back(ev);
+            }
+            c->GL.push_back(launch_ctx(c, offs[1 + g], c->gstream[g], iface_plain));
+        }
+    }
     // initial scalars: t = 0, step 0, active
     for (auto& h : c->inst) {
         int si[gfm::SI_N] = {0};
@@ -526,13 +677,6 @@ int gfm_ctx_commit(gfm_ctx* c) {
     c->committed = true;
     return GFM_OK;
 }
-
-
-
-
-
-
-
 
 
 int gfm_upload_state(gfm_ctx* c, int inst, const double* g1, const double* g2, const double* phi) {
@@ -612,7 +756,6 @@ int gfm_get_time(gfm_ctx* c, int inst, double* t, int* nstep, double* dt) {
 }
 
 
-
 int gfm_status(gfm_ctx* c, int inst, unsigned* flags) {
     int r = check_inst(c, inst);
     if (r) return r;
@@ -683,9 +826,6 @@ int gfm_mass_sums(gfm_ctx* c, int inst, double* out2) {
 }
 
 
-
-
-
 int gfm_compute_dt(gfm_ctx* c) {
     int r = ready(c);
     if (r) return r;
@@ -699,8 +839,8 @@ int gfm_set_ghost_states(gfm_ctx* c) {
     int r = ready(c);
     if (r) return r;
     if (!c->twofluid()) return fail_arg("ghost states need a two-fluid context");
-    if (c->mixed()) { gfm::launch_reset<float>(c->L, c->narrow()); ghost_chain<float>(c); }
-    else { gfm::launch_reset<double>(c->L, c->narrow()); ghost_chain<double>(c); }
+    if (c->mixed()) { gfm::launch_reset<float>(c->L, c->narrow()); ghost_chain<float>(c, c->L); }
+    else { gfm::launch_reset<double>(c->L, c->narrow()); ghost_chain<double>(c, c->L); }
     if ((r = post_launch(c))) return r;
     // keep the phase-by-phase state in buffer 0 (but not the OGFM boxes,
     // which live in the scalars)
@@ -765,10 +905,15 @@ int gfm_extrapolate(gfm_ctx* c, int N) {
 int gfm_step(gfm_ctx* c, int nsteps) {
     int r = ready(c);
     if (r) return r;
+    if (c->twofluid()) {
+        r = c->mixed() ? step_launches<float>(c, 0, nsteps) : step_launches<double>(c, 0, nsteps);
+        if (r) return r;
+        return post_launch(c);
+    }
     for (int s = 0; s < nsteps; ++s) {
-        if (c->mixed()) step_launches<float>(c, c->cur);
-        else step_launches<double>(c, c->cur);
-        if (!c->twofluid()) c->cur = 1 - c->cur;
+        r = c->mixed() ? step_launches<float>(c, c->cur) : step_launches<double>(c, c->cur);
+        if (r) return r;
+        c->cur = 1 - c->cur;
     }
     return post_launch(c);
 }
@@ -776,19 +921,25 @@ int gfm_step(gfm_ctx* c, int nsteps) {
 int gfm_step_graph(gfm_ctx* c, int nsteps) {
     int r = ready(c);
     if (r) return r;
+    if (nsteps <= 0) return GFM_OK;
     const int ng = c->twofluid() ? 1 : 2;
-    for (int k = 0; k < ng; ++k) {
-        if (c->gexec[k]) continue;
-        hipGraph_t g = nullptr;
-        HC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        if (c->mixed()) step_launches<float>(c, k);
-        else step_launches<double>(c, k);
-        HC(hipStreamEndCapture(c->stream, &g));
-        hipError_t e = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return fail("hipGraphInstantiate", e);
+    for (int k = 0; k < ng; ++k)
+        if (!c->gexec[k] && (r = capture(c->stream, &c->gexec[k], [&]() {
+                return c->mixed() ? step_launches<float>(c, k) : step_launches<double>(c, k);
+            })))
+            return r;
+    int s = 0;
+    if (c->twofluid() && c->GL.size() > 1 && nsteps >= c->graph_steps) {
+        // instance groups: blocks of graph_steps steps in one graph, in which
+        // the groups run staggered and independent (twofluid_steps)
+        if (!c->gexec_multi && (r = capture(c->stream, &c->gexec_multi, [&]() {
+                return c->mixed() ? step_launches<float>(c, 0, c->graph_steps)
+                                  : step_launches<double>(c, 0, c->graph_steps);
+            })))
+            return r;
+        for (; s + c->graph_steps <= nsteps; s += c->graph_steps) HC(hipGraphLaunch(c->gexec_multi, c->stream));
     }
-    for (int s = 0; s < nsteps; ++s) {
+    for (; s < nsteps; ++s) {
         const int k = c->twofluid() ? 0 : c->cur;
         HC(hipGraphLaunch(c->gexec[k], c->stream));
         if (!c->twofluid()) c->cur = 1 - c->cur;
@@ -811,7 +962,7 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
             if (c->mixed()) {
                 gfm::launch_dt<float>(c->L, 1, 0, c->narrow(), c->m.gfm != GFM_ORIGINAL);
                 HC(hipEventRecord(ev[1], c->stream));
-                ghost_chain<float>(c, c->m.gfm != GFM_ORIGINAL);
+                ghost_chain<float>(c, c->L, c->m.gfm != GFM_ORIGINAL);
                 HC(hipEventRecord(ev[2], c->stream));
                 gfm::launch_fs<float>(c->L, muscl, 1, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));
@@ -821,7 +972,7 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
             } else {
                 gfm::launch_dt<double>(c->L, 1, 0, c->narrow(), c->m.gfm != GFM_ORIGINAL);
                 HC(hipEventRecord(ev[1], c->stream));
-                ghost_chain<double>(c, c->m.gfm != GFM_ORIGINAL);
+                ghost_chain<double>(c, c->L, c->m.gfm != GFM_ORIGINAL);
                 HC(hipEventRecord(ev[2], c->stream));
                 gfm::launch_fs<double>(c->L, muscl, 1, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));
@@ -859,7 +1010,6 @@ int gfm_sync(gfm_ctx* c) {
     HC(hipStreamSynchronize(c->stream));
     return GFM_OK;
 }
-
 
 
 int gfm_pure_rs_solve(int n, int paralleltox, double gamma, double pinf, const double* UL, const double* UR,

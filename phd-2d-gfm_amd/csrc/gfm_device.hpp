@@ -18,7 +18,9 @@ enum { SD_T = 0, SD_DT = 1, SD_MASS1 = 2, SD_MASS2 = 3, SD_N = 8 };
 enum {
     SI_NSTEP = 0, SI_ACTIVE = 1, SI_NIFACE = 2, SI_DTCOUNT = 3,
     SI_BBOX = 4,   // 8 ints: fluid1 imin,imax,jmin,jmax ; fluid2 ...
-    SI_STATUS = 12, SI_ROUNDS = 13, SI_N = 16
+    SI_STATUS = 12, SI_ROUNDS = 13,
+    SI_NLIST = 14,   // entries of the last interfacial list recorded in imap
+    SI_N = 16
 };
 enum { ST_UNPHYS = 1, ST_RS = 2, ST_IOVF = 4, ST_IROUNDS = 8 };
 

@@ -57,51 +57,88 @@ __device__ __forceinline__ void reset_boxes(const Inst& I, int narrow) {
 }
 
 // ---------------------------------------------------------------------------
+// Row segments: the CFL/mass pass and the interfacial list work on 256-cell
+// row segments, one segment per wavefront and four cells per lane (cell
+// q*64 + lane of the segment, q = 0..3), so that every lane has its loads for
+// four cells in flight at once.
+// work: {inst, interior row, segment column, segment index within the instance}
+// ---------------------------------------------------------------------------
+constexpr int SEGW = 256, SEG_PER_WG = 4, SEG_Q = SEGW / 64;
+
+// the interfacial list of the previous build is erased from imap entry by
+// entry (imap is -1 everywhere else); wavefront s erases list entries
+// [s SEGW, (s+1) SEGW), which covers the list since it has at most one entry
+// per interior cell
+__device__ __forceinline__ void erase_prev_list(const Inst& I, int s, int lane) {
+    const int n = I.si[SI_NLIST];
+#pragma unroll
+    for (int q = 0; q < SEG_Q; ++q) {
+        const int k = s * SEGW + q * 64 + lane;
+        if (k < n) {
+            const int v = I.ilist[k];
+            const int i = v / I.W;
+            I.imap[cidx(I, i, v - i * I.W)] = -1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // A14 + A16: CFL dt (sim_twofluid.cpp:108-157, sim_onefluid.cpp:68-91) and
-// fluid masses (sim_twofluid.cpp:1803-1827).  Block partials, then the last
-// block of each instance reduces them in tile order (deterministic) and also
-// resets the per-step counters used later in the step.
-// work: {inst, interior row, segment column, segment index}, 256-cell row
-// segments.  With `count` it also counts each segment's interfacial cells
-// for the interfacial list (k_iface_count's work, from the same phi).
+// fluid masses (sim_twofluid.cpp:1803-1827).  Segment partials, then one
+// block per instance reduces them in segment order (deterministic) and also
+// resets the per-step counters used later in the step.  With `count` it also
+// counts each segment's interfacial cells for the interfacial list
+// (k_iface_count's work, from the same phi) and erases the previous list.
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, const int4* __restrict__ work,
-                                            int twofluid, int q, int count) {
-    const int4 w = work[blockIdx.x];
+                                            int nwork, int twofluid, int q, int count) {
+    const int sw = blockIdx.x * SEG_PER_WG + (int)(threadIdx.x >> 6);
+    if (sw >= nwork) return;
+    const int4 w = work[sw];
     const Inst& I = insts[w.x];
+    const int lane = threadIdx.x & 63;
+    const int i = I.g + w.y, j0 = I.g + w.z * SEGW + lane, jend = I.g + I.Nx;
+    if (count) erase_prev_list(I, w.w, lane);
+    const long long row = cidx(I, i, 0);
+    R ph[SEG_Q];
+    V4<R> U[SEG_Q];
+#pragma unroll
+    for (int k = 0; k < SEG_Q; ++k) ph[k] = (twofluid && j0 + 64 * k < jend) ? ld1<R>(I.phi, row + j0 + 64 * k) : R(0);
+#pragma unroll
+    for (int k = 0; k < SEG_Q; ++k) {
+        const int j = j0 + 64 * k;
+        const int f = ph[k] < R(0) ? 0 : 1;   // :128 strict
+        U[k] = ld4<R>(I.Q[twofluid ? f : 0][q], I, row + (j < jend ? j : I.g));
+    }
     double mu = 0.0, mv = 0.0, m1 = 0.0, m2 = 0.0;
-    bool iface = false;
-    {
-        const int i = I.g + w.y;
-        const int j = I.g + w.z * 256 + (int)threadIdx.x;
-        if (j < I.g + I.Nx) {
-            const long long c = cidx(I, i, j);
-            int f = 0;
-            double phi = 0.0;
+    const double dA = I.dx * I.dy;
+    int nc = 0;
+#pragma unroll
+    for (int k = 0; k < SEG_Q; ++k) {
+        const int j = j0 + 64 * k;
+        bool iface = false;
+        if (j < jend) {
+            const double phi = (double)ph[k];
+            const int f = twofluid ? (phi < 0.0 ? 0 : 1) : 0;
+            const double U0 = U[k].v[0], U1 = U[k].v[1], U2 = U[k].v[2], U3 = U[k].v[3];
+            const double rho = U0;
+            const double u = U1 / rho, v = U2 / rho;
+            const double e = (U3 / rho) - 0.5 * (u * u + v * v);
+            const double p = eos_p(I.gam[f], I.pinf[f], e, rho);
+            const double cs = eos_c(I.gam[f], I.pinf[f], p, rho);
+            mu = fmax(mu, fabs(U1) / rho + cs);
+            mv = fmax(mv, fabs(U2) / rho + cs);
             if (twofluid) {
-                phi = (double)ld1<R>(I.phi, c);
-                f = phi < 0.0 ? 0 : 1;   // :128 strict
-            }
-            V4<R> Ur = ld4<R>(I.Q[f][q], I, c);
-            double U0 = Ur.v[0], U1 = Ur.v[1], U2 = Ur.v[2], U3 = Ur.v[3];
-            double rho = U0;
-            double u = U1 / rho, v = U2 / rho;
-            double e = (U3 / rho) - 0.5 * (u * u + v * v);
-            double p = eos_p(I.gam[f], I.pinf[f], e, rho);
-            double cs = eos_c(I.gam[f], I.pinf[f], p, rho);
-            mu = fabs(U1) / rho + cs;
-            mv = fabs(U2) / rho + cs;
-            if (twofluid) {
-                const double dA = I.dx * I.dy;
-                if (-phi > 0.0) m1 = (double)ld4<R>(I.Q[0][q], I, c).v[0] * dA;
-                if (phi > 0.0) m2 = (double)ld4<R>(I.Q[1][q], I, c).v[0] * dA;
+                // the density of the fluid whose mass is summed is the loaded one
+                if (-phi > 0.0) m1 += U0 * dA;
+                if (phi > 0.0) m2 += U0 * dA;
             }
             if (count) iface = is_iface<R>(I, i, j);
         }
+        if (count) nc += __popcll(__ballot(iface));
     }
-    // fixed-order reductions: a butterfly within each wavefront, then the
-    // four wavefronts in order
+    // fixed-order reduction: a butterfly within the wavefront
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         mu = fmax(mu, __shfl_xor(mu, off));
@@ -109,25 +146,9 @@ __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, cons
         m1 += __shfl_xor(m1, off);
         m2 += __shfl_xor(m2, off);
     }
-    const unsigned long long mask = __ballot(iface);
-    __shared__ double red[4][4];
-    __shared__ int wc[4];
-    const int wv = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        red[wv][0] = mu; red[wv][1] = mv; red[wv][2] = m1; red[wv][3] = m2;
-        wc[wv] = __popcll(mask);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a0 = red[0][0], a1 = red[0][1], a2 = red[0][2], a3 = red[0][3];
-        int nc = wc[0];
-        for (int k = 1; k < 4; ++k) {
-            a0 = fmax(a0, red[k][0]); a1 = fmax(a1, red[k][1]);
-            a2 += red[k][2]; a3 += red[k][3];
-            nc += wc[k];
-        }
+    if (lane == 0) {
         double* pp = I.partials + 4 * (long long)w.w;
-        pp[0] = a0; pp[1] = a1; pp[2] = a2; pp[3] = a3;
+        pp[0] = mu; pp[1] = mv; pp[2] = m1; pp[3] = m2;
         if (count) I.iseg[w.w] = nc;
     }
 }
@@ -910,37 +931,42 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
 
 // The list is built in row-major cell order (the order of the reference's
 // i, j loops), so that every interfacial cell an entry depends on has a
-// smaller entry index: per 256-cell row segment a count (k_iface_count),
-// an exclusive scan of the counts per instance (k_iface_scan), then each
-// segment writes its cells in column order from its offset (k_iface_mark).
-// work (count, mark): {inst, interior row, segment column, segment index}
-constexpr int SEGW = 256;
+// smaller entry index: per row segment a count (k_iface_count, or k_dt with
+// `count`), an exclusive scan of the counts per instance (k_iface_scan), then
+// each segment that has interfacial cells writes them in column order from
+// its offset (k_iface_mark).  imap holds the list index of each listed cell
+// and -1 elsewhere; the previous list is erased entry by entry before the
+// count.
 
-// interfacial cells of one segment, as a per-wave mask and the rank of this
-// thread's cell among them
+// interfacial cells of a segment: bit q of the result is set for cell
+// q*64 + lane
 template <typename R>
-__device__ __forceinline__ bool seg_iface(const Inst& I, const int4& w, int& i, int& j) {
-    i = I.g + w.y;
-    j = I.g + w.z * SEGW + (int)threadIdx.x;
-    return j < I.g + I.Nx && is_iface<R>(I, i, j);
+__device__ __forceinline__ unsigned seg_iface(const Inst& I, const int4& w, int lane) {
+    const int i = I.g + w.y, j0 = I.g + w.z * SEGW + lane;
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < SEG_Q; ++k) {
+        const int j = j0 + 64 * k;
+        if (j < I.g + I.Nx && is_iface<R>(I, i, j)) m |= 1u << k;
+    }
+    return m;
 }
 
 template <typename R>
-__global__ __launch_bounds__(SEGW) void k_iface_count(const Inst* __restrict__ insts, const int4* __restrict__ work) {
-    __shared__ int wc[SEGW / 64];
-    const int4 w = work[blockIdx.x];
+__global__ __launch_bounds__(256) void k_iface_count(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                     int nwork) {
+    const int sw = blockIdx.x * SEG_PER_WG + (int)(threadIdx.x >> 6);
+    if (sw >= nwork) return;
+    const int4 w = work[sw];
     const Inst& I = insts[w.x];
+    const int lane = threadIdx.x & 63;
+    erase_prev_list(I, w.w, lane);
     if (!I.si[SI_ACTIVE]) return;
-    int i, j;
-    const bool f = seg_iface<R>(I, w, i, j);
-    const unsigned long long mask = __ballot(f);
-    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(mask);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int c = 0;
-        for (int q = 0; q < SEGW / 64; ++q) c += wc[q];
-        I.iseg[w.w] = c;
-    }
+    const unsigned m = seg_iface<R>(I, w, lane);
+    int nc = 0;
+#pragma unroll
+    for (int k = 0; k < SEG_Q; ++k) nc += __popcll(__ballot((m >> k) & 1u));
+    if (lane == 0) I.iseg[w.w] = nc;
 }
 
 // one workgroup per instance: exclusive scan of the segment counts in place;
@@ -968,31 +994,44 @@ __global__ __launch_bounds__(1024) void k_iface_scan(const Inst* __restrict__ in
         I.iseg[q] = acc;
         acc += c;
     }
-    if (tid == 1023) I.si[SI_NIFACE] = part[1023];
+    if (tid == 1023) {
+        I.si[SI_NIFACE] = part[1023];
+        I.si[SI_NLIST] = min(part[1023], I.icap);   // the entries k_iface_mark records
+    }
 }
 
 template <typename R>
-__global__ __launch_bounds__(SEGW) void k_iface_mark(const Inst* __restrict__ insts, const int4* __restrict__ work) {
-    __shared__ int wc[SEGW / 64];
-    const int4 w = work[blockIdx.x];
+__global__ __launch_bounds__(256) void k_iface_mark(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                    int nwork) {
+    const int sw = blockIdx.x * SEG_PER_WG + (int)(threadIdx.x >> 6);
+    if (sw >= nwork) return;
+    const int4 w = work[sw];
     const Inst& I = insts[w.x];
     if (!I.si[SI_ACTIVE]) return;
-    int i, j;
-    const bool f = seg_iface<R>(I, w, i, j);
-    const unsigned long long mask = __ballot(f);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) wc[wv] = __popcll(mask);
-    __syncthreads();
-    int k = I.iseg[w.w] + __popcll(mask & ((1ull << lane) - 1ull));
-    for (int q = 0; q < wv; ++q) k += wc[q];
-    if (j >= I.g + I.Nx) return;
-    if (f) {
-        if (k < I.icap) I.ilist[k] = i * I.W + j;
-        else { k = -1; flag(I, ST_IOVF); }
-    } else {
-        k = -1;
+    const int lane = threadIdx.x & 63;
+    int k = I.iseg[w.w];
+    const int next = w.w + 1 < I.nseg ? I.iseg[w.w + 1] : I.si[SI_NIFACE];
+    if (next == k) return;   // no interfacial cell in this segment
+    const unsigned m = seg_iface<R>(I, w, lane);
+    const int i = I.g + w.y;
+    bool ovf = false;
+#pragma unroll
+    for (int q = 0; q < SEG_Q; ++q) {
+        const bool f = (m >> q) & 1u;
+        const unsigned long long mask = __ballot(f);
+        if (f) {
+            const int kk = k + __popcll(mask & ((1ull << lane) - 1ull));
+            const int j = I.g + w.z * SEGW + q * 64 + lane;
+            if (kk < I.icap) {
+                I.ilist[kk] = i * I.W + j;
+                I.imap[cidx(I, i, j)] = kk;
+            } else {
+                ovf = true;
+            }
+        }
+        k += __popcll(mask);
     }
-    I.imap[cidx(I, i, j)] = k;
+    if (ovf) flag(I, ST_IOVF);
 }
 
 // ---------------------------------------------------------------------------
@@ -1929,8 +1968,8 @@ __global__ void k_reset(const Inst* __restrict__ insts, const int4* __restrict__
 
 template <typename R>
 void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes, int count_iface) {
-    hipLaunchKernelGGL((k_dt<R>), GRID(L.n_seg), dim3(256), 0, L.stream, L.insts, L.w_seg, twofluid, q,
-                       count_iface);
+    hipLaunchKernelGGL((k_dt<R>), GRID((L.n_seg + SEG_PER_WG - 1) / SEG_PER_WG), dim3(256), 0, L.stream, L.insts,
+                       L.w_seg, L.n_seg, twofluid, q, count_iface);
     hipLaunchKernelGGL(k_dt_final, GRID(L.n_inst), dim3(256), 0, L.stream, L.insts, L.w_inst, twofluid, narrow_boxes);
 }
 
@@ -1988,9 +2027,11 @@ bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, i
 
 template <typename R>
 void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted) {
-    if (!counted) hipLaunchKernelGGL((k_iface_count<R>), GRID(L.n_seg), dim3(SEGW), 0, L.stream, L.insts, L.w_seg);
+    const unsigned nsb = (unsigned)((L.n_seg + SEG_PER_WG - 1) / SEG_PER_WG);
+    if (!counted)
+        hipLaunchKernelGGL((k_iface_count<R>), dim3(nsb), dim3(256), 0, L.stream, L.insts, L.w_seg, L.n_seg);
     hipLaunchKernelGGL(k_iface_scan, GRID(L.n_inst), dim3(1024), 0, L.stream, L.insts, L.w_inst);
-    hipLaunchKernelGGL((k_iface_mark<R>), GRID(L.n_seg), dim3(SEGW), 0, L.stream, L.insts, L.w_seg);
+    hipLaunchKernelGGL((k_iface_mark<R>), dim3(nsb), dim3(256), 0, L.stream, L.insts, L.w_seg, L.n_seg);
     switch (variant) {
         case V_REAL:
             hipLaunchKernelGGL((k_iface_prep<R, V_REAL>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich, pbuf);

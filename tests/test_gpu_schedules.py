@@ -9,6 +9,8 @@
    context, and full steps.
 2. The interfacial solve ordered by dependency level against plain rounds
    (solve whatever is ready, repeat), for the variants with dependencies.
+3. Steps of a batch split into instance groups on separate streams (with
+   multi-step graphs and staggered group starts) against one stream.
 """
 import os
 
@@ -25,8 +27,9 @@ T, R = G.BC_TRANSMISSIVE, G.BC_REFLECTIVE
 BCS = [(T, T, R, R), (R, R, T, T), (R, T, R, T), (T, R, T, R)]
 
 
-def make_ctx(cases, variant, precision, blocked, plain_rounds=False):
-    env = {"GFM_NO_TEMPORAL_BLOCKING": "0" if blocked else "1", "GFM_IFACE_PLAIN_ROUNDS": "1" if plain_rounds else "0"}
+def make_ctx(cases, variant, precision, blocked, plain_rounds=False, groups=4):
+    env = {"GFM_NO_TEMPORAL_BLOCKING": "0" if blocked else "1", "GFM_IFACE_PLAIN_ROUNDS": "1" if plain_rounds else "0",
+           "GFM_STEP_GROUPS": str(groups)}
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
@@ -140,3 +143,23 @@ def test_reinit_blocked_matches_per_iteration(iters, precision):
         c.sync()
     for k in range(len(cases)):
         assert_same(ca.download(k)[2], cb.download(k)[2], "instance %d phi" % k)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_instance_groups_match_one_stream(variant):
+    """seven instances of different sizes in four groups: 105 steps (a
+    staggered first step, one 100-step graph and four one-step graphs per
+    group), then 7 eager steps"""
+    print("CASE groups variant=%d" % variant, flush=True)
+    sizes = [(96, 40), (64, 24), (130, 33), (72, 72), (40, 20), (110, 28), (64, 40)]
+    cases = [perturbed_bubble(nx, ny, BCS[k % len(BCS)], 20 + k) for k, (nx, ny) in enumerate(sizes)]
+    ca = make_ctx(cases, variant, G.MIXED, True, groups=4)
+    cb = make_ctx(cases, variant, G.MIXED, True, groups=1)
+    for c in (ca, cb):
+        c.step_graph(105)
+        c.step(7)
+        c.sync()
+    compare(ca, cb, len(cases))
+    for k in range(len(cases)):
+        assert ca.get_time(k) == cb.get_time(k)
+        assert ca.get_time(k)[1] == 112
