@@ -151,6 +151,11 @@ int gfm_pure_rs_solve(int n, int paralleltox, double gamma, double pinf,
                       const double* UL, const double* UR, double* flux, unsigned* status);
 int gfm_mixed_rs_solve(int n, double gammaL, double pinfL, double gammaR, double pinfR,
                        const double* UL, const double* UR, double* out4, unsigned* status);
+/* pure_RS_base::solve_RP as the mixed-precision flux update evaluates it:
+ * the conserved states rounded to FP32, the exact solver iterated in FP32,
+ * the FP32 flux returned widened to double. */
+int gfm_pure_rs_solve_f32(int n, int paralleltox, double gamma, double pinf,
+                          const double* UL, const double* UR, double* flux, unsigned* status);
 
 /* initial conditions of the settings-file test cases (host code, no GPU):
  * sim_onefluid::set_sim_ICs (sim_onefluid.cpp:196-422; test_case 1..5 =

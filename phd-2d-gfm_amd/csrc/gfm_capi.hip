@@ -320,7 +320,8 @@ static int post_launch(gfm_ctx* c) {
     return GFM_OK;
 }
 
-static int rs_common(int n, const double* UL, const double* UR, double* out, unsigned* status, bool mixed,
+// kind: 0 pure (FP64), 1 mixed, 2 pure with the mixed-precision face solver
+static int rs_common(int n, const double* UL, const double* UR, double* out, unsigned* status, int kind,
                      int px, double a, double b, double cc, double d) {
     if (n < 0 || (n > 0 && (!UL || !UR || !out))) return fail_arg("bad arguments");
     if (n == 0) { if (status) *status = 0; return GFM_OK; }
@@ -334,7 +335,8 @@ static int rs_common(int n, const double* UL, const double* UR, double* out, uns
     HC(hipMemcpy(dL, UL, bytes, hipMemcpyHostToDevice));
     HC(hipMemcpy(dR, UR, bytes, hipMemcpyHostToDevice));
     HC(hipMemset(dS, 0, sizeof(unsigned)));
-    if (mixed) gfm::launch_mixed_rs(nullptr, n, a, b, cc, d, dL, dR, dO, dS);
+    if (kind == 1) gfm::launch_mixed_rs(nullptr, n, a, b, cc, d, dL, dR, dO, dS);
+    else if (kind == 2) gfm::launch_pure_rs_f32(nullptr, n, px, a, b, dL, dR, dO, dS);
     else gfm::launch_pure_rs(nullptr, n, px, a, b, dL, dR, dO, dS);
     HC(hipGetLastError());
     HC(hipDeviceSynchronize());
@@ -1014,12 +1016,17 @@ int gfm_sync(gfm_ctx* c) {
 
 int gfm_pure_rs_solve(int n, int paralleltox, double gamma, double pinf, const double* UL, const double* UR,
                       double* flux, unsigned* status) {
-    return rs_common(n, UL, UR, flux, status, false, paralleltox, gamma, pinf, 0.0, 0.0);
+    return rs_common(n, UL, UR, flux, status, 0, paralleltox, gamma, pinf, 0.0, 0.0);
+}
+
+int gfm_pure_rs_solve_f32(int n, int paralleltox, double gamma, double pinf, const double* UL, const double* UR,
+                          double* flux, unsigned* status) {
+    return rs_common(n, UL, UR, flux, status, 2, paralleltox, gamma, pinf, 0.0, 0.0);
 }
 
 int gfm_mixed_rs_solve(int n, double gL, double pL, double gR, double pR, const double* UL, const double* UR,
                        double* out4, unsigned* status) {
-    return rs_common(n, UL, UR, out4, status, true, 0, gL, pL, gR, pR);
+    return rs_common(n, UL, UR, out4, status, 1, 0, gL, pL, gR, pR);
 }
 
 }  // extern "C"

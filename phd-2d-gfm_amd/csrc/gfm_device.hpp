@@ -33,6 +33,7 @@ struct RSFluid {
     double g, pinf, z, g6, a2, ig, k2;
     double bc[9];
     float fsk, fz, fig, fa2, fk2, fk3, fg6, fgpinf, figm1;
+    float fg, fpinf, fiz;
 };
 
 __host__ __device__ inline void rs_fluid(RSFluid& F, double g, double pinf) {
@@ -58,6 +59,9 @@ __host__ __device__ inline void rs_fluid(RSFluid& F, double g, double pinf) {
     const float gf = (float)g, pf = (float)pinf;
     F.fgpinf = gf * pf;
     F.figm1 = 1.0f / (gf - 1.0f);
+    F.fg = gf;
+    F.fpinf = pf;
+    F.fiz = (float)(1.0 / F.z);
 }
 
 // One simulation.  Plane arrays are row-major with row pitch `pitch`
@@ -402,14 +406,13 @@ __device__ __forceinline__ void rs_sample(const RSSide& L, const RSSide& R, cons
 }
 
 // ---------------------------------------------------------------------------
-// Mixed-precision face solver (FP32 storage).  The star pressure and velocity
-// come from rs_star's Newton iteration, in FP64, on the same pressure
-// functions (Toro 4.6-4.7).  What differs is the evaluation: the fluid's
-// constants are computed once per workgroup, (pbar / pbarK)^z - 1 is summed
-// as its binomial series when |pbar / pbarK - 1| < 1/64 (no cancellation)
-// and from pow_fast otherwise, the rarefaction derivative uses
-// z k r^(z-1) / pbarK = (c / g) r^z / pbar, and the star densities are left to
-// the FP32 sampling, which needs at most one of them.
+// FP64 pieces of the mixed-precision interfacial solver (rs_star_m2).  The
+// star pressure and velocity come from rs_star's Newton iteration, in FP64,
+// on the same pressure functions (Toro 4.6-4.7).  What differs is the
+// evaluation: the fluid's constants are computed once per instance,
+// (pbar / pbarK)^z - 1 is summed as its binomial series when
+// |pbar / pbarK - 1| < 1/64 (no cancellation) and from pow_fast otherwise,
+// and the rarefaction derivative uses z k r^(z-1) / pbarK = (c / g) r^z / pbar.
 // ---------------------------------------------------------------------------
 // one side: rho, u_n, p, pbar = p + pinf, 1 / pbar, 1 / rho, c; the
 // shock and rarefaction constants A = a2 / rho, B = g6 pbar, k = k2 c and
@@ -466,59 +469,6 @@ __device__ __forceinline__ double rs_f_m(const RSFluid& F, const RSSideM& s, dou
     }
     fd = s.c * F.ig * (1.0 + w) * rcp_nr(pbar);
     return F.k2 * s.c * w;
-}
-
-struct StarM { double p, u; };
-
-// rs_star's iteration (same guess, tolerance, floor and status bits) on the
-// mixed-precision evaluation
-__device__ __forceinline__ unsigned rs_star_m(const RSFluid& F, const RSSideM& L, const RSSideM& R, StarM& S) {
-    if (L.rho == R.rho && L.u == R.u && L.p == R.p) {
-        S.p = L.p; S.u = L.u;
-        return 0;
-    }
-    unsigned st = 0;
-    const double du = R.u - L.u;
-    const double plow = -F.pinf;
-    if (F.k2 * (L.c + R.c) <= du) st |= ST_RS;
-    const double pfloor = plow + 1e-6 * (0.5 * (L.p + R.p) - plow);
-    double p;
-    {
-        const double ppv = 0.5 * (L.p + R.p) - 0.125 * du * (L.rho + R.rho) * (L.c + R.c);
-        const double bmin = fmin(L.pbar, R.pbar), bmax = fmax(L.pbar, R.pbar);
-        if (bmax < 2.0 * bmin) {
-            p = ppv;
-        } else if (ppv < fmin(L.p, R.p)) {   // two rarefactions: exact for one gas
-            const double num = L.c + R.c - 0.5 * (F.g - 1.0) * du;
-            const double den = L.c / pow_fast(L.pbar, F.z) + R.c / pow_fast(R.pbar, F.z);
-            p = (num > 0.0) ? pow_fast(num / den, 1.0 / F.z) - F.pinf : pfloor;
-        } else {                             // two shocks
-            const double ps = fmax(pfloor, ppv);
-            const double gL = sqrt(F.a2 * L.irho / (ps + F.pinf + F.g6 * L.pbar));
-            const double gR = sqrt(F.a2 * R.irho / (ps + F.pinf + F.g6 * R.pbar));
-            p = (gL * L.p + gR * R.p - du) / (gL + gR);
-        }
-        p = p > pfloor ? p : pfloor;
-    }
-    bool conv = false;
-    double fL = 0.0, fR = 0.0, dL = 0.0, dR = 0.0, dp = 0.0;
-    for (int it = 0; it < 60; ++it) {
-        fL = rs_f_m(F, L, p, dL);
-        fR = rs_f_m(F, R, p, dR);
-        double pn = p - (fL + fR + du) * rcp_nr(dL + dR);
-        if (pn <= plow) pn = 0.5 * (p + plow);
-        const double chg = fabs(pn - p);
-        const double ref = 0.5 * (pn + p) - plow;
-        dp = pn - p;
-        p = pn;
-        if (chg <= 1e-6 * ref) { conv = true; break; }
-    }
-    if (!conv || p != p) st |= ST_RS;
-    // u* from the pressure functions at the converged p, to first order in
-    // the last (relative size < 1e-6) Newton step: error ~1e-12 relative
-    S.p = p;
-    S.u = 0.5 * (L.u + R.u) + 0.5 * ((fR + dR * dp) - (fL + dL * dp));
-    return st;
 }
 
 // Two-fluid form for the interfacial problems of the mixed-precision path:
@@ -583,39 +533,6 @@ __device__ __forceinline__ unsigned rs_star_m2(const RSFluid& FL, const RSFluid&
         S.rr = p > R.p ? R.rho * (r + FR.g6) * rcp_nr(FR.g6 * r + 1.0) : R.rho * pow_fast(r, FR.ig);
     }
     return st;
-}
-
-// Toro 4.5 sampling at x/t = 0 in FP32, the side mirrored onto the left-hand
-// formulas; the star density of the sampled side is computed here, in FP32
-__device__ __forceinline__ void rs_sample_m(const RSFluid& F, const RSSideM& L, const RSSideM& R, const StarM& st,
-                                            float& orho, float& ou, float& op) {
-    typedef float T;
-    const bool left = 0.0 <= st.u;
-    const double srho = left ? L.rho : R.rho, su = left ? L.u : R.u, sp = left ? L.p : R.p;
-    const double sc = left ? L.c : R.c, sipb = left ? L.ipb : R.ipb;
-    const T sg = left ? T(1) : T(-1);
-    const T pinf = (T)F.pinf, rho = (T)srho, c = (T)sc;
-    const T uk = sg * (T)su, us = sg * (T)st.u, ps = (T)st.p;
-    const T ratio = (ps + pinf) * (T)sipb;
-    if (st.p > sp) {
-        const T Sk = uk - c * sqrt(F.fsk * ratio + F.fz);
-        if (T(0) <= Sk) { orho = rho; ou = (T)su; op = (T)sp; }
-        else {
-            orho = rho * (ratio + F.fg6) / (F.fg6 * ratio + T(1));
-            ou = (T)st.u; op = ps;
-        }
-        return;
-    }
-    const T SH = uk - c;
-    if (T(0) <= SH) { orho = rho; ou = (T)su; op = (T)sp; return; }
-    const T cm = c * pow(ratio, F.fz);
-    const T ST = us - cm;
-    if (T(0) > ST) { orho = rho * pow(ratio, F.fig); ou = (T)st.u; op = ps; return; }
-    const T cf = F.fa2 * (c + T(0.5) * ((T)F.g - T(1)) * uk);
-    const T q = cf / c;
-    orho = rho * pow(q, F.fk2);
-    ou = sg * cf;
-    op = (T)(left ? L.pbar : R.pbar) * pow(q, F.fk3) - pinf;
 }
 
 template <typename T>
@@ -683,17 +600,173 @@ __device__ __forceinline__ V4<double> rs_flux_full(bool px, double g, double pin
     return rs_flux_out<double>(px, g, pinf, r, u, p, S.u, WL, WR);
 }
 
-// FP32 storage: the mixed-precision solver above
-__device__ __forceinline__ V4<float> rs_flux_full_m(const RSFluid& F, const FluidF& Ff, bool px, const V4<float>& WL,
+// ---------------------------------------------------------------------------
+// FP32 face solver of the mixed-precision path: rs_star's iteration (Toro
+// 4.6-4.7 pressure functions, guess, floor, tolerance 1e-6 on the relative
+// pressure change, status bits) evaluated in FP32 with the hardware
+// reciprocal, square root, exp2 and log2 (about 1 ulp each).  The
+// rarefaction function k ((pbar / pbarK)^z - 1) is formed without
+// cancellation (powm1_f).  The iteration also stops once its step is at FP32
+// resolution (4 ulp of p), which only matters where p + pinf << pinf.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// x^y = exp2(y log2 x), x >= 0: relative error about (2 + |y log2 x|) ulp
+__device__ __forceinline__ float pow_hw(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+
+// (1 + eps)^z - 1 for eps > -1 and |z| < 1, with r = 1 + eps as computed by
+// the caller (used only where |eps| > 1/4): log(1 + eps) = 2 atanh(s),
+// s = eps / (2 + eps), summed to s^9 for |eps| <= 1/4 (|s| <= 1/7, truncation
+// < 1e-9 relative), else ln2 log2(r); then expm1(y), y = z log(1 + eps), by
+// its Taylor series to y^7 for |y| <= 1/4 (truncation < 2e-9 relative), else
+// exp2(y / ln2) - 1 (|result| > 0.22).  About 3 ulp relative.
+__device__ __forceinline__ float powm1_f(float eps, float r, float z) {
+    float L;
+    if (fabsf(eps) <= 0.25f) {
+        const float s = eps * rcp_hw(2.0f + eps);
+        const float s2 = s * s;
+        float q = 1.0f / 9.0f;
+        q = fmaf(q, s2, 1.0f / 7.0f);
+        q = fmaf(q, s2, 1.0f / 5.0f);
+        q = fmaf(q, s2, 1.0f / 3.0f);
+        L = 2.0f * fmaf(s * s2, q, s);
+    } else {
+        L = 0.693147180559945f * __builtin_amdgcn_logf(r);
+    }
+    const float y = z * L;
+    if (fabsf(y) <= 0.25f) {
+        float e = 1.0f / 5040.0f;
+        e = fmaf(e, y, 1.0f / 720.0f);
+        e = fmaf(e, y, 1.0f / 120.0f);
+        e = fmaf(e, y, 1.0f / 24.0f);
+        e = fmaf(e, y, 1.0f / 6.0f);
+        e = fmaf(e, y, 0.5f);
+        e = fmaf(e, y, 1.0f);
+        return y * e;
+    }
+    return __builtin_amdgcn_exp2f(y * 1.44269504088896f) - 1.0f;
+}
+
+struct RSSideF { float rho, u, p, pbar, ipb, irho, c; };
+
+__device__ __forceinline__ void rs_side_f(RSSideF& s, const RSFluid& F, float rho, float u, float p) {
+    s.rho = rho; s.u = u; s.p = p;
+    s.pbar = p + F.fpinf;
+    s.ipb = rcp_hw(s.pbar);
+    s.irho = rcp_hw(rho);
+    s.c = sqrt_hw(F.fg * s.pbar * s.irho);
+}
+
+__device__ __forceinline__ float rs_f_f(const RSFluid& F, const RSSideF& s, float p, float& fd) {
+    const float pbar = p + F.fpinf;
+    const float dp = p - s.p;
+    if (p > s.p) {   // shock (Toro 4.6)
+        const float t = rcp_hw(pbar + F.fg6 * s.pbar);
+        const float q = sqrt_hw(F.fa2 * s.irho * t);
+        fd = q * (1.0f - 0.5f * dp * t);
+        return dp * q;
+    }
+    // rarefaction (Toro 4.7): f = k ((pbar / pbarK)^z - 1), f' = (c / g) (pbar / pbarK)^z / pbar
+    const float w = powm1_f(dp * s.ipb, pbar * s.ipb, F.fz);
+    fd = s.c * F.fig * (1.0f + w) * rcp_hw(pbar);
+    return F.fk2 * s.c * w;
+}
+
+struct StarF { float p, u; };
+
+__device__ __forceinline__ unsigned rs_star_f(const RSFluid& F, const RSSideF& L, const RSSideF& R, StarF& S) {
+    if (L.rho == R.rho && L.u == R.u && L.p == R.p) {
+        S.p = L.p; S.u = L.u;
+        return 0;
+    }
+    unsigned st = 0;
+    const float du = R.u - L.u;
+    const float plow = -F.fpinf;
+    if (F.fk2 * (L.c + R.c) <= du) st |= ST_RS;
+    const float pfloor = plow + 1e-6f * (0.5f * (L.p + R.p) - plow);
+    float p;
+    {
+        const float ppv = 0.5f * (L.p + R.p) - 0.125f * du * (L.rho + R.rho) * (L.c + R.c);
+        const float bmin = fminf(L.pbar, R.pbar), bmax = fmaxf(L.pbar, R.pbar);
+        if (bmax < 2.0f * bmin) {
+            p = ppv;
+        } else if (ppv < fminf(L.p, R.p)) {   // two rarefactions: exact for one gas
+            const float num = L.c + R.c - 0.5f * (F.fg - 1.0f) * du;
+            const float den = L.c * rcp_hw(pow_hw(L.pbar, F.fz)) + R.c * rcp_hw(pow_hw(R.pbar, F.fz));
+            p = (num > 0.0f) ? pow_hw(num * rcp_hw(den), F.fiz) - F.fpinf : pfloor;
+        } else {                              // two shocks
+            const float ps = fmaxf(pfloor, ppv);
+            const float gL = sqrt_hw(F.fa2 * L.irho * rcp_hw(ps + F.fpinf + F.fg6 * L.pbar));
+            const float gR = sqrt_hw(F.fa2 * R.irho * rcp_hw(ps + F.fpinf + F.fg6 * R.pbar));
+            p = (gL * L.p + gR * R.p - du) * rcp_hw(gL + gR);
+        }
+        p = p > pfloor ? p : pfloor;
+    }
+    bool conv = false;
+    float fL = 0.0f, fR = 0.0f, dL = 0.0f, dR = 0.0f, dp = 0.0f;
+    for (int it = 0; it < 60; ++it) {
+        fL = rs_f_f(F, L, p, dL);
+        fR = rs_f_f(F, R, p, dR);
+        float pn = p - (fL + fR + du) * rcp_hw(dL + dR);
+        if (pn <= plow) pn = 0.5f * (p + plow);
+        const float chg = fabsf(pn - p);
+        const float ref = 0.5f * (pn + p) - plow;
+        dp = pn - p;
+        p = pn;
+        if (chg <= 1e-6f * ref || chg <= 4.8e-7f * fabsf(p)) { conv = true; break; }
+    }
+    if (!conv || p != p) st |= ST_RS;
+    // u* from the pressure functions at the converged p, to first order in
+    // the last Newton step
+    S.p = p;
+    S.u = 0.5f * (L.u + R.u) + 0.5f * ((fR + dR * dp) - (fL + dL * dp));
+    return st;
+}
+
+// Toro 4.5 sampling at x/t = 0, the side mirrored onto the left-hand formulas
+__device__ __forceinline__ void rs_sample_f(const RSFluid& F, const RSSideF& L, const RSSideF& R, const StarF& st,
+                                            float& orho, float& ou, float& op) {
+    typedef float T;
+    const bool left = T(0) <= st.u;
+    const T rho = left ? L.rho : R.rho, su = left ? L.u : R.u, sp = left ? L.p : R.p;
+    const T c = left ? L.c : R.c, sipb = left ? L.ipb : R.ipb, spbar = left ? L.pbar : R.pbar;
+    const T sg = left ? T(1) : T(-1);
+    const T uk = sg * su, us = sg * st.u, ps = st.p;
+    const T ratio = (ps + F.fpinf) * sipb;
+    if (ps > sp) {
+        const T Sk = uk - c * sqrt_hw(F.fsk * ratio + F.fz);
+        if (T(0) <= Sk) { orho = rho; ou = su; op = sp; }
+        else {
+            orho = rho * (ratio + F.fg6) * rcp_hw(F.fg6 * ratio + T(1));
+            ou = st.u; op = ps;
+        }
+        return;
+    }
+    const T SH = uk - c;
+    if (T(0) <= SH) { orho = rho; ou = su; op = sp; return; }
+    const T cm = c * pow_hw(ratio, F.fz);
+    const T ST = us - cm;
+    if (T(0) > ST) { orho = rho * pow_hw(ratio, F.fig); ou = st.u; op = ps; return; }
+    const T cf = F.fa2 * (c + T(0.5) * (F.fg - T(1)) * uk);
+    const T q = cf * rcp_hw(c);
+    orho = rho * pow_hw(q, F.fk2);
+    ou = sg * cf;
+    op = spbar * pow_hw(q, F.fk3) - F.fpinf;
+}
+
+__device__ __forceinline__ V4<float> rs_flux_full_f(const RSFluid& F, const FluidF& Ff, bool px, const V4<float>& WL,
                                                     const V4<float>& WR, unsigned& st) {
-    RSSideM L, R;
-    rs_side_m(L, F, (double)WL.v[0], (double)norm_u(px, WL), (double)WL.v[3]);
-    rs_side_m(R, F, (double)WR.v[0], (double)norm_u(px, WR), (double)WR.v[3]);
-    StarM S;
-    st |= rs_star_m(F, L, R, S);
+    RSSideF L, R;
+    rs_side_f(L, F, WL.v[0], norm_u(px, WL), WL.v[3]);
+    rs_side_f(R, F, WR.v[0], norm_u(px, WR), WR.v[3]);
+    StarF S;
+    st |= rs_star_f(F, L, R, S);
     float r, u, p;
-    rs_sample_m(F, L, R, S, r, u, p);
-    return rs_flux_out_f(px, Ff, r, u, p, S.u, WL, WR);
+    rs_sample_f(F, L, R, S, r, u, p);
+    return rs_flux_out_f(px, Ff, r, u, p, (double)S.u, WL, WR);
 }
 
 // pure_RS_exact::solve_RP (pure_RS_exact.cpp:9-73): conserved L/R -> flux

@@ -450,7 +450,7 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
         unsigned dummy = 0;
         face_input<R, MUSCL>(I, c, i0, j0, lo_i, lim_i, lo_j, lim_j, g, pinf, Ff, sW, sE, Qi, WL, WR, px, dummy);
         V4<T> F;
-        if constexpr (sizeof(T) == 4) F = rs_flux_full_m(rf, Ff, px, WL, WR, st);
+        if constexpr (sizeof(T) == 4) F = rs_flux_full_f(rf, Ff, px, WL, WR, st);
         else F = rs_flux_full(px, g, pinf, WL, WR, st);
 #pragma unroll
         for (int k = 0; k < 4; ++k) sFl[k][c] = F.v[k];
@@ -1933,6 +1933,24 @@ __global__ void k_pure_rs(int n, int px, double g, double pinf, const double* UL
     if (st) atomicOr(status, st);
 }
 
+// the face solve of k_fs<float> (FP32 storage) for one face: FP32 conserved
+// states -> primitives -> trivial or full FP32 solve -> FP32 flux
+__global__ void k_pure_rs_f32(int n, int px, RSFluid F, const double* UL, const double* UR, double* Fl,
+                              unsigned* status) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    V4<float> a, b;
+    for (int k = 0; k < 4; ++k) { a.v[k] = (float)UL[4 * t + k]; b.v[k] = (float)UR[4 * t + k]; }
+    unsigned st = 0;
+    if (!phys_state_f(a) || !phys_state_f(b)) st |= ST_UNPHYS;
+    const FluidF Ff = fluid_f(F);
+    const V4<float> WL = cons2prims_f(Ff, a), WR = cons2prims_f(Ff, b);
+    const V4<float> f = rs_trivial(px != 0, WL, WR) ? rs_flux_trivial_f(px != 0, Ff, WL, WR)
+                                                     : rs_flux_full_f(F, Ff, px != 0, WL, WR, st);
+    for (int k = 0; k < 4; ++k) Fl[4 * t + k] = (double)f.v[k];
+    if (st) atomicOr(status, st);
+}
+
 __global__ void k_mixed_rs(int n, double gL, double pL, double gR, double pR, const double* UL, const double* UR,
                            double* out, unsigned* status) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2121,6 +2139,13 @@ void launch_bc(const LaunchCtx& L, int q, int nfluid, int advance) {
 void launch_pure_rs(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR, double* F,
                     unsigned* st) {
     hipLaunchKernelGGL(k_pure_rs, dim3((n + 255) / 256), dim3(256), 0, s, n, px, g, pinf, UL, UR, F, st);
+}
+
+void launch_pure_rs_f32(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR,
+                        double* F, unsigned* st) {
+    RSFluid rf;
+    rs_fluid(rf, g, pinf);
+    hipLaunchKernelGGL(k_pure_rs_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, px, rf, UL, UR, F, st);
 }
 
 void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, double pR, const double* UL,

@@ -38,6 +38,10 @@ template <typename R> void launch_bc(const LaunchCtx& L, int q, int nfluid, int 
 template <typename R> void launch_reset(const LaunchCtx& L, int narrow_boxes);
 void launch_pure_rs(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR,
                     double* F, unsigned* st);
+// the mixed-precision path's face solver on the same problem: FP32 states
+// (the conserved inputs rounded to FP32), FP32 flux
+void launch_pure_rs_f32(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR,
+                        double* F, unsigned* st);
 void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, double pR, const double* UL,
                      const double* UR, double* out, unsigned* st);
 

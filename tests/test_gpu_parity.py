@@ -73,6 +73,33 @@ def test_pure_rs(oracle, px, g, pinf):
     assert err <= 1e-12, err
 
 
+@pytest.mark.parametrize("px", [True, False])
+@pytest.mark.parametrize("g,pinf", [(1.4, 0.0), (4.4, 6e8 / 1e9), (1.667, 0.0)])
+def test_pure_rs_fp32(oracle, px, g, pinf):
+    """the mixed-precision path's FP32 face solver against the FP64 oracle on
+    the same FP32-rounded states: per component, max error <= 2e-6 and mean
+    error <= 5e-8 of the largest flux magnitude (measured: max 6.3e-7 for the
+    stiffened gas, 2.5e-7 otherwise; mean 1.1e-8.  FP32 rounding of the
+    inputs to the flux formula alone is 6e-8; rarefaction powers and the
+    Newton stop add a few ulp, amplified where the flux cancels)"""
+    print_case("pure_rs_fp32 %s %s %s" % (px, g, pinf))
+    rng = np.random.default_rng(3)
+    n = 4000
+    _, UL = random_states(rng, n, g, pinf)
+    _, UR = random_states(rng, n, g, pinf)
+    UR[:100] = UL[:100]
+    UL = UL.astype(np.float32).astype(np.float64)
+    UR = UR.astype(np.float32).astype(np.float64)
+    F, st = G.pure_rs_solve(px, g, pinf, UL, UR, fp32=True)
+    Fo = np.array([oracle.pure_rs_flux(px, g, pinf, UL[k], UR[k])[0] for k in range(n)])
+    ok = ~np.isnan(Fo).any(1)
+    assert ok.mean() > 0.9
+    err, mean = scaled_err(F[ok], Fo[ok])
+    print("fp32 face solver max/mean scaled error", err, mean)
+    assert err <= 2e-6 and mean <= 5e-8, (err, mean)
+    assert (F[:100] == F[:100]).all()
+
+
 def test_mixed_rs(oracle):
     print_case("mixed_rs")
     rng = np.random.default_rng(2)
