@@ -581,13 +581,20 @@ __device__ __forceinline__ void extrap_dirs(int fg, R nx, R ny, int& dj, int& di
 //   fluid 1 ghost (:98-121): c0 - ax nx dx- - ay ny dy-, d- = -+(xo - c0)
 // with the neighbours taken upwind as extrap_dirs does; both are exactly
 // c0 + wx (xo - c0) + wy (yo - c0), wx = ax |nx|, wy = ay |ny| (IEEE
-// negation is exact), which is what is evaluated
+// negation is exact), which is what is evaluated: in FP64 with the
+// reference's separate roundings, in FP32 (mixed precision) as two fused
+// multiply-adds
 template <typename R>
 __device__ __forceinline__ V4<R> extrap_combine(R wx, R wy, const V4<R>& c0, const V4<R>& xo, const V4<R>& yo) {
 #pragma clang fp contract(off)
     V4<R> Wg;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) Wg.v[k] = (c0.v[k] + wx * (xo.v[k] - c0.v[k])) + wy * (yo.v[k] - c0.v[k]);
+    for (int k = 0; k < 4; ++k) {
+        if constexpr (sizeof(R) == 4)
+            Wg.v[k] = fmaf(wy, yo.v[k] - c0.v[k], fmaf(wx, xo.v[k] - c0.v[k], c0.v[k]));
+        else
+            Wg.v[k] = (c0.v[k] + wx * (xo.v[k] - c0.v[k])) + wy * (yo.v[k] - c0.v[k]);
+    }
     return Wg;
 }
 
@@ -765,11 +772,12 @@ __device__ __forceinline__ void tb_put(V4A<R>* s, int k, const V4<R>& W) {
     s[k] = a;
 }
 
-// new ghost-fluid primitives of an LDS cell (extrap_cell's arithmetic): oc,
-// ox, oy index the ghost fluid at the cell and its upwind x and y neighbours
+// new ghost-fluid primitives of an LDS cell (extrap_cell's arithmetic) from
+// its current ones c0 and those of its upwind x and y neighbours at LDS
+// cells ox, oy
 template <typename R>
-__device__ __forceinline__ V4<R> tb_update(const V4A<R>* s, int oc, int ox, int oy, R wx, R wy) {
-    return extrap_combine<R>(wx, wy, tb_get<R>(s, oc), tb_get<R>(s, ox), tb_get<R>(s, oy));
+__device__ __forceinline__ V4<R> tb_update(const V4A<R>* s, const V4<R>& c0, int ox, int oy, R wx, R wy) {
+    return extrap_combine<R>(wx, wy, c0, tb_get<R>(s, ox), tb_get<R>(s, oy));
 }
 
 template <typename R, int NIT, int MIN, int MOUT>
@@ -791,18 +799,24 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
     const void* in0 = (MIN & MF_FIRST) ? I.Q[0][bin] : I.P[0][bin];
     const void* in1 = (MIN & MF_FIRST) ? I.Q[1][bin] : I.P[1][bin];
     const ExK<R> K = ex_k<R>(I);
-    // per cell: the update weights, the LDS offsets of the ghost fluid at the
-    // cell and its upwind neighbours, and bits (1: interior with its stencil
-    // inside the region, 2: fluid 2 real, 4: owned, 8 and up: distance to
-    // the region edge)
+    // per cell: the update weights, the ghost fluid's current primitives
+    // (only this thread changes them), the LDS cells of the ghost fluid at
+    // the cell and its upwind neighbours, and bits (1: interior with its
+    // stencil inside the region, 2: fluid 2 real, 4: owned, 8: next to the
+    // domain edge (its new state refreshes the first ghost layer), 16 and up:
+    // distance to the region edge)
     R wxs[SLOTS], wys[SLOTS];
+    V4<R> cur[SLOTS];
     int oc[SLOTS], ox[SLOTS], oy[SLOTS];
     int meta[SLOTS];
+    const bool rfb = I.bc[2] == BC_R, rft = I.bc[3] == BC_R, rfx = I.bc[0] == BC_R;
+    const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
 #pragma unroll
     for (int k = 0; k < SLOTS; ++k) {
         const int idx = threadIdx.x + k * TB_THREADS;
         wxs[k] = R(0); wys[k] = R(0); meta[k] = 0;
         oc[k] = ox[k] = oy[k] = idx;
+        cur[k] = {{R(0), R(0), R(0), R(0)}};
         if (idx < NC) {
             const int a = idx / RX, b = idx - (idx / RX) * RX;
             const int i = ri + a, j = rj + b;
@@ -815,14 +829,16 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
                     unit_normal_r<R>(I, K, i, j, nx, ny);
                     extrap_weights<R>(K, nx, ny, wxs[k], wys[k]);
                     const bool own = a >= NIT && a < NIT + TBY && b >= NIT && b < NIT + TBX;
+                    const bool edge = i == ib || i == it || j == jl || j == jr;
                     const int dist = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
                     const int fr = ld1<R>(I.phi, cidx(I, i, j)) <= R(0) ? 0 : 1;
-                    meta[k] = 1 | (fr ? 2 : 0) | (own ? 4 : 0) | (dist << 3);
+                    meta[k] = 1 | (fr ? 2 : 0) | (own ? 4 : 0) | (edge ? 8 : 0) | (dist << 4);
                     int dj, di;
                     extrap_dirs<R>(1 - fr, nx, ny, dj, di);
                     oc[k] = (1 - fr) * NC + idx;
                     ox[k] = oc[k] + dj;
                     oy[k] = oc[k] + di * RX;
+                    cur[k] = fr ? W1 : W2;
                 }
             }
             tb_put<R>(s, idx, W1);
@@ -830,31 +846,27 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
         }
     }
     __syncthreads();
-    const bool rfb = I.bc[2] == BC_R, rft = I.bc[3] == BC_R, rfx = I.bc[0] == BC_R;
-    const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
     bool bad = false;
 #pragma unroll 1
     for (int t = 1; t < NIT; ++t) {
-        V4<R> nv[SLOTS];
         int act = 0;
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
-            const int idx = threadIdx.x + k * TB_THREADS;
-            if ((meta[k] & 1) && (meta[k] >> 3) >= t) {
+            if ((meta[k] & 1) && (meta[k] >> 4) >= t) {
                 act |= 1 << k;
-                nv[k] = tb_update<R>(s, oc[k], ox[k], oy[k], wxs[k], wys[k]);
-                if ((meta[k] & 4) && !(nv[k].v[0] >= R(0))) bad = true;
+                cur[k] = tb_update<R>(s, cur[k], ox[k], oy[k], wxs[k], wys[k]);
+                if ((meta[k] & 4) && !(cur[k].v[0] >= R(0))) bad = true;
             }
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
             if (!(act & (1 << k))) continue;
-            const int idx = threadIdx.x + k * TB_THREADS;
-            const int a = idx / RX, b = idx - (idx / RX) * RX;
-            const int i = ri + a, j = rj + b;
-            tb_put<R>(s, oc[k], nv[k]);
-            if (i == ib || i == it || j == jl || j == jr) {
+            tb_put<R>(s, oc[k], cur[k]);
+            if (meta[k] & 8) {
+                const int idx = threadIdx.x + k * TB_THREADS;
+                const int a = idx / RX, b = idx - (idx / RX) * RX;
+                const int i = ri + a, j = rj + b;
                 // first ghost layer: this cell's new state, reflected
                 const V4<R> W1 = tb_get<R>(s, idx), W2 = tb_get<R>(s, NC + idx);
                 V4<R> Wv1 = W1, Wv2 = W2, Wu1 = W1, Wu2 = W2;
@@ -876,7 +888,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
         const int a = idx / RX, b = idx - (idx / RX) * RX;
         const int i = ri + a, j = rj + b;
         const int fr = (meta[k] & 2) ? 1 : 0, fg = 1 - fr;
-        const V4<R> Wg = tb_update<R>(s, oc[k], ox[k], oy[k], wxs[k], wys[k]);
+        const V4<R> Wg = tb_update<R>(s, cur[k], ox[k], oy[k], wxs[k], wys[k]);
         if (!(Wg.v[0] >= R(0))) bad = true;
         const V4<R> Wr = tb_get<R>(s, fr * NC + idx);
         V4<R> Wo = Wg;
