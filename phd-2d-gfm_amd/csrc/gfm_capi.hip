@@ -370,7 +370,7 @@ TableOffsets append_tables(const gfm_ctx* c, const std::vector<Inst>& descs, con
         int li = 0;
         for (int ty = 0; ty < tiy; ++ty)
             for (int tx = 0; tx < tix; ++tx) w_int.push_back(make_int4(ki, ty, tx, li++));
-        const int sx = (s.Nx + 31) / 32, sy = (s.Ny + 7) / 8;
+        const int sx = (s.Nx + gfm::FS_TILE_X - 1) / gfm::FS_TILE_X, sy = (s.Ny + gfm::FS_TILE_Y - 1) / gfm::FS_TILE_Y;
         for (int f = 0; f < nf; ++f)
             for (int ty = 0; ty < sy; ++ty)
                 for (int tx = 0; tx < sx; ++tx) w_fs.push_back(make_int4(ki, ty, tx, f));

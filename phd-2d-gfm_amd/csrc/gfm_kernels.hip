@@ -205,7 +205,8 @@ __global__ __launch_bounds__(256) void k_dt_final(const Inst* __restrict__ insts
 
 // ---------------------------------------------------------------------------
 // A4/A5: one unsplit MUSCL-Hancock (or first-order Godunov) update of one
-// fluid on a 32x8 tile (FS_MUSCL.cpp:148-327, FS_godunov.cpp:5-50).
+// fluid on a FS_TILE_X x FS_TILE_Y tile, one thread per cell (FS_MUSCL.cpp:148-327,
+// FS_godunov.cpp:5-50).
 //  A  primitives of the tile + 2-cell halo -> LDS
 //  B  per cell (tile + 1 halo): minmod slopes, boundary-extrapolated values,
 //     cell-level positivity fallback, Hancock half step -> LDS
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(256) void k_dt_final(const Inst* __restrict__ insts
 //  D  flux-difference update in the reference's accumulation order
 // work: {inst, tile_y, tile_x, fluid}
 // ---------------------------------------------------------------------------
-constexpr int FTX = 32, FTY = 8;
+constexpr int FTX = FS_TILE_X, FTY = FS_TILE_Y, FS_THREADS = FTX * FTY;
 constexpr int FWX = FTX + 4, FWY = FTY + 4;
 constexpr int FEX = FTX + 2, FEY = FTY + 2;
 constexpr int NFX = FTY * (FTX + 1);
@@ -306,13 +307,13 @@ __device__ __forceinline__ bool face_input(const Inst& I, int c, int i0, int j0,
     return true;
 }
 
-// FP32 storage: 4 workgroups per CU (128 VGPRs) measured fastest; FP64 is
-// insensitive to the cap
+// FP32 storage: 2 workgroups of 512 per CU (128 VGPRs); FP64 is insensitive
+// to the cap
 template <typename R>
-struct FsMinBlocks { static constexpr int value = sizeof(R) == 4 ? 4 : 1; };
+struct FsMinBlocks { static constexpr int value = sizeof(R) == 4 ? 2 : 1; };
 
 template <typename R, bool MUSCL>
-__global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* __restrict__ insts, const int4* __restrict__ work,
+__global__ __launch_bounds__(FS_THREADS, FsMinBlocks<R>::value) void k_fs(const Inst* __restrict__ insts, const int4* __restrict__ work,
                                             int qin, int qout, int copy_inactive) {
     typedef R T;
     __shared__ T sW[4][FWY][FWX];
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
         return;
     }
     // A
-    for (int c = tid; c < FWY * FWX; c += 256) {
+    for (int c = tid; c < FWY * FWX; c += FS_THREADS) {
         const int a = c / FWX, b = c % FWX;
         const int i = min(max(i0 - 2 + a, 0), I.H - 1);
         const int j = min(max(j0 - 2 + b, 0), I.W - 1);
@@ -373,7 +374,7 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
     const T hx = T(0.5) * dtodx, hy = T(0.5) * dtody;
     if (MUSCL) {
         // B
-        for (int c = tid; c < FEY * FEX; c += 256) {
+        for (int c = tid; c < FEY * FEX; c += FS_THREADS) {
             const int a = c / FEX + 1, b = c % FEX + 1;   // position in sW
             V4<T> Wc, BL, BR, BB, BT;
 #pragma unroll
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
     const int lim_i = min(bb1, i0 + FTY), lim_j = min(bb3, j0 + FTX);
     const int lo_i = max(bb0, i0), lo_j = max(bb2, j0);
     unsigned st = 0;
-    for (int c = tid; c < NFACE; c += 256) {
+    for (int c = tid; c < NFACE; c += FS_THREADS) {
         V4<T> WL, WR;
         bool px;
         if (!face_input<R, MUSCL>(I, c, i0, j0, lo_i, lim_i, lo_j, lim_j, g, pinf, Ff, sW, sE, Qi, WL, WR, px, st)) continue;
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(256, FsMinBlocks<R>::value) void k_fs(const Inst* _
     // C2: the exact Riemann solves of the listed faces
     const int nh = nHard;
     const RSFluid& rf = I.rsf[f];
-    for (int h = tid; h < nh; h += 256) {
+    for (int h = tid; h < nh; h += FS_THREADS) {
         const int c = sHard[h];
         V4<T> WL, WR;
         bool px;
@@ -2006,10 +2007,10 @@ void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes, int co
 template <typename R>
 void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive) {
     if (muscl)
-        hipLaunchKernelGGL((k_fs<R, true>), GRID(L.n_fs), dim3(256), 0, L.stream, L.insts, L.w_fs, qin, qout,
+        hipLaunchKernelGGL((k_fs<R, true>), GRID(L.n_fs), dim3(FS_THREADS), 0, L.stream, L.insts, L.w_fs, qin, qout,
                            copy_inactive);
     else
-        hipLaunchKernelGGL((k_fs<R, false>), GRID(L.n_fs), dim3(256), 0, L.stream, L.insts, L.w_fs, qin, qout,
+        hipLaunchKernelGGL((k_fs<R, false>), GRID(L.n_fs), dim3(FS_THREADS), 0, L.stream, L.insts, L.w_fs, qin, qout,
                            copy_inactive);
 }
 
