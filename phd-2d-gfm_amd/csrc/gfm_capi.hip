@@ -243,18 +243,21 @@ int copy_q(gfm_ctx* c, int from, int to) {
     return GFM_OK;
 }
 
-// XCD-aware order: MI355X dispatches workgroup k to XCD k % 8, so entry p
-// takes tile (p % 8) * ceil(n/8) + p / 8 and each XCD walks a contiguous run
-// of tiles whose halos share its L2.
-void xcd_order(std::vector<int4>& v, size_t begin) {
-    const size_t n = v.size() - begin;
+// XCD-aware order: MI355X dispatches workgroup k to XCD k % 8, so
+// workgroup p takes tile (p % 8) * ceil(n/8) + p / 8 and each XCD walks a
+// contiguous run of tiles whose halos share its L2.  `group` consecutive
+// entries form one workgroup's work.
+void xcd_order(std::vector<int4>& v, size_t begin, size_t group = 1) {
+    const size_t ne = v.size() - begin;
+    const size_t n = (ne + group - 1) / group;
     if (n < 16) return;
     const size_t per = (n + 7) / 8;
     std::vector<int4> src(v.begin() + begin, v.end()), out;
-    out.reserve(n);
-    for (size_t p = 0; out.size() < n; ++p) {
+    out.reserve(ne);
+    for (size_t p = 0; out.size() < ne; ++p) {
         const size_t t = (p % 8) * per + p / 8;
-        if (t < n) out.push_back(src[t]);
+        if (t < n)
+            for (size_t q = t * group; q < std::min(ne, (t + 1) * group); ++q) out.push_back(src[q]);
         if (p > 16 * n) break;
     }
     std::copy(out.begin(), out.end(), v.begin() + begin);
@@ -396,6 +399,7 @@ TableOffsets append_tables(const gfm_ctx* c, const std::vector<Inst>& descs, con
     xcd_order(w_fs, 0);
     xcd_order(w_tb, 0);
     xcd_order(w_rt, 0);
+    xcd_order(w_seg, 0, gfm::SEG_PER_WG);
     if (w_ich.empty()) w_ich.push_back(make_int4(members.empty() ? 0 : members[0], 1 << 30, 0, 0));
     const std::vector<int4>* t[9] = {&w_full, &w_int, &w_fs, &w_frame, &w_inst, &w_ich, &w_tb, &w_rt, &w_seg};
     TableOffsets r{};

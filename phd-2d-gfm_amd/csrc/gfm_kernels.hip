@@ -63,7 +63,7 @@ __device__ __forceinline__ void reset_boxes(const Inst& I, int narrow) {
 // four cells in flight at once.
 // work: {inst, interior row, segment column, segment index within the instance}
 // ---------------------------------------------------------------------------
-constexpr int SEGW = 256, SEG_PER_WG = 4, SEG_Q = SEGW / 64;
+constexpr int SEGW = 256, SEG_Q = SEGW / 64;
 
 // the interfacial list of the previous build is erased from imap entry by
 // entry (imap is -1 everywhere else); wavefront s erases list entries
@@ -105,6 +105,13 @@ __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, cons
     V4<R> U[SEG_Q];
 #pragma unroll
     for (int k = 0; k < SEG_Q; ++k) ph[k] = (twofluid && j0 + 64 * k < jend) ? ld1<R>(I.phi, row + j0 + 64 * k) : R(0);
+    // the interfacial test's loads do not wait for the states
+    unsigned ifm = 0;
+    if (count) {
+#pragma unroll
+        for (int k = 0; k < SEG_Q; ++k)
+            if (j0 + 64 * k < jend && is_iface<R>(I, i, j0 + 64 * k)) ifm |= 1u << k;
+    }
 #pragma unroll
     for (int k = 0; k < SEG_Q; ++k) {
         const int j = j0 + 64 * k;
@@ -114,6 +121,9 @@ __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, cons
     double mu = 0.0, mv = 0.0, m1 = 0.0, m2 = 0.0;
     const double dA = I.dx * I.dy;
     int nc = 0;
+    // FP32 storage: the wave speeds from the FP32 primitives (cons2prims_f,
+    // c = sqrt(g (p + pinf) / rho) with the hardware square root)
+    const FluidF F0 = fluid_f(I.rsf[0]), F1 = fluid_f(I.rsf[1]);
 #pragma unroll
     for (int k = 0; k < SEG_Q; ++k) {
         const int j = j0 + 64 * k;
@@ -121,20 +131,29 @@ __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, cons
         if (j < jend) {
             const double phi = (double)ph[k];
             const int f = twofluid ? (phi < 0.0 ? 0 : 1) : 0;
-            const double U0 = U[k].v[0], U1 = U[k].v[1], U2 = U[k].v[2], U3 = U[k].v[3];
-            const double rho = U0;
-            const double u = U1 / rho, v = U2 / rho;
-            const double e = (U3 / rho) - 0.5 * (u * u + v * v);
-            const double p = eos_p(I.gam[f], I.pinf[f], e, rho);
-            const double cs = eos_c(I.gam[f], I.pinf[f], p, rho);
-            mu = fmax(mu, fabs(U1) / rho + cs);
-            mv = fmax(mv, fabs(U2) / rho + cs);
+            const double U0 = U[k].v[0], U1 = U[k].v[1], U2 = U[k].v[2];
+            if constexpr (sizeof(R) == 4) {
+                const FluidF& Ff = f ? F1 : F0;
+                const V4<float> W = cons2prims_f(Ff, U[k]);
+                const float c = sqrt_hw(Ff.g * (W.v[3] + Ff.pinf) * rcp_hw(W.v[0]));
+                mu = fmax(mu, (double)(fabsf(W.v[1]) + c));
+                mv = fmax(mv, (double)(fabsf(W.v[2]) + c));
+            } else {
+                const double U3 = U[k].v[3];
+                const double rho = U0;
+                const double u = U1 / rho, v = U2 / rho;
+                const double e = (U3 / rho) - 0.5 * (u * u + v * v);
+                const double p = eos_p(I.gam[f], I.pinf[f], e, rho);
+                const double cs = eos_c(I.gam[f], I.pinf[f], p, rho);
+                mu = fmax(mu, fabs(U1) / rho + cs);
+                mv = fmax(mv, fabs(U2) / rho + cs);
+            }
             if (twofluid) {
                 // the density of the fluid whose mass is summed is the loaded one
                 if (-phi > 0.0) m1 += U0 * dA;
                 if (phi > 0.0) m2 += U0 * dA;
             }
-            if (count) iface = is_iface<R>(I, i, j);
+            iface = (ifm >> k) & 1u;
         }
         if (count) nc += __popcll(__ballot(iface));
     }

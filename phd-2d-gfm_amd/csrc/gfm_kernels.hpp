@@ -26,6 +26,7 @@ struct LaunchCtx {
 struct Houc;
 constexpr int IFREC_BYTES = 384;   // >= sizeof(IfRec), checked in the kernels
 constexpr int FS_TILE_X = 32, FS_TILE_Y = 16;   // flux-update tile (one cell per thread)
+constexpr int SEG_PER_WG = 4;   // 256-cell row segments per workgroup (one per wavefront)
 
 template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes, int count_iface);
 template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive);
