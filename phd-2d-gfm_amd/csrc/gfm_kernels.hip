@@ -222,15 +222,44 @@ __global__ __launch_bounds__(256) void k_dt_final(const Inst* __restrict__ insts
     }
 }
 
+// LDS record of four values (a cell's primitives, a face's state or flux),
+// aligned so that it moves in one 16-byte (FP32) or two (FP64) LDS accesses
+template <typename R>
+struct alignas(4 * sizeof(R) > 16 ? 16 : 4 * sizeof(R)) V4A { R v[4]; };
+
+template <typename R>
+__device__ __forceinline__ V4<R> tb_get(const V4A<R>* s, int k) {
+    const V4A<R> a = s[k];
+    V4<R> W;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) W.v[q] = a.v[q];
+    return W;
+}
+
+template <typename R>
+__device__ __forceinline__ void tb_put(V4A<R>* s, int k, const V4<R>& W) {
+    V4A<R> a;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a.v[q] = W.v[q];
+    s[k] = a;
+}
+
 // ---------------------------------------------------------------------------
 // A4/A5: one unsplit MUSCL-Hancock (or first-order Godunov) update of one
-// fluid on a FS_TILE_X x FS_TILE_Y tile, one thread per cell (FS_MUSCL.cpp:148-327,
-// FS_godunov.cpp:5-50).
-//  A  primitives of the tile + 2-cell halo -> LDS
+// fluid on a FS_TILE_X x FS_TILE_Y tile, one thread per cell
+// (FS_MUSCL.cpp:148-327, FS_godunov.cpp:5-50).  LDS holds the tile's cell
+// primitives and one record per face of the tile (x-faces first, then
+// y-faces): the face's two solver inputs, later its flux.
+//  A  primitives of the tile + 2-cell halo -> sW
 //  B  per cell (tile + 1 halo): minmod slopes, boundary-extrapolated values,
-//     cell-level positivity fallback, Hancock half step -> LDS
-//  C  x- and y-face exact Riemann fluxes (face-level fallback) -> registers
-//     -> LDS (aliasing B's storage)
+//     cell-level positivity fallback, Hancock half step; each evolved
+//     boundary state goes to the record of the face it sits on
+//  C1 per face: the face-level fallback (FS_MUSCL.cpp:276-285) and, in
+//     FP64, the W->U->W round trip give the solver inputs; identical normal
+//     states take their exact flux now, the others keep their inputs in the
+//     record and go to a compact list so that C2 runs the Newton iteration
+//     on full wavefronts
+//  C2 the exact Riemann solves of the listed faces
 //  D  flux-difference update in the reference's accumulation order
 // work: {inst, tile_y, tile_x, fluid}
 // ---------------------------------------------------------------------------
@@ -246,108 +275,63 @@ __device__ __forceinline__ T minmod1(T dL, T dR) {
     return dR > T(0) ? fmax(T(0), fmin(dL, dR)) : fmin(T(0), fmax(dL, dR));
 }
 
-// Riemann-solver input states of face c of a flux tile (x-faces first,
-// then y-faces); false if no updated cell of the tile needs this face.
-// MUSCL: evolved boundary-extrapolated values with the face-level fallback
-// (FS_MUSCL.cpp:276-285) through W->U->W; Godunov: the stored states.
-template <typename R, bool MUSCL, typename T>
-__device__ __forceinline__ bool face_input(const Inst& I, int c, int i0, int j0, int lo_i, int lim_i, int lo_j,
-                                           int lim_j, T g, T pinf, const FluidF& Ff, const T (*sW)[FWY][FWX],
-                                           const T (*sE)[FEY][FEX], const void* Qi, V4<T>& WL, V4<T>& WR,
-                                           bool& px, unsigned& st) {
-    int la, lb, ra, rb, ci_lo, cj_lo, ci_hi, cj_hi;
+// face c of a tile: its two cells (lo = left / below) in tile coordinates
+__device__ __forceinline__ void face_cells(int c, bool& px, int& ra, int& ca, int& rb, int& cb) {
     if (c < NFX) {
-        const int rr = c / (FTX + 1), fc = c % (FTX + 1);
-        px = true;
-        la = rr + 2; lb = fc + 1; ra = rr + 2; rb = fc + 2;
-        ci_lo = i0 + rr; cj_lo = j0 + fc - 1; ci_hi = ci_lo; cj_hi = j0 + fc;
+        const int r = c / (FTX + 1), fc = c - r * (FTX + 1);
+        px = true; ra = r; ca = fc - 1; rb = r; cb = fc;
     } else {
         const int cc = c - NFX;
-        const int rr = cc / FTX, fc = cc % FTX;
-        px = false;
-        la = rr + 1; lb = fc + 2; ra = rr + 2; rb = fc + 2;
-        ci_lo = i0 + rr - 1; cj_lo = j0 + fc; ci_hi = i0 + rr; cj_hi = cj_lo;
+        const int r = cc / FTX, fc = cc - r * FTX;
+        px = false; ra = r - 1; ca = fc; rb = r; cb = fc;
     }
-    const bool lo_upd = ci_lo >= lo_i && ci_lo < lim_i && cj_lo >= lo_j && cj_lo < lim_j;
-    const bool hi_upd = ci_hi >= lo_i && ci_hi < lim_i && cj_hi >= lo_j && cj_hi < lim_j;
-    if (!lo_upd && !hi_upd) return false;
-    if constexpr (sizeof(T) == 4) {
-        // FP32 storage: the evolved values (or, if unphysical, the cell
-        // values) go to the solver as primitives, without the W->U->W round
-        // trip, which only adds rounding
-        if (MUSCL) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                WL.v[k] = px ? sE[4 + k][la - 1][lb - 1] : sE[12 + k][la - 1][lb - 1];
-                WR.v[k] = px ? sE[k][ra - 1][rb - 1] : sE[8 + k][ra - 1][rb - 1];
-            }
-            if (!phys_prims_f(Ff, WR) || !phys_prims_f(Ff, WL)) {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) { WL.v[k] = sW[k][la][lb]; WR.v[k] = sW[k][ra][rb]; }
-                if (!phys_prims_f(Ff, WR) || !phys_prims_f(Ff, WL)) st |= ST_UNPHYS;
-            }
-        } else {
-            const V4<R> a = ld4<R>(Qi, I, cidx(I, ci_lo, cj_lo));
-            const V4<R> b = ld4<R>(Qi, I, cidx(I, ci_hi, cj_hi));
-            V4<float> UL, UR;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) { UL.v[k] = (float)a.v[k]; UR.v[k] = (float)b.v[k]; }
-            if (!phys_state_f(UL) || !phys_state_f(UR)) st |= ST_UNPHYS;
-            WL = cons2prims_f(Ff, UL);
-            WR = cons2prims_f(Ff, UR);
-        }
-        return true;
-    }
-    V4<T> UL, UR;
-    if (MUSCL) {
-        V4<T> SL, SR;
-        bool phys = true;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            SL.v[k] = px ? sE[4 + k][la - 1][lb - 1] : sE[12 + k][la - 1][lb - 1];   // evolved R / T of lower cell
-            SR.v[k] = px ? sE[k][ra - 1][rb - 1] : sE[8 + k][ra - 1][rb - 1];        // evolved L / B of upper cell
-        }
-        phys = phys_prims(g, pinf, SR) && phys_prims(g, pinf, SL);
-        if (!phys) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) { SL.v[k] = sW[k][la][lb]; SR.v[k] = sW[k][ra][rb]; }
-        }
-        UL = prims2cons(g, pinf, SL);
-        UR = prims2cons(g, pinf, SR);
-    } else {
-        const V4<R> a = ld4<R>(Qi, I, cidx(I, ci_lo, cj_lo));
-        const V4<R> b = ld4<R>(Qi, I, cidx(I, ci_hi, cj_hi));
-#pragma unroll
-        for (int k = 0; k < 4; ++k) { UL.v[k] = (T)a.v[k]; UR.v[k] = (T)b.v[k]; }
-    }
-    if (!phys_state(UL) || !phys_state(UR)) st |= ST_UNPHYS;   // pure_RS_exact.cpp:18-19
-    WL = cons2prims(g, pinf, UL);
-    WR = cons2prims(g, pinf, UR);
-    return true;
 }
 
-// FP32 storage: 2 workgroups of 512 per CU (128 VGPRs); FP64 is insensitive
-// to the cap
+// First-order (Godunov) solver inputs of face c from the stored states
+// (pure_RS_exact.cpp:18-19 checks, W of both cells)
+template <typename R, typename T>
+__device__ __forceinline__ void face_input_godunov(const Inst& I, int i0, int j0, int ra, int ca, int rb, int cb,
+                                                   T g, T pinf, const FluidF& Ff, const void* Qi, V4<T>& WL,
+                                                   V4<T>& WR, unsigned& st) {
+    const V4<R> a = ld4<R>(Qi, I, cidx(I, i0 + ra, j0 + ca));
+    const V4<R> b = ld4<R>(Qi, I, cidx(I, i0 + rb, j0 + cb));
+    V4<T> UL, UR;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { UL.v[k] = (T)a.v[k]; UR.v[k] = (T)b.v[k]; }
+    if constexpr (sizeof(T) == 4) {
+        if (!phys_state_f(UL) || !phys_state_f(UR)) st |= ST_UNPHYS;
+        WL = cons2prims_f(Ff, UL);
+        WR = cons2prims_f(Ff, UR);
+    } else {
+        if (!phys_state(UL) || !phys_state(UR)) st |= ST_UNPHYS;
+        WL = cons2prims(g, pinf, UL);
+        WR = cons2prims(g, pinf, UR);
+    }
+}
+
+// FP32 storage: 3 workgroups of 512 per CU (48 KB of LDS each, at most 80
+// VGPRs); FP64 is insensitive to the cap
 template <typename R>
-struct FsMinBlocks { static constexpr int value = sizeof(R) == 4 ? 2 : 1; };
+struct FsMinBlocks { static constexpr int value = sizeof(R) == 4 ? 3 : 1; };
 
 template <typename R, bool MUSCL>
-__global__ __launch_bounds__(FS_THREADS, FsMinBlocks<R>::value) void k_fs(const Inst* __restrict__ insts, const int4* __restrict__ work,
-                                            int qin, int qout, int copy_inactive) {
+__global__ __launch_bounds__(FS_THREADS, FsMinBlocks<R>::value) void k_fs(const Inst* __restrict__ insts,
+                                                                         const int4* __restrict__ work, int qin,
+                                                                         int qout, int copy_inactive) {
     typedef R T;
     __shared__ T sW[4][FWY][FWX];
-    __shared__ T sE[MUSCL ? 16 : 1][FEY][FEX];
-    __shared__ T sFl[4][NFACE];
+    __shared__ V4A<T> sF[2][NFACE];   // per face: inputs lo / hi, then sF[0] = flux
     __shared__ short sHard[NFACE];
     __shared__ int nHard;
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
     const int f = w.w;
+    const int tid = threadIdx.x;
     if (!I.si[SI_ACTIVE]) {
         // a finished instance keeps its state; the one-fluid ping-pong still
         // needs it in the output buffer
         if (copy_inactive) {
-            const int ci = I.g + w.y * FTY + (threadIdx.x / FTX), cj = I.g + w.z * FTX + (threadIdx.x % FTX);
+            const int ci = I.g + w.y * FTY + (tid / FTX), cj = I.g + w.z * FTX + (tid % FTX);
             if (ci < I.g + I.Ny && cj < I.g + I.Nx) {
                 const long long c = cidx(I, ci, cj);
                 st4<R>(I.Q[f][qout], I, c, ld4<R>(I.Q[f][qin], I, c));
@@ -369,7 +353,6 @@ __global__ __launch_bounds__(FS_THREADS, FsMinBlocks<R>::value) void k_fs(const 
     const void* Qi = I.Q[f][qin];
     void* Qo = I.Q[f][qout];
     const bool tile_active = !(i0 + FTY <= bb0 || i0 >= bb1 || j0 + FTX <= bb2 || j0 >= bb3);
-    const int tid = threadIdx.x;
     if (tid == 0) nHard = 0;
     const int ci = i0 + (tid / FTX), cj = j0 + (tid % FTX);
     if (!tile_active) {
@@ -380,21 +363,28 @@ __global__ __launch_bounds__(FS_THREADS, FsMinBlocks<R>::value) void k_fs(const 
         return;
     }
     // A
-    for (int c = tid; c < FWY * FWX; c += FS_THREADS) {
-        const int a = c / FWX, b = c % FWX;
-        const int i = min(max(i0 - 2 + a, 0), I.H - 1);
-        const int j = min(max(j0 - 2 + b, 0), I.W - 1);
-        V4<T> W;
-        if constexpr (sizeof(T) == 4) W = cons2prims_f(Ff, ld4<R>(Qi, I, cidx(I, i, j)));
-        else W = cons2prims(g, pinf, ld4<R>(Qi, I, cidx(I, i, j)));
-        sW[0][a][b] = W.v[0]; sW[1][a][b] = W.v[1]; sW[2][a][b] = W.v[2]; sW[3][a][b] = W.v[3];
+    if (MUSCL) {
+        for (int c = tid; c < FWY * FWX; c += FS_THREADS) {
+            const int a = c / FWX, b = c % FWX;
+            const int i = min(max(i0 - 2 + a, 0), I.H - 1);
+            const int j = min(max(j0 - 2 + b, 0), I.W - 1);
+            V4<T> W;
+            if constexpr (sizeof(T) == 4) W = cons2prims_f(Ff, ld4<R>(Qi, I, cidx(I, i, j)));
+            else W = cons2prims(g, pinf, ld4<R>(Qi, I, cidx(I, i, j)));
+            sW[0][a][b] = W.v[0]; sW[1][a][b] = W.v[1]; sW[2][a][b] = W.v[2]; sW[3][a][b] = W.v[3];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     const T hx = T(0.5) * dtodx, hy = T(0.5) * dtody;
     if (MUSCL) {
-        // B
-        for (int c = tid; c < FEY * FEX; c += FS_THREADS) {
-            const int a = c / FEX + 1, b = c % FEX + 1;   // position in sW
+        // B: cell (r, c) of the tile (r in -1..FTY, c in -1..FTX) is at
+        // (r + 2, c + 2) in sW; its L / R states sit on x-faces (r, c) /
+        // (r, c + 1), its B / T states on y-faces (r, c) / (r + 1, c)
+        for (int e = tid; e < FEY * FEX; e += FS_THREADS) {
+            const int r = e / FEX - 1, cc = e % FEX - 1;
+            const bool rin = r >= 0 && r < FTY, cin = cc >= 0 && cc < FTX;
+            if (!rin && !cin) continue;   // tile corners: on no face
+            const int a = r + 2, b = cc + 2;
             V4<T> Wc, BL, BR, BB, BT;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -427,53 +417,84 @@ __global__ __launch_bounds__(FS_THREADS, FsMinBlocks<R>::value) void k_fs(const 
             d[1] = hx * (u * x1 + ir * x3) + hy * (v * y1);
             d[2] = hx * (u * x2) + hy * (v * y2 + ir * y3);
             d[3] = hx * (u * x3 + rc2 * x1) + hy * (v * y3 + rc2 * y2);
-            const int ea = a - 1, eb = b - 1;
+            if (rin) {
+                V4<T> EL, ER;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                sE[k][ea][eb] = BL.v[k] + d[k];
-                sE[4 + k][ea][eb] = BR.v[k] + d[k];
-                sE[8 + k][ea][eb] = BB.v[k] + d[k];
-                sE[12 + k][ea][eb] = BT.v[k] + d[k];
+                for (int k = 0; k < 4; ++k) { EL.v[k] = BL.v[k] + d[k]; ER.v[k] = BR.v[k] + d[k]; }
+                if (cc >= 0) tb_put<T>(sF[1], r * (FTX + 1) + cc, EL);        // right input of x-face (r, cc)
+                if (cc < FTX) tb_put<T>(sF[0], r * (FTX + 1) + cc + 1, ER);   // left input of x-face (r, cc + 1)
+            }
+            if (cin) {
+                V4<T> EB, ET;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) { EB.v[k] = BB.v[k] + d[k]; ET.v[k] = BT.v[k] + d[k]; }
+                if (r >= 0) tb_put<T>(sF[1], NFX + r * FTX + cc, EB);         // upper input of y-face (r, cc)
+                if (r < FTY) tb_put<T>(sF[0], NFX + (r + 1) * FTX + cc, ET);  // lower input of y-face (r + 1, cc)
             }
         }
         __syncthreads();
     }
-    // C1: every needed face gets its Riemann-solver input states (the
-    // conserved round trip of FS_MUSCL.cpp:287-288 included); identical normal
-    // states take their exact flux now, the others go to a compact list so
-    // that C2 runs the Newton iteration on full wavefronts
+    // C1
     const int lim_i = min(bb1, i0 + FTY), lim_j = min(bb3, j0 + FTX);
     const int lo_i = max(bb0, i0), lo_j = max(bb2, j0);
     unsigned st = 0;
     for (int c = tid; c < NFACE; c += FS_THREADS) {
-        V4<T> WL, WR;
         bool px;
-        if (!face_input<R, MUSCL>(I, c, i0, j0, lo_i, lim_i, lo_j, lim_j, g, pinf, Ff, sW, sE, Qi, WL, WR, px, st)) continue;
+        int ra, ca, rb, cb;
+        face_cells(c, px, ra, ca, rb, cb);
+        const bool lo_upd = i0 + ra >= lo_i && i0 + ra < lim_i && j0 + ca >= lo_j && j0 + ca < lim_j;
+        const bool hi_upd = i0 + rb >= lo_i && i0 + rb < lim_i && j0 + cb >= lo_j && j0 + cb < lim_j;
+        if (!lo_upd && !hi_upd) continue;   // no updated cell of the tile needs this face
+        V4<T> WL, WR;
+        if (MUSCL) {
+            const V4<T> SL = tb_get<T>(sF[0], c), SR = tb_get<T>(sF[1], c);
+            if constexpr (sizeof(T) == 4) {
+                // FP32 storage: the evolved values (or, if unphysical, the
+                // cell values) go to the solver as primitives, without the
+                // W->U->W round trip, which only adds rounding
+                WL = SL; WR = SR;
+                if (!phys_prims_f(Ff, WR) || !phys_prims_f(Ff, WL)) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) { WL.v[k] = sW[k][ra + 2][ca + 2]; WR.v[k] = sW[k][rb + 2][cb + 2]; }
+                    if (!phys_prims_f(Ff, WR) || !phys_prims_f(Ff, WL)) st |= ST_UNPHYS;
+                }
+            } else {
+                V4<T> PL = SL, PR = SR;
+                if (!(phys_prims(g, pinf, PR) && phys_prims(g, pinf, PL))) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) { PL.v[k] = sW[k][ra + 2][ca + 2]; PR.v[k] = sW[k][rb + 2][cb + 2]; }
+                }
+                const V4<T> UL = prims2cons(g, pinf, PL), UR = prims2cons(g, pinf, PR);
+                if (!phys_state(UL) || !phys_state(UR)) st |= ST_UNPHYS;   // pure_RS_exact.cpp:18-19
+                WL = cons2prims(g, pinf, UL);
+                WR = cons2prims(g, pinf, UR);
+            }
+        } else {
+            face_input_godunov<R, T>(I, i0, j0, ra, ca, rb, cb, g, pinf, Ff, Qi, WL, WR, st);
+        }
         if (rs_trivial(px, WL, WR)) {
             V4<T> F;
             if constexpr (sizeof(T) == 4) F = rs_flux_trivial_f(px, Ff, WL, WR);
             else F = rs_flux_trivial<T>(px, g, pinf, WL, WR);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) sFl[k][c] = F.v[k];
+            tb_put<T>(sF[0], c, F);
         } else {
+            tb_put<T>(sF[0], c, WL);
+            tb_put<T>(sF[1], c, WR);
             sHard[atomicAdd(&nHard, 1)] = (short)c;
         }
     }
     __syncthreads();
-    // C2: the exact Riemann solves of the listed faces
+    // C2
     const int nh = nHard;
     const RSFluid& rf = I.rsf[f];
     for (int h = tid; h < nh; h += FS_THREADS) {
         const int c = sHard[h];
-        V4<T> WL, WR;
-        bool px;
-        unsigned dummy = 0;
-        face_input<R, MUSCL>(I, c, i0, j0, lo_i, lim_i, lo_j, lim_j, g, pinf, Ff, sW, sE, Qi, WL, WR, px, dummy);
+        const bool px = c < NFX;
+        const V4<T> WL = tb_get<T>(sF[0], c), WR = tb_get<T>(sF[1], c);
         V4<T> F;
         if constexpr (sizeof(T) == 4) F = rs_flux_full_f(rf, Ff, px, WL, WR, st);
         else F = rs_flux_full(px, g, pinf, WL, WR, st);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) sFl[k][c] = F.v[k];
+        tb_put<T>(sF[0], c, F);
     }
     flag(I, st);
     __syncthreads();
@@ -487,14 +508,15 @@ __global__ __launch_bounds__(FS_THREADS, FsMinBlocks<R>::value) void k_fs(const 
             else ok = phys_state(U);
             if (!ok) flag(I, ST_UNPHYS);   // FS_MUSCL.cpp:167
             const int rr = tid / FTX, fc = tid % FTX;
-            const int fxl = rr * (FTX + 1) + fc, fxr = fxl + 1;
-            const int fyb = NFX + rr * FTX + fc, fyt = fyb + FTX;
+            const int fxl = rr * (FTX + 1) + fc, fyb = NFX + rr * FTX + fc;
+            const V4<T> Fl = tb_get<T>(sF[0], fxl), Fr = tb_get<T>(sF[0], fxl + 1);
+            const V4<T> Fb = tb_get<T>(sF[0], fyb), Ft = tb_get<T>(sF[0], fyb + FTX);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 T u = (T)U.v[k];
-                u = u + (dtodx * sFl[k][fxl] + dtody * sFl[k][fyb]);   // :290
-                u = u - dtodx * sFl[k][fxr];                             // :291 of the next column
-                u = u - dtody * sFl[k][fyt];                             // :292 of the next row
+                u = u + (dtodx * Fl.v[k] + dtody * Fb.v[k]);   // :290
+                u = u - dtodx * Fr.v[k];                         // :291 of the next column
+                u = u - dtody * Ft.v[k];                         // :292 of the next row
                 U.v[k] = (R)u;
             }
         }
@@ -769,28 +791,6 @@ __global__ __launch_bounds__(256) void k_extrap(const Inst* __restrict__ insts, 
 // work: {inst, tile_y, tile_x, 0}, TBX x TBY interior tiles
 // ---------------------------------------------------------------------------
 constexpr int TBX = 64, TBY = 16, TB_THREADS = 512;
-
-// LDS cell of the blocked kernels: one fluid's four primitives, aligned so
-// that a cell moves in one 16-byte (FP32) or two (FP64) LDS accesses
-template <typename R>
-struct alignas(4 * sizeof(R) > 16 ? 16 : 4 * sizeof(R)) V4A { R v[4]; };
-
-template <typename R>
-__device__ __forceinline__ V4<R> tb_get(const V4A<R>* s, int k) {
-    const V4A<R> a = s[k];
-    V4<R> W;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) W.v[q] = a.v[q];
-    return W;
-}
-
-template <typename R>
-__device__ __forceinline__ void tb_put(V4A<R>* s, int k, const V4<R>& W) {
-    V4A<R> a;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) a.v[q] = W.v[q];
-    s[k] = a;
-}
 
 // new ghost-fluid primitives of an LDS cell (extrap_cell's arithmetic) from
 // its current ones c0 and those of its upwind x and y neighbours at LDS
