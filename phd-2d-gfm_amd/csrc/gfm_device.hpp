@@ -606,8 +606,10 @@ __device__ __forceinline__ V4<double> rs_flux_full(bool px, double g, double pin
 // pressure change, status bits) evaluated in FP32 with the hardware
 // reciprocal, square root, exp2 and log2 (about 1 ulp each).  The
 // rarefaction function k ((pbar / pbarK)^z - 1) is formed without
-// cancellation (powm1_f).  The iteration also stops once its step is at FP32
-// resolution (4 ulp of p), which only matters where p + pinf << pinf.
+// cancellation (powm1_f).  The iteration also stops once it is at FP32
+// resolution: a step of at most 4 ulp of p (matters where p + pinf << pinf),
+// or a residual fL + fR + du within 4 ulp of its terms (strong waves, where
+// the rounding of the terms moves the Newton step by more than 1e-6 p).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -710,13 +712,18 @@ __device__ __forceinline__ unsigned rs_star_f(const RSFluid& F, const RSSideF& L
     for (int it = 0; it < 60; ++it) {
         fL = rs_f_f(F, L, p, dL);
         fR = rs_f_f(F, R, p, dR);
-        float pn = p - (fL + fR + du) * rcp_hw(dL + dR);
+        const float res = fL + fR + du;
+        float pn = p - res * rcp_hw(dL + dR);
         if (pn <= plow) pn = 0.5f * (p + plow);
         const float chg = fabsf(pn - p);
         const float ref = 0.5f * (pn + p) - plow;
         dp = pn - p;
         p = pn;
-        if (chg <= 1e-6f * ref || chg <= 4.8e-7f * fabsf(p)) { conv = true; break; }
+        if (chg <= 1e-6f * ref || chg <= 4.8e-7f * fabsf(p) ||
+            fabsf(res) <= 4.8e-7f * (fabsf(fL) + fabsf(fR) + fabsf(du))) {
+            conv = true;
+            break;
+        }
     }
     if (!conv || p != p) st |= ST_RS;
     // u* from the pressure functions at the converged p, to first order in
