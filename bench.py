@@ -55,7 +55,7 @@ SEED = 20261203
 # HBM bytes per interior cell (both fluids) of one flux-kernel launch, from
 # rocprof FETCH_SIZE / WRITE_SIZE scaled by a device copy of known size
 # (profiles/k_fs_batch64_mixed_traffic.md); measured for the default workload
-K_FS_TRAFFIC_PER_CELL = {"batch": 66.5}
+K_FS_TRAFFIC_PER_CELL = {"batch": 66.1}
 GFM_LABEL = {G.GFM_ORIGINAL: "OGFM", G.GFM_REAL: "realGFM", G.GFM_RIEMANN: "riemannGFM", G.GFM_MODIFIED: "MGFM"}
 
 
