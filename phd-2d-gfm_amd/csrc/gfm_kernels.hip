@@ -309,13 +309,14 @@ __device__ __forceinline__ void face_input_godunov(const Inst& I, int i0, int j0
     }
 }
 
-// FP32 storage: 3 workgroups of 512 per CU (48 KB of LDS each, at most 80
-// VGPRs); FP64 is insensitive to the cap
+// minimum waves per SIMD (the second __launch_bounds__ argument on AMD
+// GPUs).  FP32 storage: 6, i.e. 3 workgroups of 512 per CU (48 KB of LDS
+// each) with at most 80 VGPRs; FP64 is insensitive to the cap
 template <typename R>
-struct FsMinBlocks { static constexpr int value = sizeof(R) == 4 ? 3 : 1; };
+struct FsWaves { static constexpr int value = sizeof(R) == 4 ? 6 : 1; };
 
 template <typename R, bool MUSCL>
-__global__ __launch_bounds__(FS_THREADS, FsMinBlocks<R>::value) void k_fs(const Inst* __restrict__ insts,
+__global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst* __restrict__ insts,
                                                                          const int4* __restrict__ work, int qin,
                                                                          int qout, int copy_inactive) {
     typedef R T;
@@ -1566,7 +1567,8 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
     const R dt = (R)I.sd[SD_DT];
     const R idxr = R(1.0 / I.dx), idyr = R(1.0 / I.dy);
     // per cell: phi, velocity; meta bits 0-7: distance to the region edge,
-    // 8: interior, 9: owned
+    // 8: interior, 9: owned, 10: within RAD of the domain edge (its new
+    // value refreshes mirror ghost cells)
     R p0[SLOTS], us[SLOTS], vs[SLOTS];
     int meta[SLOTS];
 #pragma unroll
@@ -1585,7 +1587,8 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
                     vs[k] = vel[c + I.plane];
                     const int d = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
                     const bool own = a >= HALO && a < HALO + LTY && b >= HALO && b < HALO + LTX;
-                    meta[k] = d | 256 | (own ? 512 : 0);
+                    const bool edge = i - g < RAD || g + I.Ny - 1 - i < RAD || j - g < RAD || g + I.Nx - 1 - j < RAD;
+                    meta[k] = d | 256 | (own ? 512 : 0) | (edge ? 1024 : 0);
                 }
             }
             p0[k] = val;
@@ -1605,13 +1608,15 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
             const R rhs = houc_rhs<R>(H, N, in + idx, 1, RX, us[k], vs[k], idxr, idyr);
             const R nv = st == 1 ? rk3_combine<R, 1>(p0[k], R(0), dt, rhs) : rk3_combine<R, 2>(p0[k], in[idx], dt, rhs);
             out[idx] = nv;
-            // mirror ghosts within the stencil radius of the boundary
-            const int a = idx / RX, b = idx - (idx / RX) * RX;
-            const int i = ri + a, j = rj + b;
-            if (i - ib < RAD) { const int ga = a - 2 * (i - ib) - 1; if (ga >= 0) out[ga * RX + b] = nv; }
-            if (it - i < RAD) { const int ga = a + 2 * (it - i) + 1; if (ga < RY) out[ga * RX + b] = nv; }
-            if (j - jl < RAD) { const int gb = b - 2 * (j - jl) - 1; if (gb >= 0) out[a * RX + gb] = nv; }
-            if (jr - j < RAD) { const int gb = b + 2 * (jr - j) + 1; if (gb < RX) out[a * RX + gb] = nv; }
+            if (meta[k] & 1024) {
+                // mirror ghosts within the stencil radius of the boundary
+                const int a = idx / RX, b = idx - (idx / RX) * RX;
+                const int i = ri + a, j = rj + b;
+                if (i - ib < RAD) { const int ga = a - 2 * (i - ib) - 1; if (ga >= 0) out[ga * RX + b] = nv; }
+                if (it - i < RAD) { const int ga = a + 2 * (it - i) + 1; if (ga < RY) out[ga * RX + b] = nv; }
+                if (j - jl < RAD) { const int gb = b - 2 * (j - jl) - 1; if (gb >= 0) out[a * RX + gb] = nv; }
+                if (jr - j < RAD) { const int gb = b + 2 * (jr - j) + 1; if (gb < RX) out[a * RX + gb] = nv; }
+            }
         }
         __syncthreads();
     }
@@ -1775,8 +1780,13 @@ __device__ __forceinline__ R* ls_buf(const Inst& I, int sel) {
     return (R*)(sel == 0 ? I.phiA : (sel == 1 ? I.phi : I.phiB));
 }
 
+// minimum waves per SIMD: FP32 8, i.e. two workgroups of 1024 per CU with
+// at most 64 VGPRs; FP64 fits one workgroup
 template <typename R>
-__global__ __launch_bounds__(RT_THREADS) void k_ls_reinit_tb(const Inst* __restrict__ insts,
+struct RtWaves { static constexpr int value = sizeof(R) == 4 ? 8 : 1; };
+
+template <typename R>
+__global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(const Inst* __restrict__ insts,
                                                              const int4* __restrict__ work, int nit, int in_sel,
                                                              int out_sel, int gate_every, int force, int copy_idle) {
     constexpr int RX = RTX + 2 * RT_MAXIT, RY = RTY + 2 * RT_MAXIT, NC = RX * RY;
@@ -1795,7 +1805,10 @@ __global__ __launch_bounds__(RT_THREADS) void k_ls_reinit_tb(const Inst* __restr
                 for (int q = threadIdx.x; q < RTX * RTY; q += RT_THREADS) {
                     const int i = g + w.y * RTY + q / RTX, j = g + w.z * RTX + q % RTX;
                     if (!interior(I, i, j)) continue;
-                    for (int m = 0; m < 9; ++m) {
+                    const long long c0 = cidx(I, i, j);
+                    phi[c0] = A[c0];
+                    if (!(i < 2 * g || i >= I.Ny || j < 2 * g || j >= I.Nx)) continue;   // no ghost image
+                    for (int m = 1; m < 9; ++m) {
                         const int pr = m / 3, pc = m - 3 * (m / 3);
                         int r = i, cc = j;
                         if (pr == 1) { if (i >= 2 * g) continue; r = 2 * g - 1 - i; }
@@ -1815,8 +1828,10 @@ __global__ __launch_bounds__(RT_THREADS) void k_ls_reinit_tb(const Inst* __restr
     const R* in = ls_buf<R>(I, in_sel);
     R* out = ls_buf<R>(I, out_sel);
     // per cell: reinit_setup's mode and constant; bit 8: interior with its
-    // stencil inside the region, bit 9: owned, bits 10+: distance to the
+    // stencil inside the region, bit 9: owned, bit 10: next to the domain
+    // edge (refreshes the first ghost layer), bits 11+: distance to the
     // region edge
+    const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
     R cs[SLOTS];
     int meta[SLOTS];
 #pragma unroll
@@ -1833,15 +1848,15 @@ __global__ __launch_bounds__(RT_THREADS) void k_ls_reinit_tb(const Inst* __restr
                     int mode;
                     reinit_cell_setup<R>(I, P0, i, j, mode, cs[k]);
                     const bool own = a >= RT_MAXIT && a < RT_MAXIT + RTY && b >= RT_MAXIT && b < RT_MAXIT + RTX;
+                    const bool edge = i == ib || i == it || j == jl || j == jr;
                     const int dist = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
-                    meta[k] = mode | 256 | (own ? 512 : 0) | (dist << 10);
+                    meta[k] = mode | 256 | (own ? 512 : 0) | (edge ? 1024 : 0) | (dist << 11);
                 }
             }
             s[idx] = v;
         }
     }
     __syncthreads();
-    const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
     const ReinitK<R> rk = reinit_k<R>(I);
     for (int t = 1; t <= nit; ++t) {
         R nv[SLOTS];
@@ -1849,7 +1864,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_ls_reinit_tb(const Inst* __restr
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
             const int idx = threadIdx.x + k * RT_THREADS;
-            if ((meta[k] & 256) && (meta[k] >> 10) >= t) {
+            if ((meta[k] & 256) && (meta[k] >> 11) >= t) {
                 act |= 1 << k;
                 nv[k] = reinit_update<R>(meta[k] & 7, cs[k], s[idx], s[idx - 1], s[idx + 1], s[idx - RX], s[idx + RX],
                                          rk.idx2, rk.idy2, rk.dtau);
@@ -1884,13 +1899,15 @@ __global__ __launch_bounds__(RT_THREADS) void k_ls_reinit_tb(const Inst* __restr
         for (int k = 0; k < SLOTS; ++k) {
             if (!(act & (1 << k))) continue;
             const int idx = threadIdx.x + k * RT_THREADS;
-            const int a = idx / RX, b = idx - (idx / RX) * RX;
-            const int i = ri + a, j = rj + b;
             s[idx] = nv[k];
-            if (i == ib) s[idx - RX] = nv[k];
-            if (i == it) s[idx + RX] = nv[k];
-            if (j == jl) s[idx - 1] = nv[k];
-            if (j == jr) s[idx + 1] = nv[k];
+            if (meta[k] & 1024) {
+                const int a = idx / RX, b = idx - (idx / RX) * RX;
+                const int i = ri + a, j = rj + b;
+                if (i == ib) s[idx - RX] = nv[k];
+                if (i == it) s[idx + RX] = nv[k];
+                if (j == jl) s[idx - 1] = nv[k];
+                if (j == jr) s[idx + 1] = nv[k];
+            }
         }
         __syncthreads();
     }
