@@ -414,6 +414,15 @@ __device__ __forceinline__ void rs_sample(const RSSide& L, const RSSide& R, cons
 // |pbar / pbarK - 1| < 1/64 (no cancellation) and from pow_fast otherwise,
 // and the rarefaction derivative uses z k r^(z-1) / pbarK = (c / g) r^z / pbar.
 // ---------------------------------------------------------------------------
+// FP32 hardware reciprocal, square root (about 1 ulp) and power
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// x^y = exp2(y log2 x), x >= 0: relative error about (2 + |y log2 x|) ulp
+__device__ __forceinline__ float pow_hw(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+
 // one side: rho, u_n, p, pbar = p + pinf, 1 / pbar, 1 / rho, c; the
 // shock and rarefaction constants A = a2 / rho, B = g6 pbar, k = k2 c and
 // c / g are formed where used
@@ -475,7 +484,8 @@ __device__ __forceinline__ double rs_f_m(const RSFluid& F, const RSSideM& s, dou
 // per-side constants, Newton start p0 when it lies above the floor (else
 // rs_guess's choice), tolerance tol, and the star densities (Toro 4.5.1,
 // 4.5.2: shock rho (r + g6) / (g6 r + 1), rarefaction rho r^(1/g), r =
-// (p* + pinf) / (p + pinf)).  Only the evaluation differs from rs_star.
+// (p* + pinf) / (p + pinf), the power in FP32: the densities are stored in
+// FP32).  Only the evaluation differs from rs_star.
 __device__ __forceinline__ unsigned rs_star_m2(const RSFluid& FL, const RSFluid& FR, const RSSideM& L,
                                                const RSSideM& R, Star& S, double p0, double tol) {
     const bool same = FL.g == FR.g && FL.pinf == FR.pinf;
@@ -526,11 +536,11 @@ __device__ __forceinline__ unsigned rs_star_m2(const RSFluid& FL, const RSFluid&
     S.u = 0.5 * (L.u + R.u) + 0.5 * ((fR + dR * dp) - (fL + dL * dp));
     {
         const double r = (p + FL.pinf) * L.ipb;
-        S.rl = p > L.p ? L.rho * (r + FL.g6) * rcp_nr(FL.g6 * r + 1.0) : L.rho * pow_fast(r, FL.ig);
+        S.rl = p > L.p ? L.rho * (r + FL.g6) * rcp_nr(FL.g6 * r + 1.0) : L.rho * (double)pow_hw((float)r, FL.fig);
     }
     {
         const double r = (p + FR.pinf) * R.ipb;
-        S.rr = p > R.p ? R.rho * (r + FR.g6) * rcp_nr(FR.g6 * r + 1.0) : R.rho * pow_fast(r, FR.ig);
+        S.rr = p > R.p ? R.rho * (r + FR.g6) * rcp_nr(FR.g6 * r + 1.0) : R.rho * (double)pow_hw((float)r, FR.fig);
     }
     return st;
 }
@@ -611,14 +621,6 @@ __device__ __forceinline__ V4<double> rs_flux_full(bool px, double g, double pin
 // or a residual fL + fR + du within 4 ulp of its terms (strong waves, where
 // the rounding of the terms moves the Newton step by more than 1e-6 p).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
-
-// x^y = exp2(y log2 x), x >= 0: relative error about (2 + |y log2 x|) ulp
-__device__ __forceinline__ float pow_hw(float x, float y) {
-    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
-}
-
 // (1 + eps)^z - 1 for eps > -1 and |z| < 1, with r = 1 + eps as computed by
 // the caller (used only where |eps| > 1/4): log(1 + eps) = 2 atanh(s),
 // s = eps / (2 + eps), summed to s^9 for |eps| <= 1/4 (|s| <= 1/7, truncation

@@ -1154,7 +1154,10 @@ __device__ __forceinline__ unsigned iface_mrp(const Inst& I, const double* ip1, 
         RSSideM L, R;
         rs_side_m(L, F1, Lp.v[0], Lp.v[1], Lp.v[3]);
         rs_side_m(R, F2, Rp.v[0], Rp.v[1], Rp.v[3]);
-        st |= rs_star_m2(F1, F2, L, R, S, p0, 1e-12);
+        // tolerance 1e-7: with Newton's quadratic convergence the star
+        // pressure is then within ~1e-13 relative, far below the FP32 storage
+        // it goes to
+        st |= rs_star_m2(F1, F2, L, R, S, p0, 1e-7);
         return st;
     }
     const V4<double> Ls = prims2cons(I.gam[0], I.pinf[0], Lp);
