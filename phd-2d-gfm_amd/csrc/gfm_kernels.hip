@@ -9,9 +9,23 @@
 
 namespace gfm {
 
+// every array of an instance is device global memory: pointers cast to the
+// global address space so that accesses compile to global (not flat)
+// instructions, which do not count against the LDS / scalar wait counter.
+// The host pass of this file only parses device code; it keeps the pointer.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+#else
+template <typename T>
+__device__ __forceinline__ T* gmem(T* p) { return p; }
+#endif
+
 template <typename R>
 __device__ __forceinline__ V4<R> ld4(const void* base, const Inst& I, long long c) {
-    const R* p = (const R*)base;
+    const auto p = gmem((const R*)base);
     V4<R> a;
     a.v[0] = p[c];
     a.v[1] = p[c + I.plane];
@@ -22,7 +36,7 @@ __device__ __forceinline__ V4<R> ld4(const void* base, const Inst& I, long long 
 
 template <typename R>
 __device__ __forceinline__ void st4(void* base, const Inst& I, long long c, const V4<R>& a) {
-    R* p = (R*)base;
+    const auto p = gmem((R*)base);
     p[c] = a.v[0];
     p[c + I.plane] = a.v[1];
     p[c + 2 * I.plane] = a.v[2];
@@ -30,7 +44,7 @@ __device__ __forceinline__ void st4(void* base, const Inst& I, long long c, cons
 }
 
 template <typename R>
-__device__ __forceinline__ R ld1(const void* base, long long c) { return ((const R*)base)[c]; }
+__device__ __forceinline__ R ld1(const void* base, long long c) { return gmem((const R*)base)[c]; }
 
 __device__ __forceinline__ void flag(const Inst& I, unsigned bits) {
     if (bits) atomicOr((unsigned*)&I.si[SI_STATUS], bits);
@@ -50,7 +64,7 @@ __device__ __forceinline__ bool is_iface(const Inst& I, int i, int j) {
 // (GFM_original.cpp:47-55)
 __device__ __forceinline__ void reset_boxes(const Inst& I, int narrow) {
     for (int f = 0; f < 2; ++f) {
-        int* b = I.si + SI_BBOX + 4 * f;
+        const auto b = gmem(I.si) + SI_BBOX + 4 * f;
         if (narrow) { b[0] = I.g + I.Ny; b[1] = I.g; b[2] = I.g + I.Nx; b[3] = I.g; }
         else { b[0] = I.g; b[1] = I.g + I.Ny; b[2] = I.g; b[3] = I.g + I.Nx; }
     }
@@ -70,7 +84,7 @@ constexpr int SEGW = 256, SEG_Q = SEGW / 64;
 // [s SEGW, (s+1) SEGW), which covers the list since it has at most one entry
 // per interior cell
 __device__ __forceinline__ void erase_prev_list(const Inst& I, int s, int lane) {
-    const int n = I.si[SI_NLIST];
+    const int n = gmem(I.si)[SI_NLIST];
 #pragma unroll
     for (int q = 0; q < SEG_Q; ++q) {
         const int k = s * SEGW + q * 64 + lane;
@@ -179,8 +193,8 @@ __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, cons
 __global__ __launch_bounds__(256) void k_dt_final(const Inst* __restrict__ insts, const int4* __restrict__ work,
                                                   int twofluid, int narrow_boxes) {
     const Inst& I = insts[work[blockIdx.x].x];
-    const int nstep = I.si[SI_NSTEP];
-    const double t = I.sd[SD_T];
+    const int nstep = gmem(I.si)[SI_NSTEP];
+    const double t = gmem(I.sd)[SD_T];
     const bool active = (t < I.T) && (nstep < I.max_steps);
     __shared__ double red[4][256];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -204,17 +218,17 @@ __global__ __launch_bounds__(256) void k_dt_final(const Inst* __restrict__ insts
     }
     if (threadIdx.x != 0) return;
     const double m0 = red[0][0], m1 = red[1][0], m2 = red[2][0], m3 = red[3][0];
-    I.si[SI_ACTIVE] = active ? 1 : 0;
-    I.si[SI_NIFACE] = 0;
+    gmem(I.si)[SI_ACTIVE] = active ? 1 : 0;
+    gmem(I.si)[SI_NIFACE] = 0;
     reset_boxes(I, narrow_boxes);
     if (active) {
         const double cfl = nstep < 5 ? fmin(I.cfl, 0.2) : I.cfl;
         double dt = cfl * fmin(I.dx / m0, I.dy / m1);
         if (t + dt > I.T) dt = I.T - t;
-        I.sd[SD_DT] = dt;
+        gmem(I.sd)[SD_DT] = dt;
     }
-    I.sd[SD_MASS1] = m2;
-    I.sd[SD_MASS2] = m3;
+    gmem(I.sd)[SD_MASS1] = m2;
+    gmem(I.sd)[SD_MASS2] = m3;
     // fluid masses of the state at t_n (output_mass_error, sim_twofluid.cpp:298-331)
     if (twofluid && nstep < I.hist_cap) {
         double* h = I.mass_hist + 3 * (long long)nstep;
@@ -328,7 +342,7 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
     const Inst& I = insts[w.x];
     const int f = w.w;
     const int tid = threadIdx.x;
-    if (!I.si[SI_ACTIVE]) {
+    if (!gmem(I.si)[SI_ACTIVE]) {
         // a finished instance keeps its state; the one-fluid ping-pong still
         // needs it in the output buffer
         if (copy_inactive) {
@@ -342,12 +356,12 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
     }
     const T g = (T)I.gam[f], pinf = (T)I.pinf[f];
     const FluidF Ff = fluid_f(I.rsf[f]);   // FP32 constants (mixed precision only)
-    const double dtd = I.sd[SD_DT];
+    const double dtd = gmem(I.sd)[SD_DT];
     const T dtodx = (T)(dtd / I.dx), dtody = (T)(dtd / I.dy);
     const int i0 = I.g + w.y * FTY, j0 = I.g + w.z * FTX;
     int bb0 = I.g, bb1 = I.g + I.Ny, bb2 = I.g, bb3 = I.g + I.Nx;
     if (I.use_bbox) {
-        const int* b = I.si + SI_BBOX + 4 * f;
+        const auto b = gmem(I.si) + SI_BBOX + 4 * f;
         bb0 = max(b[0], I.g); bb1 = min(b[1], I.g + I.Ny);
         bb2 = max(b[2], I.g); bb3 = min(b[3], I.g + I.Nx);
     }
@@ -578,7 +592,7 @@ __device__ __forceinline__ void unit_normal_r(const Inst& I, const ExK<R>& K, in
         nx = a; ny = b;
     } else {
 #pragma clang fp contract(off)
-        const float* phi = (const float*)I.phi;
+        const auto phi = gmem((const float*)I.phi);
         const float gx = (phi[cidx(I, i, j + 1)] - phi[cidx(I, i, j - 1)]) * K.i2dx;
         const float gy = (phi[cidx(I, i + 1, j)] - phi[cidx(I, i - 1, j)]) * K.i2dy;
         const float m2 = gx * gx + gy * gy;
@@ -598,11 +612,11 @@ __device__ __forceinline__ V4<R> ex_load(const Inst& I, int f, int i, int j, con
         W = ld4<R>(src, I, c);
     }
     if (MODE & MF_INJECT) {
-        const int k = I.imap[c];
+        const int k = gmem(I.imap)[c];
         if (k >= 0) {
             const int fr = ld1<R>(I.phi, c) <= R(0) ? 0 : 1;
             if (fr == f) {
-                const double* s = I.side + 8 * (long long)k;
+                const auto s = gmem(I.side) + 8 * (long long)k;
                 W.v[0] = (R)s[0]; W.v[1] = (R)s[1]; W.v[2] = (R)s[2]; W.v[3] = (R)s[3];
             }
         }
@@ -704,7 +718,7 @@ __device__ __forceinline__ void extrap_put(const Inst& I, int i, int j, int bout
     if (MODE & MF_LAST) {
         st4<R>(I.Q[0][bout], I, c, o.a);
         st4<R>(I.Q[1][bout], I, c, o.b);
-        R* vel = (R*)I.vel;
+        const auto vel = gmem((R*)I.vel);
         vel[c] = o.vu;
         vel[c + I.plane] = o.vv;
     } else if (MODE & MF_RAW) {
@@ -754,7 +768,7 @@ __global__ __launch_bounds__(256) void k_extrap(const Inst* __restrict__ insts, 
                                                 int bin, int bout) {
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!I.si[SI_ACTIVE]) return;
+    if (!gmem(I.si)[SI_ACTIVE]) return;
     __shared__ int sbb[8];
     if (MODE & MF_OGFM) {
         if (threadIdx.x < 8) sbb[threadIdx.x] = (threadIdx.x & 1) ? INT_MIN : INT_MAX;
@@ -811,7 +825,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
     __shared__ int sbb[8];
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!I.si[SI_ACTIVE]) return;
+    if (!gmem(I.si)[SI_ACTIVE]) return;
     const int g = I.g;
     const int ri = g + w.y * TBY - NIT, rj = g + w.z * TBX - NIT;   // region origin
     if (MOUT & MF_OGFM) {
@@ -994,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_iface_count(const Inst* __restrict__ in
     const Inst& I = insts[w.x];
     const int lane = threadIdx.x & 63;
     erase_prev_list(I, w.w, lane);
-    if (!I.si[SI_ACTIVE]) return;
+    if (!gmem(I.si)[SI_ACTIVE]) return;
     const unsigned m = seg_iface<R>(I, w, lane);
     int nc = 0;
 #pragma unroll
@@ -1007,7 +1021,7 @@ __global__ __launch_bounds__(256) void k_iface_count(const Inst* __restrict__ in
 __global__ __launch_bounds__(1024) void k_iface_scan(const Inst* __restrict__ insts, const int4* __restrict__ work) {
     __shared__ int part[1024];
     const Inst& I = insts[work[blockIdx.x].x];
-    if (!I.si[SI_ACTIVE]) return;
+    if (!gmem(I.si)[SI_ACTIVE]) return;
     const int n = I.nseg, tid = threadIdx.x;
     const int per = (n + 1023) / 1024;
     const int b0 = min(n, tid * per), b1 = min(n, b0 + per);
@@ -1028,8 +1042,8 @@ __global__ __launch_bounds__(1024) void k_iface_scan(const Inst* __restrict__ in
         acc += c;
     }
     if (tid == 1023) {
-        I.si[SI_NIFACE] = part[1023];
-        I.si[SI_NLIST] = min(part[1023], I.icap);   // the entries k_iface_mark records
+        gmem(I.si)[SI_NIFACE] = part[1023];
+        gmem(I.si)[SI_NLIST] = min(part[1023], I.icap);   // the entries k_iface_mark records
     }
 }
 
@@ -1040,10 +1054,10 @@ __global__ __launch_bounds__(256) void k_iface_mark(const Inst* __restrict__ ins
     if (sw >= nwork) return;
     const int4 w = work[sw];
     const Inst& I = insts[w.x];
-    if (!I.si[SI_ACTIVE]) return;
+    if (!gmem(I.si)[SI_ACTIVE]) return;
     const int lane = threadIdx.x & 63;
     int k = I.iseg[w.w];
-    const int next = w.w + 1 < I.nseg ? I.iseg[w.w + 1] : I.si[SI_NIFACE];
+    const int next = w.w + 1 < I.nseg ? I.iseg[w.w + 1] : gmem(I.si)[SI_NIFACE];
     if (next == k) return;   // no interfacial cell in this segment
     const unsigned m = seg_iface<R>(I, w, lane);
     const int i = I.g + w.y;
@@ -1258,8 +1272,8 @@ __global__ __launch_bounds__(256) void k_iface_prep(const Inst* __restrict__ ins
                                                     int pbuf) {
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!I.si[SI_ACTIVE]) return;
-    const int n = min(I.si[SI_NIFACE], I.icap);
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const int n = min(gmem(I.si)[SI_NIFACE], I.icap);
     const int k = w.y + threadIdx.x;
     if (k >= n) return;
     const int lin = I.ilist[k];
@@ -1321,11 +1335,11 @@ __global__ __launch_bounds__(256) void k_iface_prep(const Inst* __restrict__ ins
     probe_values<VAR>(I, r, false, ip1, ip2);
     st |= iface_finish<VAR, (sizeof(R) == 4)>(I, r, ip1, ip2, -1e300, out, &ps);
     r.pspec = ps;
-    IfRec* recs = (IfRec*)I.irec;
+    const auto recs = gmem((IfRec*)I.irec);
     recs[k] = r;
-    I.idone[k] = has_dep ? 0 : 1;
+    gmem(I.idone)[k] = has_dep ? 0 : 1;
     if (!has_dep) {
-        double* o = I.side + 8 * (long long)k;
+        const auto o = gmem(I.side) + 8 * (long long)k;
         for (int q = 0; q < 5; ++q) o[q] = out[q];
     }
     flag(I, st);
@@ -1363,8 +1377,8 @@ __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ i
     __shared__ int nclaim;                    // next unclaimed entry
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!I.si[SI_ACTIVE]) return;
-    const int n = min(I.si[SI_NIFACE], I.icap);
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const int n = min(gmem(I.si)[SI_NIFACE], I.icap);
     if (n > ICAP_LDS) { if (threadIdx.x == 0) flag(I, ST_IOVF); return; }
     const IfRec* recs = (const IfRec*)I.irec;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -1436,7 +1450,7 @@ __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ i
         }
     }
     flag(I, st);
-    if (tid == 0) I.si[SI_ROUNDS] = rounds + 1;
+    if (tid == 0) gmem(I.si)[SI_ROUNDS] = rounds + 1;
 }
 
 // GFM_modified write-back (GFM_modified.cpp:37-47, :230-239): the grids are
@@ -1449,7 +1463,7 @@ __global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ ins
                                                     int qin, int qout) {
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!I.si[SI_ACTIVE]) return;
+    if (!gmem(I.si)[SI_ACTIVE]) return;
     const int i = w.y * 4 + (threadIdx.x >> 6);
     const int j = w.z * 64 + (threadIdx.x & 63);
     if (i >= I.H || j >= I.W) return;
@@ -1457,7 +1471,7 @@ __global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ ins
     V4<R> W1 = cons2prims<R>((R)I.gam[0], (R)I.pinf[0], ld4<R>(I.Q[0][qin], I, c));
     V4<R> W2 = cons2prims<R>((R)I.gam[1], (R)I.pinf[1], ld4<R>(I.Q[1][qin], I, c));
     const int fr = ld1<R>(I.phi, c) <= R(0) ? 0 : 1;
-    R* vel = (R*)I.vel;
+    const auto vel = gmem((R*)I.vel);
     vel[c] = fr == 0 ? W1.v[1] : W2.v[1];
     vel[c + I.plane] = fr == 0 ? W1.v[2] : W2.v[2];
     if (interior(I, i, j)) {
@@ -1505,7 +1519,7 @@ __device__ __forceinline__ R houc_rhs(const Houc& H, int n, const R* psi, long l
 template <typename R>
 __device__ __forceinline__ R ls_rhs(const Inst& I, const Houc& H, const R* psi, int i, int j) {
     const long long c = cidx(I, i, j);
-    const R* vel = (const R*)I.vel;
+    const auto vel = gmem((const R*)I.vel);
     return houc_rhs<R>(H, H.n, psi + c, 1, I.pitch, vel[c], vel[c + I.plane], R(1.0 / I.dx), R(1.0 / I.dy));
 }
 
@@ -1523,16 +1537,16 @@ __global__ __launch_bounds__(256) void k_ls_stage(const Inst* __restrict__ insts
                                                   Houc H) {
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!I.si[SI_ACTIVE]) return;
+    if (!gmem(I.si)[SI_ACTIVE]) return;
     const int i = w.y * 4 + (threadIdx.x >> 6);
     const int j = w.z * 64 + (threadIdx.x & 63);
     if (i >= I.H || j >= I.W) return;
     bool f0, f1;
     const int si = src_row(I, i, f0), sj = src_col(I, j, f1);
-    const R dt = (R)I.sd[SD_DT];
-    const R* phi = (const R*)I.phi;
-    const R* A = (const R*)I.phiA;
-    const R* B = (const R*)I.phiB;
+    const R dt = (R)gmem(I.sd)[SD_DT];
+    const auto phi = gmem((const R*)I.phi);
+    const auto A = gmem((const R*)I.phiA);
+    const auto B = gmem((const R*)I.phiB);
     const long long cs = cidx(I, si, sj);
     R val;
     if (STAGE == 1) val = rk3_combine<R, 1>(phi[cs], R(0), dt, ls_rhs<R>(I, H, phi, si, sj));
@@ -1562,12 +1576,12 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
     __shared__ R s0[NC], s1[NC];
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!I.si[SI_ACTIVE]) return;
+    if (!gmem(I.si)[SI_ACTIVE]) return;
     const int g = I.g;
     const int ri = g + w.y * LTY - HALO, rj = g + w.z * LTX - HALO;
-    const R* phi = (const R*)I.phi;
-    const R* vel = (const R*)I.vel;
-    const R dt = (R)I.sd[SD_DT];
+    const auto phi = gmem((const R*)I.phi);
+    const auto vel = gmem((const R*)I.vel);
+    const R dt = (R)gmem(I.sd)[SD_DT];
     const R idxr = R(1.0 / I.dx), idyr = R(1.0 / I.dy);
     // per cell: phi, velocity; meta bits 0-7: distance to the region edge,
     // 8: interior, 9: owned, 10: within RAD of the domain edge (its new
@@ -1623,7 +1637,7 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
         }
         __syncthreads();
     }
-    R* A = (R*)I.phiA;
+    const auto A = gmem((R*)I.phiA);
 #pragma unroll
     for (int k = 0; k < SLOTS; ++k) {
         if (!(meta[k] & 512)) continue;
@@ -1649,7 +1663,7 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
 }
 
 __device__ __forceinline__ bool reinit_step(const Inst& I, int every) {
-    return every > 0 && ((I.si[SI_NSTEP] + 1) % every) == 0;
+    return every > 0 && ((gmem(I.si)[SI_NSTEP] + 1) % every) == 0;
 }
 
 // copy A -> phi on steps without reinitialisation
@@ -1658,7 +1672,7 @@ __global__ __launch_bounds__(256) void k_ls_copy(const Inst* __restrict__ insts,
                                                  int every) {
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!I.si[SI_ACTIVE] || reinit_step(I, every)) return;
+    if (!gmem(I.si)[SI_ACTIVE] || reinit_step(I, every)) return;
     const int i = w.y * 4 + (threadIdx.x >> 6);
     const int j = w.z * 64 + (threadIdx.x & 63);
     if (i >= I.H || j >= I.W) return;
@@ -1730,7 +1744,8 @@ __device__ __forceinline__ ReinitK<R> reinit_k(const Inst& I) {
 }
 
 template <typename R>
-__device__ __forceinline__ void reinit_cell_setup(const Inst& I, const R* P0, int si, int sj, int& mode, R& cst) {
+__device__ __forceinline__ void reinit_cell_setup(const Inst& I, const R* P0g, int si, int sj, int& mode, R& cst) {
+    const auto P0 = gmem(P0g);
     const double h = fmin(I.dx, I.dy);
     reinit_setup<R>(P0[cidx(I, si, sj)], P0[cidx(I, si, sj + 1)], P0[cidx(I, si, sj - 1)], P0[cidx(I, si + 1, sj)],
                     P0[cidx(I, si - 1, sj)], I.dx, I.dy, h, mode, cst);
@@ -1744,13 +1759,13 @@ __global__ __launch_bounds__(256) void k_ls_reinit(const Inst* __restrict__ inst
                                                    int k, int K, int gate_every, int force) {
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!force && (!I.si[SI_ACTIVE] || !reinit_step(I, gate_every))) return;
+    if (!force && (!gmem(I.si)[SI_ACTIVE] || !reinit_step(I, gate_every))) return;
     const int i = w.y * 4 + (threadIdx.x >> 6);
     const int j = w.z * 64 + (threadIdx.x & 63);
     if (i >= I.H || j >= I.W) return;
     bool f0, f1;
     const int si = src_row(I, i, f0), sj = src_col(I, j, f1);
-    const R* P0 = (const R*)I.phiA;
+    const auto P0 = gmem((const R*)I.phiA);
     const bool out_phi = ((K - 1 - k) % 2) == 0;
     const R* in = k == 0 ? P0 : (const R*)(out_phi ? I.phiB : I.phi);
     R* out = (R*)(out_phi ? I.phi : I.phiB);
@@ -1799,12 +1814,12 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
     const Inst& I = insts[w.x];
     const int g = I.g;
     if (!force) {
-        if (!I.si[SI_ACTIVE]) return;
+        if (!gmem(I.si)[SI_ACTIVE]) return;
         if (!reinit_step(I, gate_every)) {
             // a step without reinitialisation: phi = A (k_ls_copy's work)
             if (copy_idle) {
-                const R* A = (const R*)I.phiA;
-                R* phi = (R*)I.phi;
+                const auto A = gmem((const R*)I.phiA);
+                const auto phi = gmem((R*)I.phi);
                 for (int q = threadIdx.x; q < RTX * RTY; q += RT_THREADS) {
                     const int i = g + w.y * RTY + q / RTX, j = g + w.z * RTX + q % RTX;
                     if (!interior(I, i, j)) continue;
@@ -1827,9 +1842,9 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
         }
     }
     const int ri = g + w.y * RTY - RT_MAXIT, rj = g + w.z * RTX - RT_MAXIT;
-    const R* P0 = (const R*)I.phiA;
-    const R* in = ls_buf<R>(I, in_sel);
-    R* out = ls_buf<R>(I, out_sel);
+    const auto P0 = gmem((const R*)I.phiA);
+    const auto in = gmem((const R*)ls_buf<R>(I, in_sel));
+    const auto out = gmem(ls_buf<R>(I, out_sel));
     // per cell: reinit_setup's mode and constant; bit 8: interior with its
     // stencil inside the region, bit 9: owned, bit 10: next to the domain
     // edge (refreshes the first ghost layer), bits 11+: distance to the
@@ -1952,7 +1967,7 @@ __global__ __launch_bounds__(256) void k_bc(const Inst* __restrict__ insts, cons
                                             int q, int nfluid, int advance) {
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    const bool active = I.si[SI_ACTIVE] != 0;
+    const bool active = gmem(I.si)[SI_ACTIVE] != 0;
     int i, j;
     if (frame_cell(I, w.y, w.z + threadIdx.x, i, j)) {
         bool fv, fu;
@@ -1965,8 +1980,8 @@ __global__ __launch_bounds__(256) void k_bc(const Inst* __restrict__ insts, cons
         }
     }
     if (active && advance && w.y == 0 && w.z == 0 && threadIdx.x == 0) {
-        I.sd[SD_T] += I.sd[SD_DT];
-        I.si[SI_NSTEP] += 1;
+        gmem(I.sd)[SD_T] += gmem(I.sd)[SD_DT];
+        gmem(I.si)[SI_NSTEP] += 1;
     }
 }
 
@@ -2025,9 +2040,9 @@ __global__ void k_mixed_rs(int n, double gL, double pL, double gR, double pR, co
 __global__ void k_reset(const Inst* __restrict__ insts, const int4* __restrict__ work, int narrow_boxes) {
     const Inst& I = insts[work[blockIdx.x].x];
     if (threadIdx.x != 0) return;
-    I.si[SI_ACTIVE] = 1;
-    I.si[SI_NIFACE] = 0;
-    I.si[SI_DTCOUNT] = 0;
+    gmem(I.si)[SI_ACTIVE] = 1;
+    gmem(I.si)[SI_NIFACE] = 0;
+    gmem(I.si)[SI_DTCOUNT] = 0;
     reset_boxes(I, narrow_boxes);
 }
 
