@@ -1202,7 +1202,7 @@ __device__ __forceinline__ void orig_prims(const Inst& I, int f, int ii, int jj,
 // 0: fluid 1 at -n, 1: fluid 2 at +n; GFM_real uses corner 0 with weight 1)
 // and which corners are earlier interfacial cells whose star state replaces
 // the stored value
-struct IfRec {
+struct alignas(16) IfRec {
     double nx, ny, lsval, pspec, preal;
     double tx[2], ty[2];
     double cv[2][4][4];
@@ -1218,8 +1218,8 @@ __device__ __forceinline__ void bilerp(const double (*cv)[4], double tx, double 
 // star state of one interfacial cell from its probe states; writes the
 // side-buffer entry (GFM_riemann.cpp:97-145, GFM_real.cpp:152-205,
 // GFM_modified.cpp:113-196)
-template <int VAR, bool FAST>
-__device__ __forceinline__ unsigned iface_finish(const Inst& I, const IfRec& r, const double* ip1, const double* ip2,
+template <int VAR, bool FAST, typename Rec>
+__device__ __forceinline__ unsigned iface_finish(const Inst& I, const Rec& r, const double* ip1, const double* ip2,
                                                  double p0, double* out, double* pstar) {
     Star S;
     double t1[2], t2[2];
@@ -1253,6 +1253,26 @@ __device__ __forceinline__ void probe_values(const Inst& I, const IfRec& r, bool
         for (int q = 0; q < 4; ++q) {
             const int d = r.dep[p][q];
             const double* src = (use_side && d >= 0) ? I.side + 8 * (long long)d : r.cv[p][q];
+            for (int k = 0; k < 4; ++k) cv[p][q][k] = src[k];
+        }
+    if (VAR == V_REAL) {
+        for (int k = 0; k < 4; ++k) { ip1[k] = cv[0][0][k]; ip2[k] = cv[1][0][k]; }
+    } else {
+        bilerp(cv[0], r.tx[0], r.ty[0], ip1);
+        bilerp(cv[1], r.tx[1], r.ty[1], ip2);
+    }
+}
+
+// probe states of a record in global memory, dependent corners from the
+// side buffer
+template <int VAR, typename Rec>
+__device__ __forceinline__ void probe_values_side(const Inst& I, const Rec& r, double* ip1, double* ip2) {
+    double cv[2][4][4];
+    const auto side = gmem(I.side);
+    for (int p = 0; p < 2; ++p)
+        for (int q = 0; q < 4; ++q) {
+            const int d = r.dep[p][q];
+            const auto src = d >= 0 ? side + 8 * (long long)d : &r.cv[p][q][0];
             for (int k = 0; k < 4; ++k) cv[p][q][k] = src[k];
         }
     if (VAR == V_REAL) {
@@ -1347,12 +1367,12 @@ __global__ __launch_bounds__(256) void k_iface_prep(const Inst* __restrict__ ins
 
 // star state of pending entry k once every earlier interfacial cell it reads
 // has its own (the side buffer holds those)
-template <int VAR, bool FAST>
-__device__ __forceinline__ unsigned iface_solve_pending(const Inst& I, const IfRec& r, int k) {
+template <int VAR, bool FAST, typename Rec>
+__device__ __forceinline__ unsigned iface_solve_pending(const Inst& I, const Rec& r, int k) {
     double ip1[4], ip2[4], out[8], ps;
-    probe_values<VAR>(I, r, true, ip1, ip2);
+    probe_values_side<VAR>(I, r, ip1, ip2);
     const unsigned st = iface_finish<VAR, FAST>(I, r, ip1, ip2, r.pspec, out, &ps);
-    double* o = I.side + 8 * (long long)k;
+    const auto o = gmem(I.side) + 8 * (long long)k;
     for (int q = 0; q < 4; ++q) o[q] = out[q];
     return st;
 }
@@ -1380,7 +1400,7 @@ __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ i
     if (!gmem(I.si)[SI_ACTIVE]) return;
     const int n = min(gmem(I.si)[SI_NIFACE], I.icap);
     if (n > ICAP_LDS) { if (threadIdx.x == 0) flag(I, ST_IOVF); return; }
-    const IfRec* recs = (const IfRec*)I.irec;
+    const auto recs = gmem((const IfRec*)I.irec);
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int k = tid; k < n; k += nt) fin[k] = I.idone[k] ? 1 : 0;
     if (tid == 0) nclaim = nt;
