@@ -582,23 +582,32 @@ __device__ __forceinline__ ExK<R> ex_k(const Inst& I) {
     return k;
 }
 
-// unit normal of phi in the storage precision: FP64 as unit_normal; FP32
-// (mixed precision) from one reciprocal square root
+// unit normal of phi in the storage precision from phi at the east, west,
+// north and south neighbours: FP64 as unit_normal; FP32 (mixed precision)
+// from one reciprocal square root
 template <typename R>
-__device__ __forceinline__ void unit_normal_r(const Inst& I, const ExK<R>& K, int i, int j, R& nx, R& ny) {
+__device__ __forceinline__ void unit_normal_v(const Inst& I, const ExK<R>& K, R pe, R pw, R pn, R ps, R& nx, R& ny) {
+#pragma clang fp contract(off)
     if constexpr (sizeof(R) == 8) {
-        double a, b;
-        unit_normal<R>(I, i, j, a, b);
+        double a = ((double)pe - (double)pw) / (2.0 * I.dx);
+        double b = ((double)pn - (double)ps) / (2.0 * I.dy);
+        const double m = sqrt(a * a + b * b);
+        if (m < 1e-12) { a = 1.0; b = 0.0; }
+        else { a /= m; b /= m; }
         nx = a; ny = b;
     } else {
-#pragma clang fp contract(off)
-        const auto phi = gmem((const float*)I.phi);
-        const float gx = (phi[cidx(I, i, j + 1)] - phi[cidx(I, i, j - 1)]) * K.i2dx;
-        const float gy = (phi[cidx(I, i + 1, j)] - phi[cidx(I, i - 1, j)]) * K.i2dy;
+        const float gx = (pe - pw) * K.i2dx;
+        const float gy = (pn - ps) * K.i2dy;
         const float m2 = gx * gx + gy * gy;
         if (m2 < 1e-24f) { nx = 1.0f; ny = 0.0f; }
         else { const float r = rsqrtf(m2); nx = gx * r; ny = gy * r; }
     }
+}
+
+template <typename R>
+__device__ __forceinline__ void unit_normal_r(const Inst& I, const ExK<R>& K, int i, int j, R& nx, R& ny) {
+    unit_normal_v<R>(I, K, ld1<R>(I.phi, cidx(I, i, j + 1)), ld1<R>(I.phi, cidx(I, i, j - 1)),
+                     ld1<R>(I.phi, cidx(I, i + 1, j)), ld1<R>(I.phi, cidx(I, i - 1, j)), nx, ny);
 }
 
 template <typename R, int MODE>
@@ -846,6 +855,36 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
     int meta[SLOTS];
     const bool rfb = I.bc[2] == BC_R, rft = I.bc[3] == BC_R, rfx = I.bc[0] == BC_R;
     const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
+    // in two passes, so that every load for the thread's cells is in flight
+    // before any is used: the raw values (states, phi and its neighbours,
+    // the interfacial index), then ex_load's conversion and injection,
+    // normals, weights and the LDS writes
+    V4<R> L1[SLOTS], L2[SLOTS];
+    R pc[SLOTS], pe[SLOTS], pw[SLOTS], pn[SLOTS], ps[SLOTS];
+    int im[SLOTS];
+    const auto phig = gmem((const R*)I.phi);
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+        const int idx = threadIdx.x + k * TB_THREADS;
+        const int a = idx / RX, b = idx - (idx / RX) * RX;
+        const int i = ri + a, j = rj + b;
+        const bool valid = idx < NC && i >= 0 && i < I.H && j >= 0 && j < I.W;
+        const bool inner = valid && interior(I, i, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1;
+        const long long c = valid ? cidx(I, i, j) : 0;
+        L1[k] = L2[k] = {{R(0), R(0), R(0), R(0)}};
+        pc[k] = pe[k] = pw[k] = pn[k] = ps[k] = R(0);
+        im[k] = -1;
+        if (valid) {
+            L1[k] = ld4<R>(in0, I, c);
+            L2[k] = ld4<R>(in1, I, c);
+            if (LM & MF_INJECT) im[k] = gmem(I.imap)[c];
+            if ((LM & MF_INJECT) || inner) pc[k] = phig[c];
+        }
+        if (inner) {
+            pe[k] = phig[c + 1]; pw[k] = phig[c - 1];
+            pn[k] = phig[c + I.pitch]; ps[k] = phig[c - I.pitch];
+        }
+    }
 #pragma unroll
     for (int k = 0; k < SLOTS; ++k) {
         const int idx = threadIdx.x + k * TB_THREADS;
@@ -857,16 +896,32 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
             const int i = ri + a, j = rj + b;
             V4<R> W1 = {{R(0), R(0), R(0), R(0)}}, W2 = W1;
             if (i >= 0 && i < I.H && j >= 0 && j < I.W) {
-                W1 = ex_load<R, LM>(I, 0, i, j, in0);
-                W2 = ex_load<R, LM>(I, 1, i, j, in1);
+                // ex_load<R, LM> of both fluids on the loaded values
+                W1 = L1[k]; W2 = L2[k];
+                if (LM & MF_FIRST) {
+                    if constexpr (sizeof(R) == 4) {
+                        W1 = cons2prims_f(fluid_f(I.rsf[0]), L1[k]);
+                        W2 = cons2prims_f(fluid_f(I.rsf[1]), L2[k]);
+                    } else {
+                        W1 = cons2prims<R>((R)I.gam[0], (R)I.pinf[0], L1[k]);
+                        W2 = cons2prims<R>((R)I.gam[1], (R)I.pinf[1], L2[k]);
+                    }
+                }
+                if ((LM & MF_INJECT) && im[k] >= 0) {
+                    const auto sd = gmem(I.side) + 8 * (long long)im[k];
+                    V4<R> Ws;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) Ws.v[q] = (R)sd[q];
+                    if (pc[k] <= R(0)) W1 = Ws; else W2 = Ws;
+                }
                 if (interior(I, i, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1) {
                     R nx, ny;
-                    unit_normal_r<R>(I, K, i, j, nx, ny);
+                    unit_normal_v<R>(I, K, pe[k], pw[k], pn[k], ps[k], nx, ny);
                     extrap_weights<R>(K, nx, ny, wxs[k], wys[k]);
                     const bool own = a >= NIT && a < NIT + TBY && b >= NIT && b < NIT + TBX;
                     const bool edge = i == ib || i == it || j == jl || j == jr;
                     const int dist = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
-                    const int fr = ld1<R>(I.phi, cidx(I, i, j)) <= R(0) ? 0 : 1;
+                    const int fr = pc[k] <= R(0) ? 0 : 1;
                     meta[k] = 1 | (fr ? 2 : 0) | (own ? 4 : 0) | (edge ? 8 : 0) | (dist << 4);
                     int dj, di;
                     extrap_dirs<R>(1 - fr, nx, ny, dj, di);
