@@ -377,16 +377,36 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
         }
         return;
     }
-    // A
+    // the state of this thread's own cell, for D, loaded now
+    const bool own_in = ci < I.g + I.Ny && cj < I.g + I.Nx;
+    const long long cown = own_in ? cidx(I, ci, cj) : 0;
+    V4<R> Uown = {{R(0), R(0), R(0), R(0)}};
+    if (own_in) Uown = ld4<R>(Qi, I, cown);
+    // A: the loads of both passes first, then the conversions
     if (MUSCL) {
-        for (int c = tid; c < FWY * FWX; c += FS_THREADS) {
-            const int a = c / FWX, b = c % FWX;
-            const int i = min(max(i0 - 2 + a, 0), I.H - 1);
-            const int j = min(max(j0 - 2 + b, 0), I.W - 1);
-            V4<T> W;
-            if constexpr (sizeof(T) == 4) W = cons2prims_f(Ff, ld4<R>(Qi, I, cidx(I, i, j)));
-            else W = cons2prims(g, pinf, ld4<R>(Qi, I, cidx(I, i, j)));
-            sW[0][a][b] = W.v[0]; sW[1][a][b] = W.v[1]; sW[2][a][b] = W.v[2]; sW[3][a][b] = W.v[3];
+        constexpr int NA = (FWY * FWX + FS_THREADS - 1) / FS_THREADS;
+        V4<R> UA[NA];
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int c = tid + q * FS_THREADS;
+            UA[q] = {{R(0), R(0), R(0), R(0)}};
+            if (c < FWY * FWX) {
+                const int a = c / FWX, b = c % FWX;
+                const int i = min(max(i0 - 2 + a, 0), I.H - 1);
+                const int j = min(max(j0 - 2 + b, 0), I.W - 1);
+                UA[q] = ld4<R>(Qi, I, cidx(I, i, j));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int c = tid + q * FS_THREADS;
+            if (c < FWY * FWX) {
+                const int a = c / FWX, b = c % FWX;
+                V4<T> W;
+                if constexpr (sizeof(T) == 4) W = cons2prims_f(Ff, UA[q]);
+                else W = cons2prims(g, pinf, UA[q]);
+                sW[0][a][b] = W.v[0]; sW[1][a][b] = W.v[1]; sW[2][a][b] = W.v[2]; sW[3][a][b] = W.v[3];
+            }
         }
         __syncthreads();
     }
@@ -514,9 +534,9 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
     flag(I, st);
     __syncthreads();
     // D
-    if (ci < I.g + I.Ny && cj < I.g + I.Nx) {
-        const long long c = cidx(I, ci, cj);
-        V4<R> U = ld4<R>(Qi, I, c);
+    if (own_in) {
+        const long long c = cown;
+        V4<R> U = Uown;
         if (ci >= bb0 && ci < bb1 && cj >= bb2 && cj < bb3) {
             bool ok;
             if constexpr (sizeof(T) == 4) ok = phys_state_f(U);
