@@ -406,8 +406,8 @@ __device__ __forceinline__ void rs_sample(const RSSide& L, const RSSide& R, cons
 }
 
 // ---------------------------------------------------------------------------
-// FP64 pieces of the mixed-precision interfacial solver (rs_star_m2).  The
-// star pressure and velocity come from rs_star's Newton iteration, in FP64,
+// FP64 pieces of the mixed-precision solvers (rs_star_m2).  The star
+// pressure and velocity come from rs_star's Newton iteration, in FP64,
 // on the same pressure functions (Toro 4.6-4.7).  What differs is the
 // evaluation: the fluid's constants are computed once per instance,
 // (pbar / pbarK)^z - 1 is summed as its binomial series when
@@ -480,12 +480,14 @@ __device__ __forceinline__ double rs_f_m(const RSFluid& F, const RSSideM& s, dou
     return F.k2 * s.c * w;
 }
 
-// Two-fluid form for the interfacial problems of the mixed-precision path:
-// per-side constants, Newton start p0 when it lies above the floor (else
-// rs_guess's choice), tolerance tol, and the star densities (Toro 4.5.1,
-// 4.5.2: shock rho (r + g6) / (g6 r + 1), rarefaction rho r^(1/g), r =
-// (p* + pinf) / (p + pinf), the power in FP32: the densities are stored in
-// FP32).  Only the evaluation differs from rs_star.
+// rs_star's FP64 Newton iteration for two stiffened gases on the
+// mixed-precision evaluation: per-side constants, Newton start p0 when it
+// lies above the floor (else rs_guess's choice), tolerance tol, and with
+// DENS the star densities (Toro 4.5.1, 4.5.2: shock rho (r + g6) / (g6 r +
+// 1), rarefaction rho r^(1/g), r = (p* + pinf) / (p + pinf)).  Only the
+// evaluation differs from rs_star.  Used for the interfacial problems and,
+// with one fluid and the FP32 iteration's p* as the start, for the faces.
+template <bool DENS = true>
 __device__ __forceinline__ unsigned rs_star_m2(const RSFluid& FL, const RSFluid& FR, const RSSideM& L,
                                                const RSSideM& R, Star& S, double p0, double tol) {
     const bool same = FL.g == FR.g && FL.pinf == FR.pinf;
@@ -534,13 +536,15 @@ __device__ __forceinline__ unsigned rs_star_m2(const RSFluid& FL, const RSFluid&
     if (!conv || p != p) st |= ST_RS;
     S.p = p;
     S.u = 0.5 * (L.u + R.u) + 0.5 * ((fR + dR * dp) - (fL + dL * dp));
-    {
-        const double r = (p + FL.pinf) * L.ipb;
-        S.rl = p > L.p ? L.rho * (r + FL.g6) * rcp_nr(FL.g6 * r + 1.0) : L.rho * (double)pow_hw((float)r, FL.fig);
-    }
-    {
-        const double r = (p + FR.pinf) * R.ipb;
-        S.rr = p > R.p ? R.rho * (r + FR.g6) * rcp_nr(FR.g6 * r + 1.0) : R.rho * (double)pow_hw((float)r, FR.fig);
+    if (DENS) {
+        {
+            const double r = (p + FL.pinf) * L.ipb;
+            S.rl = p > L.p ? L.rho * (r + FL.g6) * rcp_nr(FL.g6 * r + 1.0) : L.rho * pow_fast(r, FL.ig);
+        }
+        {
+            const double r = (p + FR.pinf) * R.ipb;
+            S.rr = p > R.p ? R.rho * (r + FR.g6) * rcp_nr(FR.g6 * r + 1.0) : R.rho * pow_fast(r, FR.ig);
+        }
     }
     return st;
 }
@@ -611,7 +615,7 @@ __device__ __forceinline__ V4<double> rs_flux_full(bool px, double g, double pin
 }
 
 // ---------------------------------------------------------------------------
-// FP32 face solver of the mixed-precision path: rs_star's iteration (Toro
+// FP32 pre-iteration of the mixed-precision face solver: rs_star's iteration (Toro
 // 4.6-4.7 pressure functions, guess, floor, tolerance 1e-6 on the relative
 // pressure change, status bits) evaluated in FP32 with the hardware
 // reciprocal, square root, exp2 and log2 (about 1 ulp each).  The
@@ -766,16 +770,28 @@ __device__ __forceinline__ void rs_sample_f(const RSFluid& F, const RSSideF& L, 
     op = spbar * pow_hw(q, F.fk3) - F.fpinf;
 }
 
+// The mixed-precision face solve: the FP32 iteration above gives the start
+// of rs_star's FP64 Newton iteration (rs_star_m2, tolerance 1e-6), which
+// then needs about one step; p* and u* come from the FP64 pressure
+// functions; the sampling is FP32.
 __device__ __forceinline__ V4<float> rs_flux_full_f(const RSFluid& F, const FluidF& Ff, bool px, const V4<float>& WL,
                                                     const V4<float>& WR, unsigned& st) {
-    RSSideF L, R;
-    rs_side_f(L, F, WL.v[0], norm_u(px, WL), WL.v[3]);
-    rs_side_f(R, F, WR.v[0], norm_u(px, WR), WR.v[3]);
-    StarF S;
-    st |= rs_star_f(F, L, R, S);
+    RSSideF Lf, Rf;
+    rs_side_f(Lf, F, WL.v[0], norm_u(px, WL), WL.v[3]);
+    rs_side_f(Rf, F, WR.v[0], norm_u(px, WR), WR.v[3]);
+    StarF Sf;
+    (void)rs_star_f(F, Lf, Rf, Sf);   // a start only: the FP64 iteration decides the status
+    RSSideM L, R;
+    rs_side_m(L, F, (double)WL.v[0], (double)norm_u(px, WL), (double)WL.v[3]);
+    rs_side_m(R, F, (double)WR.v[0], (double)norm_u(px, WR), (double)WR.v[3]);
+    Star S;
+    st |= rs_star_m2<false>(F, F, L, R, S, (Sf.p == Sf.p) ? (double)Sf.p : -1e300, 1e-6);
+    StarF S2;
+    S2.p = (float)S.p;
+    S2.u = (float)S.u;
     float r, u, p;
-    rs_sample_f(F, L, R, S, r, u, p);
-    return rs_flux_out_f(px, Ff, r, u, p, (double)S.u, WL, WR);
+    rs_sample_f(F, Lf, Rf, S2, r, u, p);
+    return rs_flux_out_f(px, Ff, r, u, p, S.u, WL, WR);
 }
 
 // pure_RS_exact::solve_RP (pure_RS_exact.cpp:9-73): conserved L/R -> flux
