@@ -685,7 +685,8 @@ __device__ __forceinline__ float rs_f_f(const RSFluid& F, const RSSideF& s, floa
 
 struct StarF { float p, u; };
 
-__device__ __forceinline__ unsigned rs_star_f(const RSFluid& F, const RSSideF& L, const RSSideF& R, StarF& S) {
+__device__ __forceinline__ unsigned rs_star_f(const RSFluid& F, const RSSideF& L, const RSSideF& R, StarF& S,
+                                              float tol = 1e-6f) {
     if (L.rho == R.rho && L.u == R.u && L.p == R.p) {
         S.p = L.p; S.u = L.u;
         return 0;
@@ -725,7 +726,7 @@ __device__ __forceinline__ unsigned rs_star_f(const RSFluid& F, const RSSideF& L
         const float ref = 0.5f * (pn + p) - plow;
         dp = pn - p;
         p = pn;
-        if (chg <= 1e-6f * ref || chg <= 4.8e-7f * fabsf(p) ||
+        if (chg <= tol * ref || chg <= 4.8e-7f * fabsf(p) ||
             fabsf(res) <= 4.8e-7f * (fabsf(fL) + fabsf(fR) + fabsf(du))) {
             conv = true;
             break;
@@ -780,7 +781,10 @@ __device__ __forceinline__ V4<float> rs_flux_full_f(const RSFluid& F, const Flui
     rs_side_f(Lf, F, WL.v[0], norm_u(px, WL), WL.v[3]);
     rs_side_f(Rf, F, WR.v[0], norm_u(px, WR), WR.v[3]);
     StarF Sf;
-    (void)rs_star_f(F, Lf, Rf, Sf);   // a start only: the FP64 iteration decides the status
+    // a start only (the FP64 iteration decides the status): stopped at a
+    // relative step of 1e-4, after which the start is within ~1e-8 and the
+    // FP64 iteration's first step is below its 1e-6 tolerance
+    (void)rs_star_f(F, Lf, Rf, Sf, 1e-4f);
     RSSideM L, R;
     rs_side_m(L, F, (double)WL.v[0], (double)norm_u(px, WL), (double)WL.v[3]);
     rs_side_m(R, F, (double)WR.v[0], (double)norm_u(px, WR), (double)WR.v[3]);
