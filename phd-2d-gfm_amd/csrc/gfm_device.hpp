@@ -774,9 +774,10 @@ __device__ __forceinline__ void rs_sample_f(const RSFluid& F, const RSSideF& L, 
 // The mixed-precision face solve: the FP32 iteration above gives the start
 // of rs_star's FP64 Newton iteration (rs_star_m2, tolerance 1e-6), which
 // then needs about one step; p* and u* come from the FP64 pressure
-// functions; the sampling is FP32.
-__device__ __forceinline__ V4<float> rs_flux_full_f(const RSFluid& F, const FluidF& Ff, bool px, const V4<float>& WL,
-                                                    const V4<float>& WR, unsigned& st) {
+// functions; the sampling is FP32.  In two parts (the flux kernel runs each
+// over all its faces in turn); rs_flux_full_f is both.
+__device__ __forceinline__ float rs_face_start_f(const RSFluid& F, bool px, const V4<float>& WL,
+                                                 const V4<float>& WR) {
     RSSideF Lf, Rf;
     rs_side_f(Lf, F, WL.v[0], norm_u(px, WL), WL.v[3]);
     rs_side_f(Rf, F, WR.v[0], norm_u(px, WR), WR.v[3]);
@@ -785,17 +786,31 @@ __device__ __forceinline__ V4<float> rs_flux_full_f(const RSFluid& F, const Flui
     // relative step of 1e-4, after which the start is within ~1e-8 and the
     // FP64 iteration's first step is below its 1e-6 tolerance
     (void)rs_star_f(F, Lf, Rf, Sf, 1e-4f);
+    return Sf.p;
+}
+
+__device__ __forceinline__ V4<float> rs_face_finish_f(const RSFluid& F, const FluidF& Ff, bool px,
+                                                      const V4<float>& WL, const V4<float>& WR, float p0,
+                                                      unsigned& st) {
     RSSideM L, R;
     rs_side_m(L, F, (double)WL.v[0], (double)norm_u(px, WL), (double)WL.v[3]);
     rs_side_m(R, F, (double)WR.v[0], (double)norm_u(px, WR), (double)WR.v[3]);
     Star S;
-    st |= rs_star_m2<false>(F, F, L, R, S, (Sf.p == Sf.p) ? (double)Sf.p : -1e300, 1e-6);
+    st |= rs_star_m2<false>(F, F, L, R, S, (p0 == p0) ? (double)p0 : -1e300, 1e-6);
+    RSSideF Lf, Rf;
+    rs_side_f(Lf, F, WL.v[0], norm_u(px, WL), WL.v[3]);
+    rs_side_f(Rf, F, WR.v[0], norm_u(px, WR), WR.v[3]);
     StarF S2;
     S2.p = (float)S.p;
     S2.u = (float)S.u;
     float r, u, p;
     rs_sample_f(F, Lf, Rf, S2, r, u, p);
     return rs_flux_out_f(px, Ff, r, u, p, S.u, WL, WR);
+}
+
+__device__ __forceinline__ V4<float> rs_flux_full_f(const RSFluid& F, const FluidF& Ff, bool px, const V4<float>& WL,
+                                                    const V4<float>& WR, unsigned& st) {
+    return rs_face_finish_f(F, Ff, px, WL, WR, rs_face_start_f(F, px, WL, WR), st);
 }
 
 // pure_RS_exact::solve_RP (pure_RS_exact.cpp:9-73): conserved L/R -> flux

@@ -337,6 +337,7 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
     __shared__ T sW[4][FWY][FWX];
     __shared__ V4A<T> sF[2][NFACE];   // per face: inputs lo / hi, then sF[0] = flux
     __shared__ short sHard[NFACE];
+    __shared__ float sP0[sizeof(T) == 4 ? NFACE : 1];   // FP32 storage: start of each hard face's FP64 Newton
     __shared__ int nHard;
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
@@ -522,14 +523,29 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
     // C2
     const int nh = nHard;
     const RSFluid& rf = I.rsf[f];
-    for (int h = tid; h < nh; h += FS_THREADS) {
-        const int c = sHard[h];
-        const bool px = c < NFX;
-        const V4<T> WL = tb_get<T>(sF[0], c), WR = tb_get<T>(sF[1], c);
-        V4<T> F;
-        if constexpr (sizeof(T) == 4) F = rs_flux_full_f(rf, Ff, px, WL, WR, st);
-        else F = rs_flux_full(px, g, pinf, WL, WR, st);
-        tb_put<T>(sF[0], c, F);
+    if constexpr (sizeof(T) == 4) {
+        // FP32 storage, in two passes: the FP32 start of every hard face,
+        // then the FP64 Newton step, sampling and flux (rs_flux_full_f split)
+        for (int h = tid; h < nh; h += FS_THREADS) {
+            const int c = sHard[h];
+            const bool px = c < NFX;
+            const V4<T> WL = tb_get<T>(sF[0], c), WR = tb_get<T>(sF[1], c);
+            sP0[c] = rs_face_start_f(rf, px, WL, WR);
+        }
+        __syncthreads();
+        for (int h = tid; h < nh; h += FS_THREADS) {
+            const int c = sHard[h];
+            const bool px = c < NFX;
+            const V4<T> WL = tb_get<T>(sF[0], c), WR = tb_get<T>(sF[1], c);
+            tb_put<T>(sF[0], c, rs_face_finish_f(rf, Ff, px, WL, WR, sP0[c], st));
+        }
+    } else {
+        for (int h = tid; h < nh; h += FS_THREADS) {
+            const int c = sHard[h];
+            const bool px = c < NFX;
+            const V4<T> WL = tb_get<T>(sF[0], c), WR = tb_get<T>(sF[1], c);
+            tb_put<T>(sF[0], c, rs_flux_full(px, g, pinf, WL, WR, st));
+        }
     }
     flag(I, st);
     __syncthreads();
