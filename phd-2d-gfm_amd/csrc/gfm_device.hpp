@@ -431,22 +431,36 @@ struct RSSideM { double rho, u, p, pbar, ipb, irho, c; };
 // FP64 reciprocal and square root for the mixed-precision Newton iteration:
 // the hardware estimates refined by two Newton-Raphson steps (a few ulp from
 // the correctly rounded results, against one ulp-exact but several times
-// longer IEEE sequences); zero, infinite and NaN arguments take the IEEE
-// operation
+// longer IEEE sequences), for normal arguments (|x| >= 2^-1022; the pressures,
+// densities and sound speeds here are far inside that range).  Zero,
+// infinite and NaN arguments give the IEEE results (1/0 = inf, 1/inf = 0,
+// sqrt(-0) = -0, sqrt(x < 0) = NaN) by a select, not by an IEEE fallback: the
+// compiler if-converts a fallback branch (these are hoisted out of loops),
+// which would then run the whole IEEE sequence for every argument.
 __device__ __forceinline__ double rcp_nr(double x) {
-    if (!(fabs(x) < INFINITY) || x == 0.0) return 1.0 / x;
-    double r = __builtin_amdgcn_rcp(x);
+    const double r0 = __builtin_amdgcn_rcp(x);         // exact for 0, inf, NaN
+    double r = fma(fma(-x, r0, 1.0), r0, r0);
     r = fma(fma(-x, r, 1.0), r, r);
-    return fma(fma(-x, r, 1.0), r, r);
+    return (fabs(x) < INFINITY && x != 0.0) ? r : r0;
 }
 
 __device__ __forceinline__ double sqrt_nr(double x) {
-    if (!(x > 0.0) || !(x < INFINITY)) return sqrt(x);
     double y = __builtin_amdgcn_rsq(x);                // ~ 1/sqrt(x)
     y = y * fma(-0.5 * x * y, y, 1.5);
     y = y * fma(-0.5 * x * y, y, 1.5);
     const double s = x * y;                            // ~ sqrt(x)
-    return fma(fma(-s, s, x), 0.5 * y, s);             // one Newton step on s
+    const double r = fma(fma(-s, s, x), 0.5 * y, s);   // one Newton step on s
+    return (x > 0.0 && x < INFINITY) ? r : (x >= 0.0 ? x : __builtin_nan(""));
+}
+
+// 1 / sqrt(x) for normal x > 0: the hardware estimate refined by two
+// Newton-Raphson steps (a few ulp); 0, inf, x < 0 and NaN give the
+// hardware estimate, which is the IEEE result there, by a select (rcp_nr)
+__device__ __forceinline__ double rsq_nr(double x) {
+    const double y0 = __builtin_amdgcn_rsq(x);
+    double y = y0 * fma(-0.5 * x * y0, y0, 1.5);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    return (x > 0.0 && x < INFINITY) ? y : y0;         // y0 exact for 0, inf, x < 0, NaN
 }
 
 __device__ __forceinline__ void rs_side_m(RSSideM& s, const RSFluid& F, double rho, double u, double p) {
@@ -455,6 +469,18 @@ __device__ __forceinline__ void rs_side_m(RSSideM& s, const RSFluid& F, double r
     s.ipb = rcp_nr(s.pbar);
     s.irho = rcp_nr(rho);
     s.c = sqrt_nr(F.g * s.pbar * s.irho);
+}
+
+// (1 + eps)^z - 1, r = 1 + eps: the binomial series for |eps| < 1/64, else
+// pow_fast
+__device__ __forceinline__ double powm1_m(const RSFluid& F, double eps, double r) {
+    if (fabs(eps) < 0.015625) {
+        double q = F.bc[8];
+#pragma unroll
+        for (int k = 7; k >= 0; --k) q = fma(q, eps, F.bc[k]);
+        return q * eps;
+    }
+    return pow_fast(r, F.z) - 1.0;
 }
 
 __device__ __forceinline__ double rs_f_m(const RSFluid& F, const RSSideM& s, double p, double& fd) {
@@ -466,16 +492,7 @@ __device__ __forceinline__ double rs_f_m(const RSFluid& F, const RSSideM& s, dou
         return (p - s.p) * q;
     }
     // rarefaction (Toro 4.7): f = k ((pbar/pbarK)^z - 1)
-    const double eps = (p - s.p) * s.ipb;
-    double w;
-    if (fabs(eps) < 0.015625) {
-        double q = F.bc[8];
-#pragma unroll
-        for (int k = 7; k >= 0; --k) q = fma(q, eps, F.bc[k]);
-        w = q * eps;
-    } else {
-        w = pow_fast(pbar * s.ipb, F.z) - 1.0;
-    }
+    const double w = powm1_m(F, (p - s.p) * s.ipb, pbar * s.ipb);
     fd = s.c * F.ig * (1.0 + w) * rcp_nr(pbar);
     return F.k2 * s.c * w;
 }
@@ -740,6 +757,64 @@ __device__ __forceinline__ unsigned rs_star_f(const RSFluid& F, const RSSideF& L
     return st;
 }
 
+// The faces' FP64 pressure function of one side (Toro 4.6-4.7, evaluated as
+// rs_f_m) with only what its value needs: shock
+// f = (p - pK) / sqrt(rho (pbar + g6 pbarK) / a2), rarefaction
+// f = k2 c ((pbar / pbarK)^z - 1) with c = g pbarK / sqrt(g pbarK rho).  The
+// derivative uses the hardware FP64 reciprocal estimate: f' only scales the
+// Newton step, so a relative error e in it leaves about e times the step as
+// error after the step; the fixed point is that of the FP64 values and the
+// stopping test is on the FP64 step.
+__device__ __forceinline__ double rs_f_face(const RSFluid& F, double rho, double pK, double p, double& fd) {
+    const double pbK = pK + F.pinf, pbar = p + F.pinf, dp = p - pK;
+    if (p > pK) {
+        const double y = pbar + F.g6 * pbK;
+        const double q = rsq_nr(y * rho * (0.5 * F.g + 0.5));
+        fd = q * (1.0 - 0.5 * dp * __builtin_amdgcn_rcp(y));
+        return dp * q;
+    }
+    const double ipb = rcp_nr(pbK);
+    const double w = powm1_m(F, dp * ipb, pbar * ipb);
+    const double gp = F.g * pbK;
+    const double c = gp * rsq_nr(gp * rho);
+    fd = c * F.ig * (1.0 + w) * __builtin_amdgcn_rcp(pbar);
+    return F.k2 * c * w;
+}
+
+// rs_star's FP64 Newton iteration for a face (one fluid) from the FP32
+// iteration's p*, p0: rs_star_m2's floor, iteration, tolerance and status,
+// on the pressure functions of rs_f_face, with the vacuum test on the FP32
+// sound speeds.  A start at or below the floor (the FP32 iteration ended
+// there, near vacuum) or NaN is replaced by the floor: the pressure function
+// is increasing and concave, so the iteration converges to the same p* from
+// any start above -pinf, and from the floor within a few steps.
+__device__ __forceinline__ unsigned rs_star_face_m(const RSFluid& F, const RSSideF& Lf, const RSSideF& Rf, float p0,
+                                                   Star& S, double tol) {
+    const double rl = Lf.rho, ul = Lf.u, pl = Lf.p, rr = Rf.rho, ur = Rf.u, pr = Rf.p;
+    const double du = ur - ul;
+    const double plow = -F.pinf;
+    const double pfloor = plow + 1e-6 * (0.5 * (pl + pr) - plow);
+    unsigned st = ((double)(F.fk2 * (Lf.c + Rf.c)) <= du) ? ST_RS : 0u;
+    double p = ((double)p0 > pfloor) ? (double)p0 : pfloor;
+    bool conv = false;
+    double fL = 0.0, fR = 0.0, dL = 0.0, dR = 0.0, dp = 0.0;
+    for (int it = 0; it < 60; ++it) {
+        fL = rs_f_face(F, rl, pl, p, dL);
+        fR = rs_f_face(F, rr, pr, p, dR);
+        double pn = p - (fL + fR + du) * __builtin_amdgcn_rcp(dL + dR);
+        if (pn <= plow) pn = 0.5 * (p + plow);
+        const double chg = fabs(pn - p);
+        const double ref = 0.5 * (pn + p) - plow;
+        dp = pn - p;
+        p = pn;
+        if (chg <= tol * ref) { conv = true; break; }
+    }
+    if (!conv || p != p) st |= ST_RS;
+    S.p = p;
+    S.u = 0.5 * (ul + ur) + 0.5 * ((fR + dR * dp) - (fL + dL * dp));
+    return st;
+}
+
 // Toro 4.5 sampling at x/t = 0, the side mirrored onto the left-hand formulas
 __device__ __forceinline__ void rs_sample_f(const RSFluid& F, const RSSideF& L, const RSSideF& R, const StarF& st,
                                             float& orho, float& ou, float& op) {
@@ -772,8 +847,8 @@ __device__ __forceinline__ void rs_sample_f(const RSFluid& F, const RSSideF& L, 
 }
 
 // The mixed-precision face solve: the FP32 iteration above gives the start
-// of rs_star's FP64 Newton iteration (rs_star_m2, tolerance 1e-6), which
-// then needs about one step; p* and u* come from the FP64 pressure
+// of rs_star's FP64 Newton iteration (rs_star_face_m, tolerance 1e-6),
+// which then needs about one step; p* and u* come from the FP64 pressure
 // functions; the sampling is FP32.  In two parts (the flux kernel runs each
 // over all its faces in turn); rs_flux_full_f is both.
 __device__ __forceinline__ float rs_face_start_f(const RSFluid& F, bool px, const V4<float>& WL,
@@ -792,14 +867,11 @@ __device__ __forceinline__ float rs_face_start_f(const RSFluid& F, bool px, cons
 __device__ __forceinline__ V4<float> rs_face_finish_f(const RSFluid& F, const FluidF& Ff, bool px,
                                                       const V4<float>& WL, const V4<float>& WR, float p0,
                                                       unsigned& st) {
-    RSSideM L, R;
-    rs_side_m(L, F, (double)WL.v[0], (double)norm_u(px, WL), (double)WL.v[3]);
-    rs_side_m(R, F, (double)WR.v[0], (double)norm_u(px, WR), (double)WR.v[3]);
-    Star S;
-    st |= rs_star_m2<false>(F, F, L, R, S, (p0 == p0) ? (double)p0 : -1e300, 1e-6);
     RSSideF Lf, Rf;
     rs_side_f(Lf, F, WL.v[0], norm_u(px, WL), WL.v[3]);
     rs_side_f(Rf, F, WR.v[0], norm_u(px, WR), WR.v[3]);
+    Star S;
+    st |= rs_star_face_m(F, Lf, Rf, p0, S, 1e-6);
     StarF S2;
     S2.p = (float)S.p;
     S2.u = (float)S.u;
