@@ -338,6 +338,7 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
     __shared__ V4A<T> sF[2][NFACE];   // per face: inputs lo / hi, then sF[0] = flux
     __shared__ short sHard[NFACE];
     __shared__ float sP0[sizeof(T) == 4 ? NFACE : 1];   // FP32 storage: start of each hard face's FP64 Newton
+    __shared__ T sLam[2];                               // dt / dx, dt / dy for D
     __shared__ int nHard;
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
@@ -369,7 +370,7 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
     const void* Qi = I.Q[f][qin];
     void* Qo = I.Q[f][qout];
     const bool tile_active = !(i0 + FTY <= bb0 || i0 >= bb1 || j0 + FTX <= bb2 || j0 >= bb3);
-    if (tid == 0) nHard = 0;
+    if (tid == 0) { nHard = 0; sLam[0] = dtodx; sLam[1] = dtody; }
     const int ci = i0 + (tid / FTX), cj = j0 + (tid % FTX);
     if (!tile_active) {
         if (ci < I.g + I.Ny && cj < I.g + I.Nx) {
@@ -520,6 +521,14 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
         }
     }
     __syncthreads();
+    // the own cell's state waits in the primitive tile, unused from here on,
+    // while the face solves need the registers (D reads it back, and
+    // recomputes its cell index and takes dt / dx, dt / dy from LDS)
+    T* const park = &sW[0][0][0];
+    if (own_in) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) park[k * FWY * FWX + tid] = Uown.v[k];
+    }
     // C2
     const int nh = nHard;
     const RSFluid& rf = I.rsf[f];
@@ -550,15 +559,21 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
     flag(I, st);
     __syncthreads();
     // D
-    if (own_in) {
-        const long long c = cown;
-        V4<R> U = Uown;
-        if (ci >= bb0 && ci < bb1 && cj >= bb2 && cj < bb3) {
+    int t = tid;
+    asm volatile("" : "+v"(t));   // recomputed below, not kept live through C2
+    const int di = i0 + t / FTX, dj = j0 + t % FTX;
+    if (di < I.g + I.Ny && dj < I.g + I.Nx) {
+        const long long c = cidx(I, di, dj);
+        V4<R> U;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) U.v[k] = park[k * FWY * FWX + t];
+        const T dtodx = sLam[0], dtody = sLam[1];
+        if (di >= bb0 && di < bb1 && dj >= bb2 && dj < bb3) {
             bool ok;
             if constexpr (sizeof(T) == 4) ok = phys_state_f(U);
             else ok = phys_state(U);
             if (!ok) flag(I, ST_UNPHYS);   // FS_MUSCL.cpp:167
-            const int rr = tid / FTX, fc = tid % FTX;
+            const int rr = t / FTX, fc = t % FTX;
             const int fxl = rr * (FTX + 1) + fc, fyb = NFX + rr * FTX + fc;
             const V4<T> Fl = tb_get<T>(sF[0], fxl), Fr = tb_get<T>(sF[0], fxl + 1);
             const V4<T> Fb = tb_get<T>(sF[0], fyb), Ft = tb_get<T>(sF[0], fyb + FTX);
