@@ -1936,6 +1936,7 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
     constexpr int RX = RTX + 2 * RT_MAXIT, RY = RTY + 2 * RT_MAXIT, NC = RX * RY;
     constexpr int SLOTS = (NC + RT_THREADS - 1) / RT_THREADS;
     __shared__ R s[NC];
+    __shared__ R scs[NC];   // reinit_setup's constant per cell (in LDS: it would not fit the registers)
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
     const int g = I.g;
@@ -1976,12 +1977,11 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
     // edge (refreshes the first ghost layer), bits 11+: distance to the
     // region edge
     const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
-    R cs[SLOTS];
     int meta[SLOTS];
 #pragma unroll
     for (int k = 0; k < SLOTS; ++k) {
         const int idx = threadIdx.x + k * RT_THREADS;
-        cs[k] = R(0); meta[k] = 0;
+        meta[k] = 0;
         if (idx < NC) {
             const int a = idx / RX, b = idx - (idx / RX) * RX;
             const int i = ri + a, j = rj + b;
@@ -1990,7 +1990,9 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
                 v = in[cidx(I, i, j)];
                 if (interior(I, i, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1) {
                     int mode;
-                    reinit_cell_setup<R>(I, P0, i, j, mode, cs[k]);
+                    R cs;
+                    reinit_cell_setup<R>(I, P0, i, j, mode, cs);
+                    scs[idx] = cs;
                     const bool own = a >= RT_MAXIT && a < RT_MAXIT + RTY && b >= RT_MAXIT && b < RT_MAXIT + RTX;
                     const bool edge = i == ib || i == it || j == jl || j == jr;
                     const int dist = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
@@ -2010,8 +2012,8 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
             const int idx = threadIdx.x + k * RT_THREADS;
             if ((meta[k] & 256) && (meta[k] >> 11) >= t) {
                 act |= 1 << k;
-                nv[k] = reinit_update<R>(meta[k] & 7, cs[k], s[idx], s[idx - 1], s[idx + 1], s[idx - RX], s[idx + RX],
-                                         rk.idx2, rk.idy2, rk.dtau);
+                nv[k] = reinit_update<R>(meta[k] & 7, scs[idx], s[idx], s[idx - 1], s[idx + 1], s[idx - RX],
+                                         s[idx + RX], rk.idx2, rk.idy2, rk.dtau);
             }
         }
         if (t == nit) {
