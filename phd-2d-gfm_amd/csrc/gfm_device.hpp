@@ -502,8 +502,8 @@ __device__ __forceinline__ double rs_f_m(const RSFluid& F, const RSSideM& s, dou
 // lies above the floor (else rs_guess's choice), tolerance tol, and with
 // DENS the star densities (Toro 4.5.1, 4.5.2: shock rho (r + g6) / (g6 r +
 // 1), rarefaction rho r^(1/g), r = (p* + pinf) / (p + pinf)).  Only the
-// evaluation differs from rs_star.  Used for the interfacial problems and,
-// with one fluid and the FP32 iteration's p* as the start, for the faces.
+// evaluation differs from rs_star.  Used for the interfacial problems (the
+// faces use rs_star_face_m below).
 template <bool DENS = true>
 __device__ __forceinline__ unsigned rs_star_m2(const RSFluid& FL, const RSFluid& FR, const RSSideM& L,
                                                const RSSideM& R, Star& S, double p0, double tol) {
