@@ -51,7 +51,8 @@ enum {
     GFM_ST_UNPHYSICAL = 1,      /* an assert of the reference would fire      */
     GFM_ST_RS_FAIL = 2,         /* exact RS: no convergence, vacuum or NaN    */
     GFM_ST_IFACE_OVERFLOW = 4,  /* interfacial list larger than its capacity  */
-    GFM_ST_IFACE_ROUNDS = 8     /* interfacial dependency rounds not drained  */
+    GFM_ST_IFACE_ROUNDS = 8,    /* interfacial dependency rounds not drained  */
+    GFM_ST_IFACE_LIST = 16      /* interfacial cell counts and list disagree  */
 };
 
 /* sim_info (external PhD-Common sim_info.hpp; fields used at

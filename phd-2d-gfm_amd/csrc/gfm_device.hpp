@@ -22,7 +22,7 @@ enum {
     SI_NLIST = 14,   // entries of the last interfacial list recorded in imap
     SI_N = 16
 };
-enum { ST_UNPHYS = 1, ST_RS = 2, ST_IOVF = 4, ST_IROUNDS = 8 };
+enum { ST_UNPHYS = 1, ST_RS = 2, ST_IOVF = 4, ST_IROUNDS = 8, ST_ILIST = 16 };
 
 // Per-fluid constants of the mixed-precision face Riemann solver, set on the
 // host at commit: z = (g-1)/(2g), g6 = (g-1)/(g+1), a2 = 2/(g+1), ig = 1/g,

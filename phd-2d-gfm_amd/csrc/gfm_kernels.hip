@@ -90,8 +90,8 @@ __device__ __forceinline__ void erase_prev_list(const Inst& I, int s, int lane) 
         const int k = s * SEGW + q * 64 + lane;
         if (k < n) {
             const int v = I.ilist[k];
-            const int i = v / I.W;
-            I.imap[cidx(I, i, v - i * I.W)] = -1;
+            const int i = v / I.W, j = v - (v / I.W) * I.W;
+            if (v >= 0 && interior(I, i, j)) I.imap[cidx(I, i, j)] = -1;   // a bad entry was flagged when listed
         }
     }
 }
@@ -117,14 +117,50 @@ __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, cons
     const long long row = cidx(I, i, 0);
     R ph[SEG_Q];
     V4<R> U[SEG_Q];
+    // phi of the segment's cells and of the first ghost column after the
+    // last interior cell (a right neighbour)
 #pragma unroll
-    for (int k = 0; k < SEG_Q; ++k) ph[k] = (twofluid && j0 + 64 * k < jend) ? ld1<R>(I.phi, row + j0 + 64 * k) : R(0);
-    // the interfacial test's loads do not wait for the states
+    for (int k = 0; k < SEG_Q; ++k)
+        ph[k] = ((twofluid || count) && j0 + 64 * k <= jend) ? ld1<R>(I.phi, row + j0 + 64 * k) : R(0);
+    // the interfacial test (is_iface): the rows above and below are loaded,
+    // the left and right neighbours come from the adjacent lanes (the
+    // segment is contiguous), plus one load at each end of the segment; its
+    // loads do not wait for the states
     unsigned ifm = 0;
     if (count) {
+        R up[SEG_Q], dn[SEG_Q];
 #pragma unroll
-        for (int k = 0; k < SEG_Q; ++k)
-            if (j0 + 64 * k < jend && is_iface<R>(I, i, j0 + 64 * k)) ifm |= 1u << k;
+        for (int k = 0; k < SEG_Q; ++k) {
+            const int j = j0 + 64 * k;
+            up[k] = dn[k] = R(0);
+            if (j < jend) { up[k] = ld1<R>(I.phi, row + I.pitch + j); dn[k] = ld1<R>(I.phi, row - I.pitch + j); }
+        }
+        R el = R(0), er = R(0);
+        if (lane == 0) el = ld1<R>(I.phi, row + j0 - 1);
+        // j0 is this lane's column in chunk 0: lane 63's last cell is j0 + 64 (SEG_Q - 1)
+        if (lane == 63 && j0 + 64 * (SEG_Q - 1) < jend) er = ld1<R>(I.phi, row + j0 + 64 * (SEG_Q - 1) + 1);
+        R lft[SEG_Q], rgt[SEG_Q];
+        R carry = el;   // lane 0: the left neighbour of chunk k's first cell
+#pragma unroll
+        for (int k = 0; k < SEG_Q; ++k) {
+            const R rot = __shfl(ph[k], (lane + 63) & 63);   // lane l: lane l - 1 (lane 0: lane 63)
+            lft[k] = lane == 0 ? carry : rot;
+            carry = rot;
+        }
+        carry = er;     // lane 63: the right neighbour of chunk k's last cell
+#pragma unroll
+        for (int k = SEG_Q - 1; k >= 0; --k) {
+            const R rot = __shfl(ph[k], (lane + 1) & 63);    // lane l: lane l + 1 (lane 63: lane 0)
+            rgt[k] = lane == 63 ? carry : rot;
+            carry = rot;
+        }
+#pragma unroll
+        for (int k = 0; k < SEG_Q; ++k) {
+            const bool s0 = ph[k] <= R(0);
+            const bool ifc = ((rgt[k] <= R(0)) != s0) || ((lft[k] <= R(0)) != s0) || ((up[k] <= R(0)) != s0) ||
+                             ((dn[k] <= R(0)) != s0);
+            if (j0 + 64 * k < jend && ifc) ifm |= 1u << k;
+        }
     }
 #pragma unroll
     for (int k = 0; k < SEG_Q; ++k) {
@@ -1167,7 +1203,7 @@ __global__ __launch_bounds__(256) void k_iface_mark(const Inst* __restrict__ ins
     if (next == k) return;   // no interfacial cell in this segment
     const unsigned m = seg_iface<R>(I, w, lane);
     const int i = I.g + w.y;
-    bool ovf = false;
+    bool ovf = false, bad = false;
 #pragma unroll
     for (int q = 0; q < SEG_Q; ++q) {
         const bool f = (m >> q) & 1u;
@@ -1175,7 +1211,9 @@ __global__ __launch_bounds__(256) void k_iface_mark(const Inst* __restrict__ ins
         if (f) {
             const int kk = k + __popcll(mask & ((1ull << lane) - 1ull));
             const int j = I.g + w.z * SEGW + q * 64 + lane;
-            if (kk < I.icap) {
+            if (kk >= next) {
+                bad = true;   // more cells than the segment's count: never write past its range
+            } else if (kk < I.icap) {
                 I.ilist[kk] = i * I.W + j;
                 I.imap[cidx(I, i, j)] = kk;
             } else {
@@ -1184,7 +1222,9 @@ __global__ __launch_bounds__(256) void k_iface_mark(const Inst* __restrict__ ins
         }
         k += __popcll(mask);
     }
+    if (k < next) bad = true;   // fewer: entries of the range would keep stale cells
     if (ovf) flag(I, ST_IOVF);
+    if (bad) flag(I, ST_ILIST);
 }
 
 // ---------------------------------------------------------------------------
@@ -1402,8 +1442,13 @@ __global__ __launch_bounds__(256) void k_iface_prep(const Inst* __restrict__ ins
     const int n = min(gmem(I.si)[SI_NIFACE], I.icap);
     const int k = w.y + threadIdx.x;
     if (k >= n) return;
-    const int lin = I.ilist[k];
-    const int i = lin / I.W, j = lin % I.W;
+    int lin = I.ilist[k];
+    int i = lin / I.W, j = lin % I.W;
+    if (lin < 0 || !interior(I, i, j)) {
+        // only if the counts and the list disagreed (flagged): a valid cell keeps every access in bounds
+        flag(I, ST_ILIST);
+        i = I.g; j = I.g; lin = i * I.W + j;
+    }
     IfRec r;
     r.lsval = (double)ld1<R>(I.phi, cidx(I, i, j));
     unit_normal<R>(I, i, j, r.nx, r.ny);

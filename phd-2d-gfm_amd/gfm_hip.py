@@ -19,7 +19,7 @@ SIM_ONEFLUID, SIM_TWOFLUID = 0, 1
 FS_MUSCL, FS_GODUNOV = 0, 1
 GFM_ORIGINAL, GFM_REAL, GFM_RIEMANN, GFM_MODIFIED = 0, 1, 2, 3
 FP64, MIXED = 0, 1
-ST_UNPHYSICAL, ST_RS_FAIL, ST_IFACE_OVERFLOW, ST_IFACE_ROUNDS = 1, 2, 4, 8
+ST_UNPHYSICAL, ST_RS_FAIL, ST_IFACE_OVERFLOW, ST_IFACE_ROUNDS, ST_IFACE_LIST = 1, 2, 4, 8, 16
 
 GFM_NAMES = {"OGFM": GFM_ORIGINAL, "realGFM": GFM_REAL, "riemannGFM": GFM_RIEMANN, "MGFM": GFM_MODIFIED}
 
