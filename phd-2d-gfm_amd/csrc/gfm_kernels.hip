@@ -50,14 +50,6 @@ __device__ __forceinline__ void flag(const Inst& I, unsigned bits) {
     if (bits) atomicOr((unsigned*)&I.si[SI_STATUS], bits);
 }
 
-// interfacial cell (external Levelset::is_interfacial_cell, restated as a
-// 4-neighbour change of side with the side taken as phi <= 0)
-template <typename R>
-__device__ __forceinline__ bool is_iface(const Inst& I, int i, int j) {
-    const bool s = ld1<R>(I.phi, cidx(I, i, j)) <= R(0);
-    return ((ld1<R>(I.phi, cidx(I, i, j + 1)) <= R(0)) != s) || ((ld1<R>(I.phi, cidx(I, i, j - 1)) <= R(0)) != s) ||
-           ((ld1<R>(I.phi, cidx(I, i + 1, j)) <= R(0)) != s) || ((ld1<R>(I.phi, cidx(I, i - 1, j)) <= R(0)) != s);
-}
 
 // update boxes (realcells1/2): the full domain, or empty when GFM_original is
 // about to narrow them to the bounding box of each fluid's real cells
@@ -96,6 +88,53 @@ __device__ __forceinline__ void erase_prev_list(const Inst& I, int s, int lane) 
     }
 }
 
+// interfacial cells of a row segment (external Levelset::is_interfacial_cell,
+// restated as a 4-neighbour change of side with the side taken as phi <= 0)
+// for cells q*64 + lane, as a bit per q, given phi of the segment's cells and of the first ghost column
+// after the last interior cell (ph[q], loaded where j <= jend): the rows
+// above and below are loaded, the left and right neighbours come from the
+// adjacent lanes (the segment is contiguous), plus one load at each end of
+// the segment.  row = cidx(I, i, 0); j0 = this lane's column in chunk 0.
+template <typename R>
+__device__ __forceinline__ unsigned seg_iface_ph(const Inst& I, long long row, int j0, int jend, int lane,
+                                                 const R (&ph)[SEG_Q]) {
+    R up[SEG_Q], dn[SEG_Q];
+#pragma unroll
+    for (int k = 0; k < SEG_Q; ++k) {
+        const int j = j0 + 64 * k;
+        up[k] = dn[k] = R(0);
+        if (j < jend) { up[k] = ld1<R>(I.phi, row + I.pitch + j); dn[k] = ld1<R>(I.phi, row - I.pitch + j); }
+    }
+    R el = R(0), er = R(0);
+    if (lane == 0) el = ld1<R>(I.phi, row + j0 - 1);
+    // lane 63's last cell is j0 + 64 (SEG_Q - 1)
+    if (lane == 63 && j0 + 64 * (SEG_Q - 1) < jend) er = ld1<R>(I.phi, row + j0 + 64 * (SEG_Q - 1) + 1);
+    R lft[SEG_Q], rgt[SEG_Q];
+    R carry = el;   // lane 0: the left neighbour of chunk k's first cell
+#pragma unroll
+    for (int k = 0; k < SEG_Q; ++k) {
+        const R rot = __shfl(ph[k], (lane + 63) & 63);   // lane l: lane l - 1 (lane 0: lane 63)
+        lft[k] = lane == 0 ? carry : rot;
+        carry = rot;
+    }
+    carry = er;     // lane 63: the right neighbour of chunk k's last cell
+#pragma unroll
+    for (int k = SEG_Q - 1; k >= 0; --k) {
+        const R rot = __shfl(ph[k], (lane + 1) & 63);    // lane l: lane l + 1 (lane 63: lane 0)
+        rgt[k] = lane == 63 ? carry : rot;
+        carry = rot;
+    }
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < SEG_Q; ++k) {
+        const bool s0 = ph[k] <= R(0);
+        const bool ifc = ((rgt[k] <= R(0)) != s0) || ((lft[k] <= R(0)) != s0) || ((up[k] <= R(0)) != s0) ||
+                         ((dn[k] <= R(0)) != s0);
+        if (j0 + 64 * k < jend && ifc) m |= 1u << k;
+    }
+    return m;
+}
+
 // ---------------------------------------------------------------------------
 // A14 + A16: CFL dt (sim_twofluid.cpp:108-157, sim_onefluid.cpp:68-91) and
 // fluid masses (sim_twofluid.cpp:1803-1827).  Segment partials, then one
@@ -122,46 +161,8 @@ __global__ __launch_bounds__(256) void k_dt(const Inst* __restrict__ insts, cons
 #pragma unroll
     for (int k = 0; k < SEG_Q; ++k)
         ph[k] = ((twofluid || count) && j0 + 64 * k <= jend) ? ld1<R>(I.phi, row + j0 + 64 * k) : R(0);
-    // the interfacial test (is_iface): the rows above and below are loaded,
-    // the left and right neighbours come from the adjacent lanes (the
-    // segment is contiguous), plus one load at each end of the segment; its
-    // loads do not wait for the states
-    unsigned ifm = 0;
-    if (count) {
-        R up[SEG_Q], dn[SEG_Q];
-#pragma unroll
-        for (int k = 0; k < SEG_Q; ++k) {
-            const int j = j0 + 64 * k;
-            up[k] = dn[k] = R(0);
-            if (j < jend) { up[k] = ld1<R>(I.phi, row + I.pitch + j); dn[k] = ld1<R>(I.phi, row - I.pitch + j); }
-        }
-        R el = R(0), er = R(0);
-        if (lane == 0) el = ld1<R>(I.phi, row + j0 - 1);
-        // j0 is this lane's column in chunk 0: lane 63's last cell is j0 + 64 (SEG_Q - 1)
-        if (lane == 63 && j0 + 64 * (SEG_Q - 1) < jend) er = ld1<R>(I.phi, row + j0 + 64 * (SEG_Q - 1) + 1);
-        R lft[SEG_Q], rgt[SEG_Q];
-        R carry = el;   // lane 0: the left neighbour of chunk k's first cell
-#pragma unroll
-        for (int k = 0; k < SEG_Q; ++k) {
-            const R rot = __shfl(ph[k], (lane + 63) & 63);   // lane l: lane l - 1 (lane 0: lane 63)
-            lft[k] = lane == 0 ? carry : rot;
-            carry = rot;
-        }
-        carry = er;     // lane 63: the right neighbour of chunk k's last cell
-#pragma unroll
-        for (int k = SEG_Q - 1; k >= 0; --k) {
-            const R rot = __shfl(ph[k], (lane + 1) & 63);    // lane l: lane l + 1 (lane 63: lane 0)
-            rgt[k] = lane == 63 ? carry : rot;
-            carry = rot;
-        }
-#pragma unroll
-        for (int k = 0; k < SEG_Q; ++k) {
-            const bool s0 = ph[k] <= R(0);
-            const bool ifc = ((rgt[k] <= R(0)) != s0) || ((lft[k] <= R(0)) != s0) || ((up[k] <= R(0)) != s0) ||
-                             ((dn[k] <= R(0)) != s0);
-            if (j0 + 64 * k < jend && ifc) ifm |= 1u << k;
-        }
-    }
+    // the interfacial test's loads do not wait for the states
+    const unsigned ifm = count ? seg_iface_ph<R>(I, row, j0, jend, lane, ph) : 0u;
 #pragma unroll
     for (int k = 0; k < SEG_Q; ++k) {
         const int j = j0 + 64 * k;
@@ -1131,14 +1132,12 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
 // q*64 + lane
 template <typename R>
 __device__ __forceinline__ unsigned seg_iface(const Inst& I, const int4& w, int lane) {
-    const int i = I.g + w.y, j0 = I.g + w.z * SEGW + lane;
-    unsigned m = 0;
+    const int i = I.g + w.y, j0 = I.g + w.z * SEGW + lane, jend = I.g + I.Nx;
+    const long long row = cidx(I, i, 0);
+    R ph[SEG_Q];
 #pragma unroll
-    for (int k = 0; k < SEG_Q; ++k) {
-        const int j = j0 + 64 * k;
-        if (j < I.g + I.Nx && is_iface<R>(I, i, j)) m |= 1u << k;
-    }
-    return m;
+    for (int k = 0; k < SEG_Q; ++k) ph[k] = j0 + 64 * k <= jend ? ld1<R>(I.phi, row + j0 + 64 * k) : R(0);
+    return seg_iface_ph<R>(I, row, j0, jend, lane, ph);
 }
 
 template <typename R>
