@@ -146,6 +146,23 @@ def states_of(ctx, n):
     return [ctx.download(k) for k in range(n)]
 
 
+def host_io_run(ctx, states, steps, init_reinit):
+    """the rate through the per-phase drop-in boundary: every step uploads
+    each instance's host grids (FP64 cells, both fluids and phi), runs one
+    step (eager launches) and downloads all grids again"""
+    load_all(ctx, states, init_reinit)
+    ctx.sync()
+    cur = list(states)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for k, st in enumerate(cur):
+            ctx.upload(k, *st)
+        ctx.step(1)
+        cur = states_of(ctx, len(cur))
+    ctx.sync()
+    return time.perf_counter() - t0
+
+
 def accuracy(infos, got, ref):
     res = [A.compare(info, a, b) for info, a, b in zip(infos, got, ref)]
     return A.aggregate(res, [i.Nx * i.Ny for i in infos])
@@ -177,6 +194,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=None,
                     help="steps of the per-phase HIP-event pass (default: the timed step count)")
     ap.add_argument("--eager", action="store_true", help="launch kernels directly instead of one hipGraph per step")
+    ap.add_argument("--host-io", type=int, default=0, metavar="K",
+                    help="also time K steps that move every instance's grids host -> device -> host around each "
+                         "step (the per-phase drop-in's rate, PCIe included; reported beside value, never as it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run device-to-device copies of known size (the workload's conserved grids, "
                          "through gfm_set_ghost_states) for scaling rocprof FETCH_SIZE / WRITE_SIZE; prints "
@@ -224,6 +244,13 @@ def main():
     step_bytes = 2 * (4 * 2 + 1) * es
     step_gbs = step_bytes * value / 1e9
     final = states_of(ctx, len(cases)) if nprof == steps else None
+    host_io = None
+    if a.host_io > 0:
+        el_io = host_io_run(ctx, states, a.host_io, init_reinit)
+        host_io = {"value": cells * a.host_io / el_io, "unit": "cell-updates/s", "steps": a.host_io,
+                   "ms_per_step": el_io * 1e3 / a.host_io,
+                   "what": "per step: upload of every instance's FP64 host grids (both fluids, phi), one eager "
+                           "step, download of all grids, from Python through the C ABI"}
     ctx.close()
 
     out = {
@@ -253,6 +280,8 @@ def main():
         "phase_steps": nprof,
         "status": sorted(set(status)),
     }
+    if host_io is not None:
+        out["host_io_inclusive"] = host_io
     if prec == G.MIXED and not a.no_fp64:
         ctx64, st64, _ = build(builder, cases, variant, G.FP64, steps + a.warmup)
         el64 = timed_run(ctx64, st64, steps, a.warmup, init_reinit, a.eager)
