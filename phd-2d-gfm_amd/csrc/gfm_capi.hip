@@ -611,14 +611,19 @@ int gfm_ctx_commit(gfm_ctx* c) {
     const int iface_plain = (plain && plain[0] == '1') ? 1 : 0;
     HC(hipMalloc((void**)&c->d_insts, descs.size() * sizeof(Inst)));
     HC(hipMemcpyAsync(c->d_insts, descs.data(), descs.size() * sizeof(Inst), hipMemcpyHostToDevice, c->stream));
-    // step groups: GFM_STEP_GROUPS (default and maximum 4, the hardware
-    // queues a process gets; 6 and 8 groups measured slower) groups of
-    // instances with about equal cell counts, largest instances first
+    // step groups: GFM_STEP_GROUPS (maximum 4, the hardware queues a
+    // process gets) groups of instances with about equal cell counts,
+    // largest instances first.  Default 2 in mixed precision, 4 in FP64: on
+    // 64 shocked bubbles the mixed path measured 3.80 ms per step with two
+    // groups, 3.92 with three, 3.94 with four and 4.51 on one stream; the
+    // FP64 path, whose kernels take longer next to the same interfacial
+    // solve, 12.7 ms with four groups and 13.9 with two (6 and 8 groups were
+    // slower for both).
     const int ninst = (int)c->inst.size();
     int ng = 1;
     if (c->twofluid()) {
         const char* eg = std::getenv("GFM_STEP_GROUPS");
-        ng = eg ? std::atoi(eg) : 4;
+        ng = eg ? std::atoi(eg) : (c->mixed() ? 2 : 4);
         ng = std::max(1, std::min(ng, std::min(ninst, 4)));
     }
     std::vector<std::vector<int>> members(ng);
