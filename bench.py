@@ -3,8 +3,10 @@
 A "step" is one full two-fluid time step of every instance: CFL dt, ghost-
 fluid fill, MUSCL-Hancock exact-RS update of both fluids, HOUC-5 level-set
 advection with reinitialisation every 5 steps, boundary conditions.  Inputs
-are resident in HBM before the timed region; each step is one hipGraph
-launch (--eager: direct launches).
+are resident in HBM before the timed region; a single instance runs one
+hipGraph per step, a batch runs its instance groups (two streams in mixed
+precision, four in FP64) from one graph per 100-step block (--eager: direct
+launches).
 
 Workloads (--workload):
   batch   (default) 64 seeded shocked bubbles, grids s*(256x72), s~U[1,8]
@@ -34,6 +36,9 @@ initial state perturbed by 1e-7 relative, which measures how much the flow
 itself amplifies a last-digit change.
 `cpu_baseline` times the single-threaded FP64 CPU oracle on a bounded
 sample (first steps of the smallest instances).
+--host-io K adds `host_io_inclusive`: K steps with every instance's grids
+moved host -> device -> host around each step (the per-phase drop-in
+boundary, PCIe included), reported beside `value`, never as it.
 """
 import argparse
 import json
