@@ -36,6 +36,9 @@ initial state perturbed by 1e-7 relative, which measures how much the flow
 itself amplifies a last-digit change.
 `cpu_baseline` times the single-threaded FP64 CPU oracle on a bounded
 sample (first steps of the smallest instances).
+Before timing, each precision's schedule (instance groups, blocked or
+one-sweep sweeps) is autotuned on short runs of the same job (autotune.py;
+`schedule` holds the trials; --no-autotune keeps the library default).
 --host-io K adds `host_io_inclusive`: K steps with every instance's grids
 moved host -> device -> host around each step (the per-phase drop-in
 boundary, PCIe included), reported beside `value`, never as it.
@@ -52,6 +55,7 @@ sys.path.insert(0, os.path.join(ROOT, "phd-2d-gfm_amd"))
 import numpy as np  # noqa: E402
 
 import accuracy as A  # noqa: E402
+import autotune as T  # noqa: E402
 import gfm_hip as G  # noqa: E402
 import problems as P  # noqa: E402
 
@@ -86,22 +90,47 @@ def make_cases(workload, n_inst):
             % len(cases), 40)
 
 
-def build(builder, cases, variant, precision, steps, perturb=0.0):
-    m = G.methods(sim_type=G.SIM_TWOFLUID, fs=G.FS_MUSCL, gfm=variant, ls_order=5,
-                  reinit_every=5, reinit_iters=10, precision=precision)
-    ctx = G.Context(m)
-    states, infos = [], []
+def make_setup(builder, cases, perturb=0.0):
+    """per instance (sim info, EOS) and the initial host grids"""
+    setup, states = [], []
     rng = np.random.default_rng(7)
     for c in cases:
         info, eos, st = builder(**c)
         if perturb:
             st = tuple(np.ascontiguousarray(a * (1.0 + perturb * rng.uniform(-1.0, 1.0, a.shape)))
                        if k < 2 else a for k, a in enumerate(st))
-        ctx.add(info, eos, 0.6, 1e300, steps)
+        setup.append((info, eos))
         states.append(st)
-        infos.append(info)
+    return setup, states
+
+
+def new_ctx(setup, variant, precision, steps):
+    m = G.methods(sim_type=G.SIM_TWOFLUID, fs=G.FS_MUSCL, gfm=variant, ls_order=5,
+                  reinit_every=5, reinit_iters=10, precision=precision)
+    ctx = G.Context(m)
+    for info, eos in setup:
+        ctx.add(info, eos, 0.6, 1e300, steps)
     ctx.commit()
-    return ctx, states, infos
+    return ctx
+
+
+def build(builder, cases, variant, precision, steps, perturb=0.0):
+    setup, states = make_setup(builder, cases, perturb)
+    return new_ctx(setup, variant, precision, steps), states, [s[0] for s in setup]
+
+
+def tuned_ctx(a, setup, states, variant, precision, steps, init_reinit):
+    """a context on the autotuned schedule (autotune.py), unless --no-autotune, --eager, or the
+    schedule is set in the environment; returns (ctx, schedule record)"""
+    if a.no_autotune or a.eager or any(k in os.environ for k in T.KEYS):
+        rec = {"tuned": False, "env": {k: os.environ[k] for k in T.KEYS if k in os.environ}}
+        return new_ctx(setup, variant, precision, steps), rec
+    ts = min(a.tune_steps, steps)
+    choice = T.tune(lambda: new_ctx(setup, variant, precision, ts + a.warmup),
+                    lambda c: load_all(c, states, init_reinit), len(setup), steps=ts, warmup=a.warmup)
+    with T.applied(choice):
+        ctx = new_ctx(setup, variant, precision, steps)
+    return ctx, dict(choice, tuned=True, tune_steps=ts)
 
 
 def load_all(ctx, states, init_reinit):
@@ -199,6 +228,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=None,
                     help="steps of the per-phase HIP-event pass (default: the timed step count)")
     ap.add_argument("--eager", action="store_true", help="launch kernels directly instead of one hipGraph per step")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default schedule instead of autotuning it (autotune.py)")
+    ap.add_argument("--tune-steps", type=int, default=100, help="steps per autotuning trial")
     ap.add_argument("--host-io", type=int, default=0, metavar="K",
                     help="also time K steps that move every instance's grids host -> device -> host around each "
                          "step (the per-phase drop-in's rate, PCIe included; reported beside value, never as it)")
@@ -216,9 +248,11 @@ def main():
     builder, cases, variant, prec, dsteps, name, init_reinit = make_cases(a.workload, n_inst)
     steps = a.steps or dsteps
 
-    ctx, states, infos = build(builder, cases, variant, prec, steps + a.warmup)
+    setup, states = make_setup(builder, cases)
+    infos = [s[0] for s in setup]
     cells = sum(i.Nx * i.Ny for i in infos)
     if a.copy_calibration:
+        ctx = new_ctx(setup, variant, prec, steps + a.warmup)
         # the phase API ends a ghost fill with Q[.][1] -> Q[.][0], one
         # hipMemcpyAsync per instance and fluid of 4 padded planes
         es = 8 if prec == G.FP64 else 4
@@ -233,6 +267,7 @@ def main():
         ctx.sync()
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 2 * len(infos)}))
         return
+    ctx, schedule = tuned_ctx(a, setup, states, variant, prec, steps + a.warmup, init_reinit)
     el = timed_run(ctx, states, steps, a.warmup, init_reinit, a.eager)
     status = [ctx.status(k) for k in range(len(cases))]
     value = cells * steps / el
@@ -284,14 +319,15 @@ def main():
         "phase_ms_per_step": {n: v / nprof for n, v in zip(("dt", "ghost", "fs", "ls", "bc"), ms)},
         "phase_steps": nprof,
         "status": sorted(set(status)),
+        "schedule": schedule,
     }
     if host_io is not None:
         out["host_io_inclusive"] = host_io
     if prec == G.MIXED and not a.no_fp64:
-        ctx64, st64, _ = build(builder, cases, variant, G.FP64, steps + a.warmup)
-        el64 = timed_run(ctx64, st64, steps, a.warmup, init_reinit, a.eager)
+        ctx64, sched64 = tuned_ctx(a, setup, states, variant, G.FP64, steps + a.warmup, init_reinit)
+        el64 = timed_run(ctx64, states, steps, a.warmup, init_reinit, a.eager)
         out["fp64_plain"] = {"value": cells * steps / el64, "steps": steps,
-                             "ratio_mixed_over_fp64": value / (cells * steps / el64)}
+                             "ratio_mixed_over_fp64": value / (cells * steps / el64), "schedule": sched64}
         ref = states_of(ctx64, len(cases))
         ctx64.close()
         if final is not None:

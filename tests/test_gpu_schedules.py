@@ -163,3 +163,41 @@ def test_instance_groups_match_one_stream(variant):
     for k in range(len(cases)):
         assert ca.get_time(k) == cb.get_time(k)
         assert ca.get_time(k)[1] == 112
+
+
+def test_autotune_small_batch():
+    """the schedule autotuner (autotune.py) on a real batch: five trials (four group counts with the
+    blocked sweeps, then the one-sweep schedule at the best count), and a context on the chosen
+    schedule steps the batch to the same bits as one stream"""
+    print("CASE autotune", flush=True)
+    import autotune as AT
+    sizes = [(96, 40), (64, 24), (130, 33), (72, 72), (40, 20)]
+    cases = [perturbed_bubble(nx, ny, BCS[k % len(BCS)], 40 + k) for k, (nx, ny) in enumerate(sizes)]
+
+    def build():
+        ctx = G.Context(G.methods(sim_type=G.SIM_TWOFLUID, fs=G.FS_MUSCL, gfm=G.GFM_RIEMANN, ls_order=5,
+                                  reinit_every=5, reinit_iters=10, precision=G.MIXED))
+        for info, eos, _ in cases:
+            ctx.add(info, eos, 0.6, 1e300, 1 << 30)
+        ctx.commit()
+        return ctx
+
+    def load(ctx):
+        for k, (_, _, st) in enumerate(cases):
+            ctx.upload(k, *st)
+            ctx.set_time(k, 0.0, 0)
+
+    before = {k: os.environ.get(k) for k in AT.KEYS}
+    choice = AT.tune(build, load, len(cases), steps=6, warmup=2)
+    assert {k: os.environ.get(k) for k in AT.KEYS} == before
+    assert len(choice["trials"]) == 5
+    assert {"groups": choice["groups"], "blocking": choice["blocking"], "ms_per_step": choice["ms_per_step"]} \
+        in choice["trials"]
+    with AT.applied(choice):
+        ca = build()
+    load(ca)
+    cb = make_ctx(cases, G.GFM_RIEMANN, G.MIXED, True, groups=1)
+    for c in (ca, cb):
+        c.step_graph(12)
+        c.sync()
+    compare(ca, cb, len(cases))
