@@ -469,6 +469,62 @@ void sim_twofluid::run_sim(GFM_settingsfile SF) {
         }
     }
     {
+        // the velocity field of the last ghost fill, the one the level set
+        // was advected with (sim_twofluid.cpp:181,198)
+        std::vector<double> vf(2 * (size_t)H * W);
+        check(gfm_download_vfield(c, 0, vf.data()), "gfm_download_vfield");
+        std::ofstream out(SF.basename + "-" + std::to_string(nstep) + "-vfield.dat");
+        for (int i = 2; i < H - 2; ++i) {
+            for (int j = 2; j < W - 2; ++j) {
+                const vec2 cc = p.cellcentre_coord(i, j);
+                const size_t q = (size_t)i * W + j;
+                out << cc[0] << " " << cc[1] << " " << vf[2 * q] << " " << vf[2 * q + 1] << "\n";
+            }
+            out << "\n";
+        }
+    }
+    {
+        // numerical schlieren (sim_twofluid.cpp:226-271): |grad rho| by
+        // central differences over the interior, each neighbour's density
+        // from the fluid real there (phi <= 0: fluid 1), over its maximum
+        const int g = p.numGC;
+        auto rho = [&](int i, int j) {
+            const size_t q = (size_t)i * W + j;
+            return phi[q] <= 0.0 ? g1[4 * q] : g2[4 * q];
+        };
+        std::vector<double> grad;
+        grad.reserve((size_t)p.Nx * p.Ny);
+        for (int i = g; i < p.Ny + g; ++i)
+            for (int j = g; j < p.Nx + g; ++j) {
+                const double gx = (rho(i, j + 1) - rho(i, j - 1)) / (2.0 * p.dx);
+                const double gy = (rho(i + 1, j) - rho(i - 1, j)) / (2.0 * p.dy);
+                grad.push_back(std::sqrt(gx * gx + gy * gy));
+            }
+        const double gmax = *std::max_element(grad.begin(), grad.end());
+        std::ofstream out(SF.basename + "-" + std::to_string(nstep) + "-schlieren.dat");
+        size_t k = 0;
+        for (int i = g; i < p.Ny + g; ++i) {
+            for (int j = g; j < p.Nx + g; ++j) {
+                const vec2 cc = p.cellcentre_coord(i, j);
+                out << cc[0] << " " << cc[1] << " " << grad[k++] / gmax << "\n";
+            }
+            out << "\n";
+        }
+        // L1 distance of fluid 1's density to the Gaussian density bump of
+        // the advection test, 1 + 1000 exp(-|x - (0.5, 0.5)|^2 / (2 0.1^2)),
+        // written for every case as the reference does (:273-291)
+        double l1 = 0.0;
+        for (int i = g; i < p.Ny + g; ++i)
+            for (int j = g; j < p.Nx + g; ++j) {
+                const vec2 cc = p.cellcentre_coord(i, j);
+                const double r2 = (cc[0] - 0.5) * (cc[0] - 0.5) + (cc[1] - 0.5) * (cc[1] - 0.5);
+                const double exact = 1.0 + 1000.0 * std::exp(-r2 / (2.0 * 0.1 * 0.1));
+                l1 += std::fabs(g1[4 * ((size_t)i * W + j)] - exact) * p.dx * p.dy;
+            }
+        std::ofstream err(SF.basename + "-GDAerr.dat");
+        err << p.Nx << " " << l1 << "\n";
+    }
+    {
         const int nh = std::min(nstep + 1, max_steps + 1);
         std::vector<double> h(3 * (size_t)nh);
         check(gfm_mass_history(c, 0, nh, h.data()), "gfm_mass_history");

@@ -1,5 +1,6 @@
 """The settings-file driver gfm2d (main.cpp / sim_base flow of the reference)
-run end to end on the GPU, its output files checked against the oracle.
+run end to end on the GPU, its output files (prims, masschange, vfield,
+schlieren, GDAerr; sim_twofluid.cpp:163-331) checked against the oracle.
 The output format prints 6 significant digits (as the reference's ofstream
 default does), so the bar is 2e-5 relative per printed field."""
 import os
@@ -67,12 +68,41 @@ def test_gfm2d_helium_bubble(tmp_path, oracle):
                               oracle._p(oracle.arr(1, 0, 0, 1)), oracle._p(oracle.arr(1.3764, -0.394, 0, 1.5698)),
                               225.0, oracle._p(g1), oracle._p(g2), oracle._p(phi))
     run = oracle.Run(2, 0, 5, 5, 10, 0.4, 280.0, 40)
+    # or_twofluid_run's loop, step by step, to keep the last step's velocity field
     s = np.zeros(1, dtype=np.int32)
     t = np.zeros(1)
-    assert oracle.lib().or_twofluid_run(G, E, run, oracle._p(g1), oracle._p(g2), oracle._p(phi), oracle._p(s),
-                                        oracle._p(t)) == 0
+    vf, s1, s2 = np.zeros((H, W, 2)), np.zeros((H, W, 4)), np.zeros((H, W, 4))
+    while t[0] < run.T and s[0] < run.max_steps:
+        assert oracle.lib().or_twofluid_step(G, E, run, int(s[0]), oracle._p(t), oracle._p(g1), oracle._p(g2),
+                                             oracle._p(phi), oracle._p(vf), oracle._p(s1), oracle._p(s2),
+                                             None) == 0
+        s[0] += 1
     rho = np.where(phi < 0, g1[..., 0], g2[..., 0])[2:H - 2, 2:W - 2].reshape(-1)
     assert np.allclose(got[:, 2], rho, rtol=2e-5, atol=0)
+    # -vfield.dat: the velocity field of the last ghost fill
+    vfield = read_dat(os.path.join(tmp_path, prims.replace("-prims.dat", "-vfield.dat")))
+    ref = vf[2:H - 2, 2:W - 2].reshape(-1, 2)
+    assert vfield.shape == (ref.shape[0], 4)
+    assert np.allclose(vfield[:, :2], got[:, :2], rtol=0, atol=0)
+    assert np.allclose(vfield[:, 2:], ref, rtol=2e-5, atol=1e-7 * np.abs(ref).max())
+    # -schlieren.dat (sim_twofluid.cpp:226-271) from the oracle's final state, restated here
+    g = 8
+    rr = np.where(phi <= 0, g1[..., 0], g2[..., 0])
+    gx = (rr[g:g + 36, g + 1:g + 131] - rr[g:g + 36, g - 1:g + 129]) / (2 * G.dx)
+    gy = (rr[g + 1:g + 37, g:g + 130] - rr[g - 1:g + 35, g:g + 130]) / (2 * G.dy)
+    gm = np.hypot(gx, gy).reshape(-1)
+    sch = read_dat(os.path.join(tmp_path, prims.replace("-prims.dat", "-schlieren.dat")))
+    assert sch.shape == (130 * 36, 3)
+    assert np.allclose(sch[:, 2], gm / gm.max(), rtol=2e-5, atol=2e-6)
+    # -GDAerr.dat: Nx and the L1 distance of fluid 1's density to the advection test's Gaussian
+    xs = 0.0 + (np.arange(W) - g + 0.5) * G.dx
+    ys = 0.0 + (np.arange(H) - g + 0.5) * G.dy
+    X, Y = np.meshgrid(xs, ys)
+    exact = 1.0 + 1000.0 * np.exp(-((X - 0.5) ** 2 + (Y - 0.5) ** 2) / (2 * 0.1 * 0.1))
+    l1 = (np.abs(g1[..., 0] - exact)[g:g + 36, g:g + 130]).sum() * G.dx * G.dy
+    gda = read_dat(str(tmp_path / [p for p in os.listdir(tmp_path) if p.endswith("-GDAerr.dat")][0]))
+    assert gda.shape == (1, 2) and gda[0, 0] == 130
+    assert abs(gda[0, 1] - l1) <= 2e-5 * l1
     mass = read_dat(os.path.join(tmp_path, [p for p in os.listdir(tmp_path) if p.endswith("masschange.dat")][0]))
     assert mass.shape == (41, 3) and mass[0, 1] == 0.0 and mass[0, 2] == 0.0
     m1 = oracle.lib().or_fluid_mass(G, -1.0, oracle._p(g1), oracle._p(phi))
