@@ -1181,7 +1181,14 @@ int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const do
                     np1[q * 4] = star[2]; np1[q * 4 + 1] = usx; np1[q * 4 + 2] = usy; np1[q * 4 + 3] = star[0];
                 }
             }
+#ifdef OR_MGFM_EXTRAPOLATE_NEW_STATES
+        /* not the reference: the extrapolation of :202 applied to the new
+         * states, which the thesis's MGFM convergence results imply
+         * (liboracle_mgfm_new.so, used only to check that claim) */
+        st |= or_extrapolate_vector(G, phi, 6, np1, np2, 1);
+#else
         st |= or_extrapolate_vector(G, phi, 6, prims1, prims2, 1);   /* :202, result unused below */
+#endif
         store_vfield_real(G, phi, prims1, prims2, vfield);
         for (size_t q = 0; q < ncell; ++q) {               /* :230-239 writes new_prims */
             or_prims_to_cons(E->gamma1, E->pinf1, np1 + q * 4, grid1 + q * 4);

@@ -42,6 +42,7 @@ class Run(C.Structure):
 
 
 _lib = None
+_other = {}
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)
 
@@ -61,56 +62,66 @@ def build():
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
 
-def lib():
+def lib(path=None):
+    """the oracle (or, with path, another build of the same source, e.g. liboracle_mgfm_new.so)"""
     global _lib
+    if path is not None and path != LIB_PATH:
+        if path not in _other:
+            if not os.path.exists(path):
+                build()
+            _other[path] = _bind(C.CDLL(path))
+        return _other[path]
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        dbl, i32 = C.c_double, C.c_int
-        sig = {
-            "or_eos_pressure": (dbl, [dbl, dbl, dbl, dbl]),
-            "or_eos_soundspeed": (dbl, [dbl, dbl, dbl, dbl]),
-            "or_eos_isentropic": (dbl, [dbl, dbl, dbl, dbl, dbl]),
-            "or_cons_to_prims": (None, [dbl, dbl, D, D]),
-            "or_prims_to_cons": (None, [dbl, dbl, D, D]),
-            "or_flux_prims": (None, [dbl, dbl, D, D]),
-            "or_exact_rs_star": (i32, [dbl, dbl, dbl, dbl, D, D, D, I]),
-            "or_exact_rs_sample": (None, [dbl, dbl, dbl, dbl, D, D, D, dbl, D]),
-            "or_pure_rs_flux": (i32, [i32, dbl, dbl, D, D, D]),
-            "or_mixed_rs": (i32, [dbl, dbl, dbl, dbl, D, D, D]),
-            "or_apply_bcs": (None, [C.POINTER(Grid), D]),
-            "or_apply_bcs_scalar": (None, [C.POINTER(Grid), D]),
-            "or_fs_muscl": (i32, [C.POINTER(Grid), dbl, dbl, dbl, I, D, D]),
-            "or_fs_godunov": (i32, [C.POINTER(Grid), dbl, dbl, dbl, I, D, D]),
-            "or_compute_dt_onefluid": (dbl, [C.POINTER(Grid), dbl, dbl, dbl, D, dbl, dbl]),
-            "or_compute_dt_twofluid": (dbl, [C.POINTER(Grid), C.POINTER(Eos), dbl, D, D, D, dbl, dbl]),
-            "or_fluid_mass": (dbl, [C.POINTER(Grid), dbl, D, D]),
-            "or_ls_normal": (None, [C.POINTER(Grid), D, i32, i32, D]),
-            "or_ls_is_interfacial": (i32, [C.POINTER(Grid), D, i32, i32]),
-            "or_bilinear": (None, [C.POINTER(Grid), D, dbl, dbl, D]),
-            "or_houc_coeffs": (None, [i32, D, I]),
-            "or_ls_advect": (None, [C.POINTER(Grid), i32, D, D, dbl]),
-            "or_ls_reinit": (None, [C.POINTER(Grid), D, i32]),
-            "or_extrapolate_vector": (i32, [C.POINTER(Grid), D, i32, D, D, i32]),
-            "or_set_ghost_states": (i32, [C.POINTER(Grid), C.POINTER(Eos), i32, D, D, D, I, I, D]),
-            "or_onefluid_step": (i32, [C.POINTER(Grid), dbl, dbl, i32, dbl, dbl, D, D, D, D]),
-            "or_twofluid_step": (i32, [C.POINTER(Grid), C.POINTER(Eos), C.POINTER(Run), i32, D, D, D, D, D, D, D, D]),
-            "or_twofluid_run": (i32, [C.POINTER(Grid), C.POINTER(Eos), C.POINTER(Run), D, D, D, I, D]),
-            "or_onefluid_run": (i32, [C.POINTER(Grid), dbl, dbl, i32, dbl, dbl, i32, D, I, D]),
-            "or_cellcentre": (None, [C.POINTER(Grid), i32, i32, D, D]),
-            "or_ic_onefluid": (i32, [C.POINTER(Grid), dbl, dbl, i32, D]),
-            "or_ic_bubble": (None, [C.POINTER(Grid), C.POINTER(Eos), dbl, dbl, dbl, D, D, D, dbl, D, D, D]),
-            "or_ic_rmi": (None, [C.POINTER(Grid), C.POINTER(Eos), dbl, dbl, i32, D, D, D, dbl, D, D, D]),
-            "or_ic_planar_twofluid": (None, [C.POINTER(Grid), C.POINTER(Eos), i32, dbl, D, D, D, D, D]),
-            "or_shock_state": (None, [dbl, dbl, D, dbl, i32, D]),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = _bind(C.CDLL(LIB_PATH))
     return _lib
+
+
+def _bind(L):
+    dbl, i32 = C.c_double, C.c_int
+    sig = {
+        "or_eos_pressure": (dbl, [dbl, dbl, dbl, dbl]),
+        "or_eos_soundspeed": (dbl, [dbl, dbl, dbl, dbl]),
+        "or_eos_isentropic": (dbl, [dbl, dbl, dbl, dbl, dbl]),
+        "or_cons_to_prims": (None, [dbl, dbl, D, D]),
+        "or_prims_to_cons": (None, [dbl, dbl, D, D]),
+        "or_flux_prims": (None, [dbl, dbl, D, D]),
+        "or_exact_rs_star": (i32, [dbl, dbl, dbl, dbl, D, D, D, I]),
+        "or_exact_rs_sample": (None, [dbl, dbl, dbl, dbl, D, D, D, dbl, D]),
+        "or_pure_rs_flux": (i32, [i32, dbl, dbl, D, D, D]),
+        "or_mixed_rs": (i32, [dbl, dbl, dbl, dbl, D, D, D]),
+        "or_apply_bcs": (None, [C.POINTER(Grid), D]),
+        "or_apply_bcs_scalar": (None, [C.POINTER(Grid), D]),
+        "or_fs_muscl": (i32, [C.POINTER(Grid), dbl, dbl, dbl, I, D, D]),
+        "or_fs_godunov": (i32, [C.POINTER(Grid), dbl, dbl, dbl, I, D, D]),
+        "or_compute_dt_onefluid": (dbl, [C.POINTER(Grid), dbl, dbl, dbl, D, dbl, dbl]),
+        "or_compute_dt_twofluid": (dbl, [C.POINTER(Grid), C.POINTER(Eos), dbl, D, D, D, dbl, dbl]),
+        "or_fluid_mass": (dbl, [C.POINTER(Grid), dbl, D, D]),
+        "or_ls_normal": (None, [C.POINTER(Grid), D, i32, i32, D]),
+        "or_ls_is_interfacial": (i32, [C.POINTER(Grid), D, i32, i32]),
+        "or_bilinear": (None, [C.POINTER(Grid), D, dbl, dbl, D]),
+        "or_houc_coeffs": (None, [i32, D, I]),
+        "or_ls_advect": (None, [C.POINTER(Grid), i32, D, D, dbl]),
+        "or_ls_reinit": (None, [C.POINTER(Grid), D, i32]),
+        "or_extrapolate_vector": (i32, [C.POINTER(Grid), D, i32, D, D, i32]),
+        "or_set_ghost_states": (i32, [C.POINTER(Grid), C.POINTER(Eos), i32, D, D, D, I, I, D]),
+        "or_onefluid_step": (i32, [C.POINTER(Grid), dbl, dbl, i32, dbl, dbl, D, D, D, D]),
+        "or_twofluid_step": (i32, [C.POINTER(Grid), C.POINTER(Eos), C.POINTER(Run), i32, D, D, D, D, D, D, D, D]),
+        "or_twofluid_run": (i32, [C.POINTER(Grid), C.POINTER(Eos), C.POINTER(Run), D, D, D, I, D]),
+        "or_onefluid_run": (i32, [C.POINTER(Grid), dbl, dbl, i32, dbl, dbl, i32, D, I, D]),
+        "or_cellcentre": (None, [C.POINTER(Grid), i32, i32, D, D]),
+        "or_ic_onefluid": (i32, [C.POINTER(Grid), dbl, dbl, i32, D]),
+        "or_ic_bubble": (None, [C.POINTER(Grid), C.POINTER(Eos), dbl, dbl, dbl, D, D, D, dbl, D, D, D]),
+        "or_ic_rmi": (None, [C.POINTER(Grid), C.POINTER(Eos), dbl, dbl, i32, D, D, D, dbl, D, D, D]),
+        "or_ic_planar_twofluid": (None, [C.POINTER(Grid), C.POINTER(Eos), i32, dbl, D, D, D, D, D]),
+        "or_shock_state": (None, [dbl, dbl, D, dbl, i32, D]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
 
 
 def arr(*v):
