@@ -20,6 +20,7 @@ here. Prints one JSON line.
 """
 import argparse
 import json
+import os
 import time
 
 import numpy as np
@@ -105,12 +106,18 @@ def main():
     ap.add_argument("--ls-order", type=int, default=9)
     ap.add_argument("--reinit-every", type=int, default=0)
     ap.add_argument("--reinit-iters", type=int, default=10)
+    ap.add_argument("--mgfm-extrapolate", action="store_true",
+                    help="MGFM with the extrapolation of GFM_modified.cpp:202 applied to the new states "
+                         "(GFM_MGFM_EXTRAPOLATE=1; the thesis's form, not the reference code's)")
     a = ap.parse_args()
+    if a.mgfm_extrapolate:
+        os.environ["GFM_MGFM_EXTRAPOLATE"] = "1"
     t0 = time.perf_counter()
     ref_info, ref_rho, ref_steps, ref_status = reference(a.ref_n, a.cfl)
     out = {"cfl": a.cfl, "ref_n": a.ref_n, "ref_steps": ref_steps, "ref_status": ref_status,
            "ref_seconds": time.perf_counter() - t0, "ls_order": a.ls_order,
-           "reinit_every": a.reinit_every, "reinit_iters": a.reinit_iters, "res": a.res, "errors": {}}
+           "reinit_every": a.reinit_every, "reinit_iters": a.reinit_iters, "mgfm_extrapolate": a.mgfm_extrapolate,
+           "res": a.res, "errors": {}}
     for gfm in a.gfms:
         rows = []
         for k, n in enumerate(a.res):

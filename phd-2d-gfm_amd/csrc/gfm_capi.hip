@@ -81,6 +81,7 @@ struct gfm_ctx {
     int cur = 0;                       // current conserved buffer (one-fluid step path)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     bool tb = false;                   // temporally blocked extrapolation (no periodic instance)
+    bool mgfm_extrapolate = false;     // GFM_MGFM_EXTRAPOLATE=1 (GFM_modified only; not the reference)
     // two-fluid step path: the instances in groups, each group's step on its
     // own stream (group 0 on `stream`), forked from and joined back to
     // `stream`, so that the latency-bound phases of one group (the ordered
@@ -107,11 +108,13 @@ void dispatch_dt(gfm_ctx* c, int q) { gfm::launch_dt<R>(c->L, c->twofluid() ? 1 
 // N Jacobi iterations of the extrapolation.  Source: conserved Q[.][qin]
 // (from_q) or primitives P[.][pstart]; destination: conserved Q[.][qout]
 // with W->U and the velocity field (to_q) or the P ping-pong.  Returns the
-// P buffer holding the result when !to_q.
+// P buffer holding the result when !to_q.  all_extra: mode bits of every
+// iteration (32 = MF_HOLDIF: interfacial cells keep their ghost state), for
+// which there is no blocked kernel.
 template <typename R>
 int extrap_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int N, bool from_q, int qin, int pstart, bool inject_first,
-                 bool to_q, int qout, int last_extra) {
-    if (L.tb && N >= 2) {
+                 bool to_q, int qout, int last_extra, int all_extra = 0) {
+    if (L.tb && N >= 2 && all_extra == 0) {
         // one launch for all N iterations; it writes only the last one's
         // destination, which is the same buffer as in the loop below
         int cur_p = pstart;
@@ -122,7 +125,7 @@ int extrap_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int N, bool from_q, int qi
     }
     int cur_p = pstart;
     for (int k = 0; k < N; ++k) {
-        int mode = 0, bin, bout;
+        int mode = all_extra, bin, bout;
         if (k == 0) {
             bin = from_q ? qin : pstart;
             if (from_q) mode |= 1;            // MF_FIRST
@@ -169,7 +172,15 @@ int ghost_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int counted = 0, hipEvent_t
             const int p = extrap_chain<R>(c, L, 2, true, 0, 0, false, false, 0, 0);
             gfm::launch_iface<R>(L, gfm::V_MODIFIED, p, counted);
             if (mark) HC(hipEventRecord(mark, L.stream));
-            gfm::launch_mgfm_final<R>(L, 0, 1);
+            if (c->mgfm_extrapolate) {
+                // not the reference (GFM_MGFM_EXTRAPOLATE=1): the extrapolation of
+                // GFM_modified.cpp:202 applied to the new states, which :230-239
+                // otherwise discard; the interfacial cells keep theirs
+                gfm::launch_mgfm_final<R>(L, 0, p, true);
+                extrap_chain<R>(c, L, 6, false, 0, p, false, true, 1, 0, 32);
+            } else {
+                gfm::launch_mgfm_final<R>(L, 0, 1);
+            }
             break;
         }
         default: break;
@@ -603,6 +614,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
         descs.push_back(d);
         if (s.BC_L == GFM_BC_PERIODIC || s.BC_B == GFM_BC_PERIODIC) periodic = true;
     }
+    const char* mx = std::getenv("GFM_MGFM_EXTRAPOLATE");
+    c->mgfm_extrapolate = c->m.gfm == GFM_MODIFIED && mx && mx[0] == '1';
     const char* no_tb = std::getenv("GFM_NO_TEMPORAL_BLOCKING");
     c->tb = c->twofluid() && !periodic && !(no_tb && no_tb[0] == '1');
     // GFM_NO_TEMPORAL_BLOCKING=1 / GFM_IFACE_PLAIN_ROUNDS=1 select the simpler

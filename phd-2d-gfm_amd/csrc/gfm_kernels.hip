@@ -640,7 +640,7 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
 //   MF_RAW     read/write Q buffers as primitives (standalone extrapolation)
 // work: {inst, tile_y, tile_x, 0}, 64x4 tiles over the padded array
 // ---------------------------------------------------------------------------
-enum { MF_FIRST = 1, MF_INJECT = 2, MF_LAST = 4, MF_OGFM = 8, MF_RAW = 16 };
+enum { MF_FIRST = 1, MF_INJECT = 2, MF_LAST = 4, MF_OGFM = 8, MF_RAW = 16, MF_HOLDIF = 32 };
 
 template <typename R>
 __device__ __forceinline__ void unit_normal(const Inst& I, int i, int j, double& nx, double& ny) {
@@ -849,6 +849,9 @@ __device__ __forceinline__ void extrap_cell(const Inst& I, int i, int j, int bin
     V4<R> Wg = extrap_combine<R>(wx, wy, ex_load<R, MODE>(I, fg, si, sj, in_g),
                                  ex_load<R, MODE>(I, fg, si, sj + dj, in_g),
                                  ex_load<R, MODE>(I, fg, si + di, sj, in_g));
+    // MF_HOLDIF: interfacial cells keep their ghost state (extrapolate_vector_mgfm,
+    // extrapolate_vector.cpp:215)
+    if ((MODE & MF_HOLDIF) && interior(I, si, sj) && gmem(I.imap)[cs] >= 0) Wg = ex_load<R, MODE>(I, fg, si, sj, in_g);
     if (!(Wg.v[0] >= R(0))) flag(I, ST_UNPHYS);
     const bool own = (si == i && sj == j);
     if ((MODE & MF_OGFM) && own) {
@@ -1628,7 +1631,10 @@ __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ i
 // velocity field is the real fluid's (the 6-iteration extrapolation at :202
 // only changes ghost values and its result is not written back).
 // work: full-array 64x4 tiles
-template <typename R>
+// PRIMS: the new states go to the primitives P[.][qout] (for the extrapolation
+// that follows) and the velocity field is left to that extrapolation's last
+// iteration
+template <typename R, bool PRIMS>
 __global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ insts, const int4* __restrict__ work,
                                                     int qin, int qout) {
     const int4 w = work[blockIdx.x];
@@ -1641,9 +1647,11 @@ __global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ ins
     V4<R> W1 = cons2prims<R>((R)I.gam[0], (R)I.pinf[0], ld4<R>(I.Q[0][qin], I, c));
     V4<R> W2 = cons2prims<R>((R)I.gam[1], (R)I.pinf[1], ld4<R>(I.Q[1][qin], I, c));
     const int fr = ld1<R>(I.phi, c) <= R(0) ? 0 : 1;
-    const auto vel = gmem((R*)I.vel);
-    vel[c] = fr == 0 ? W1.v[1] : W2.v[1];
-    vel[c + I.plane] = fr == 0 ? W1.v[2] : W2.v[2];
+    if (!PRIMS) {
+        const auto vel = gmem((R*)I.vel);
+        vel[c] = fr == 0 ? W1.v[1] : W2.v[1];
+        vel[c + I.plane] = fr == 0 ? W1.v[2] : W2.v[2];
+    }
     if (interior(I, i, j)) {
         const int k = I.imap[c];
         if (k >= 0) {
@@ -1656,6 +1664,11 @@ __global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ ins
                 W1.v[0] = (R)s[1]; W1.v[1] = (R)s[2]; W1.v[2] = (R)s[3]; W1.v[3] = (R)s[4];
             }
         }
+    }
+    if (PRIMS) {
+        st4<R>(I.P[0][qout], I, c, W1);
+        st4<R>(I.P[1][qout], I, c, W2);
+        return;
     }
     st4<R>(I.Q[0][qout], I, c, prims2cons<R>((R)I.gam[0], (R)I.pinf[0], W1));
     st4<R>(I.Q[1][qout], I, c, prims2cons<R>((R)I.gam[1], (R)I.pinf[1], W2));
@@ -2259,6 +2272,8 @@ void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout) {
         EXC(MF_INJECT | MF_LAST)
         EXC(MF_FIRST | MF_INJECT | MF_LAST)
         EXC(MF_RAW)
+        EXC(MF_HOLDIF)
+        EXC(MF_HOLDIF | MF_LAST)
         default: break;
     }
 #undef EXC
@@ -2308,8 +2323,11 @@ void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted) {
 }
 
 template <typename R>
-void launch_mgfm_final(const LaunchCtx& L, int qin, int qout) {
-    hipLaunchKernelGGL((k_mgfm_final<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, qin, qout);
+void launch_mgfm_final(const LaunchCtx& L, int qin, int qout, bool prims_out) {
+    if (prims_out)
+        hipLaunchKernelGGL((k_mgfm_final<R, true>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, qin, qout);
+    else
+        hipLaunchKernelGGL((k_mgfm_final<R, false>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, qin, qout);
 }
 
 // K reinitialisation iterations from A into phi: blocked launches of up to
@@ -2398,7 +2416,7 @@ void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, doub
     template void launch_extrap<R>(const LaunchCtx&, int, int, int);          \
     template bool launch_extrap_tb<R>(const LaunchCtx&, int, int, int, int, int); \
     template void launch_iface<R>(const LaunchCtx&, int, int, int);           \
-    template void launch_mgfm_final<R>(const LaunchCtx&, int, int);           \
+    template void launch_mgfm_final<R>(const LaunchCtx&, int, int, bool);     \
     template void launch_ls<R>(const LaunchCtx&, const Houc&, int, int);      \
     template void launch_reinit_now<R>(const LaunchCtx&, int);                \
     template void launch_bc<R>(const LaunchCtx&, int, int, int);               \

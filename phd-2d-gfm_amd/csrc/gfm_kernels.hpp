@@ -33,7 +33,7 @@ template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, in
 template <typename R> void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout);
 template <typename R> bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout);
 template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted);
-template <typename R> void launch_mgfm_final(const LaunchCtx& L, int qin, int qout);
+template <typename R> void launch_mgfm_final(const LaunchCtx& L, int qin, int qout, bool prims_out = false);
 template <typename R> void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters);
 template <typename R> void launch_reinit_now(const LaunchCtx& L, int iters);
 template <typename R> void launch_bc(const LaunchCtx& L, int q, int nfluid, int advance);

@@ -7,7 +7,8 @@ use CFL 0.4 and LS9 without periodic reinitialisation; the thesis records neithe
 - MGFM reproduces the reference code, which discards the 6-iteration extrapolation of
   GFM_modified.cpp:202; the GPU matches the oracle. The thesis's MGFM errors are reproduced only
   when that extrapolation is applied to the new ghost states (the oracle built with
-  OR_MGFM_EXTRAPOLATE_NEW_STATES), so the thesis runs evidently used a version that applied it.
+  OR_MGFM_EXTRAPOLATE_NEW_STATES, and the library with GFM_MGFM_EXTRAPOLATE=1), so the thesis
+  runs evidently used a version that applied it.
 """
 import os
 
@@ -72,3 +73,14 @@ def test_mgfm_reference_code_and_thesis(ref, oracle):
     assert gpu > 2.0 * CT.THESIS["MGFM"][0]          # which is not what the thesis plots
     applied = l1_of(oracle_mgfm(oracle, VARIANT, n), ref)
     assert abs(applied / CT.THESIS["MGFM"][0] - 1.0) <= 0.02
+    # the library's switch for that form (GFM_MGFM_EXTRAPOLATE=1) on the GPU
+    os.environ["GFM_MGFM_EXTRAPOLATE"] = "1"
+    try:
+        for k, n in enumerate((50, 100, 200)):
+            sw, _, status = CT.gfm_error("MGFM", n, CFL, ref[0], ref[1], 9, 0, 10)
+            assert status == 0
+            assert abs(sw / CT.THESIS["MGFM"][k] - 1.0) <= 0.05, (n, sw, CT.THESIS["MGFM"][k])
+            if k == 0:
+                assert abs(sw / applied - 1.0) <= 1e-6
+    finally:
+        del os.environ["GFM_MGFM_EXTRAPOLATE"]

@@ -15,6 +15,8 @@ last-ulp behaviour of pow/acos in the device math library, so:
     (no cell changes side).
 Integer results (interfacial cells, update boxes, step counts) are exact.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -408,6 +410,48 @@ def test_twofluid_steps(oracle, variant):
     pi, pr = P.interior(info, op), P.interior(info, rp)
     assert np.array_equal(pi <= 0, pr <= 0)
     assert np.abs(pi - pr).max() <= 1e-8 * np.abs(pr).max()
+
+
+def test_mgfm_extrapolate_steps(oracle):
+    """GFM_MGFM_EXTRAPOLATE=1 (not the reference: the extrapolation of GFM_modified.cpp:202 applied
+    to the new states, the form the thesis's MGFM results imply) against the oracle built the same
+    way (liboracle_mgfm_new.so) over 60 steps, and different from the reference's MGFM"""
+    print_case("mgfm_extrapolate_steps")
+    nsteps = 60
+    info, eos, g1, g2, phi = bubble_state(oracle, G.GFM_MODIFIED)
+    os.environ["GFM_MGFM_EXTRAPOLATE"] = "1"
+    try:
+        ctx, k = two_ctx(info, eos, G.GFM_MODIFIED)
+    finally:
+        del os.environ["GFM_MGFM_EXTRAPOLATE"]
+    ctx.upload(k, g1, g2, phi)
+    ctx.step_graph(nsteps)
+    ctx.sync()
+    o1, o2, op = ctx.download(k)
+    t, n, _ = ctx.get_time(k)
+    assert n == nsteps and ctx.status(k) == 0
+    variant_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+                               "liboracle_mgfm_new.so")
+    outs = []
+    for path in (variant_lib, None):
+        run = oracle.Run(G.GFM_MODIFIED, 0, 5, 5, 10, 0.6, 1e300, nsteps)
+        s = np.zeros(1, dtype=np.int32)
+        to = np.zeros(1)
+        r1, r2, rp = g1.copy(), g2.copy(), phi.copy()
+        st = oracle.lib(path).or_twofluid_run(ogrid(oracle, info), oeos(oracle, eos), run, oracle._p(r1),
+                                              oracle._p(r2), oracle._p(rp), oracle._p(s), oracle._p(to))
+        assert st == 0
+        outs.append((r1, r2, rp, to[0]))
+    r1, r2, rp, to = outs[0]
+    assert abs(t - to) <= 1e-12 * to
+    for a, b in ((o1, r1), (o2, r2)):
+        linf, l1 = scaled_err(P.interior(info, a), P.interior(info, b))
+        assert linf <= 1e-8 and l1 <= 1e-10, (linf, l1)
+    pi, pr = P.interior(info, op), P.interior(info, rp)
+    assert np.array_equal(pi <= 0, pr <= 0)
+    assert np.abs(pi - pr).max() <= 1e-8 * np.abs(pr).max()
+    linf, _ = scaled_err(P.interior(info, o1), P.interior(info, outs[1][0]))
+    assert linf > 1e-6                                   # the switch changes the result
 
 
 def test_graph_matches_eager():
