@@ -1745,12 +1745,15 @@ __global__ __launch_bounds__(256) void k_ls_stage(const Inst* __restrict__ insts
 // exact on the region shrunk by t r cells, the r ghost layers the stencil
 // reads refreshed from their mirror sources after stages 1 and 2, the result
 // written to A for the owned cells and the ghost cells they source.  Same
-// arithmetic as k_ls_stage, bit for bit.  Mirror BCs only.
+// arithmetic as k_ls_stage, bit for bit.  Mirror BCs in x; in y mirror or
+// periodic (region rows wrapped to their interior rows).
 // work: the RTX x RTY interior tiles of the reinitialisation
 // ---------------------------------------------------------------------------
 constexpr int LTX = 64, LTY = 32;
 
-template <typename R, int ORDER>
+// PY: the instances periodic in y (the others return at once), so that
+// their extra reads cost the mirror-bounded instances no registers
+template <typename R, int ORDER, bool PY>
 __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ insts, const int4* __restrict__ work,
                                                     Houc H) {
     constexpr int N = ORDER + 1, RAD = N / 2, HALO = 3 * RAD;
@@ -1759,16 +1762,25 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
     __shared__ R s0[NC], s1[NC];
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!gmem(I.si)[SI_ACTIVE]) return;
+    if ((I.bc[2] == BC_P && I.bc[3] == BC_P) != PY || !gmem(I.si)[SI_ACTIVE]) return;
     const int g = I.g;
     const int ri = g + w.y * LTY - HALO, rj = g + w.z * LTX - HALO;
     const auto phi = gmem((const R*)I.phi);
     const auto vel = gmem((const R*)I.vel);
     const R dt = (R)gmem(I.sd)[SD_DT];
     const R idxr = R(1.0 / I.dx), idyr = R(1.0 / I.dy);
+    // periodic in y (x then has mirror boundaries): a region row outside the
+    // interior is the interior row it wraps to (src_row), computed like any
+    // interior row, and no y ghost layer is refreshed.  Stage 1 of a row
+    // within RAD of the y boundary reads phi's stored ghost rows, as
+    // k_ls_stage does (they need not be periodic images: after an upload they
+    // hold the initial condition's own values); the stages after it read
+    // stage values, whose ghost rows are images
     // per cell: phi, velocity; meta bits 0-7: distance to the region edge,
     // 8: interior, 9: owned, 10: within RAD of the domain edge (its new
-    // value refreshes mirror ghost cells)
+    // value refreshes mirror ghost cells), 11: periodic in y and within RAD
+    // of the y boundary (stage 1 from global phi)
+    constexpr bool py = PY;
     R p0[SLOTS], us[SLOTS], vs[SLOTS];
     int meta[SLOTS];
 #pragma unroll
@@ -1778,17 +1790,21 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
         if (idx < NC) {
             const int a = idx / RX, b = idx - (idx / RX) * RX;
             const int i = ri + a, j = rj + b;
+            bool fl;
+            const int is = py ? src_row(I, i, fl) : i;
             R val = R(0);
-            if (i >= 0 && i < I.H && j >= 0 && j < I.W) {
-                const long long c = cidx(I, i, j);
+            if (is >= 0 && is < I.H && j >= 0 && j < I.W) {
+                const long long c = cidx(I, is, j);
                 val = phi[c];
-                if (interior(I, i, j)) {
+                if (interior(I, is, j)) {
                     us[k] = vel[c];
                     vs[k] = vel[c + I.plane];
                     const int d = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
-                    const bool own = a >= HALO && a < HALO + LTY && b >= HALO && b < HALO + LTX;
-                    const bool edge = i - g < RAD || g + I.Ny - 1 - i < RAD || j - g < RAD || g + I.Nx - 1 - j < RAD;
-                    meta[k] = d | 256 | (own ? 512 : 0) | (edge ? 1024 : 0);
+                    const bool own = a >= HALO && a < HALO + LTY && b >= HALO && b < HALO + LTX && interior(I, i, j);
+                    const bool edge = (!py && (i - g < RAD || g + I.Ny - 1 - i < RAD)) || j - g < RAD ||
+                                      g + I.Nx - 1 - j < RAD;
+                    const bool yb = py && (is - g < RAD || g + I.Ny - 1 - is < RAD);
+                    meta[k] = d | 256 | (own ? 512 : 0) | (edge ? 1024 : 0) | (yb ? 2048 : 0);
                 }
             }
             p0[k] = val;
@@ -1805,15 +1821,23 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
         for (int k = 0; k < SLOTS; ++k) {
             if (!(meta[k] & 256) || (meta[k] & 255) < st * RAD) continue;
             const int idx = threadIdx.x + k * 1024;
-            const R rhs = houc_rhs<R>(H, N, in + idx, 1, RX, us[k], vs[k], idxr, idyr);
+            R rhs;
+            if (PY && st == 1 && (meta[k] & 2048)) {
+                const int a = idx / RX, b = idx - (idx / RX) * RX;
+                bool fl;
+                const int is = src_row(I, ri + a, fl);
+                rhs = houc_rhs<R>(H, N, (const R*)I.phi + cidx(I, is, rj + b), 1, I.pitch, us[k], vs[k], idxr, idyr);
+            } else {
+                rhs = houc_rhs<R>(H, N, in + idx, 1, RX, us[k], vs[k], idxr, idyr);
+            }
             const R nv = st == 1 ? rk3_combine<R, 1>(p0[k], R(0), dt, rhs) : rk3_combine<R, 2>(p0[k], in[idx], dt, rhs);
             out[idx] = nv;
             if (meta[k] & 1024) {
                 // mirror ghosts within the stencil radius of the boundary
                 const int a = idx / RX, b = idx - (idx / RX) * RX;
                 const int i = ri + a, j = rj + b;
-                if (i - ib < RAD) { const int ga = a - 2 * (i - ib) - 1; if (ga >= 0) out[ga * RX + b] = nv; }
-                if (it - i < RAD) { const int ga = a + 2 * (it - i) + 1; if (ga < RY) out[ga * RX + b] = nv; }
+                if (!py && i - ib < RAD) { const int ga = a - 2 * (i - ib) - 1; if (ga >= 0) out[ga * RX + b] = nv; }
+                if (!py && it - i < RAD) { const int ga = a + 2 * (it - i) + 1; if (ga < RY) out[ga * RX + b] = nv; }
                 if (j - jl < RAD) { const int gb = b - 2 * (j - jl) - 1; if (gb >= 0) out[a * RX + gb] = nv; }
                 if (jr - j < RAD) { const int gb = b + 2 * (jr - j) + 1; if (gb < RX) out[a * RX + gb] = nv; }
             }
@@ -1835,8 +1859,8 @@ __global__ __launch_bounds__(1024) void k_ls_rk3_tb(const Inst* __restrict__ ins
             for (int m = 1; m < 9; ++m) {
                 const int pr = m / 3, pc = m - 3 * (m / 3);
                 int r = i, cc = j;
-                if (pr == 1) { if (i >= 2 * g) continue; r = 2 * g - 1 - i; }
-                if (pr == 2) { if (i < I.Ny) continue; r = 2 * (I.Ny + g) - 1 - i; }
+                if (pr == 1) { if (i >= 2 * g) continue; r = py ? i + I.Ny : 2 * g - 1 - i; }
+                if (pr == 2) { if (i < I.Ny) continue; r = py ? i - I.Ny : 2 * (I.Ny + g) - 1 - i; }
                 if (pc == 1) { if (j >= 2 * g) continue; cc = 2 * g - 1 - j; }
                 if (pc == 2) { if (j < I.Nx) continue; cc = 2 * (I.Nx + g) - 1 - j; }
                 A[cidx(I, r, cc)] = nv;
@@ -1986,7 +2010,8 @@ __device__ __forceinline__ R* ls_buf(const Inst& I, int sel) {
 template <typename R>
 struct RtWaves { static constexpr int value = sizeof(R) == 4 ? 8 : 1; };
 
-template <typename R>
+// PY: the instances periodic in y, as in k_ls_rk3_tb
+template <typename R, bool PY>
 __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(const Inst* __restrict__ insts,
                                                              const int4* __restrict__ work, int nit, int in_sel,
                                                              int out_sel, int gate_every, int force, int copy_idle) {
@@ -1996,7 +2021,12 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
     __shared__ R scs[NC];   // reinit_setup's constant per cell (in LDS: it would not fit the registers)
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
+    if ((I.bc[2] == BC_P && I.bc[3] == BC_P) != PY) return;
     const int g = I.g;
+    // periodic in y, as in k_ls_rk3_tb: wrapped region rows, no y ghost
+    // refresh; the first iteration of a row on the y boundary reads its
+    // outer neighbour from the input's stored ghost row, as k_ls_reinit does
+    constexpr bool py = PY;
     if (!force) {
         if (!gmem(I.si)[SI_ACTIVE]) return;
         if (!reinit_step(I, gate_every)) {
@@ -2013,8 +2043,8 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
                     for (int m = 1; m < 9; ++m) {
                         const int pr = m / 3, pc = m - 3 * (m / 3);
                         int r = i, cc = j;
-                        if (pr == 1) { if (i >= 2 * g) continue; r = 2 * g - 1 - i; }
-                        if (pr == 2) { if (i < I.Ny) continue; r = 2 * (I.Ny + g) - 1 - i; }
+                        if (pr == 1) { if (i >= 2 * g) continue; r = py ? i + I.Ny : 2 * g - 1 - i; }
+                        if (pr == 2) { if (i < I.Ny) continue; r = py ? i - I.Ny : 2 * (I.Ny + g) - 1 - i; }
                         if (pc == 1) { if (j >= 2 * g) continue; cc = 2 * g - 1 - j; }
                         if (pc == 2) { if (j < I.Nx) continue; cc = 2 * (I.Nx + g) - 1 - j; }
                         const long long c = cidx(I, r, cc);
@@ -2029,10 +2059,11 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
     const auto P0 = gmem((const R*)I.phiA);
     const auto in = gmem((const R*)ls_buf<R>(I, in_sel));
     const auto out = gmem(ls_buf<R>(I, out_sel));
-    // per cell: reinit_setup's mode and constant; bit 8: interior with its
-    // stencil inside the region, bit 9: owned, bit 10: next to the domain
-    // edge (refreshes the first ghost layer), bits 11+: distance to the
-    // region edge
+    // per cell: reinit_setup's mode and constant; bits 3 / 4: periodic in y
+    // and on the bottom / top interior row, bit 8: interior with its stencil
+    // inside the region, bit 9: owned, bit 10: next to the domain edge
+    // (refreshes the first ghost layer), bits 11+: distance to the region
+    // edge
     const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
     int meta[SLOTS];
 #pragma unroll
@@ -2042,18 +2073,22 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
         if (idx < NC) {
             const int a = idx / RX, b = idx - (idx / RX) * RX;
             const int i = ri + a, j = rj + b;
+            bool fl;
+            const int is = py ? src_row(I, i, fl) : i;
             R v = R(0);
-            if (i >= 0 && i < I.H && j >= 0 && j < I.W) {
-                v = in[cidx(I, i, j)];
-                if (interior(I, i, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1) {
+            if (is >= 0 && is < I.H && j >= 0 && j < I.W) {
+                v = in[cidx(I, is, j)];
+                if (interior(I, is, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1) {
                     int mode;
                     R cs;
-                    reinit_cell_setup<R>(I, P0, i, j, mode, cs);
+                    reinit_cell_setup<R>(I, P0, is, j, mode, cs);
                     scs[idx] = cs;
-                    const bool own = a >= RT_MAXIT && a < RT_MAXIT + RTY && b >= RT_MAXIT && b < RT_MAXIT + RTX;
-                    const bool edge = i == ib || i == it || j == jl || j == jr;
+                    const bool own = a >= RT_MAXIT && a < RT_MAXIT + RTY && b >= RT_MAXIT && b < RT_MAXIT + RTX &&
+                                     interior(I, i, j);
+                    const bool edge = (!py && (i == ib || i == it)) || j == jl || j == jr;
                     const int dist = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
-                    meta[k] = mode | 256 | (own ? 512 : 0) | (edge ? 1024 : 0) | (dist << 11);
+                    const int yb = py ? ((is == ib ? 8 : 0) | (is == it ? 16 : 0)) : 0;
+                    meta[k] = mode | yb | 256 | (own ? 512 : 0) | (edge ? 1024 : 0) | (dist << 11);
                 }
             }
             s[idx] = v;
@@ -2069,8 +2104,16 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
             const int idx = threadIdx.x + k * RT_THREADS;
             if ((meta[k] & 256) && (meta[k] >> 11) >= t) {
                 act |= 1 << k;
-                nv[k] = reinit_update<R>(meta[k] & 7, scs[idx], s[idx], s[idx - 1], s[idx + 1], s[idx - RX],
-                                         s[idx + RX], rk.idx2, rk.idy2, rk.dtau);
+                R fs = s[idx - RX], fn = s[idx + RX];
+                if (PY && t == 1 && (meta[k] & 24)) {
+                    const int a = idx / RX, b = idx - (idx / RX) * RX;
+                    bool fl;
+                    const int is = src_row(I, ri + a, fl);
+                    if (meta[k] & 8) fs = in[cidx(I, is - 1, rj + b)];
+                    if (meta[k] & 16) fn = in[cidx(I, is + 1, rj + b)];
+                }
+                nv[k] = reinit_update<R>(meta[k] & 7, scs[idx], s[idx], s[idx - 1], s[idx + 1], fs, fn, rk.idx2,
+                                         rk.idy2, rk.dtau);
             }
         }
         if (t == nit) {
@@ -2087,8 +2130,8 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
                     for (int m = 1; m < 9; ++m) {
                         const int pr = m / 3, pc = m - 3 * (m / 3);
                         int r = i, cc = j;
-                        if (pr == 1) { if (i >= 2 * g) continue; r = 2 * g - 1 - i; }
-                        if (pr == 2) { if (i < I.Ny) continue; r = 2 * (I.Ny + g) - 1 - i; }
+                        if (pr == 1) { if (i >= 2 * g) continue; r = py ? i + I.Ny : 2 * g - 1 - i; }
+                        if (pr == 2) { if (i < I.Ny) continue; r = py ? i - I.Ny : 2 * (I.Ny + g) - 1 - i; }
                         if (pc == 1) { if (j >= 2 * g) continue; cc = 2 * g - 1 - j; }
                         if (pc == 2) { if (j < I.Nx) continue; cc = 2 * (I.Nx + g) - 1 - j; }
                         out[cidx(I, r, cc)] = nv[k];
@@ -2106,8 +2149,8 @@ __global__ __launch_bounds__(RT_THREADS, RtWaves<R>::value) void k_ls_reinit_tb(
             if (meta[k] & 1024) {
                 const int a = idx / RX, b = idx - (idx / RX) * RX;
                 const int i = ri + a, j = rj + b;
-                if (i == ib) s[idx - RX] = nv[k];
-                if (i == it) s[idx + RX] = nv[k];
+                if (!py && i == ib) s[idx - RX] = nv[k];
+                if (!py && i == it) s[idx + RX] = nv[k];
                 if (j == jl) s[idx - 1] = nv[k];
                 if (j == jr) s[idx + 1] = nv[k];
             }
@@ -2331,17 +2374,21 @@ void launch_mgfm_final(const LaunchCtx& L, int qin, int qout, bool prims_out) {
 }
 
 // K reinitialisation iterations from A into phi: blocked launches of up to
-// RT_MAXIT iterations each, or one launch per iteration (periodic contexts)
+// RT_MAXIT iterations each, or one launch per iteration (x-periodic contexts)
 template <typename R>
 void launch_reinit_chain(const LaunchCtx& L, int K, int every, int force, int copy_idle) {
-    if (L.tb) {
+    if (L.tb_ls) {
         const int m = (K + RT_MAXIT - 1) / RT_MAXIT;
         int in = 0, done = 0;
         for (int c = 0; c < m; ++c) {
             const int nit = (K - done + (m - c) - 1) / (m - c);   // even split
             const int out = ((m - 1 - c) % 2 == 0) ? 1 : 2;
-            hipLaunchKernelGGL((k_ls_reinit_tb<R>), GRID(L.n_rt), dim3(RT_THREADS), 0, L.stream, L.insts, L.w_rt, nit,
-                               in, out, every, force, c == 0 ? copy_idle : 0);
+            if (L.ls_mirror_y)
+                hipLaunchKernelGGL((k_ls_reinit_tb<R, false>), GRID(L.n_rt), dim3(RT_THREADS), 0, L.stream, L.insts,
+                                   L.w_rt, nit, in, out, every, force, c == 0 ? copy_idle : 0);
+            if (L.ls_periodic_y)
+                hipLaunchKernelGGL((k_ls_reinit_tb<R, true>), GRID(L.n_rt), dim3(RT_THREADS), 0, L.stream, L.insts,
+                                   L.w_rt, nit, in, out, every, force, c == 0 ? copy_idle : 0);
             in = out;
             done += nit;
         }
@@ -2355,9 +2402,15 @@ void launch_reinit_chain(const LaunchCtx& L, int K, int every, int force, int co
 template <typename R>
 void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters) {
     if (iters <= 0) every = 0;   // no reinitialisation iterations: phi = A every step
-    if (L.tb) {
+    if (L.tb_ls) {
         static_assert(LTX == RTX && LTY == RTY, "the fused level-set kernel uses the reinitialisation tiles");
-#define RK3(O) hipLaunchKernelGGL((k_ls_rk3_tb<R, O>), GRID(L.n_rt), dim3(1024), 0, L.stream, L.insts, L.w_rt, H)
+#define RK3(O)                                                                                                         \
+    do {                                                                                                               \
+        if (L.ls_mirror_y)                                                                                             \
+            hipLaunchKernelGGL((k_ls_rk3_tb<R, O, false>), GRID(L.n_rt), dim3(1024), 0, L.stream, L.insts, L.w_rt, H); \
+        if (L.ls_periodic_y)                                                                                           \
+            hipLaunchKernelGGL((k_ls_rk3_tb<R, O, true>), GRID(L.n_rt), dim3(1024), 0, L.stream, L.insts, L.w_rt, H);  \
+    } while (0)
         switch (H.n) {
             case 2: RK3(1); break;
             case 4: RK3(3); break;

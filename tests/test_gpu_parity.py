@@ -384,6 +384,38 @@ def test_compute_dt_and_mass(oracle):
     assert abs(m[0] - m1) <= 1e-13 * m1 and abs(m[1] - m2) <= 1e-13 * m2
 
 
+@pytest.mark.parametrize("variant", [G.GFM_ORIGINAL, G.GFM_MODIFIED])
+def test_twofluid_steps_periodic_y(oracle, variant):
+    """single-mode Richtmyer-Meshkov instance, transmissive in x and periodic
+    in y (the interface crosses the periodic boundary): 40 steps, reinitialised
+    every 5, against the oracle; the first step reads the initial condition's
+    own level-set ghost rows"""
+    print_case("twofluid_steps_periodic_y variant=%d" % variant)
+    nsteps = 40
+    info, eos, (g1, g2, phi) = P.rmi(128, 32, numGC=6)
+    ctx, k = two_ctx(info, eos, variant)
+    ctx.upload(k, g1, g2, phi)
+    ctx.step_graph(nsteps)
+    ctx.sync()
+    o1, o2, op = ctx.download(k)
+    t, n, _ = ctx.get_time(k)
+    assert n == nsteps
+    run = oracle.Run(variant, 0, 5, 5, 10, 0.6, 1e300, nsteps)
+    s = np.zeros(1, dtype=np.int32)
+    to = np.zeros(1)
+    r1, r2, rp = g1.copy(), g2.copy(), phi.copy()
+    st = oracle.lib().or_twofluid_run(ogrid(oracle, info), oeos(oracle, eos), run, oracle._p(r1), oracle._p(r2),
+                                      oracle._p(rp), oracle._p(s), oracle._p(to))
+    assert st == 0 and ctx.status(k) == 0
+    assert abs(t - to[0]) <= 1e-12 * to[0]
+    for a, b in ((o1, r1), (o2, r2)):
+        linf, l1 = scaled_err(P.interior(info, a), P.interior(info, b))
+        assert linf <= 1e-8 and l1 <= 1e-10, (linf, l1)
+    pi, pr = P.interior(info, op), P.interior(info, rp)
+    assert np.array_equal(pi <= 0, pr <= 0)
+    assert np.abs(pi - pr).max() <= 1e-8 * np.abs(pr).max()
+
+
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_twofluid_steps(oracle, variant):
     print_case("twofluid_steps variant=%d" % variant)

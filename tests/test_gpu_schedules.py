@@ -11,6 +11,8 @@
    (solve whatever is ready, repeat), for the variants with dependencies.
 3. Steps of a batch split into instance groups on separate streams (with
    multi-step graphs and staggered group starts) against one stream.
+4. The blocked level-set kernels on periodic-in-y instances (wrapped region
+   rows) against the per-iteration ones.
 """
 import os
 
@@ -143,6 +145,32 @@ def test_reinit_blocked_matches_per_iteration(iters, precision):
         c.sync()
     for k in range(len(cases)):
         assert_same(ca.download(k)[2], cb.download(k)[2], "instance %d phi" % k)
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
+@pytest.mark.parametrize("variant", [G.GFM_RIEMANN, G.GFM_MODIFIED])
+def test_periodic_y_level_set_blocked_matches_per_iteration(variant, precision):
+    """periodic-in-y instances (the blocked level-set kernels wrap their
+    region rows; the ghost extrapolation stays per iteration) beside a
+    mirror-bounded one: 12 steps with reinitialisation every 5, then an
+    explicit 23-iteration reinitialisation.  Ny = 12 makes one region wrap
+    several times."""
+    print("CASE periodic-y level set variant=%d precision=%d" % (variant, precision), flush=True)
+    P_ = G.BC_PERIODIC
+    cases = [perturbed_bubble(150, 37, (T, T, P_, P_), 31), perturbed_bubble(96, 40, BCS[0], 32, numGC=8),
+             perturbed_bubble(80, 12, (T, R, P_, P_), 33)]
+    ca = make_ctx(cases, variant, precision, True)
+    cb = make_ctx(cases, variant, precision, False)
+    for c in (ca, cb):
+        c.step_graph(12)
+        c.sync()
+    compare(ca, cb, len(cases))
+    for c in (ca, cb):
+        c.ls_reinit(23)
+        c.sync()
+    for k in range(len(cases)):
+        assert_same(ca.download(k)[2], cb.download(k)[2], "instance %d phi after reinit" % k)
+        assert ca.get_time(k) == cb.get_time(k)
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
