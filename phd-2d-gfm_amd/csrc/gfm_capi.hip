@@ -80,8 +80,7 @@ struct gfm_ctx {
     gfm::Houc houc{};
     int cur = 0;                       // current conserved buffer (one-fluid step path)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    bool tb = false;                   // temporally blocked extrapolation (no periodic instance)
-    bool tb_ls = false;                // temporally blocked level set (no x-periodic instance)
+    bool tb = false;                   // temporally blocked kernels (no x-periodic instance)
     bool mgfm_extrapolate = false;     // GFM_MGFM_EXTRAPOLATE=1 (GFM_modified only; not the reference)
     // two-fluid step path: the instances in groups, each group's step on its
     // own stream (group 0 on `stream`), forked from and joined back to
@@ -439,10 +438,9 @@ gfm::LaunchCtx launch_ctx(const gfm_ctx* c, const TableOffsets& t, hipStream_t s
     L.w_rt = w + t.o[7]; L.n_rt = t.n[7];
     L.w_seg = w + t.o[8]; L.n_seg = t.n[8];
     L.tb = c->tb ? 1 : 0;
-    L.tb_ls = c->tb_ls ? 1 : 0;
     for (int k : members) {
-        if (c->inst[k].info.BC_B == GFM_BC_PERIODIC) L.ls_periodic_y = 1;
-        else L.ls_mirror_y = 1;
+        if (c->inst[k].info.BC_B == GFM_BC_PERIODIC) L.periodic_y = 1;
+        else L.mirror_y = 1;
     }
     L.iface_plain = iface_plain;
     return L;
@@ -550,7 +548,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
     if (c->inst.empty()) return fail_arg("no instances");
     const size_t es = c->es();
     std::vector<Inst> descs;
-    bool periodic = false, periodic_x = false;
+    bool periodic_x = false;
     for (size_t k = 0; k < c->inst.size(); ++k) {
         InstHost& h = c->inst[k];
         const gfm_sim_info& s = h.info;
@@ -619,15 +617,13 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.si = (int*)(base + osi);
         h.d = d;
         descs.push_back(d);
-        if (s.BC_L == GFM_BC_PERIODIC || s.BC_B == GFM_BC_PERIODIC) periodic = true;
         if (s.BC_L == GFM_BC_PERIODIC || s.BC_R == GFM_BC_PERIODIC) periodic_x = true;
     }
     const char* mx = std::getenv("GFM_MGFM_EXTRAPOLATE");
     c->mgfm_extrapolate = c->m.gfm == GFM_MODIFIED && mx && mx[0] == '1';
     const char* no_tb = std::getenv("GFM_NO_TEMPORAL_BLOCKING");
-    c->tb = c->twofluid() && !periodic && !(no_tb && no_tb[0] == '1');
-    // the blocked level-set kernels wrap periodic rows themselves
-    c->tb_ls = c->twofluid() && !periodic_x && !(no_tb && no_tb[0] == '1');
+    // the blocked kernels wrap the rows of y-periodic instances themselves
+    c->tb = c->twofluid() && !periodic_x && !(no_tb && no_tb[0] == '1');
     // GFM_NO_TEMPORAL_BLOCKING=1 / GFM_IFACE_PLAIN_ROUNDS=1 select the simpler
     // schedules that the tests compare the default ones against
     const char* plain = std::getenv("GFM_IFACE_PLAIN_ROUNDS");

@@ -899,7 +899,13 @@ __global__ __launch_bounds__(256) void k_extrap(const Inst* __restrict__ insts, 
 // layer, whose source is the adjacent interior cell, so that cell refreshes
 // it after each iteration; the first iteration reads the arrays as they are,
 // ghost frame included.  The last iteration writes every owned cell and each
-// ghost cell whose source it is.  Periodic contexts use k_extrap.
+// ghost cell whose source it is.  PY: the instances periodic in y (the
+// others return at once).  Their region rows wrap to the interior rows they
+// stand for and are computed like them, so no y ghost layer is refreshed;
+// only the first iteration of a row on the y boundary whose upwind y
+// neighbour lies across it reads that neighbour from the input's stored
+// ghost row (kept in an LDS strip), as k_extrap does.  x-periodic contexts
+// use k_extrap.
 // MIN: read-side mode bits of the first iteration (MF_FIRST, MF_INJECT);
 // MOUT: write-side bits of the last (MF_LAST, MF_OGFM; none = primitives to
 // P[.][bout]).
@@ -915,17 +921,20 @@ __device__ __forceinline__ V4<R> tb_update(const V4A<R>* s, const V4<R>& c0, int
     return extrap_combine<R>(wx, wy, c0, tb_get<R>(s, ox), tb_get<R>(s, oy));
 }
 
-template <typename R, int NIT, int MIN, int MOUT>
+template <typename R, int NIT, int MIN, int MOUT, bool PY>
 __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict__ insts,
                                                           const int4* __restrict__ work, int bin, int bout) {
+    static_assert(NIT >= 2, "the first iteration is not the last");
     constexpr int RX = TBX + 2 * NIT, RY = TBY + 2 * NIT, NC = RX * RY;
     constexpr int SLOTS = (NC + TB_THREADS - 1) / TB_THREADS;
     constexpr int LM = MIN & (MF_FIRST | MF_INJECT);
-    __shared__ V4A<R> s[2 * NC];   // s[f NC + cell]: the primitives of fluid f
+    // s[f NC + cell]: the primitives of fluid f; PY: then the stored ghost
+    // rows below and above the interior, s[2 NC + (2 side + f) RX + column]
+    __shared__ V4A<R> s[2 * NC + (PY ? 4 * RX : 0)];
     __shared__ int sbb[8];
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
-    if (!gmem(I.si)[SI_ACTIVE]) return;
+    if ((I.bc[2] == BC_P && I.bc[3] == BC_P) != PY || !gmem(I.si)[SI_ACTIVE]) return;
     const int g = I.g;
     const int ri = g + w.y * TBY - NIT, rj = g + w.z * TBX - NIT;   // region origin
     if (MOUT & MF_OGFM) {
@@ -954,14 +963,48 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
     R pc[SLOTS], pe[SLOTS], pw[SLOTS], pn[SLOTS], ps[SLOTS];
     int im[SLOTS];
     const auto phig = gmem((const R*)I.phi);
+    if constexpr (PY) {
+        // the stored ghost rows g-1 and g+Ny as the first iteration reads them
+        for (int q = threadIdx.x; q < 2 * RX; q += TB_THREADS) {
+            const int side = q / RX, b = q - side * RX, j = rj + b;
+            const int r = side ? g + I.Ny : g - 1;
+            V4<R> W1 = {{R(0), R(0), R(0), R(0)}}, W2 = W1;
+            if (j >= 0 && j < I.W) {
+                const long long c = cidx(I, r, j);
+                W1 = ld4<R>(in0, I, c);
+                W2 = ld4<R>(in1, I, c);
+                if (LM & MF_FIRST) {
+                    if constexpr (sizeof(R) == 4) {
+                        W1 = cons2prims_f(fluid_f(I.rsf[0]), W1);
+                        W2 = cons2prims_f(fluid_f(I.rsf[1]), W2);
+                    } else {
+                        W1 = cons2prims<R>((R)I.gam[0], (R)I.pinf[0], W1);
+                        W2 = cons2prims<R>((R)I.gam[1], (R)I.pinf[1], W2);
+                    }
+                }
+                const int m = (LM & MF_INJECT) ? gmem(I.imap)[c] : -1;
+                if (m >= 0) {
+                    const auto sd = gmem(I.side) + 8 * (long long)m;
+                    V4<R> Ws;
+#pragma unroll
+                    for (int qq = 0; qq < 4; ++qq) Ws.v[qq] = (R)sd[qq];
+                    if (phig[c] <= R(0)) W1 = Ws; else W2 = Ws;
+                }
+            }
+            tb_put<R>(s, 2 * NC + (2 * side) * RX + b, W1);
+            tb_put<R>(s, 2 * NC + (2 * side + 1) * RX + b, W2);
+        }
+    }
 #pragma unroll
     for (int k = 0; k < SLOTS; ++k) {
         const int idx = threadIdx.x + k * TB_THREADS;
         const int a = idx / RX, b = idx - (idx / RX) * RX;
         const int i = ri + a, j = rj + b;
-        const bool valid = idx < NC && i >= 0 && i < I.H && j >= 0 && j < I.W;
-        const bool inner = valid && interior(I, i, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1;
-        const long long c = valid ? cidx(I, i, j) : 0;
+        bool fl;
+        const int is = PY ? src_row(I, i, fl) : i;   // the row the region row stands for
+        const bool valid = idx < NC && is >= 0 && is < I.H && j >= 0 && j < I.W;
+        const bool inner = valid && interior(I, is, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1;
+        const long long c = valid ? cidx(I, is, j) : 0;
         L1[k] = L2[k] = {{R(0), R(0), R(0), R(0)}};
         pc[k] = pe[k] = pw[k] = pn[k] = ps[k] = R(0);
         im[k] = -1;
@@ -985,8 +1028,10 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
         if (idx < NC) {
             const int a = idx / RX, b = idx - (idx / RX) * RX;
             const int i = ri + a, j = rj + b;
+            bool fl;
+            const int is = PY ? src_row(I, i, fl) : i;
             V4<R> W1 = {{R(0), R(0), R(0), R(0)}}, W2 = W1;
-            if (i >= 0 && i < I.H && j >= 0 && j < I.W) {
+            if (is >= 0 && is < I.H && j >= 0 && j < I.W) {
                 // ex_load<R, LM> of both fluids on the loaded values
                 W1 = L1[k]; W2 = L2[k];
                 if (LM & MF_FIRST) {
@@ -1005,17 +1050,21 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
                     for (int q = 0; q < 4; ++q) Ws.v[q] = (R)sd[q];
                     if (pc[k] <= R(0)) W1 = Ws; else W2 = Ws;
                 }
-                if (interior(I, i, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1) {
+                if (interior(I, is, j) && a > 0 && a < RY - 1 && b > 0 && b < RX - 1) {
                     R nx, ny;
                     unit_normal_v<R>(I, K, pe[k], pw[k], pn[k], ps[k], nx, ny);
                     extrap_weights<R>(K, nx, ny, wxs[k], wys[k]);
-                    const bool own = a >= NIT && a < NIT + TBY && b >= NIT && b < NIT + TBX;
-                    const bool edge = i == ib || i == it || j == jl || j == jr;
+                    const bool own = a >= NIT && a < NIT + TBY && b >= NIT && b < NIT + TBX && interior(I, i, j);
+                    const bool edge = (!PY && (i == ib || i == it)) || j == jl || j == jr;
                     const int dist = min(min(a, RY - 1 - a), min(b, RX - 1 - b));
                     const int fr = pc[k] <= R(0) ? 0 : 1;
                     meta[k] = 1 | (fr ? 2 : 0) | (own ? 4 : 0) | (edge ? 8 : 0) | (dist << 4);
                     int dj, di;
                     extrap_dirs<R>(1 - fr, nx, ny, dj, di);
+                    // bit 12: the first iteration's upwind y neighbour is the
+                    // stored ghost row, below (bit 13 clear) or above (set)
+                    if (PY && ((is == ib && di < 0) || (is == it && di > 0)))
+                        meta[k] |= 4096 | (di > 0 ? 8192 : 0);
                     oc[k] = (1 - fr) * NC + idx;
                     ox[k] = oc[k] + dj;
                     oy[k] = oc[k] + di * RX;
@@ -1033,9 +1082,15 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
         int act = 0;
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
-            if ((meta[k] & 1) && (meta[k] >> 4) >= t) {
+            if ((meta[k] & 1) && ((meta[k] >> 4) & 255) >= t) {
                 act |= 1 << k;
-                cur[k] = tb_update<R>(s, cur[k], ox[k], oy[k], wxs[k], wys[k]);
+                int oyk = oy[k];
+                if (PY && t == 1 && (meta[k] & 4096)) {
+                    const int idx = threadIdx.x + k * TB_THREADS;
+                    const int b = idx - (idx / RX) * RX;
+                    oyk = 2 * NC + (2 * ((meta[k] & 8192) ? 1 : 0) + (oc[k] >= NC ? 1 : 0)) * RX + b;
+                }
+                cur[k] = tb_update<R>(s, cur[k], ox[k], oyk, wxs[k], wys[k]);
                 if ((meta[k] & 4) && !(cur[k].v[0] >= R(0))) bad = true;
             }
         }
@@ -1053,8 +1108,8 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
                 V4<R> Wv1 = W1, Wv2 = W2, Wu1 = W1, Wu2 = W2;
                 Wv1.v[2] = -Wv1.v[2]; Wv2.v[2] = -Wv2.v[2];
                 Wu1.v[1] = -Wu1.v[1]; Wu2.v[1] = -Wu2.v[1];
-                if (i == ib) { tb_put<R>(s, idx - RX, rfb ? Wv1 : W1); tb_put<R>(s, NC + idx - RX, rfb ? Wv2 : W2); }
-                if (i == it) { tb_put<R>(s, idx + RX, rft ? Wv1 : W1); tb_put<R>(s, NC + idx + RX, rft ? Wv2 : W2); }
+                if (!PY && i == ib) { tb_put<R>(s, idx - RX, rfb ? Wv1 : W1); tb_put<R>(s, NC + idx - RX, rfb ? Wv2 : W2); }
+                if (!PY && i == it) { tb_put<R>(s, idx + RX, rft ? Wv1 : W1); tb_put<R>(s, NC + idx + RX, rft ? Wv2 : W2); }
                 if (j == jl) { tb_put<R>(s, idx - 1, rfx ? Wu1 : W1); tb_put<R>(s, NC + idx - 1, rfx ? Wu2 : W2); }
                 if (j == jr) { tb_put<R>(s, idx + 1, rfx ? Wu1 : W1); tb_put<R>(s, NC + idx + 1, rfx ? Wu2 : W2); }
             }
@@ -1096,8 +1151,8 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
                 const int pr = m / 3, pc = m - 3 * (m / 3);
                 int r = i, cc = j;
                 bool fv = false, fu = false;
-                if (pr == 1) { if (i >= 2 * g) continue; r = 2 * g - 1 - i; fv = rfb; }
-                if (pr == 2) { if (i < I.Ny) continue; r = 2 * (I.Ny + g) - 1 - i; fv = rft; }
+                if (pr == 1) { if (i >= 2 * g) continue; r = PY ? i + I.Ny : 2 * g - 1 - i; fv = !PY && rfb; }
+                if (pr == 2) { if (i < I.Ny) continue; r = PY ? i - I.Ny : 2 * (I.Ny + g) - 1 - i; fv = !PY && rft; }
                 if (pc == 1) { if (j >= 2 * g) continue; cc = 2 * g - 1 - j; fu = rfx; }
                 if (pc == 2) { if (j < I.Nx) continue; cc = 2 * (I.Nx + g) - 1 - j; fu = rfx; }
                 extrap_put<R, MOUT & MF_LAST>(I, r, cc, bout, oa, fu, fv);
@@ -2326,11 +2381,15 @@ void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout) {
 // has no temporally blocked instance (the caller then launches per iteration)
 template <typename R>
 bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout) {
-#define EXT(N, MI, MO)                                                                                     \
-    if (nit == N && min == (MI) && mout == (MO)) {                                                         \
-        hipLaunchKernelGGL((k_extrap_tb<R, N, MI, MO>), GRID(L.n_tb), dim3(TB_THREADS), 0, L.stream, L.insts, \
-                           L.w_tb, bin, bout);                                                             \
-        return true;                                                                                       \
+#define EXT(N, MI, MO)                                                                                              \
+    if (nit == N && min == (MI) && mout == (MO)) {                                                                  \
+        if (L.mirror_y)                                                                                             \
+            hipLaunchKernelGGL((k_extrap_tb<R, N, MI, MO, false>), GRID(L.n_tb), dim3(TB_THREADS), 0, L.stream,     \
+                               L.insts, L.w_tb, bin, bout);                                                         \
+        if (L.periodic_y)                                                                                           \
+            hipLaunchKernelGGL((k_extrap_tb<R, N, MI, MO, true>), GRID(L.n_tb), dim3(TB_THREADS), 0, L.stream,      \
+                               L.insts, L.w_tb, bin, bout);                                                         \
+        return true;                                                                                                \
     }
     EXT(6, MF_FIRST, MF_LAST | MF_OGFM)           // GFM_original
     EXT(6, MF_FIRST | MF_INJECT, MF_LAST)         // GFM_real
@@ -2377,16 +2436,16 @@ void launch_mgfm_final(const LaunchCtx& L, int qin, int qout, bool prims_out) {
 // RT_MAXIT iterations each, or one launch per iteration (x-periodic contexts)
 template <typename R>
 void launch_reinit_chain(const LaunchCtx& L, int K, int every, int force, int copy_idle) {
-    if (L.tb_ls) {
+    if (L.tb) {
         const int m = (K + RT_MAXIT - 1) / RT_MAXIT;
         int in = 0, done = 0;
         for (int c = 0; c < m; ++c) {
             const int nit = (K - done + (m - c) - 1) / (m - c);   // even split
             const int out = ((m - 1 - c) % 2 == 0) ? 1 : 2;
-            if (L.ls_mirror_y)
+            if (L.mirror_y)
                 hipLaunchKernelGGL((k_ls_reinit_tb<R, false>), GRID(L.n_rt), dim3(RT_THREADS), 0, L.stream, L.insts,
                                    L.w_rt, nit, in, out, every, force, c == 0 ? copy_idle : 0);
-            if (L.ls_periodic_y)
+            if (L.periodic_y)
                 hipLaunchKernelGGL((k_ls_reinit_tb<R, true>), GRID(L.n_rt), dim3(RT_THREADS), 0, L.stream, L.insts,
                                    L.w_rt, nit, in, out, every, force, c == 0 ? copy_idle : 0);
             in = out;
@@ -2402,13 +2461,13 @@ void launch_reinit_chain(const LaunchCtx& L, int K, int every, int force, int co
 template <typename R>
 void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters) {
     if (iters <= 0) every = 0;   // no reinitialisation iterations: phi = A every step
-    if (L.tb_ls) {
+    if (L.tb) {
         static_assert(LTX == RTX && LTY == RTY, "the fused level-set kernel uses the reinitialisation tiles");
 #define RK3(O)                                                                                                         \
     do {                                                                                                               \
-        if (L.ls_mirror_y)                                                                                             \
+        if (L.mirror_y)                                                                                             \
             hipLaunchKernelGGL((k_ls_rk3_tb<R, O, false>), GRID(L.n_rt), dim3(1024), 0, L.stream, L.insts, L.w_rt, H); \
-        if (L.ls_periodic_y)                                                                                           \
+        if (L.periodic_y)                                                                                           \
             hipLaunchKernelGGL((k_ls_rk3_tb<R, O, true>), GRID(L.n_rt), dim3(1024), 0, L.stream, L.insts, L.w_rt, H);  \
     } while (0)
         switch (H.n) {

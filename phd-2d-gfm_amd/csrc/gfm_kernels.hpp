@@ -19,9 +19,8 @@ struct LaunchCtx {
     const int4* w_tb;    int n_tb;      // 64x16 interior tiles (temporally blocked extrapolation)
     const int4* w_rt;    int n_rt;      // 64x32 interior tiles (temporally blocked reinitialisation)
     const int4* w_seg;   int n_seg;     // 256-cell interior row segments (dt, interfacial list)
-    int tb;                             // temporally blocked extrapolation (no periodic instance)
-    int tb_ls;                          // temporally blocked level set (no x-periodic instance)
-    int ls_mirror_y, ls_periodic_y;     // some instance of the group has mirror / periodic y BCs
+    int tb;                             // temporally blocked kernels (no x-periodic instance)
+    int mirror_y, periodic_y;           // some instance of the group has mirror / periodic y BCs
     int iface_plain;                    // interfacial solve by plain rounds instead of levels
 };
 

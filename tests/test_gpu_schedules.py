@@ -11,8 +11,8 @@
    (solve whatever is ready, repeat), for the variants with dependencies.
 3. Steps of a batch split into instance groups on separate streams (with
    multi-step graphs and staggered group starts) against one stream.
-4. The blocked level-set kernels on periodic-in-y instances (wrapped region
-   rows) against the per-iteration ones.
+4. The blocked extrapolation and level-set kernels on periodic-in-y
+   instances (wrapped region rows) against the per-iteration ones.
 """
 import os
 
@@ -148,13 +148,31 @@ def test_reinit_blocked_matches_per_iteration(iters, precision):
 
 
 @pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
-@pytest.mark.parametrize("variant", [G.GFM_RIEMANN, G.GFM_MODIFIED])
-def test_periodic_y_level_set_blocked_matches_per_iteration(variant, precision):
-    """periodic-in-y instances (the blocked level-set kernels wrap their
-    region rows; the ghost extrapolation stays per iteration) beside a
-    mirror-bounded one: 12 steps with reinitialisation every 5, then an
-    explicit 23-iteration reinitialisation.  Ny = 12 makes one region wrap
-    several times."""
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_periodic_y_ghost_fill_blocked_matches_per_iteration(variant, precision):
+    """blocked ghost extrapolation of periodic-in-y instances (wrapped region
+    rows) straight after the upload: the perturbed ghost rows are not periodic
+    images, so the first iteration must read them as stored"""
+    print("CASE periodic-y ghost_fill variant=%d precision=%d" % (variant, precision), flush=True)
+    P_ = G.BC_PERIODIC
+    cases = [perturbed_bubble(150, 37, (T, T, P_, P_), 1), perturbed_bubble(70, 12, (R, T, P_, P_), 2, numGC=7),
+             perturbed_bubble(96, 40, BCS[1], 3)]
+    ca = make_ctx(cases, variant, precision, True)
+    cb = make_ctx(cases, variant, precision, False)
+    ca.set_ghost_states()
+    cb.set_ghost_states()
+    ca.sync()
+    cb.sync()
+    compare(ca, cb, len(cases))
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_periodic_y_steps_blocked_matches_per_iteration(variant, precision):
+    """periodic-in-y instances (the blocked extrapolation and level-set
+    kernels wrap their region rows) beside a mirror-bounded one: 12 steps
+    with reinitialisation every 5, then an explicit 23-iteration
+    reinitialisation.  Ny = 12 makes one region wrap several times."""
     print("CASE periodic-y level set variant=%d precision=%d" % (variant, precision), flush=True)
     P_ = G.BC_PERIODIC
     cases = [perturbed_bubble(150, 37, (T, T, P_, P_), 31), perturbed_bubble(96, 40, BCS[0], 32, numGC=8),
