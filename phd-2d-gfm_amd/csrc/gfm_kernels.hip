@@ -2049,7 +2049,8 @@ __global__ __launch_bounds__(256) void k_ls_reinit(const Inst* __restrict__ inst
 // halo of the iterate in LDS; iteration t is exact on the region shrunk by t
 // cells; the first ghost layer (the only one the stencil reads) is refreshed
 // from its interior source after each iteration; the last iteration writes
-// the owned cells and the ghost cells they source.  Mirror BCs only.
+// the owned cells and the ghost cells they source.  Mirror BCs in x; in y
+// mirror or periodic (PY: wrapped region rows, as k_ls_rk3_tb).
 // in_sel / out_sel: 0 = A, 1 = phi, 2 = B.
 // work: {inst, tile_y, tile_x, 0}, RTX x RTY interior tiles
 // ---------------------------------------------------------------------------
