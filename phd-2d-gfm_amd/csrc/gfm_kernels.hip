@@ -421,6 +421,9 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
     const long long cown = own_in ? cidx(I, ci, cj) : 0;
     V4<R> Uown = {{R(0), R(0), R(0), R(0)}};
     if (own_in) Uown = ld4<R>(Qi, I, cown);
+    // the first-order form has no barrier in A or B: the counter and dt/dx,
+    // dt/dy set by thread 0 must be visible before any wave reaches C1
+    if (!MUSCL) __syncthreads();
     // A: the loads of both passes first, then the conversions
     if (MUSCL) {
         constexpr int NA = (FWY * FWX + FS_THREADS - 1) / FS_THREADS;
