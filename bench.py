@@ -2,52 +2,68 @@
 
 A "step" is one full two-fluid time step of every instance: CFL dt, ghost-
 fluid fill, MUSCL-Hancock exact-RS update of both fluids, HOUC-5 level-set
-advection with reinitialisation every 5 steps, boundary conditions.  Inputs
-are resident in HBM before the timed region; a single instance runs one
-hipGraph per step, a batch runs its instance groups (two streams in mixed
-precision, four in FP64) from one graph per 100-step block (--eager: direct
-launches).
+advection with reinitialisation every 5 steps, boundary conditions.
+
+Timed window.  Each precision's job starts from the initial condition and
+is advanced untimed to mid-job (--advance, default half the job length), so
+that the timed steps do the per-step work of a developed flow rather than of
+the quiescent start.  The state there is checkpointed (downloaded with its
+time and step count); the schedule autotuner times its candidates from that
+checkpoint; then --warmup untimed steps, and the --steps timed steps.
+Inputs are resident in HBM before the timed region; a single instance runs
+one hipGraph per step, a batch runs its instance groups (two streams in
+mixed precision, four in FP64) from one graph per 100-step block (--eager:
+direct launches).
 
 Workloads (--workload):
   batch   (default) 64 seeded shocked bubbles, grids s*(256x72), s~U[1,8]
           (2.55e7 interior cells), per-instance Mach/radius/density/gamma,
           GFM_riemann, mixed precision (FP32 state and sweeps, FP64 Newton,
-          FP64 interfacial states, FP64 sums), per-instance dt, 1500 steps;
+          FP64 interfacial states, FP64 sums), per-instance dt; 1500-step job;
   bubble  one 1024x288 shocked bubble, plain FP64, GFM_riemann, 2000 steps;
   large   one 4096x1152 shocked bubble (bubble gamma and density drawn from
           seed 20261203), mixed precision, GFM_riemann, 2000 steps; with
           --accuracy also GFM_original and GFM_real, each against FP64;
   rmi     256 seeded single-mode Richtmyer-Meshkov instances, grids
           s*(512x128), s~U[1,4], periodic in y, GFM_modified, mixed
-          precision; 300 steps unless --steps says otherwise.
-`value` = interior cells x steps / wall time, summed over instances.
-`roofline` is for the dominant kernel, the fused MUSCL-Hancock flux kernel:
-algorithmic bytes per launch = 2 fluids x cells x 8 words (4 read, 4
+          precision; 300-step job unless --job-steps says otherwise.
+`value` = interior cells x timed steps / wall time, summed over instances.
+
+`roofline` is for the dominant kernel, the fused MUSCL-Hancock flux kernel
+k_fs: algorithmic bytes per launch = 2 fluids x cells x 8 words (4 read, 4
 written) x word size, over its mean duration from HIP events on the
-context's stream during a replay of the same workload (same steps, from the
-same initial state, with events around each phase).  `step_roofline` is
-the whole step against the compulsory state traffic 2 (4 F + L) s bytes
-per cell-update (F = 2 fluids, L = 1 level set, s = word size).
-For mixed-precision workloads the FP64 path runs the same job (`fp64_plain`)
-and the final states are compared (`accuracy`: relative L1 / Linf per
-conserved variable of the composite real-fluid state, interface-contour
-L1); --perturb adds the same comparison between FP64 and FP64 from an
-initial state perturbed by 1e-7 relative, which measures how much the flow
-itself amplifies a last-digit change.
-`cpu_baseline` times the single-threaded FP64 CPU oracle on a bounded
-sample (first steps of the smallest instances).
-Before timing, each precision's schedule (instance groups, blocked or
-one-sweep sweeps) is autotuned on short runs of the same job (autotune.py;
-`schedule` holds the trials; --no-autotune keeps the library default).
+context's stream.  Those events come from a replay of exactly the timed
+steps: the checkpoint taken at the start of the timed window is restored
+and the same steps are launched eagerly with events around each phase
+(`replay_bitwise` says whether the replay reproduced the timed run's final
+state bit for bit).  `traffic` is HBM bytes per k_fs launch from rocprof
+FETCH_SIZE / WRITE_SIZE over the same replay (--traffic-kib, measured and
+passed in; see the profiles), scaled by what those counters read for a cold
+device copy of known size (--copy-calibration).  `step_roofline` is the
+whole step against the compulsory state traffic 2 (4 F + L) s bytes per
+cell-update (F = 2 fluids, L = 1 level set, s = word size).
+
+For mixed-precision workloads the FP64 path runs the same job the same way
+(`fp64_plain`) and the states after the timed window are compared
+(`accuracy`: relative L1 / Linf per conserved variable of the composite
+real-fluid state, interface-contour L1); --perturb adds the same comparison
+between FP64 and FP64 from an initial state perturbed by 1e-7 relative,
+which measures how much the flow itself amplifies a last-digit change.
+`cpu_baseline` times the FP64 CPU oracle on a bounded sample of the same
+window (the checkpointed mid-job states of the smallest instances, a few
+steps each): serially, as the reference runs, and with one host thread per
+instance on up to 16 cores (`cpu_baseline.all_cores`).
 --host-io K adds `host_io_inclusive`: K steps with every instance's grids
 moved host -> device -> host around each step (the per-phase drop-in
 boundary, PCIe included), reported beside `value`, never as it.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "phd-2d-gfm_amd"))
@@ -61,15 +77,12 @@ import problems as P  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 SEED = 20261203
-# HBM bytes per interior cell (both fluids) of one flux-kernel launch, from
-# rocprof FETCH_SIZE / WRITE_SIZE scaled by a device copy of known size
-# (profiles/k_fs_batch64_mixed_traffic.md); measured for the default workload
-K_FS_TRAFFIC_PER_CELL = {"batch": 66.1}
 GFM_LABEL = {G.GFM_ORIGINAL: "OGFM", G.GFM_REAL: "realGFM", G.GFM_RIEMANN: "riemannGFM", G.GFM_MODIFIED: "MGFM"}
+HOST_THREADS_MAX = 16   # the host cores one GPU's share of a box offers
 
 
 def make_cases(workload, n_inst):
-    """(problem builder, per-instance kwargs, variant, precision, steps, name, init reinit iterations)"""
+    """(problem builder, per-instance kwargs, variant, precision, job steps, name, init reinit iterations)"""
     if workload == "batch":
         cases = P.bubble_batch(n_inst)
         return (P.shocked_bubble, cases, G.GFM_RIEMANN, G.MIXED, 1500,
@@ -104,36 +117,17 @@ def make_setup(builder, cases, perturb=0.0):
     return setup, states
 
 
-def new_ctx(setup, variant, precision, steps):
+def new_ctx(setup, variant, precision, max_steps):
     m = G.methods(sim_type=G.SIM_TWOFLUID, fs=G.FS_MUSCL, gfm=variant, ls_order=5,
                   reinit_every=5, reinit_iters=10, precision=precision)
     ctx = G.Context(m)
     for info, eos in setup:
-        ctx.add(info, eos, 0.6, 1e300, steps)
+        ctx.add(info, eos, 0.6, 1e300, max_steps)
     ctx.commit()
     return ctx
 
 
-def build(builder, cases, variant, precision, steps, perturb=0.0):
-    setup, states = make_setup(builder, cases, perturb)
-    return new_ctx(setup, variant, precision, steps), states, [s[0] for s in setup]
-
-
-def tuned_ctx(a, setup, states, variant, precision, steps, init_reinit):
-    """a context on the autotuned schedule (autotune.py), unless --no-autotune, --eager, or the
-    schedule is set in the environment; returns (ctx, schedule record)"""
-    if a.no_autotune or a.eager or any(k in os.environ for k in T.KEYS):
-        rec = {"tuned": False, "env": {k: os.environ[k] for k in T.KEYS if k in os.environ}}
-        return new_ctx(setup, variant, precision, steps), rec
-    ts = min(a.tune_steps, steps)
-    choice = T.tune(lambda: new_ctx(setup, variant, precision, ts + a.warmup),
-                    lambda c: load_all(c, states, init_reinit), len(setup), steps=ts, warmup=a.warmup)
-    with T.applied(choice):
-        ctx = new_ctx(setup, variant, precision, steps)
-    return ctx, dict(choice, tuned=True, tune_steps=ts)
-
-
-def load_all(ctx, states, init_reinit):
+def load_ic(ctx, states, init_reinit):
     for k, st in enumerate(states):
         ctx.upload(k, *st)
         ctx.set_time(k, 0.0, 0)
@@ -141,58 +135,124 @@ def load_all(ctx, states, init_reinit):
         ctx.ls_reinit(init_reinit)   # ls->initialisation() (sim_twofluid.cpp:47)
 
 
-def timed_run(ctx, states, steps, warmup, init_reinit, eager=False):
+def checkpoint(ctx, n):
+    """every instance's grids, time and step count"""
+    ctx.sync()
+    return [(ctx.download(k), ctx.get_time(k)[:2]) for k in range(n)]
+
+
+def restore(ctx, ck):
+    for k, (st, (t, nstep)) in enumerate(ck):
+        ctx.upload(k, *st)
+        ctx.set_time(k, t, nstep)
+    ctx.sync()
+
+
+def advanced(setup, states, variant, precision, advance, cap, init_reinit, eager=False):
+    """the job advanced `advance` steps from its initial condition (default schedule), checkpointed"""
+    ctx = new_ctx(setup, variant, precision, cap)
+    load_ic(ctx, states, init_reinit)
+    (ctx.step if eager else ctx.step_graph)(advance)
+    ck = checkpoint(ctx, len(setup))
+    status = sorted({ctx.status(k) for k in range(len(setup))})
+    ctx.close()
+    return ck, status
+
+
+def tuned_ctx(a, setup, ck, variant, precision, cap):
+    """a context on the autotuned schedule (autotune.py), trials timed from the checkpoint `ck`, unless
+    --no-autotune, --eager, or the schedule is set in the environment; returns (ctx, schedule record)"""
+    if a.no_autotune or a.eager or any(k in os.environ for k in T.KEYS):
+        rec = {"tuned": False, "env": {k: os.environ[k] for k in T.KEYS if k in os.environ}}
+        return new_ctx(setup, variant, precision, cap), rec
+    ts = a.tune_steps if a.tune_steps else a.steps
+    choice = T.tune(lambda: new_ctx(setup, variant, precision, cap), lambda c: restore(c, ck), len(setup),
+                    steps=ts, warmup=a.warmup)
+    with T.applied(choice):
+        ctx = new_ctx(setup, variant, precision, cap)
+    return ctx, dict(choice, tuned=True, tune_steps=ts, from_step=ck[0][1][1])
+
+
+def timed_window(ctx, ck, steps, warmup, eager=False):
+    """restore `ck`, `warmup` untimed steps, checkpoint, `steps` timed steps; returns
+    (seconds, checkpoint at the window start, final states)"""
     step = ctx.step if eager else ctx.step_graph
-    load_all(ctx, states, init_reinit)
+    if not eager:
+        # capture every graph the timed call replays (blocks of steps are
+        # captured at their first use), then start again from `ck`
+        restore(ctx, ck)
+        step(steps)
+    restore(ctx, ck)
     step(warmup)
-    ctx.sync()
-    load_all(ctx, states, init_reinit)
-    ctx.sync()
+    win = checkpoint(ctx, len(ck))
     t0 = time.perf_counter()
     step(steps)
     ctx.sync()
-    return time.perf_counter() - t0
+    el = time.perf_counter() - t0
+    return el, win, [ctx.download(k) for k in range(len(ck))]
 
 
-def cpu_baseline(builder, cases, variant, steps):
-    """FP64 CPU oracle, one core, first `steps` steps of the given instances"""
+def same_states(a, b):
+    return all(all(np.array_equal(x, y) for x, y in zip(sa, sb)) for sa, sb in zip(a, b))
+
+
+def oracle_job(builder_args):
+    """one instance's FP64 oracle run from a checkpointed state (called from host threads; the oracle's
+    scratch is thread-local and ctypes releases the GIL)"""
+    import oracle_lib as O
+    info, eos, st, variant, steps = builder_args
+    g1, g2, phi = (np.ascontiguousarray(x, dtype=np.float64).copy() for x in st)
+    Gr = O.Grid(info.Nx, info.Ny, info.numGC, info.x0, info.y0, info.dx, info.dy,
+                info.BC_L, info.BC_R, info.BC_B, info.BC_T)
+    E = O.Eos(eos.gamma1, eos.pinf1, eos.gamma2, eos.pinf2)
+    run = O.Run(variant, 0, 5, 5, 10, 0.6, 1e300, steps)
+    s = np.zeros(1, dtype=np.int32)
+    t = np.zeros(1)
+    O.lib().or_twofluid_run(Gr, E, run, O._p(g1), O._p(g2), O._p(phi), O._p(s), O._p(t))
+    return info.Nx * info.Ny * int(s[0])
+
+
+def cpu_baseline(setup, ck, variant, steps, n_serial, n_threads):
+    """FP64 CPU oracle on the smallest instances' checkpointed mid-job states: serially (one core, as the
+    reference runs) and with one host thread per instance"""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    tot_cells, el = 0, 0.0
-    for c in cases:
-        info, eos, (g1, g2, phi) = builder(**c)
-        Gr = O.Grid(info.Nx, info.Ny, info.numGC, info.x0, info.y0, info.dx, info.dy,
-                    info.BC_L, info.BC_R, info.BC_B, info.BC_T)
-        E = O.Eos(eos.gamma1, eos.pinf1, eos.gamma2, eos.pinf2)
-        run = O.Run(variant, 0, 5, 5, 10, 0.6, 1e300, steps)
-        s = np.zeros(1, dtype=np.int32)
-        t = np.zeros(1)
-        t0 = time.perf_counter()
-        O.lib().or_twofluid_run(Gr, E, run, O._p(g1), O._p(g2), O._p(phi), O._p(s), O._p(t))
-        el += time.perf_counter() - t0
-        tot_cells += info.Nx * info.Ny * int(s[0])
-    desc = ", ".join("%dx%d" % (c["Nx"], c["Ny"]) for c in cases)
-    return {"value": tot_cells / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "FP64 oracle (single thread), first %d steps of %s, %.1f s" % (steps, desc, el)}
+    O.lib()
+    order = sorted(range(len(setup)), key=lambda k: setup[k][0].Nx * setup[k][0].Ny)
+    jobs = [(setup[k][0], setup[k][1], ck[k][0], variant, steps) for k in order]
+    t0 = time.perf_counter()
+    cells_s = sum(oracle_job(j) for j in jobs[:n_serial])
+    el_s = time.perf_counter() - t0
+    threads = max(1, min(n_threads, HOST_THREADS_MAX, os.cpu_count() or 1))
+    par = jobs[:max(threads, 1)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        cells_p = sum(ex.map(oracle_job, par))
+    el_p = time.perf_counter() - t0
+    desc = lambda js: ", ".join("%dx%d" % (j[0].Nx, j[0].Ny) for j in js)  # noqa: E731
+    start = ck[0][1][1]
+    return {"value": cells_s / el_s, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "FP64 oracle, one thread, %d steps from the step-%d checkpoint of %s: %.1f s"
+                      % (steps, start, desc(jobs[:n_serial]), el_s),
+            "all_cores": {"value": cells_p / el_p, "unit": "cell-updates/s", "cores": threads,
+                          "nproc": os.cpu_count(),
+                          "sample": "FP64 oracle, one host thread per instance, %d steps from the step-%d "
+                                    "checkpoint of %d instances (%s): %.1f s"
+                                    % (steps, start, len(par), desc(par), el_p)}}
 
 
-def states_of(ctx, n):
-    return [ctx.download(k) for k in range(n)]
-
-
-def host_io_run(ctx, states, steps, init_reinit):
+def host_io_run(ctx, ck, steps):
     """the rate through the per-phase drop-in boundary: every step uploads
     each instance's host grids (FP64 cells, both fluids and phi), runs one
     step (eager launches) and downloads all grids again"""
-    load_all(ctx, states, init_reinit)
-    ctx.sync()
-    cur = list(states)
+    restore(ctx, ck)
+    cur = [c[0] for c in ck]
     t0 = time.perf_counter()
     for _ in range(steps):
         for k, st in enumerate(cur):
             ctx.upload(k, *st)
         ctx.step(1)
-        cur = states_of(ctx, len(cur))
+        cur = [ctx.download(k) for k in range(len(cur))]
     ctx.sync()
     return time.perf_counter() - t0
 
@@ -202,149 +262,171 @@ def accuracy(infos, got, ref):
     return A.aggregate(res, [i.Nx * i.Ny for i in infos])
 
 
-def run_job(builder, cases, variant, precision, steps, init_reinit, perturb=0.0):
-    ctx, states, _ = build(builder, cases, variant, precision, steps, perturb)
-    load_all(ctx, states, init_reinit)
+def run_job(setup, states, variant, precision, steps, init_reinit):
+    ctx = new_ctx(setup, variant, precision, steps + 1)
+    load_ic(ctx, states, init_reinit)
     ctx.step_graph(steps)
     ctx.sync()
-    out = states_of(ctx, len(cases))
+    out = [ctx.download(k) for k in range(len(setup))]
     ctx.close()
     return out
+
+
+def copy_calibration(nbytes):
+    """one cold device-to-device copy of nbytes (hipMemcpy, the runtime's copy kernel): the source is
+    written first, then a buffer of the same size, so that at most the last-written bytes sit in the
+    caches when the copy reads the source.  rocprof's FETCH_SIZE / WRITE_SIZE of that dispatch over
+    nbytes are the counters' scale factors."""
+    hip = C.CDLL("libamdhip64.so")
+    vp = C.c_void_p
+    bufs = [vp(), vp(), vp()]
+    for b in bufs:
+        if hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc")
+    src, dst, flush = bufs
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    hip.hipMemset(flush, 2, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    for b in bufs:
+        hip.hipFree(b)
+    return {"copy_calibration_bytes": nbytes, "dispatch": "the last __amd_rocclr_copyBuffer of the run"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=100, help="timed steps")
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--advance", type=int, default=None,
+                    help="untimed steps from the initial condition before the window (default: half the job)")
+    ap.add_argument("--job-steps", type=int, default=None, help="job length whose middle is timed")
     ap.add_argument("--workload", default="batch", choices=["batch", "bubble", "large", "rmi"])
     ap.add_argument("--instances", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run of mixed workloads")
+    ap.add_argument("--no-replay", action="store_true", help="skip the per-phase HIP-event replay of the window")
     ap.add_argument("--accuracy", action="store_true",
                     help="large: also run GFM_original and GFM_real, mixed against FP64")
     ap.add_argument("--perturb", action="store_true", help="also compare FP64 against FP64 from a perturbed start")
-    ap.add_argument("--profile-steps", type=int, default=None,
-                    help="steps of the per-phase HIP-event pass (default: the timed step count)")
-    ap.add_argument("--eager", action="store_true", help="launch kernels directly instead of one hipGraph per step")
+    ap.add_argument("--eager", action="store_true", help="launch kernels directly instead of hipGraphs")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default schedule instead of autotuning it (autotune.py)")
-    ap.add_argument("--tune-steps", type=int, default=100, help="steps per autotuning trial")
+    ap.add_argument("--tune-steps", type=int, default=None, help="steps per autotuning trial (default: --steps)")
+    ap.add_argument("--traffic-kib", type=float, nargs=2, default=None, metavar=("FETCH", "WRITE"),
+                    help="rocprof FETCH_SIZE and WRITE_SIZE of k_fs per launch over the replay (KiB), with the "
+                         "copy scale factors from --copy-factors")
+    ap.add_argument("--copy-factors", type=float, nargs=2, default=(1.0, 1.0), metavar=("FETCH", "WRITE"))
     ap.add_argument("--host-io", type=int, default=0, metavar="K",
                     help="also time K steps that move every instance's grids host -> device -> host around each "
                          "step (the per-phase drop-in's rate, PCIe included; reported beside value, never as it)")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run device-to-device copies of known size (the workload's conserved grids, "
-                         "through gfm_set_ghost_states) for scaling rocprof FETCH_SIZE / WRITE_SIZE; prints "
-                         "the bytes copied")
+    ap.add_argument("--copy-calibration", type=float, default=None, metavar="GIB",
+                    help="only run one cold device-to-device copy of GIB GiB, for scaling FETCH_SIZE / WRITE_SIZE")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one MI355X: the per-timestep path runs on one GPU here "
               "(instances would be bin-packed across GPUs; not built)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        print(json.dumps(copy_calibration(int(a.copy_calibration * (1 << 30)))))
+        return
 
     n_inst = a.instances or (64 if a.workload == "batch" else 256)
-    builder, cases, variant, prec, dsteps, name, init_reinit = make_cases(a.workload, n_inst)
-    steps = a.steps or dsteps
+    builder, cases, variant, prec, job, name, init_reinit = make_cases(a.workload, n_inst)
+    job = a.job_steps or job
+    advance = job // 2 if a.advance is None else a.advance
+    cap = advance + a.warmup + max(a.steps, a.tune_steps or 0) + 8
 
     setup, states = make_setup(builder, cases)
     infos = [s[0] for s in setup]
     cells = sum(i.Nx * i.Ny for i in infos)
-    if a.copy_calibration:
-        ctx = new_ctx(setup, variant, prec, steps + a.warmup)
-        # the phase API ends a ghost fill with Q[.][1] -> Q[.][0], one
-        # hipMemcpyAsync per instance and fluid of 4 padded planes
-        es = 8 if prec == G.FP64 else 4
-        nbytes = 0
-        for i in infos:
-            H, W = i.shape
-            pitch = (W + 63) // 64 * 64
-            plane = (pitch * H + 63) // 64 * 64
-            nbytes += 2 * 4 * plane * es
-        load_all(ctx, states, init_reinit)
-        ctx.set_ghost_states()
-        ctx.sync()
-        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 2 * len(infos)}))
-        return
-    ctx, schedule = tuned_ctx(a, setup, states, variant, prec, steps + a.warmup, init_reinit)
-    el = timed_run(ctx, states, steps, a.warmup, init_reinit, a.eager)
-    status = [ctx.status(k) for k in range(len(cases))]
-    value = cells * steps / el
+    n = len(setup)
 
-    # per-phase HIP-event timing: the same workload replayed from the initial
-    # state with events around each phase (eager launches)
-    nprof = a.profile_steps or steps
-    load_all(ctx, states, init_reinit)
-    ms = ctx.profile_steps(nprof)
-    fs_ms = ms[2] / nprof
+    ck, st_adv = advanced(setup, states, variant, prec, advance, cap, init_reinit, a.eager)
+    ctx, schedule = tuned_ctx(a, setup, ck, variant, prec, cap)
+    el, win, final = timed_window(ctx, ck, a.steps, a.warmup, a.eager)
+    status = sorted({ctx.status(k) for k in range(n)} | set(st_adv))
+    value = cells * a.steps / el
+
     es = 8 if prec == G.FP64 else 4
-    fs_bytes = 2 * cells * 8 * es
-    achieved = fs_bytes / (fs_ms * 1e-3) / 1e9
-    step_bytes = 2 * (4 * 2 + 1) * es
-    step_gbs = step_bytes * value / 1e9
-    final = states_of(ctx, len(cases)) if nprof == steps else None
-    host_io = None
-    if a.host_io > 0:
-        el_io = host_io_run(ctx, states, a.host_io, init_reinit)
-        host_io = {"value": cells * a.host_io / el_io, "unit": "cell-updates/s", "steps": a.host_io,
-                   "ms_per_step": el_io * 1e3 / a.host_io,
-                   "what": "per step: upload of every instance's FP64 host grids (both fluids, phi), one eager "
-                           "step, download of all grids, from Python through the C ABI"}
-    ctx.close()
-
     out = {
         "metric": "cell-updates/s on 1 MI355X (two-fluid GFM step; mixed+batched vs FP64 plain)",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
-        "steps": steps,
+        "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": el * 1e3 / steps,
+        "ms_per_step": el * 1e3 / a.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if prec == G.FP64 else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": name, "instances": len(cases), "cells": cells, "steps": steps,
+        "config": {"workload": name, "instances": n, "cells": cells, "job_steps": job,
+                   "window": [advance + a.warmup, advance + a.warmup + a.steps],
                    "gfm": GFM_LABEL[variant], "fs": "MUSCL", "ls": "LS5", "reinit_every": 5, "cfl": 0.6},
-        "roofline": {"bound": "hbm", "kernel": "k_fs (MUSCL-Hancock + exact RS, both fluids)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (K_FS_TRAFFIC_PER_CELL[a.workload] * cells
-                                 if a.workload in K_FS_TRAFFIC_PER_CELL and prec == G.MIXED else None),
-                     "traffic_source": "rocprof FETCH_SIZE+WRITE_SIZE, copy-scaled (profiles/k_fs_batch64_mixed_traffic.md)"},
-        "step_roofline": {"bytes_per_cell_update": step_bytes, "achieved": step_gbs, "peak": HBM_PEAK_GBS,
-                          "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
-        "phase_ms_per_step": {n: v / nprof for n, v in zip(("dt", "ghost", "fs", "ls", "bc"), ms)},
-        "phase_steps": nprof,
-        "status": sorted(set(status)),
-        "schedule": schedule,
     }
-    if host_io is not None:
-        out["host_io_inclusive"] = host_io
+    if not a.no_replay:
+        # the same steps again from the window's checkpoint, eager, events around each phase
+        restore(ctx, win)
+        ms = ctx.profile_steps(a.steps)
+        replay = [ctx.download(k) for k in range(n)]
+        fs_ms = ms[2] / a.steps
+        fs_bytes = 2 * cells * 8 * es
+        achieved = fs_bytes / (fs_ms * 1e-3) / 1e9
+        traffic = None
+        if a.traffic_kib:
+            traffic = 1024.0 * (a.traffic_kib[0] * a.copy_factors[0] + a.traffic_kib[1] * a.copy_factors[1])
+        out["roofline"] = {"bound": "hbm", "kernel": "k_fs (MUSCL-Hancock + exact RS, both fluids)",
+                           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                           "bytes_per_launch": fs_bytes, "ms_per_launch": fs_ms,
+                           "launches": a.steps, "window": "the timed steps, replayed from their checkpoint"}
+        if traffic is not None:
+            out["roofline"]["traffic_raw_kib"] = list(a.traffic_kib)
+            out["roofline"]["copy_factors"] = list(a.copy_factors)
+        out["phase_ms_per_step"] = {k: v / a.steps for k, v in zip(("dt", "ghost", "fs", "ls", "bc"), ms)}
+        out["replay_bitwise"] = same_states(final, replay)
+    step_bytes = 2 * (4 * 2 + 1) * es
+    step_gbs = step_bytes * value / 1e9
+    out["step_roofline"] = {"bytes_per_cell_update": step_bytes, "achieved": step_gbs, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS}
+    out["status"] = status
+    out["schedule"] = schedule
+    if a.host_io > 0:
+        el_io = host_io_run(ctx, win, a.host_io)
+        out["host_io_inclusive"] = {
+            "value": cells * a.host_io / el_io, "unit": "cell-updates/s", "steps": a.host_io,
+            "ms_per_step": el_io * 1e3 / a.host_io,
+            "what": "per step: upload of every instance's FP64 host grids (both fluids, phi), one eager "
+                    "step, download of all grids, from Python through the C ABI"}
+    ctx.close()
+
     if prec == G.MIXED and not a.no_fp64:
-        ctx64, sched64 = tuned_ctx(a, setup, states, variant, G.FP64, steps + a.warmup, init_reinit)
-        el64 = timed_run(ctx64, states, steps, a.warmup, init_reinit, a.eager)
-        out["fp64_plain"] = {"value": cells * steps / el64, "steps": steps,
-                             "ratio_mixed_over_fp64": value / (cells * steps / el64), "schedule": sched64}
-        ref = states_of(ctx64, len(cases))
+        ck64, st64 = advanced(setup, states, variant, G.FP64, advance, cap, init_reinit, a.eager)
+        ctx64, sched64 = tuned_ctx(a, setup, ck64, variant, G.FP64, cap)
+        el64, _, ref = timed_window(ctx64, ck64, a.steps, a.warmup, a.eager)
         ctx64.close()
-        if final is not None:
-            acc = {"vs": "FP64 GPU path, same job, after %d steps" % steps,
-                   GFM_LABEL[variant]: accuracy(infos, final, ref)}
-            if a.perturb:
-                pert = run_job(builder, cases, variant, G.FP64, steps, init_reinit, perturb=1e-7)
-                acc["fp64_perturbed_1e-7"] = accuracy(infos, pert, ref)
-            if a.accuracy and a.workload == "large":
-                for v in (G.GFM_ORIGINAL, G.GFM_REAL):
-                    mixed = run_job(builder, cases, v, G.MIXED, steps, init_reinit)
-                    fp64 = run_job(builder, cases, v, G.FP64, steps, init_reinit)
-                    acc[GFM_LABEL[v]] = accuracy(infos, mixed, fp64)
-            out["accuracy"] = acc
+        out["fp64_plain"] = {"value": cells * a.steps / el64, "steps": a.steps,
+                             "ratio_mixed_over_fp64": value / (cells * a.steps / el64), "schedule": sched64,
+                             "status": st64}
+        total = advance + a.warmup + a.steps
+        acc = {"vs": "FP64 GPU path, same job, after %d steps from the initial condition" % total,
+               GFM_LABEL[variant]: accuracy(infos, final, ref)}
+        if a.perturb:
+            _, pstates = make_setup(builder, cases, perturb=1e-7)
+            pert = run_job(setup, pstates, variant, G.FP64, total, init_reinit)
+            acc["fp64_perturbed_1e-7"] = accuracy(infos, pert, ref)
+        if a.accuracy and a.workload == "large":
+            for v in (G.GFM_ORIGINAL, G.GFM_REAL):
+                mixed = run_job(setup, states, v, G.MIXED, total, init_reinit)
+                fp64 = run_job(setup, states, v, G.FP64, total, init_reinit)
+                acc[GFM_LABEL[v]] = accuracy(infos, mixed, fp64)
+        out["accuracy"] = acc
     if not a.no_cpu:
-        small = sorted(cases, key=lambda c: c["Nx"] * c["Ny"])[:2]
-        out["cpu_baseline"] = cpu_baseline(builder, small, variant, a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(setup, ck, variant, a.cpu_steps, 4, HOST_THREADS_MAX)
     print(json.dumps(out))
 
 

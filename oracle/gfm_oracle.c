@@ -35,7 +35,8 @@
 
 /* ------------------------------------------------------------------ */
 /* scratch buffers (the reference keeps function-local statics too,     */
-/* GFM_riemann.cpp:20-22, extrapolate_vector.cpp:18-19)                 */
+/* GFM_riemann.cpp:20-22, extrapolate_vector.cpp:18-19); thread-local   */
+/* here, so that host threads may run independent instances at once     */
 /* ------------------------------------------------------------------ */
 typedef struct { double* p; size_t n; } buf_t;
 
@@ -804,7 +805,7 @@ static void ls_rhs(const or_grid* G, int order, const double* phi, const double*
 /* update_interface_coupled(vfield, t, dt), sim_twofluid.cpp:72 */
 void or_ls_advect(const or_grid* G, int order, double* phi, const double* vf, double dt)
 {
-    static buf_t b0, b1, b2;
+    static _Thread_local buf_t b0, b1, b2;
     size_t n = (size_t)H_OF(G) * W_OF(G);
     double* p1 = buf_get(&b0, n);
     double* p2 = buf_get(&b1, n);
@@ -834,7 +835,7 @@ void or_ls_advect(const or_grid* G, int order, double* phi, const double* vf, do
 
 void or_ls_reinit(const or_grid* G, double* phi, int iters)
 {
-    static buf_t b0, b1;
+    static _Thread_local buf_t b0, b1;
     size_t n = (size_t)H_OF(G) * W_OF(G);
     double* p0 = buf_get(&b0, n);
     double* pn = buf_get(&b1, n);
@@ -895,7 +896,7 @@ void or_ls_reinit(const or_grid* G, double* phi, int iters)
 int or_extrapolate_vector(const or_grid* G, const double* phi, int N,
                           double* g1, double* g2, int mgfm)
 {
-    static buf_t b1, b2;
+    static _Thread_local buf_t b1, b2;
     int st = 0;
     size_t n = (size_t)H_OF(G) * W_OF(G) * 4;
     double* u1 = buf_get(&b1, n);
@@ -1003,7 +1004,7 @@ static void store_vfield_real(const or_grid* G, const double* phi, const double*
 int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const double* phi,
                         double* grid1, double* grid2, int* bb1, int* bb2, double* vfield)
 {
-    static buf_t bp1, bp2, bn1, bn2;
+    static _Thread_local buf_t bp1, bp2, bn1, bn2;
     int st = 0;
     int H = H_OF(G), Wd = W_OF(G), g = G->g;
     size_t ncell = (size_t)H * Wd;

@@ -45,8 +45,10 @@ def _time(build_ctx, load, settings, steps, warmup):
     with _env(**settings):
         ctx = build_ctx()
     try:
+        # the warmup has the timed call's shape, so that every graph the
+        # timed call replays is captured before the clock starts
         load(ctx)
-        ctx.step_graph(warmup)
+        ctx.step_graph(max(warmup, steps))
         ctx.sync()
         load(ctx)
         ctx.sync()
