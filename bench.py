@@ -37,9 +37,10 @@ steps: the checkpoint taken at the start of the timed window is restored
 and the same steps are launched eagerly with events around each phase
 (`replay_bitwise` says whether the replay reproduced the timed run's final
 state bit for bit).  `traffic` is HBM bytes per k_fs launch from rocprof
-FETCH_SIZE / WRITE_SIZE over the same replay (--traffic-kib, measured and
-passed in; see the profiles), scaled by what those counters read for a cold
-device copy of known size (--copy-calibration).  `step_roofline` is the
+FETCH_SIZE / WRITE_SIZE per launch (--traffic-kib; for the default workload
+the values measured over its default window are built in), scaled by what
+those counters read for a cold device copy of known size
+(--copy-calibration): FETCH_SIZE x 2, WRITE_SIZE x 1.  `step_roofline` is the
 whole step against the compulsory state traffic 2 (4 F + L) s bytes per
 cell-update (F = 2 fluids, L = 1 level set, s = word size).
 
@@ -79,6 +80,12 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20261203
 GFM_LABEL = {G.GFM_ORIGINAL: "OGFM", G.GFM_REAL: "realGFM", G.GFM_RIEMANN: "riemannGFM", G.GFM_MODIFIED: "MGFM"}
 HOST_THREADS_MAX = 16   # the host cores one GPU's share of a box offers
+# rocprof counters of the flux kernel per launch over the default workload's window (steps 760-860),
+# profiles/batch64_window_counters.md: FETCH_SIZE and WRITE_SIZE in KiB, with the factors a cold 2 GiB
+# device copy gives them (FETCH_SIZE counts gfx950's 128-byte read requests as 64 bytes), VALU busy
+K_FS_COUNTERS = {"batch": {"fetch_kib": 4.241e5, "write_kib": 7.982e5, "factors": (2.0, 1.0),
+                           "valu_busy": 0.962, "lds_bank_conflict_per_lds_cycle": 0.71,
+                           "measured_window": [760, 860], "source": "profiles/batch64_window_counters.md"}}
 
 
 def make_cases(workload, n_inst):
@@ -376,17 +383,22 @@ def main():
         fs_ms = ms[2] / a.steps
         fs_bytes = 2 * cells * 8 * es
         achieved = fs_bytes / (fs_ms * 1e-3) / 1e9
-        traffic = None
+        traffic, counters = None, None
         if a.traffic_kib:
-            traffic = 1024.0 * (a.traffic_kib[0] * a.copy_factors[0] + a.traffic_kib[1] * a.copy_factors[1])
+            counters = {"fetch_kib": a.traffic_kib[0], "write_kib": a.traffic_kib[1],
+                        "factors": tuple(a.copy_factors), "source": "--traffic-kib"}
+        elif a.workload in K_FS_COUNTERS and prec == G.MIXED and n == 64:
+            counters = K_FS_COUNTERS[a.workload]
+        if counters:
+            traffic = 1024.0 * (counters["fetch_kib"] * counters["factors"][0]
+                                + counters["write_kib"] * counters["factors"][1])
         out["roofline"] = {"bound": "hbm", "kernel": "k_fs (MUSCL-Hancock + exact RS, both fluids)",
                            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                            "bytes_per_launch": fs_bytes, "ms_per_launch": fs_ms,
                            "launches": a.steps, "window": "the timed steps, replayed from their checkpoint"}
-        if traffic is not None:
-            out["roofline"]["traffic_raw_kib"] = list(a.traffic_kib)
-            out["roofline"]["copy_factors"] = list(a.copy_factors)
+        if counters:
+            out["roofline"]["traffic_counters"] = counters
         out["phase_ms_per_step"] = {k: v / a.steps for k, v in zip(("dt", "ghost", "fs", "ls", "bc"), ms)}
         out["replay_bitwise"] = same_states(final, replay)
     step_bytes = 2 * (4 * 2 + 1) * es
