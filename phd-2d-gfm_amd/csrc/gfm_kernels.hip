@@ -810,6 +810,25 @@ __device__ __forceinline__ ExOut<R> extrap_out(const Inst& I, int fr, const V4<R
     return o;
 }
 
+// FP32 storage (mixed precision): the real fluid of a cell that is not
+// interfacial leaves the ghost fill as it came in.  Its primitives are the
+// conversion of the fill's input conserved state Q[.][0] and are never
+// changed, so the reference's W->U at the end of the fill returns that state
+// up to rounding; in FP32 that round trip (E - rho |u|^2 / 2 and back) costs
+// several ulp of the real fluid on every step, the largest error of the mixed
+// step, so the stored state is written instead.  FP64 keeps the reference's
+// conversion.  c: the cell whose real fluid this is.
+template <typename R, int MODE>
+__device__ __forceinline__ void keep_real(const Inst& I, long long c, int fr, ExOut<R>& o) {
+    if constexpr (sizeof(R) == 4 && (MODE & MF_LAST) != 0) {
+        if (gmem(I.imap)[c] < 0) {
+            const V4<R> U = ld4<R>(I.Q[fr][0], I, c);
+            if (fr == 0) o.a = U;
+            else o.b = U;
+        }
+    }
+}
+
 template <typename R, int MODE>
 __device__ __forceinline__ void extrap_put(const Inst& I, int i, int j, int bout, ExOut<R> o, bool fu, bool fv) {
     if (fu) { o.a.v[1] = -o.a.v[1]; o.b.v[1] = -o.b.v[1]; o.vu = -o.vu; }
@@ -863,7 +882,9 @@ __device__ __forceinline__ void extrap_cell(const Inst& I, int i, int j, int bin
         atomicMin(&bb[0], i - 2); atomicMax(&bb[1], i + 2);
         atomicMin(&bb[2], j - 2); atomicMax(&bb[3], j + 2);
     }
-    extrap_put<R, MODE>(I, i, j, bout, extrap_out<R, MODE>(I, fr, Wr, Wg), fu, fv);
+    ExOut<R> o = extrap_out<R, MODE>(I, fr, Wr, Wg);
+    keep_real<R, MODE>(I, cs, fr, o);
+    extrap_put<R, MODE>(I, i, j, bout, o, fu, fv);
 }
 
 template <typename R, int MODE>
@@ -1137,7 +1158,8 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
             atomicMin(&bb[0], i - 2); atomicMax(&bb[1], i + 2);
             atomicMin(&bb[2], j - 2); atomicMax(&bb[3], j + 2);
         }
-        const ExOut<R> o = extrap_out<R, MOUT>(I, fr, Wr, Wo);
+        ExOut<R> o = extrap_out<R, MOUT>(I, fr, Wr, Wo);
+        keep_real<R, MOUT>(I, cidx(I, i, j), fr, o);
         extrap_put<R, MOUT>(I, i, j, bout, o, false, false);
         if (i < 2 * g || i >= I.Ny || j < 2 * g || j >= I.Nx) {
             // ghost cells sourced here (inverse of src_row / src_col): rows
@@ -1398,9 +1420,17 @@ __device__ __forceinline__ unsigned iface_mrp(const Inst& I, const double* ip1, 
 template <typename R, int VAR>
 __device__ __forceinline__ void orig_prims(const Inst& I, int f, int ii, int jj, int pbuf, double* W) {
     const long long c = cidx(I, ii, jj);
-    V4<R> P;
-    if (VAR == V_REAL) P = cons2prims<R>((R)I.gam[f], (R)I.pinf[f], ld4<R>(I.Q[f][0], I, c));
-    else P = ld4<R>(I.P[f][pbuf], I, c);
+    if (VAR == V_REAL) {
+        // GFM_real converts the stored state; in FP64 also for FP32 storage
+        // (an interfacial star-state input, O(perimeter) work)
+        const V4<R> U = ld4<R>(I.Q[f][0], I, c);
+        V4<double> Ud;
+        for (int k = 0; k < 4; ++k) Ud.v[k] = (double)U.v[k];
+        const V4<double> P = cons2prims<double>(I.gam[f], I.pinf[f], Ud);
+        W[0] = P.v[0]; W[1] = P.v[1]; W[2] = P.v[2]; W[3] = P.v[3];
+        return;
+    }
+    const V4<R> P = ld4<R>(I.P[f][pbuf], I, c);
     W[0] = P.v[0]; W[1] = P.v[1]; W[2] = P.v[2]; W[3] = P.v[3];
 }
 
@@ -1702,8 +1732,9 @@ __global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ ins
     const int j = w.z * 64 + (threadIdx.x & 63);
     if (i >= I.H || j >= I.W) return;
     const long long c = cidx(I, i, j);
-    V4<R> W1 = cons2prims<R>((R)I.gam[0], (R)I.pinf[0], ld4<R>(I.Q[0][qin], I, c));
-    V4<R> W2 = cons2prims<R>((R)I.gam[1], (R)I.pinf[1], ld4<R>(I.Q[1][qin], I, c));
+    const V4<R> U1 = ld4<R>(I.Q[0][qin], I, c), U2 = ld4<R>(I.Q[1][qin], I, c);
+    V4<R> W1 = cons2prims<R>((R)I.gam[0], (R)I.pinf[0], U1);
+    V4<R> W2 = cons2prims<R>((R)I.gam[1], (R)I.pinf[1], U2);
     const int fr = ld1<R>(I.phi, c) <= R(0) ? 0 : 1;
     if (!PRIMS) {
         const auto vel = gmem((R*)I.vel);
@@ -1726,6 +1757,14 @@ __global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ ins
     if (PRIMS) {
         st4<R>(I.P[0][qout], I, c, W1);
         st4<R>(I.P[1][qout], I, c, W2);
+        return;
+    }
+    if (sizeof(R) == 4 && !(interior(I, i, j) && I.imap[c] >= 0)) {
+        // FP32 storage: both fluids of a cell that is not interfacial are
+        // unchanged, so their stored states go out as they came in (the
+        // W->U of :230-239 would return them up to an FP32 round trip)
+        st4<R>(I.Q[0][qout], I, c, U1);
+        st4<R>(I.Q[1][qout], I, c, U2);
         return;
     }
     st4<R>(I.Q[0][qout], I, c, prims2cons<R>((R)I.gam[0], (R)I.pinf[0], W1));
