@@ -178,3 +178,31 @@ def make_grid(Nx, Ny, g, x0, y0, Lx, Ly, bcs=(BC_T, BC_T, BC_T, BC_T)):
 def new_state(G):
     H, W = G.shape
     return np.zeros((H, W, 4)), np.zeros((H, W))
+
+
+def twofluid_run(info, eos, state, variant, steps, fs=FS_MUSCL, ls_order=5, reinit_every=5, reinit_iters=10,
+                 cfl=0.6, init_reinit=0):
+    """`steps` oracle steps of one two-fluid instance from `state` (g1, g2, phi; copied) after
+    `init_reinit` reinitialisation iterations of phi; returns (g1, g2, phi, status, t)"""
+    g1, g2, phi = (np.ascontiguousarray(a, dtype=np.float64).copy() for a in state)
+    Gr = Grid(info.Nx, info.Ny, info.numGC, info.x0, info.y0, info.dx, info.dy,
+              info.BC_L, info.BC_R, info.BC_B, info.BC_T)
+    E = Eos(eos.gamma1, eos.pinf1, eos.gamma2, eos.pinf2)
+    if init_reinit:
+        lib().or_ls_reinit(Gr, _p(phi), init_reinit)
+    run = Run(variant, fs, ls_order, reinit_every, reinit_iters, cfl, 1e300, steps)
+    s = np.zeros(1, dtype=np.int32)
+    t = np.zeros(1)
+    st = lib().or_twofluid_run(Gr, E, run, _p(g1), _p(g2), _p(phi), _p(s), _p(t))
+    assert int(s[0]) == steps
+    return g1, g2, phi, st, float(t[0])
+
+
+def run_many(fn, jobs, threads=None):
+    """fn(*job) for every job on host threads (the oracle's scratch is thread-local and ctypes
+    releases the GIL while a call runs)"""
+    from concurrent.futures import ThreadPoolExecutor
+    lib()
+    n = max(1, min(len(jobs), threads or min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(n) as ex:
+        return list(ex.map(lambda j: fn(*j), jobs))
