@@ -93,6 +93,19 @@ int gfm_instance_add(gfm_ctx* ctx, const gfm_sim_info* info, const gfm_eos* eos,
                      double cfl, double T, int max_steps, int* inst_id);
 int gfm_ctx_commit(gfm_ctx* ctx);
 int gfm_num_instances(const gfm_ctx* ctx);
+/* GFM_base's public flags use_extension_vfield and use_slip_BCs
+ * (GFM_base.hpp:20-21), set between gfm_ctx_create and gfm_ctx_commit.  Both
+ * default to 0 here, as sim_twofluid::set_sim_methods sets them
+ * (sim_twofluid.cpp:551-552); GFM_base's own constructor defaults
+ * use_extension_vfield to 1.  use_extension_vfield: the stored velocity field
+ * is the interfacial cells' velocity (u* n, or the real fluid's for
+ * GFM_original) carried away from the interface by 20 iterations of
+ * extrapolate_extension_vfield (extrapolate_extension_vfield.cpp:4-122)
+ * instead of the real fluid's velocity.  use_slip_BCs: GFM_modified's ghost
+ * fluid keeps its own tangential velocity (GFM_modified.cpp:152-169); no
+ * effect on GFM_real and GFM_riemann, refused for GFM_original, whose
+ * reference asserts it is off (GFM_original.cpp:21). */
+int gfm_set_gfm_flags(gfm_ctx* ctx, int use_extension_vfield, int use_slip_BCs);
 
 /* state transfer (host FP64, reference layout).  grid2 and phi are ignored
  * for one-fluid contexts and may be NULL. */

@@ -437,7 +437,9 @@ static int bc_src_row(const or_grid* G, int i, int* flip)
     return i;
 }
 
-static void apply_bcs_n(const or_grid* G, double* grid, int nc, int flipcomp)
+/* nc values per cell; on a reflective side component fx (x sides) or fy (y
+ * sides) is negated (-1: none) */
+static void apply_bcs_n(const or_grid* G, double* grid, int nc, int fx, int fy)
 {
     int H = H_OF(G), Wd = W_OF(G), g = G->g;
     int allper = G->bcL == OR_BC_PERIODIC && G->bcR == OR_BC_PERIODIC &&
@@ -460,7 +462,7 @@ static void apply_bcs_n(const or_grid* G, double* grid, int nc, int flipcomp)
             int sj = bc_src_col(G, j, &fc);
             double* dst = grid + IDX(G, i, j) * nc;
             memcpy(dst, grid + IDX(G, i, sj) * nc, nc * sizeof(double));
-            if (fc && flipcomp) dst[1] = -dst[1];
+            if (fc && fx >= 0) dst[fx] = -dst[fx];
         }
     for (int i = 0; i < H; ++i) {
         if (i >= g && i < G->Ny + g) continue;
@@ -469,15 +471,21 @@ static void apply_bcs_n(const or_grid* G, double* grid, int nc, int flipcomp)
         for (int j = 0; j < Wd; ++j) {
             double* dst = grid + IDX(G, i, j) * nc;
             memcpy(dst, grid + IDX(G, si, j) * nc, nc * sizeof(double));
-            if (fr && flipcomp) dst[2] = -dst[2];
+            if (fr && fy >= 0) dst[fy] = -dst[fy];
         }
     }
 }
 
-void or_apply_bcs(const or_grid* G, double* grid) { apply_bcs_n(G, grid, 4, 1); }
+void or_apply_bcs(const or_grid* G, double* grid) { apply_bcs_n(G, grid, 4, 1, 2); }
 
 /* level-set BCs: same mirror/wrap map, no sign flips (external Levelset) */
-void or_apply_bcs_scalar(const or_grid* G, double* phi) { apply_bcs_n(G, phi, 1, 0); }
+void or_apply_bcs_scalar(const or_grid* G, double* phi) { apply_bcs_n(G, phi, 1, -1, -1); }
+
+/* apply_gridVector2dtype_BCs (external PhD-Common boundary_conditions.hpp,
+ * called at extrapolate_extension_vfield.cpp:116), restated with the map of
+ * apply_BCs_euler: the x component negated across a reflective x side, the y
+ * component across a reflective y side */
+void or_apply_bcs_vector(const or_grid* G, double* v) { apply_bcs_n(G, v, 2, 0, 1); }
 
 /* ------------------------------------------------------------------ */
 /* A4: FS_MUSCL.cpp:148-327                                             */
@@ -950,6 +958,74 @@ int or_extrapolate_vector(const or_grid* G, const double* phi, int N,
 }
 
 /* ------------------------------------------------------------------ */
+/* extrapolate_extension_vfield.cpp:4-122: N Jacobi iterations carrying the */
+/* velocity of the interfacial cells (held) into every other interior cell  */
+/* along the normal, away from the interface on both sides                  */
+/* ------------------------------------------------------------------ */
+void or_extrapolate_extension_vfield(const or_grid* G, const double* phi, int N, double* vel)
+{
+    static _Thread_local buf_t b;
+    size_t n = (size_t)H_OF(G) * W_OF(G) * 2;
+    double* up = buf_get(&b, n);
+    double dt = 0.5 * fmin(G->dx, G->dy);                 /* :15 */
+    memcpy(up, vel, n * sizeof(double));                  /* :23-29 */
+    for (int num = 0; num < N; ++num) {
+        memcpy(vel, up, n * sizeof(double));              /* :34-40 */
+        for (int i = G->g; i < G->Ny + G->g; ++i)
+            for (int j = G->g; j < G->Nx + G->g; ++j) {
+                if (or_ls_is_interfacial(G, phi, i, j)) continue;   /* :46 */
+                double lsval = phi[IDX(G, i, j)];
+                double nrm[2];
+                or_ls_normal(G, phi, i, j, nrm);
+                double m = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1]);
+                if (!(m > 1e-12)) continue;               /* :52 */
+                nrm[0] /= m;
+                nrm[1] /= m;
+                const double* c = vel + IDX(G, i, j) * 2;
+                const double* e = vel + IDX(G, i, j + 1) * 2;
+                const double* w = vel + IDX(G, i, j - 1) * 2;
+                const double* no = vel + IDX(G, i + 1, j) * 2;
+                const double* so = vel + IDX(G, i - 1, j) * 2;
+                double dvx[2], dvy[2];
+                double* o = up + IDX(G, i, j) * 2;
+                if (lsval <= 0.0) {                       /* :59-83: along -normal */
+                    for (int k = 0; k < 2; ++k) {
+                        dvx[k] = nrm[0] > 0.0 ? e[k] - c[k] : c[k] - w[k];
+                        dvy[k] = nrm[1] > 0.0 ? no[k] - c[k] : c[k] - so[k];
+                        o[k] = c[k] + (dt / G->dx) * nrm[0] * dvx[k] + (dt / G->dy) * nrm[1] * dvy[k];
+                    }
+                } else {                                  /* :85-109: along +normal */
+                    for (int k = 0; k < 2; ++k) {
+                        dvx[k] = nrm[0] > 0.0 ? c[k] - w[k] : e[k] - c[k];
+                        dvy[k] = nrm[1] > 0.0 ? c[k] - so[k] : no[k] - c[k];
+                        o[k] = c[k] - (dt / G->dx) * nrm[0] * dvx[k] - (dt / G->dy) * nrm[1] * dvy[k];
+                    }
+                }
+            }
+        or_apply_bcs_vector(G, up);                       /* :116 */
+    }
+    memcpy(vel, up, n * sizeof(double));                  /* :121-127 */
+}
+
+/* the velocity field of a ghost fill with use_extension_vfield set: zero,
+ * the interfacial cells' own value (iv, two per cell, set where a cell is
+ * interfacial), then 20 extension iterations (e.g. GFM_riemann.cpp:40,
+ * :145, :152-155) */
+static void store_vfield_extension(const or_grid* G, const double* phi, const double* iv, double* vf)
+{
+    size_t n = (size_t)H_OF(G) * W_OF(G);
+    memset(vf, 0, n * 2 * sizeof(double));
+    for (int i = G->g; i < G->Ny + G->g; ++i)
+        for (int j = G->g; j < G->Nx + G->g; ++j)
+            if (or_ls_is_interfacial(G, phi, i, j)) {
+                size_t q = IDX(G, i, j);
+                vf[q * 2] = iv[q * 2];
+                vf[q * 2 + 1] = iv[q * 2 + 1];
+            }
+    or_extrapolate_extension_vfield(G, phi, 20, vf);
+}
+
+/* ------------------------------------------------------------------ */
 /* A7: GFM_base.cpp:4-33                                                */
 /* ------------------------------------------------------------------ */
 static void probe_states(const or_grid* G, const double* prims1, const double* prims2,
@@ -1000,17 +1076,22 @@ static void store_vfield_real(const or_grid* G, const double* phi, const double*
     }
 }
 
-/* A8-A11 */
-int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const double* phi,
-                        double* grid1, double* grid2, int* bb1, int* bb2, double* vfield)
+/* A8-A11, with the GFM_base flags use_extension_vfield and use_slip_BCs
+ * (GFM_base.hpp:20-21, defaults true and false; sim_twofluid.cpp:551-552
+ * sets both false) */
+int or_set_ghost_states_flags(const or_grid* G, const or_eos* E, int variant, const double* phi,
+                              double* grid1, double* grid2, int* bb1, int* bb2, double* vfield,
+                              int use_extension_vfield, int use_slip_BCs)
 {
-    static _Thread_local buf_t bp1, bp2, bn1, bn2;
+    static _Thread_local buf_t bp1, bp2, bn1, bn2, biv;
     int st = 0;
     int H = H_OF(G), Wd = W_OF(G), g = G->g;
     size_t ncell = (size_t)H * Wd;
     double* prims1 = buf_get(&bp1, ncell * 4);
     double* prims2 = buf_get(&bp2, ncell * 4);
+    double* iv = buf_get(&biv, ncell * 2);   /* newvelocities of the interfacial cells */
     double ds = fmin(G->dx, G->dy);
+    if (variant == OR_GFM_ORIGINAL && use_slip_BCs) return OR_ST_BADARG;   /* GFM_original.cpp:21 */
 
     for (size_t q = 0; q < ncell; ++q) {
         if (variant == OR_GFM_ORIGINAL || variant == OR_GFM_RIEMANN) {
@@ -1033,6 +1114,7 @@ int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const do
                 double* P1 = prims1 + q * 4;
                 double* P2 = prims2 + q * 4;
                 if (phi[q] <= 0.0) {
+                    iv[q * 2] = P1[1]; iv[q * 2 + 1] = P1[2];   /* :67-71 (used where interfacial) */
                     double pn = P1[3];
                     double rn = or_eos_isentropic(E->gamma2, E->pinf2, P2[0], P2[3], pn);
                     if (!(rn >= 0.0)) st |= OR_ST_UNPHYSICAL;
@@ -1042,6 +1124,7 @@ int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const do
                     if (j - 2 < bb1[2]) bb1[2] = j - 2;
                     if (j + 2 > bb1[3]) bb1[3] = j + 2;
                 } else {
+                    iv[q * 2] = P2[1]; iv[q * 2 + 1] = P2[2];   /* :93-97 */
                     double pn = P2[3];
                     double rn = or_eos_isentropic(E->gamma1, E->pinf1, P1[0], P1[3], pn);
                     if (!(rn >= 0.0)) st |= OR_ST_UNPHYSICAL;
@@ -1065,7 +1148,8 @@ int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const do
             or_prims_to_cons(E->gamma2, E->pinf2, prims2 + q * 4, grid2 + q * 4);
             if (!is_physical_state(grid1 + q * 4) || !is_physical_state(grid2 + q * 4)) st |= OR_ST_UNPHYSICAL;
         }
-        store_vfield_real(G, phi, prims1, prims2, vfield); /* :147-163 */
+        if (use_extension_vfield) store_vfield_extension(G, phi, iv, vfield);   /* :141-144 */
+        else store_vfield_real(G, phi, prims1, prims2, vfield);                 /* :147-163 */
         return st;
     }
 
@@ -1113,9 +1197,12 @@ int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const do
                     double* P = prims2 + IDX(G, i, j) * 4;
                     P[0] = star[3]; P[1] = star[1] * nrm[0] + t2[0]; P[2] = star[1] * nrm[1] + t2[1]; P[3] = star[0];
                 }
+                iv[IDX(G, i, j) * 2] = star[1] * nrm[0];   /* :205 */
+                iv[IDX(G, i, j) * 2 + 1] = star[1] * nrm[1];
             }
         st |= or_extrapolate_vector(G, phi, 6, prims1, prims2, 0);
-        store_vfield_real(G, phi, prims1, prims2, vfield);
+        if (use_extension_vfield) store_vfield_extension(G, phi, iv, vfield);   /* :212-215 */
+        else store_vfield_real(G, phi, prims1, prims2, vfield);
         for (size_t q = 0; q < ncell; ++q) {
             or_prims_to_cons(E->gamma1, E->pinf1, prims1 + q * 4, grid1 + q * 4);
             or_prims_to_cons(E->gamma2, E->pinf2, prims2 + q * 4, grid2 + q * 4);
@@ -1143,9 +1230,12 @@ int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const do
                     double* P = prims2 + IDX(G, i, j) * 4;
                     P[0] = star[3]; P[1] = star[1] * nrm[0] + t2[0]; P[2] = star[1] * nrm[1] + t2[1]; P[3] = star[0];
                 }
+                iv[IDX(G, i, j) * 2] = star[1] * nrm[0];   /* :145 */
+                iv[IDX(G, i, j) * 2 + 1] = star[1] * nrm[1];
             }
         st |= or_extrapolate_vector(G, phi, 6, prims1, prims2, 0);
-        store_vfield_real(G, phi, prims1, prims2, vfield);
+        if (use_extension_vfield) store_vfield_extension(G, phi, iv, vfield);   /* :152-155 */
+        else store_vfield_real(G, phi, prims1, prims2, vfield);
         for (size_t q = 0; q < ncell; ++q) {
             or_prims_to_cons(E->gamma1, E->pinf1, prims1 + q * 4, grid1 + q * 4);
             or_prims_to_cons(E->gamma2, E->pinf2, prims2 + q * 4, grid2 + q * 4);
@@ -1170,9 +1260,13 @@ int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const do
                 probe_states(G, prims1, prims2, phi, i, j, ds, ip2, ip1);
                 double star[4], t1[2], t2[2];
                 st |= interface_mrp(E, ip1, ip2, nrm, star, t1, t2);
-                /* no-slip: both star velocities take the real fluid's tangential part (:152-169) */
-                const double* tr = lsval <= 0.0 ? t1 : t2;
+                /* :152-169: with slip BCs each fluid keeps its own tangential
+                 * part, else both take the real fluid's; the real fluid's
+                 * velocity is not written, so only the ghost's matters */
+                const double* tr = lsval <= 0.0 ? (use_slip_BCs ? t2 : t1) : (use_slip_BCs ? t1 : t2);
                 double usx = star[1] * nrm[0] + tr[0], usy = star[1] * nrm[1] + tr[1];
+                iv[IDX(G, i, j) * 2] = star[1] * nrm[0];   /* :196 */
+                iv[IDX(G, i, j) * 2 + 1] = star[1] * nrm[1];
                 size_t q = IDX(G, i, j);
                 if (lsval <= 0.0) {
                     np1[q * 4] = or_eos_isentropic(E->gamma1, E->pinf1, star[2], star[0], prims1[q * 4 + 3]);
@@ -1190,7 +1284,8 @@ int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const do
 #else
         st |= or_extrapolate_vector(G, phi, 6, prims1, prims2, 1);   /* :202, result unused below */
 #endif
-        store_vfield_real(G, phi, prims1, prims2, vfield);
+        if (use_extension_vfield) store_vfield_extension(G, phi, iv, vfield);   /* :204-207 */
+        else store_vfield_real(G, phi, prims1, prims2, vfield);
         for (size_t q = 0; q < ncell; ++q) {               /* :230-239 writes new_prims */
             or_prims_to_cons(E->gamma1, E->pinf1, np1 + q * 4, grid1 + q * 4);
             or_prims_to_cons(E->gamma2, E->pinf2, np2 + q * 4, grid2 + q * 4);
@@ -1199,6 +1294,12 @@ int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const do
         return st;
     }
     return OR_ST_BADARG;
+}
+
+int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const double* phi,
+                        double* grid1, double* grid2, int* bb1, int* bb2, double* vfield)
+{
+    return or_set_ghost_states_flags(G, E, variant, phi, grid1, grid2, bb1, bb2, vfield, 0, 0);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1243,7 +1344,8 @@ int or_twofluid_step(const or_grid* G, const or_eos* E, const or_run* R, int ste
     double cfl = step_index < 5 ? fmin(R->cfl, 0.2) : R->cfl;             /* :66 */
     double dt = or_compute_dt_twofluid(G, E, cfl, phi, grid1, grid2, R->T, *t);
     int bb1[4], bb2[4];
-    st |= or_set_ghost_states(G, E, R->gfm, phi, grid1, grid2, bb1, bb2, vfield);
+    st |= or_set_ghost_states_flags(G, E, R->gfm, phi, grid1, grid2, bb1, bb2, vfield,
+                                    R->use_extension_vfield, R->use_slip_BCs);
     if (R->fs == OR_FS_GODUNOV) {
         st |= or_fs_godunov(G, E->gamma1, E->pinf1, dt, bb1, grid1, scratch1);
         st |= or_fs_godunov(G, E->gamma2, E->pinf2, dt, bb2, grid2, scratch2);

@@ -50,6 +50,8 @@ typedef struct {
     double cfl;
     double T;           /* end time (loop runs while t < T and steps < max_steps) */
     int max_steps;
+    int use_extension_vfield;   /* GFM_base flags (GFM_base.hpp:20-21); 0 as sim_twofluid.cpp:551 */
+    int use_slip_BCs;
 } or_run;
 
 /* A1/A2: EOS + conversions */
@@ -76,6 +78,7 @@ int or_mixed_rs(double gL, double pinfL, double gR, double pinfR,
 /* A13: boundary conditions */
 void or_apply_bcs(const or_grid* G, double* grid);
 void or_apply_bcs_scalar(const or_grid* G, double* phi);
+void or_apply_bcs_vector(const or_grid* G, double* vfield);
 
 /* A4/A5: flux solvers; bbox = {imin, imax, jmin, jmax} (half-open) */
 int or_fs_muscl(const or_grid* G, double g, double pinf, double dt, const int* bbox,
@@ -105,6 +108,11 @@ int or_extrapolate_vector(const or_grid* G, const double* phi, int N,
 /* A7-A11 */
 int or_set_ghost_states(const or_grid* G, const or_eos* E, int variant, const double* phi,
                         double* grid1, double* grid2, int* bbox1, int* bbox2, double* vfield);
+int or_set_ghost_states_flags(const or_grid* G, const or_eos* E, int variant, const double* phi,
+                              double* grid1, double* grid2, int* bbox1, int* bbox2, double* vfield,
+                              int use_extension_vfield, int use_slip_BCs);
+/* extrapolate_extension_vfield.cpp:4-122 */
+void or_extrapolate_extension_vfield(const or_grid* G, const double* phi, int N, double* vfield);
 
 /* A17: step orchestration */
 int or_onefluid_step(const or_grid* G, double g, double pinf, int fs, double cfl_eff,

@@ -82,6 +82,8 @@ struct gfm_ctx {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     bool tb = false;                   // temporally blocked kernels (no x-periodic instance)
     bool mgfm_extrapolate = false;     // GFM_MGFM_EXTRAPOLATE=1 (GFM_modified only; not the reference)
+    bool use_extension_vfield = false; // GFM_base::use_extension_vfield (sim_twofluid.cpp:551 sets it false)
+    bool use_slip_BCs = false;         // GFM_base::use_slip_BCs
     // two-fluid step path: the instances in groups, each group's step on its
     // own stream (group 0 on `stream`), forked from and joined back to
     // `stream`, so that the latency-bound phases of one group (the ordered
@@ -185,6 +187,9 @@ int ghost_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int counted = 0, hipEvent_t
         }
         default: break;
     }
+    // the extension velocity field replaces the real fluid's velocity the
+    // chain wrote (e.g. GFM_riemann.cpp:152-174)
+    if (c->use_extension_vfield) gfm::launch_vext<R>(L, c->m.gfm == GFM_ORIGINAL);
     return GFM_OK;
 }
 
@@ -542,6 +547,16 @@ int gfm_instance_add(gfm_ctx* c, const gfm_sim_info* info, const gfm_eos* eos, d
 
 int gfm_num_instances(const gfm_ctx* c) { return c ? (int)c->inst.size() : 0; }
 
+int gfm_set_gfm_flags(gfm_ctx* c, int use_extension_vfield, int use_slip_BCs) {
+    if (!c) return fail_arg("null ctx");
+    if (c->committed) return fail_arg("the GFM flags are set before gfm_ctx_commit");
+    if (!c->twofluid() && (use_extension_vfield || use_slip_BCs)) return fail_arg("GFM flags need a two-fluid context");
+    if (use_slip_BCs && c->m.gfm == GFM_ORIGINAL) return fail_arg("GFM_original has no slip BCs (GFM_original.cpp:21)");
+    c->use_extension_vfield = use_extension_vfield != 0;
+    c->use_slip_BCs = use_slip_BCs != 0;
+    return GFM_OK;
+}
+
 int gfm_ctx_commit(gfm_ctx* c) {
     if (!c) return fail_arg("null ctx");
     if (c->committed) return GFM_OK;
@@ -566,6 +581,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         for (int f = 0; f < 2; ++f) gfm::rs_fluid(d.rsf[f], d.gam[f], d.pinf[f]);
         d.cfl = h.cfl; d.T = h.T; d.max_steps = h.max_steps;
         d.use_bbox = 1;   // boxes are reset to the full domain each step unless OGFM narrows them
+        d.use_slip = c->use_slip_BCs ? 1 : 0;
         d.ntile_dt = s.Ny * ((s.Nx + 255) / 256);   // dt partials: one per 256-cell row segment
         d.icap = c->twofluid() ? std::min(std::min(s.Nx * s.Ny, 16 * (s.Nx + s.Ny) + 1024), 49152) : 1;
         // layout of the single allocation

@@ -78,6 +78,7 @@ struct Inst {
     int ntile_dt;
     int icap;                  // interfacial list capacity
     int use_bbox;              // OGFM narrows the per-fluid update boxes
+    int use_slip;              // GFM_base::use_slip_BCs (GFM_modified: the ghost keeps its tangential velocity)
     void* Q[2][2];             // conserved [fluid][buffer]
     void* P[2][2];             // primitives [fluid][buffer]
     void* phi;

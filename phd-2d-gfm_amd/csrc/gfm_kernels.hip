@@ -1461,9 +1461,10 @@ __device__ __forceinline__ unsigned iface_finish(const Inst& I, const Rec& r, co
     double t1[2], t2[2];
     unsigned st = iface_mrp<FAST>(I, ip1, ip2, r.nx, r.ny, S, t1, t2, p0);
     if (VAR == V_MODIFIED) {
-        // no-slip: the real fluid keeps its state except the density, the
-        // ghost takes the star state with the real fluid's tangential velocity
-        const double* tr = r.lsval <= 0.0 ? t1 : t2;
+        // the real fluid keeps its state except the density; the ghost takes
+        // the star state with the real fluid's tangential velocity (no-slip)
+        // or, with slip BCs, its own (GFM_modified.cpp:152-169)
+        const double* tr = r.lsval <= 0.0 ? (I.use_slip ? t2 : t1) : (I.use_slip ? t1 : t2);
         const int fr = r.lsval <= 0.0 ? 0 : 1;
         const double rs_real = fr == 0 ? S.rl : S.rr;
         out[0] = rs_real * rpow<FAST>((r.preal + I.pinf[fr]) / (S.p + I.pinf[fr]), 1.0 / I.gam[fr]);
@@ -1476,6 +1477,8 @@ __device__ __forceinline__ unsigned iface_finish(const Inst& I, const Rec& r, co
     } else {
         out[0] = S.rr; out[1] = S.u * r.nx + t2[0]; out[2] = S.u * r.ny + t2[1]; out[3] = S.p;
     }
+    if (VAR != V_MODIFIED) out[4] = S.p;
+    out[5] = S.u;   // u* n is the cell's extension velocity (e.g. GFM_riemann.cpp:145)
     if (S.p != S.p) st |= ST_RS;
     *pstar = S.p;
     return st;
@@ -1601,7 +1604,7 @@ __global__ __launch_bounds__(256) void k_iface_prep(const Inst* __restrict__ ins
     gmem(I.idone)[k] = has_dep ? 0 : 1;
     if (!has_dep) {
         const auto o = gmem(I.side) + 8 * (long long)k;
-        for (int q = 0; q < 5; ++q) o[q] = out[q];
+        for (int q = 0; q < 6; ++q) o[q] = out[q];
     }
     flag(I, st);
 }
@@ -1614,7 +1617,7 @@ __device__ __forceinline__ unsigned iface_solve_pending(const Inst& I, const Rec
     probe_values_side<VAR>(I, r, ip1, ip2);
     const unsigned st = iface_finish<VAR, FAST>(I, r, ip1, ip2, r.pspec, out, &ps);
     const auto o = gmem(I.side) + 8 * (long long)k;
-    for (int q = 0; q < 4; ++q) o[q] = out[q];
+    for (int q = 0; q < 6; ++q) o[q] = out[q];
     return st;
 }
 
@@ -1769,6 +1772,135 @@ __global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ ins
     }
     st4<R>(I.Q[0][qout], I, c, prims2cons<R>((R)I.gam[0], (R)I.pinf[0], W1));
     st4<R>(I.Q[1][qout], I, c, prims2cons<R>((R)I.gam[1], (R)I.pinf[1], W2));
+}
+
+// ---------------------------------------------------------------------------
+// Extension velocity field (GFM_base::use_extension_vfield, GFM_base.hpp:20;
+// extrapolate_extension_vfield.cpp:4-122, called with 20 iterations at
+// GFM_original.cpp:141-144, GFM_real.cpp:212-215, GFM_riemann.cpp:152-155,
+// GFM_modified.cpp:204-207).  The field starts at zero except in interior
+// interfacial cells, which hold u* n from the interfacial solve
+// (GFM_real.cpp:205, GFM_riemann.cpp:145, GFM_modified.cpp:196) or, for
+// GFM_original, the real fluid's velocity (GFM_original.cpp:67-71, :93-97).
+// Each Jacobi iteration moves every other interior cell's value along the
+// normal, away from the interface on both sides; a cell whose phi gradient
+// vanishes keeps its value.  BCs are folded in as a gather: a ghost cell takes
+// its BC source cell's new value with the component normal to a reflective
+// side negated (the external apply_gridVector2dtype_BCs, restated with the
+// map of apply_BCs_euler).  The two planes ping-pong through P[0][0] and
+// P[0][1], free once the fill has written the conserved grids; the last
+// iteration writes the velocity field.  work: full-array 64x4 tiles
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ bool iface_cell(const Inst& I, int i, int j) {
+    const auto ph = gmem((const R*)I.phi);
+    const long long c = cidx(I, i, j);
+    const bool s0 = ph[c] <= R(0);
+    return ((ph[c + 1] <= R(0)) != s0) || ((ph[c - 1] <= R(0)) != s0) || ((ph[c + I.pitch] <= R(0)) != s0) ||
+           ((ph[c - I.pitch] <= R(0)) != s0);
+}
+
+template <typename R, bool OGFM>
+__global__ __launch_bounds__(256) void k_vext_init(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    const long long c = cidx(I, i, j);
+    R vx = R(0), vy = R(0);
+    if (interior(I, i, j) && iface_cell<R>(I, i, j)) {
+        if (OGFM) {
+            const int fr = ld1<R>(I.phi, c) <= R(0) ? 0 : 1;
+            V4<R> W;
+            if constexpr (sizeof(R) == 4) W = cons2prims_f(fluid_f(I.rsf[fr]), ld4<R>(I.Q[fr][0], I, c));
+            else W = cons2prims<R>((R)I.gam[fr], (R)I.pinf[fr], ld4<R>(I.Q[fr][0], I, c));
+            vx = W.v[1];
+            vy = W.v[2];
+        } else {
+            const int k = gmem(I.imap)[c];
+            if (k >= 0 && k < I.icap) {
+                const double us = gmem(I.side)[8 * (long long)k + 5];
+                const auto r = gmem((const IfRec*)I.irec) + k;
+                vx = (R)(us * r->nx);
+                vy = (R)(us * r->ny);
+            } else {
+                flag(I, ST_ILIST);
+            }
+        }
+    }
+    const auto out = gmem((R*)I.P[0][0]);
+    out[c] = vx;
+    out[c + I.plane] = vy;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_vext_iter(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                   int vin, int last) {
+#pragma clang fp contract(off)
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    bool fv, fu;
+    const int si = src_row(I, i, fv);
+    const int sj = src_col(I, j, fu);
+    const auto v = gmem((const R*)I.P[0][vin]);
+    const long long cs = cidx(I, si, sj), P2 = I.plane;
+    R vx = v[cs], vy = v[cs + P2];
+    if (!iface_cell<R>(I, si, sj)) {   // extrapolate_extension_vfield.cpp:46
+        const auto ph = gmem((const R*)I.phi);
+        const R pe = ph[cs + 1], pw = ph[cs - 1], pn = ph[cs + I.pitch], ps = ph[cs - I.pitch];
+        R nx, ny;
+        bool ok;
+        if constexpr (sizeof(R) == 8) {
+            nx = (pe - pw) / (2.0 * I.dx);
+            ny = (pn - ps) / (2.0 * I.dy);
+            const double m = sqrt(nx * nx + ny * ny);
+            ok = m > 1e-12;   // :52
+            if (ok) { nx /= m; ny /= m; }
+        } else {
+            nx = (pe - pw) * (float)(0.5 / I.dx);
+            ny = (pn - ps) * (float)(0.5 / I.dy);
+            const float m2 = nx * nx + ny * ny;
+            ok = m2 > 1e-24f;
+            if (ok) { const float r = rsqrtf(m2); nx *= r; ny *= r; }
+        }
+        if (ok) {
+            const double pdt = 0.5 * fmin(I.dx, I.dy);   // :15
+            const R ax = (R)(pdt / I.dx) * nx, ay = (R)(pdt / I.dy) * ny;
+            const R ve[2] = {v[cs + 1], v[cs + 1 + P2]}, vw[2] = {v[cs - 1], v[cs - 1 + P2]};
+            const R vn[2] = {v[cs + I.pitch], v[cs + I.pitch + P2]}, vs[2] = {v[cs - I.pitch], v[cs - I.pitch + P2]};
+            R o[2];
+            const R c0[2] = {vx, vy};
+            if (ph[cs] <= R(0)) {
+                // information along -n (:59-83)
+                for (int k = 0; k < 2; ++k) {
+                    const R dx = nx > R(0) ? ve[k] - c0[k] : c0[k] - vw[k];
+                    const R dy = ny > R(0) ? vn[k] - c0[k] : c0[k] - vs[k];
+                    o[k] = (c0[k] + ax * dx) + ay * dy;
+                }
+            } else {
+                // information along +n (:85-109)
+                for (int k = 0; k < 2; ++k) {
+                    const R dx = nx > R(0) ? c0[k] - vw[k] : ve[k] - c0[k];
+                    const R dy = ny > R(0) ? c0[k] - vs[k] : vn[k] - c0[k];
+                    o[k] = (c0[k] - ax * dx) - ay * dy;
+                }
+            }
+            vx = o[0];
+            vy = o[1];
+        }
+    }
+    if (fu) vx = -vx;
+    if (fv) vy = -vy;
+    const auto out = gmem((R*)(last ? I.vel : I.P[0][1 - vin]));
+    const long long c = cidx(I, i, j);
+    out[c] = vx;
+    out[c + P2] = vy;
 }
 
 // ---------------------------------------------------------------------------
@@ -2475,6 +2607,16 @@ void launch_mgfm_final(const LaunchCtx& L, int qin, int qout, bool prims_out) {
         hipLaunchKernelGGL((k_mgfm_final<R, false>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, qin, qout);
 }
 
+template <typename R>
+void launch_vext(const LaunchCtx& L, bool ogfm) {
+    if (ogfm) hipLaunchKernelGGL((k_vext_init<R, true>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full);
+    else hipLaunchKernelGGL((k_vext_init<R, false>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full);
+    constexpr int N = 20;   // extrapolate_extension_vfield(params, ls, 20, newvelocities)
+    for (int k = 0; k < N; ++k)
+        hipLaunchKernelGGL((k_vext_iter<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, k & 1,
+                           k == N - 1 ? 1 : 0);
+}
+
 // K reinitialisation iterations from A into phi: blocked launches of up to
 // RT_MAXIT iterations each, or one launch per iteration (x-periodic contexts)
 template <typename R>
@@ -2572,6 +2714,7 @@ void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, doub
     template bool launch_extrap_tb<R>(const LaunchCtx&, int, int, int, int, int); \
     template void launch_iface<R>(const LaunchCtx&, int, int, int);           \
     template void launch_mgfm_final<R>(const LaunchCtx&, int, int, bool);     \
+    template void launch_vext<R>(const LaunchCtx&, bool);                       \
     template void launch_ls<R>(const LaunchCtx&, const Houc&, int, int);      \
     template void launch_reinit_now<R>(const LaunchCtx&, int);                \
     template void launch_bc<R>(const LaunchCtx&, int, int, int);               \

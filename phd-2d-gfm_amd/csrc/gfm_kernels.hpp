@@ -35,6 +35,10 @@ template <typename R> void launch_extrap(const LaunchCtx& L, int mode, int bin, 
 template <typename R> bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout);
 template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted);
 template <typename R> void launch_mgfm_final(const LaunchCtx& L, int qin, int qout, bool prims_out = false);
+// the extension velocity field of a ghost fill (use_extension_vfield): the
+// interfacial cells' velocity carried into the rest of the grid by 20
+// iterations, written to the velocity field
+template <typename R> void launch_vext(const LaunchCtx& L, bool ogfm);
 template <typename R> void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters);
 template <typename R> void launch_reinit_now(const LaunchCtx& L, int iters);
 template <typename R> void launch_bc(const LaunchCtx& L, int q, int nfluid, int advance);

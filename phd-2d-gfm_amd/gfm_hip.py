@@ -26,6 +26,7 @@ GFM_NAMES = {"OGFM": GFM_ORIGINAL, "realGFM": GFM_REAL, "riemannGFM": GFM_RIEMAN
 # every entry point declared in include/gfm_hip.h
 EXPORTS = [
     "gfm_ctx_create", "gfm_ctx_destroy", "gfm_instance_add", "gfm_ctx_commit", "gfm_num_instances",
+    "gfm_set_gfm_flags",
     "gfm_upload_state", "gfm_download_state", "gfm_upload_vfield", "gfm_download_vfield",
     "gfm_set_time", "gfm_get_time", "gfm_set_dt",
     "gfm_compute_dt", "gfm_set_ghost_states", "gfm_fs_update", "gfm_ls_update", "gfm_apply_bcs",
@@ -82,6 +83,7 @@ def load():
         "gfm_instance_add": [vp, C.POINTER(SimInfo), C.POINTER(Eos), C.c_double, C.c_double, C.c_int, _I],
         "gfm_ctx_commit": [vp],
         "gfm_num_instances": [vp],
+        "gfm_set_gfm_flags": [vp, C.c_int, C.c_int],
         "gfm_upload_state": [vp, C.c_int, _D, _D, _D],
         "gfm_download_state": [vp, C.c_int, _D, _D, _D],
         "gfm_upload_vfield": [vp, C.c_int, _D],
@@ -168,6 +170,10 @@ class Context:
                "gfm_instance_add")
         self.infos.append(info)
         return k.value
+
+    def set_gfm_flags(self, use_extension_vfield=False, use_slip_BCs=False):
+        """GFM_base::use_extension_vfield / use_slip_BCs, before commit"""
+        _check(load().gfm_set_gfm_flags(self.h, int(use_extension_vfield), int(use_slip_BCs)), "gfm_set_gfm_flags")
 
     def commit(self):
         _check(load().gfm_ctx_commit(self.h), "gfm_ctx_commit")

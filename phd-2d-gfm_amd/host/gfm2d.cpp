@@ -100,18 +100,21 @@ static std::vector<double> ls_flat(const sim_info& p, const GFM_ITM_interface& l
 
 // ---------------------------------------------------------------- context cache
 // Phase-by-phase drop-ins reuse one committed context per (methods, grid, EOS).
-static gfm_ctx* cached_ctx(const gfm_methods& m, const sim_info& p, const gfm_eos& e) {
+// ext, slip: GFM_base::use_extension_vfield / use_slip_BCs (two-fluid contexts)
+static gfm_ctx* cached_ctx(const gfm_methods& m, const sim_info& p, const gfm_eos& e, bool ext = false,
+                           bool slip = false) {
     static std::map<std::string, gfm_ctx*> cache;   // contexts live until exit
     char key[512];
     const gfm_sim_info s = p.c_info();
-    std::snprintf(key, sizeof key, "%d %d %d %d %d %d %d|%d %d %d %.17g %.17g %.17g %.17g %d %d %d %d|%.17g %.17g %.17g %.17g",
+    std::snprintf(key, sizeof key, "%d %d %d %d %d %d %d|%d %d %d %.17g %.17g %.17g %.17g %d %d %d %d|%.17g %.17g %.17g %.17g|%d %d",
                   m.sim_type, m.fs, m.gfm, m.ls_order, m.reinit_every, m.reinit_iters, m.precision, s.Nx, s.Ny,
                   s.numGC, s.x0, s.y0, s.dx, s.dy, s.BC_L, s.BC_R, s.BC_B, s.BC_T, e.gamma1, e.pinf1, e.gamma2,
-                  e.pinf2);
+                  e.pinf2, ext ? 1 : 0, slip ? 1 : 0);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     gfm_ctx* c = nullptr;
     check(gfm_ctx_create(&m, &c), "gfm_ctx_create");
+    if (ext || slip) check(gfm_set_gfm_flags(c, ext ? 1 : 0, slip ? 1 : 0), "gfm_set_gfm_flags");
     check(gfm_instance_add(c, &s, &e, 0.5, 1e300, 1 << 30, nullptr), "gfm_instance_add");
     check(gfm_ctx_commit(c), "gfm_ctx_commit");
     cache[key] = c;
@@ -214,7 +217,8 @@ void Levelset::update_interface_coupled(const solved_velocity_field& vf, double 
 void GFM_base::run_variant(int variant, const sim_info& p, const GFM_ITM_interface& ls, const binarySGparams& eos,
                            double t, grideuler2type& grid1, grideuler2type& grid2, BBrange& rc1, BBrange& rc2) {
     gfm_eos e{eos.gamma1, eos.pinf1, eos.gamma2, eos.pinf2};
-    gfm_ctx* c = cached_ctx(methods_of(GFM_SIM_TWOFLUID, GFM_FS_MUSCL, variant), p, e);
+    gfm_ctx* c = cached_ctx(methods_of(GFM_SIM_TWOFLUID, GFM_FS_MUSCL, variant), p, e, use_extension_vfield,
+                            use_slip_BCs);
     std::vector<double> g1 = flatten(grid1), g2 = flatten(grid2), phi = ls_flat(p, ls);
     check(gfm_upload_state(c, 0, g1.data(), g2.data(), phi.data()), "gfm_upload_state");
     check(gfm_set_ghost_states(c), "gfm_set_ghost_states");

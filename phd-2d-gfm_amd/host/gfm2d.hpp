@@ -157,6 +157,8 @@ public:
 // ---------------------------------------------------------------- ghost fluid methods
 class GFM_base {
 public:
+    // GFM_base.hpp:20-21 and the constructor's defaults there; forwarded to
+    // the library (gfm_set_gfm_flags) by every set_ghost_states call
     bool use_extension_vfield = true;
     bool use_slip_BCs = false;
     std::shared_ptr<solved_velocity_field> vfield;

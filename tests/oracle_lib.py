@@ -38,7 +38,8 @@ class Eos(C.Structure):
 class Run(C.Structure):
     _fields_ = [("gfm", C.c_int), ("fs", C.c_int), ("ls_order", C.c_int),
                 ("reinit_every", C.c_int), ("reinit_iters", C.c_int),
-                ("cfl", C.c_double), ("T", C.c_double), ("max_steps", C.c_int)]
+                ("cfl", C.c_double), ("T", C.c_double), ("max_steps", C.c_int),
+                ("use_extension_vfield", C.c_int), ("use_slip_BCs", C.c_int)]
 
 
 _lib = None
@@ -93,6 +94,9 @@ def _bind(L):
         "or_mixed_rs": (i32, [dbl, dbl, dbl, dbl, D, D, D]),
         "or_apply_bcs": (None, [C.POINTER(Grid), D]),
         "or_apply_bcs_scalar": (None, [C.POINTER(Grid), D]),
+        "or_apply_bcs_vector": (None, [C.POINTER(Grid), D]),
+        "or_extrapolate_extension_vfield": (None, [C.POINTER(Grid), D, i32, D]),
+        "or_set_ghost_states_flags": (i32, [C.POINTER(Grid), C.POINTER(Eos), i32, D, D, D, I, I, D, i32, i32]),
         "or_fs_muscl": (i32, [C.POINTER(Grid), dbl, dbl, dbl, I, D, D]),
         "or_fs_godunov": (i32, [C.POINTER(Grid), dbl, dbl, dbl, I, D, D]),
         "or_compute_dt_onefluid": (dbl, [C.POINTER(Grid), dbl, dbl, dbl, D, dbl, dbl]),

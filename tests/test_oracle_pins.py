@@ -228,3 +228,37 @@ def test_uniform_flow_stays_uniform(oracle, variant):
         dev = np.abs(a[g:g + 40, g:g + 48] - U).reshape(-1, 4).max(0) / np.abs(U).max()
         assert dev.max() <= 1e-13, dev
     assert not np.array_equal(phi <= 0, p0 <= 0)          # the interface moved with the flow
+
+
+def test_extension_vfield_planar_closed_form(oracle):
+    """extrapolate_extension_vfield (extrapolate_extension_vfield.cpp:4-122) on a planar interface
+    phi = x - 0.5 (normal (1, 0) exactly): the two interfacial columns hold (a, b), every other cell
+    starts at zero, and with the pseudo-time step dx / 2 each iteration sets a cell to the mean of
+    itself and its neighbour towards the interface.  After N iterations a cell m columns from its
+    interfacial column holds (a, b) P(Binomial(N, 1/2) >= m) on either side.  Reflective bottom and
+    top: the ghost rows mirror the rows next to them with the y component negated."""
+    from math import comb
+    g, Nx, Ny, N = 8, 60, 8, 20
+    G = O.Grid(Nx, Ny, g, 0.0, 0.0, 1.0 / Nx, 1.0 / Nx, 0, 0, 1, 1)
+    H, W = G.shape
+    x = (np.arange(W) - g + 0.5) / Nx
+    phi = np.ascontiguousarray(np.broadcast_to(x - 0.5, (H, W)).copy())
+    jl, jr = g + Nx // 2 - 1, g + Nx // 2          # the interfacial columns, phi < 0 and phi > 0
+    a, b = 0.7, -0.3
+    vel = np.zeros((H, W, 2))
+    vel[g:g + Ny, jl] = (a, b)
+    vel[g:g + Ny, jr] = (a, b)
+    O.lib().or_extrapolate_extension_vfield(G, O._p(phi), N, O._p(vel))
+    tail = lambda m: sum(comb(N, k) for k in range(m, N + 1)) / 2.0 ** N  # noqa: E731
+    want = np.zeros(W)
+    for j in range(g, g + Nx):
+        m = jl - j if j <= jl else j - jr
+        want[j] = tail(m)
+    for i in range(g, g + Ny):
+        assert np.abs(vel[i, g:g + Nx, 0] - a * want[g:g + Nx]).max() <= 1e-15
+        assert np.abs(vel[i, g:g + Nx, 1] - b * want[g:g + Nx]).max() <= 1e-15
+    assert vel[g, jr + N + 1, 0] == 0.0 and vel[g, jl - N - 1, 0] == 0.0 and vel[g, jr + N, 0] > 0.0   # N cells reach
+    for k in range(g):   # reflective y: mirrored rows, y component negated
+        assert np.array_equal(vel[g - 1 - k, :, 0], vel[g + k, :, 0])
+        assert np.array_equal(vel[g - 1 - k, :, 1], -vel[g + k, :, 1])
+        assert np.array_equal(vel[g + Ny + k, :, 1], -vel[g + Ny - 1 - k, :, 1])
