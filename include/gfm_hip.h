@@ -171,6 +171,17 @@ int gfm_mixed_rs_solve(int n, double gammaL, double pinfL, double gammaR, double
 int gfm_pure_rs_solve_f32(int n, int paralleltox, double gamma, double pinf,
                           const double* UL, const double* UR, double* flux, unsigned* status);
 
+/* pure_RS_base::solve_RP over n faces, as gfm_pure_rs_solve, without
+ * allocating: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The host arrays are staged through a
+ * fixed device array of the library (24 MiB per GPU, held until the process
+ * ends), so n is at most 262144; larger n returns GFM_E_ARG.  Calls that
+ * overlap on one GPU share that array and race; the status word is cleared
+ * by every call.  Runs on the null stream of the current device. */
+int gfm_pure_rs_solve_noalloc(int n, int paralleltox, double gamma, double pinf,
+                              const double* UL, const double* UR, double* flux, unsigned* status);
+
 /* initial conditions of the settings-file test cases (host code, no GPU):
  * sim_onefluid::set_sim_ICs (sim_onefluid.cpp:196-422; test_case 1..5 =
  * TTC1..5_x, 11..15 = TTC1..5_y, 100 = circularexplosion) and

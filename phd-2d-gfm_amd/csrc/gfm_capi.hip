@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -366,6 +367,40 @@ static int rs_common(int n, const double* UL, const double* UR, double* out, uns
     if (status) *status = st;
     return GFM_OK;
 }
+
+// Staging of the allocation-free face solver (gfm_pure_rs_solve_noalloc):
+// the status word, then the left states, right states and fluxes of up to
+// NOALLOC_MAX_FACES faces.  Every GPU the library runs on holds its own copy
+// from the load of the code object until the process ends; host code reaches
+// it only through the per-device address from hipGetSymbolAddress.
+constexpr int NOALLOC_MAX_FACES = 1 << 18;
+constexpr size_t NOALLOC_FACE_BYTES = (size_t)NOALLOC_MAX_FACES * 4 * sizeof(double);
+constexpr size_t NOALLOC_BYTES = 256 + 3 * NOALLOC_FACE_BYTES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[NOALLOC_MAX_DEVICES] = {};
+
+// the current device's copy of `scratch`, looked up once per device
+int scratch_of_current_device(unsigned char** out) {
+    int dev = 0;
+    HC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail_arg("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        HC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail_arg("no device address for the scratch array");
+        g_scratch[dev] = p;
+    }
+    *out = (unsigned char*)g_scratch[dev];
+    return GFM_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -1072,6 +1107,33 @@ int gfm_pure_rs_solve_f32(int n, int paralleltox, double gamma, double pinf, con
 int gfm_mixed_rs_solve(int n, double gL, double pL, double gR, double pR, const double* UL, const double* UR,
                        double* out4, unsigned* status) {
     return rs_common(n, UL, UR, out4, status, 1, 0, gL, pL, gR, pR);
+}
+
+int gfm_pure_rs_solve_noalloc(int n, int paralleltox, double gamma, double pinf, const double* UL, const double* UR,
+                              double* flux, unsigned* status) {
+    if (n < 0 || (n > 0 && (!UL || !UR || !flux))) return fail_arg("bad arguments");
+    if (n > NOALLOC_MAX_FACES) return fail_arg("more faces than the allocation-free solver stages (262144)");
+    if (n == 0) { if (status) *status = 0; return GFM_OK; }
+    unsigned char* base = nullptr;
+    int r = scratch_of_current_device(&base);
+    if (r) return r;
+    unsigned* dS = (unsigned*)base;
+    double* dL = (double*)(base + 256);
+    double* dR = (double*)(base + 256 + NOALLOC_FACE_BYTES);
+    double* dO = (double*)(base + 256 + 2 * NOALLOC_FACE_BYTES);
+    const size_t bytes = (size_t)n * 4 * sizeof(double);
+    const hipStream_t s = nullptr;
+    gfm::launch_fill_u32(s, dS, 1, 0u);   // the previous call's status stays in the array
+    HC(hipMemcpyAsync(dL, UL, bytes, hipMemcpyHostToDevice, s));
+    HC(hipMemcpyAsync(dR, UR, bytes, hipMemcpyHostToDevice, s));
+    gfm::launch_pure_rs(s, n, paralleltox, gamma, pinf, dL, dR, dO, dS);
+    HC(hipGetLastError());
+    unsigned st = 0;
+    HC(hipMemcpyAsync(flux, dO, bytes, hipMemcpyDeviceToHost, s));
+    HC(hipMemcpyAsync(&st, dS, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HC(hipStreamSynchronize(s));
+    if (status) *status = st;
+    return GFM_OK;
 }
 
 }  // extern "C"

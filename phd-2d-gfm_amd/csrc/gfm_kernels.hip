@@ -2695,6 +2695,15 @@ void launch_pure_rs(hipStream_t s, int n, int px, double g, double pinf, const d
     hipLaunchKernelGGL(k_pure_rs, dim3((n + 255) / 256), dim3(256), 0, s, n, px, g, pinf, UL, UR, F, st);
 }
 
+__global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+void launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v) {
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+}
+
 void launch_pure_rs_f32(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR,
                         double* F, unsigned* st) {
     RSFluid rf;

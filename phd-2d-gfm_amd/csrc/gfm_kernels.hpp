@@ -45,6 +45,8 @@ template <typename R> void launch_bc(const LaunchCtx& L, int q, int nfluid, int 
 template <typename R> void launch_reset(const LaunchCtx& L, int narrow_boxes);
 void launch_pure_rs(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR,
                     double* F, unsigned* st);
+// p[0..n) = v (a kernel, so that it may be captured in a graph)
+void launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
 // the mixed-precision path's face solver on the same problem: FP32 states
 // (the conserved inputs rounded to FP32), FP32 flux
 void launch_pure_rs_f32(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR,

@@ -32,7 +32,7 @@ EXPORTS = [
     "gfm_compute_dt", "gfm_set_ghost_states", "gfm_fs_update", "gfm_ls_update", "gfm_apply_bcs",
     "gfm_extrapolate", "gfm_ls_reinit", "gfm_get_bbox", "gfm_set_bbox",
     "gfm_step", "gfm_step_graph", "gfm_sync", "gfm_profile_steps", "gfm_status", "gfm_mass_sums", "gfm_mass_history", "gfm_iface_info",
-    "gfm_pure_rs_solve", "gfm_pure_rs_solve_f32", "gfm_mixed_rs_solve",
+    "gfm_pure_rs_solve", "gfm_pure_rs_solve_f32", "gfm_mixed_rs_solve", "gfm_pure_rs_solve_noalloc",
     "gfm_ic_onefluid", "gfm_ic_bubble", "gfm_ic_rmi", "gfm_ic_planar", "gfm_shock_state",
     "gfm_last_error",
 ]
@@ -103,6 +103,7 @@ def load():
         "gfm_iface_info": [vp, C.c_int, _I, _I],
         "gfm_pure_rs_solve": [C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _D, _U],
         "gfm_pure_rs_solve_f32": [C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _D, _U],
+        "gfm_pure_rs_solve_noalloc": [C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _D, _U],
         "gfm_mixed_rs_solve": [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _D, _D, _D, _U],
         "gfm_ic_onefluid": [C.POINTER(SimInfo), C.POINTER(Eos), C.c_int, _D],
         "gfm_ic_bubble": [C.POINTER(SimInfo), C.POINTER(Eos), C.c_double, C.c_double, C.c_double, _D, _D, _D,
@@ -275,14 +276,15 @@ class Context:
         return list(b1), list(b2)
 
 
-def pure_rs_solve(paralleltox, gamma, pinf, UL, UR, fp32=False):
+def pure_rs_solve(paralleltox, gamma, pinf, UL, UR, fp32=False, noalloc=False):
     """face fluxes of the FP64 solver, or (fp32) of the mixed-precision
-    path's FP32 face solver on the states rounded to FP32"""
+    path's FP32 face solver on the states rounded to FP32, or (noalloc) of
+    the FP64 solver through the entry point that allocates nothing"""
     UL = np.ascontiguousarray(UL, dtype=np.float64).reshape(-1, 4)
     UR = np.ascontiguousarray(UR, dtype=np.float64).reshape(-1, 4)
     F = np.zeros_like(UL)
     st = C.c_uint(0)
-    name = "gfm_pure_rs_solve_f32" if fp32 else "gfm_pure_rs_solve"
+    name = "gfm_pure_rs_solve_f32" if fp32 else ("gfm_pure_rs_solve_noalloc" if noalloc else "gfm_pure_rs_solve")
     _check(getattr(load(), name)(len(UL), int(paralleltox), gamma, pinf, _dp(UL), _dp(UR), _dp(F), C.byref(st)),
            name)
     return F, st.value
