@@ -95,6 +95,8 @@ struct gfm_ctx {
     hipEvent_t ev_fork = nullptr;
     std::vector<hipEvent_t> ev_join, ev_stag;
     hipGraphExec_t gexec_multi = nullptr;   // graph_steps two-fluid steps of all groups
+    gfm::IcDesc* d_ic = nullptr;       // initial-condition descriptors, one slot per instance
+    int* d_icmap = nullptr;            // per instance: its descriptor, or -1
     int graph_steps = 0;
     bool mixed() const { return m.precision == GFM_MIXED; }
     bool twofluid() const { return m.sim_type == GFM_SIM_TWOFLUID; }
@@ -203,7 +205,7 @@ int twofluid_step(gfm_ctx* c, const gfm::LaunchCtx& L, hipEvent_t mark) {
     gfm::launch_dt<R>(L, 1, 0, c->narrow(), count);
     int r = ghost_chain<R>(c, L, count, mark);
     if (r) return r;
-    gfm::launch_fs<R>(L, muscl, 1, 0, 0);
+    gfm::launch_fs<R>(L, muscl, 1, 0, 0, 0);   // the fill read Q[.][0]
     gfm::launch_ls<R>(L, c->houc, c->m.reinit_every, c->m.reinit_iters);
     gfm::launch_bc<R>(L, 0, 2, 1);
     return GFM_OK;
@@ -490,18 +492,37 @@ gfm::LaunchCtx launch_ctx(const gfm_ctx* c, const TableOffsets& t, hipStream_t s
 
 namespace {
 
-// graph of `launch`'s work, captured on `stream`
+// graph of `launch`'s work, captured on `stream`; `forked` are the streams
+// the work forks to, each of which must have joined back before the capture
+// ends.  *out is set only to a checked, instantiated graph.
 template <typename F>
-int capture(hipStream_t stream, hipGraphExec_t* out, F launch) {
+int capture(hipStream_t stream, const std::vector<hipStream_t>& forked, hipGraphExec_t* out, F launch) {
     hipGraph_t g = nullptr;
     HC(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int r = launch();
     hipError_t ec = hipStreamEndCapture(stream, &g);
     if (r) { if (g) (void)hipGraphDestroy(g); return r; }
     if (ec != hipSuccess) return fail("hipStreamEndCapture", ec);
-    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    if (!g) return fail_arg("capture produced no graph");
+    for (hipStream_t s : forked) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hipError_t e = hipStreamIsCapturing(s, &cs);
+        if (e != hipSuccess || cs != hipStreamCaptureStatusNone) {
+            (void)hipGraphDestroy(g);
+            return e != hipSuccess ? fail("hipStreamIsCapturing", e) : fail_arg("a forked stream did not join the capture");
+        }
+    }
+    size_t nodes = 0;
+    hipError_t en = hipGraphGetNodes(g, nullptr, &nodes);
+    if (en != hipSuccess || nodes == 0) {
+        (void)hipGraphDestroy(g);
+        return en != hipSuccess ? fail("hipGraphGetNodes", en) : fail_arg("captured graph is empty");
+    }
+    hipGraphExec_t x = nullptr;
+    hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail("hipGraphInstantiate", e);
+    if (e != hipSuccess || !x) return e != hipSuccess ? fail("hipGraphInstantiate", e) : fail_arg("no graph executable");
+    *out = x;
     return GFM_OK;
 }
 
@@ -542,6 +563,8 @@ int gfm_ctx_destroy(gfm_ctx* c) {
     for (auto& h : c->inst)
         if (h.mem) (void)hipFree(h.mem);
     if (c->d_insts) (void)hipFree(c->d_insts);
+    if (c->d_ic) (void)hipFree(c->d_ic);
+    if (c->d_icmap) (void)hipFree(c->d_icmap);
     if (c->d_work) (void)hipFree(c->d_work);
     for (size_t g = 1; g < c->gstream.size(); ++g) (void)hipStreamDestroy(c->gstream[g]);
     for (size_t g = 1; g < c->ev_join.size(); ++g) (void)hipEventDestroy(c->ev_join[g]);
@@ -679,6 +702,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
     // schedules that the tests compare the default ones against
     const char* plain = std::getenv("GFM_IFACE_PLAIN_ROUNDS");
     const int iface_plain = (plain && plain[0] == '1') ? 1 : 0;
+    HC(hipMalloc((void**)&c->d_ic, descs.size() * sizeof(gfm::IcDesc)));
+    HC(hipMalloc((void**)&c->d_icmap, descs.size() * sizeof(int)));
     HC(hipMalloc((void**)&c->d_insts, descs.size() * sizeof(Inst)));
     HC(hipMemcpyAsync(c->d_insts, descs.data(), descs.size() * sizeof(Inst), hipMemcpyHostToDevice, c->stream));
     // step groups: GFM_STEP_GROUPS (maximum 4, the hardware queues a
@@ -786,6 +811,36 @@ int gfm_download_state(gfm_ctx* c, int inst, double* g1, double* g2, double* phi
         if (g2 && (r = xfer(c, inst, h.d.Q[1][0], g2, 4, false))) return r;
         if (phi && (r = xfer(c, inst, h.d.phi, phi, 1, false))) return r;
     }
+    return GFM_OK;
+}
+
+int gfm_ic_device(gfm_ctx* c, int n, const int* inst, const gfm_ic_desc* desc) {
+    int r = ready(c);
+    if (r) return r;
+    if (n < 0 || (n > 0 && (!inst || !desc))) return fail_arg("bad arguments");
+    static_assert(sizeof(gfm::IcDesc) == sizeof(gfm_ic_desc), "descriptor layouts differ");
+    const int ninst = (int)c->inst.size();
+    std::vector<int> map(ninst, -1);
+    for (int k = 0; k < n; ++k) {
+        if (inst[k] < 0 || inst[k] >= ninst) return fail_arg("instance out of range");
+        const int kind = desc[k].kind;
+        if (c->twofluid() ? (kind < GFM_IC_BUBBLE || kind > GFM_IC_PLANAR) : (kind != GFM_IC_TTC && kind != GFM_IC_EXPLOSION))
+            return fail_arg("initial condition kind does not fit the context");
+        if (kind == GFM_IC_TTC && (desc[k].test_case % 10 < 1 || desc[k].test_case % 10 > 5 || desc[k].test_case > 15))
+            return fail_arg("TTC test case");
+        if (map[inst[k]] >= 0) return fail_arg("instance listed twice");
+        map[inst[k]] = k;
+    }
+    if (n == 0) return GFM_OK;
+    HC(hipMemcpyAsync(c->d_ic, desc, (size_t)n * sizeof(gfm_ic_desc), hipMemcpyHostToDevice, c->stream));
+    HC(hipMemcpyAsync(c->d_icmap, map.data(), (size_t)ninst * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const int q = c->twofluid() ? 0 : c->cur;
+    if (c->mixed()) gfm::launch_ic<float>(c->L, c->d_ic, c->d_icmap, q, c->twofluid() ? 1 : 0);
+    else gfm::launch_ic<double>(c->L, c->d_ic, c->d_icmap, q, c->twofluid() ? 1 : 0);
+    if ((r = post_launch(c))) return r;
+    HC(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < n; ++k)
+        if ((r = gfm_set_time(c, inst[k], 0.0, 0))) return r;
     return GFM_OK;
 }
 
@@ -1006,7 +1061,7 @@ int gfm_step_graph(gfm_ctx* c, int nsteps) {
     if (nsteps <= 0) return GFM_OK;
     const int ng = c->twofluid() ? 1 : 2;
     for (int k = 0; k < ng; ++k)
-        if (!c->gexec[k] && (r = capture(c->stream, &c->gexec[k], [&]() {
+        if (!c->gexec[k] && (r = capture(c->stream, c->gstream, &c->gexec[k], [&]() {
                 return c->mixed() ? step_launches<float>(c, k) : step_launches<double>(c, k);
             })))
             return r;
@@ -1014,7 +1069,7 @@ int gfm_step_graph(gfm_ctx* c, int nsteps) {
     if (c->twofluid() && c->GL.size() > 1 && nsteps >= c->graph_steps) {
         // instance groups: blocks of graph_steps steps in one graph, in which
         // the groups run staggered and independent (twofluid_steps)
-        if (!c->gexec_multi && (r = capture(c->stream, &c->gexec_multi, [&]() {
+        if (!c->gexec_multi && (r = capture(c->stream, c->gstream, &c->gexec_multi, [&]() {
                 return c->mixed() ? step_launches<float>(c, 0, c->graph_steps)
                                   : step_launches<double>(c, 0, c->graph_steps);
             })))
@@ -1046,7 +1101,7 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
                 HC(hipEventRecord(ev[1], c->stream));
                 ghost_chain<float>(c, c->L, c->m.gfm != GFM_ORIGINAL);
                 HC(hipEventRecord(ev[2], c->stream));
-                gfm::launch_fs<float>(c->L, muscl, 1, 0, 0);
+                gfm::launch_fs<float>(c->L, muscl, 1, 0, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));
                 gfm::launch_ls<float>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
                 HC(hipEventRecord(ev[4], c->stream));
@@ -1056,7 +1111,7 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
                 HC(hipEventRecord(ev[1], c->stream));
                 ghost_chain<double>(c, c->L, c->m.gfm != GFM_ORIGINAL);
                 HC(hipEventRecord(ev[2], c->stream));
-                gfm::launch_fs<double>(c->L, muscl, 1, 0, 0);
+                gfm::launch_fs<double>(c->L, muscl, 1, 0, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));
                 gfm::launch_ls<double>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
                 HC(hipEventRecord(ev[4], c->stream));

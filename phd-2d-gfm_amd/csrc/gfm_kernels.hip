@@ -366,10 +366,17 @@ __device__ __forceinline__ void face_input_godunov(const Inst& I, int i0, int j0
 template <typename R>
 struct FsWaves { static constexpr int value = sizeof(R) == 4 ? 6 : 1; };
 
+// qorig >= 0 (FP32 storage, whole two-fluid steps): Q[.][qorig] holds the
+// ghost fill's input.  A cell whose fluid f is real there and not interfacial
+// leaves the fill unchanged, and its fill output Q[.][qin] is that input
+// through the fill's U->W->U conversion, which in FP32 costs several ulp on
+// every step (E - rho |u|^2 / 2 and back): the cell's own state for the update
+// D is therefore taken from Q[.][qorig], exactly as stored.  The
+// reconstruction keeps reading Q[.][qin].  FP64 takes every state from qin.
 template <typename R, bool MUSCL>
 __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst* __restrict__ insts,
                                                                          const int4* __restrict__ work, int qin,
-                                                                         int qout, int copy_inactive) {
+                                                                         int qout, int copy_inactive, int qorig) {
     typedef R T;
     __shared__ T sW[4][FWY][FWX];
     __shared__ V4A<T> sF[2][NFACE];   // per face: inputs lo / hi, then sF[0] = flux
@@ -420,10 +427,27 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
     const bool own_in = ci < I.g + I.Ny && cj < I.g + I.Nx;
     const long long cown = own_in ? cidx(I, ci, cj) : 0;
     V4<R> Uown = {{R(0), R(0), R(0), R(0)}};
-    if (own_in) Uown = ld4<R>(Qi, I, cown);
+    // the own cell's side and interfacial index (FP32 with qorig) are loaded
+    // first, the tile's states next, and the own state, whose buffer depends
+    // on them, last, so that no load waits behind another's latency
+    R ph_own = R(0);
+    int im_own = 0;
+    const bool pick = sizeof(R) == 4 && qorig >= 0 && own_in;
+    if (pick) {
+        ph_own = ld1<R>(I.phi, cown);
+        im_own = gmem(I.imap)[cown];
+    }
+    auto load_own = [&]() {
+        if (!own_in) return;
+        const bool orig = pick && ((ph_own <= R(0)) == (f == 0)) && im_own < 0;
+        Uown = ld4<R>(orig ? I.Q[f][qorig] : Qi, I, cown);
+    };
     // the first-order form has no barrier in A or B: the counter and dt/dx,
     // dt/dy set by thread 0 must be visible before any wave reaches C1
-    if (!MUSCL) __syncthreads();
+    if (!MUSCL) {
+        load_own();
+        __syncthreads();
+    }
     // A: the loads of both passes first, then the conversions
     if (MUSCL) {
         constexpr int NA = (FWY * FWX + FS_THREADS - 1) / FS_THREADS;
@@ -439,6 +463,7 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
                 UA[q] = ld4<R>(Qi, I, cidx(I, i, j));
             }
         }
+        load_own();
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
             const int c = tid + q * FS_THREADS;
@@ -810,25 +835,6 @@ __device__ __forceinline__ ExOut<R> extrap_out(const Inst& I, int fr, const V4<R
     return o;
 }
 
-// FP32 storage (mixed precision): the real fluid of a cell that is not
-// interfacial leaves the ghost fill as it came in.  Its primitives are the
-// conversion of the fill's input conserved state Q[.][0] and are never
-// changed, so the reference's W->U at the end of the fill returns that state
-// up to rounding; in FP32 that round trip (E - rho |u|^2 / 2 and back) costs
-// several ulp of the real fluid on every step, the largest error of the mixed
-// step, so the stored state is written instead.  FP64 keeps the reference's
-// conversion.  c: the cell whose real fluid this is.
-template <typename R, int MODE>
-__device__ __forceinline__ void keep_real(const Inst& I, long long c, int fr, ExOut<R>& o) {
-    if constexpr (sizeof(R) == 4 && (MODE & MF_LAST) != 0) {
-        if (gmem(I.imap)[c] < 0) {
-            const V4<R> U = ld4<R>(I.Q[fr][0], I, c);
-            if (fr == 0) o.a = U;
-            else o.b = U;
-        }
-    }
-}
-
 template <typename R, int MODE>
 __device__ __forceinline__ void extrap_put(const Inst& I, int i, int j, int bout, ExOut<R> o, bool fu, bool fv) {
     if (fu) { o.a.v[1] = -o.a.v[1]; o.b.v[1] = -o.b.v[1]; o.vu = -o.vu; }
@@ -882,9 +888,7 @@ __device__ __forceinline__ void extrap_cell(const Inst& I, int i, int j, int bin
         atomicMin(&bb[0], i - 2); atomicMax(&bb[1], i + 2);
         atomicMin(&bb[2], j - 2); atomicMax(&bb[3], j + 2);
     }
-    ExOut<R> o = extrap_out<R, MODE>(I, fr, Wr, Wg);
-    keep_real<R, MODE>(I, cs, fr, o);
-    extrap_put<R, MODE>(I, i, j, bout, o, fu, fv);
+    extrap_put<R, MODE>(I, i, j, bout, extrap_out<R, MODE>(I, fr, Wr, Wg), fu, fv);
 }
 
 template <typename R, int MODE>
@@ -1158,8 +1162,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict
             atomicMin(&bb[0], i - 2); atomicMax(&bb[1], i + 2);
             atomicMin(&bb[2], j - 2); atomicMax(&bb[3], j + 2);
         }
-        ExOut<R> o = extrap_out<R, MOUT>(I, fr, Wr, Wo);
-        keep_real<R, MOUT>(I, cidx(I, i, j), fr, o);
+        const ExOut<R> o = extrap_out<R, MOUT>(I, fr, Wr, Wo);
         extrap_put<R, MOUT>(I, i, j, bout, o, false, false);
         if (i < 2 * g || i >= I.Ny || j < 2 * g || j >= I.Nx) {
             // ghost cells sourced here (inverse of src_row / src_col): rows
@@ -1904,6 +1907,120 @@ __global__ __launch_bounds__(256) void k_vext_iter(const Inst* __restrict__ inst
 }
 
 // ---------------------------------------------------------------------------
+// Initial conditions on the device (the host library gfm_ic.cpp, restated
+// per cell): sim_onefluid.cpp:196-422 (TTC1-5, circular explosion with 10x10
+// sub-cell mixing), sim_twofluid.cpp:774-859 (shocked bubble), :1245-1347
+// (RMI interface), :675-742 (planar tubes).  FP64, every operation rounded on
+// its own as the host code is compiled; work: full-array 64x4 tiles.
+// ---------------------------------------------------------------------------
+__constant__ double c_ttc[5][2][4] = {
+    {{1.0, 0.0, 0.0, 1.0}, {0.125, 0.0, 0.0, 0.1}},
+    {{1.0, -2.0, 0.0, 0.4}, {1.0, 2.0, 0.0, 0.4}},
+    {{1.0, 0.0, 0.0, 1000.0}, {1.0, 0.0, 0.0, 0.01}},
+    {{1.0, 0.0, 0.0, 0.1}, {1.0, 0.0, 0.0, 100.0}},
+    {{5.99924, 19.5975, 0.0, 460.894}, {5.99242, -6.19633, 0.0, 46.0950}},
+};
+
+// euler_misc.hpp:89-108 with the stiffened-gas energy
+__device__ __forceinline__ V4<double> ic_cons(double g, double pinf, const double* W) {
+#pragma clang fp contract(off)
+    const double rho = W[0], u = W[1], v = W[2], p = W[3];
+    const double e = (p + g * pinf) / ((g - 1.0) * rho);
+    V4<double> U;
+    U.v[0] = rho;
+    U.v[1] = rho * u;
+    U.v[2] = rho * v;
+    U.v[3] = rho * e + 0.5 * rho * (u * u + v * v);
+    return U;
+}
+
+template <typename R>
+__device__ __forceinline__ void ic_put(void* base, const Inst& I, long long c, const V4<double>& U) {
+    V4<R> a;
+    for (int k = 0; k < 4; ++k) a.v[k] = (R)U.v[k];
+    st4<R>(base, I, c, a);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_ic(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                            const IcDesc* __restrict__ descs, const int* __restrict__ map, int q,
+                                            int twofluid) {
+#pragma clang fp contract(off)
+    const int4 w = work[blockIdx.x];
+    const int d = map[w.x];
+    if (d < 0) return;
+    const Inst& I = insts[w.x];
+    const IcDesc& D = descs[d];
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    const long long c = cidx(I, i, j);
+    const double x = I.x0 + (j - I.g + 0.5) * I.dx, y = I.y0 + (i - I.g + 0.5) * I.dy;   // cellcentre_coord
+    const double g1 = I.gam[0], p1 = I.pinf[0], g2 = I.gam[1], p2 = I.pinf[1];
+    double phi = 0.0;
+    switch (D.kind) {
+        case 1: {   // TTC: set_planar_IC (sim_onefluid.cpp:426-449), a x + b y <= 0.5 -> left
+            const int k = D.test_case % 10 - 1;
+            const bool ydir = D.test_case > 10;
+            double W[4];
+            const int side = (ydir ? y : x) <= 0.5 ? 0 : 1;
+            for (int m = 0; m < 4; ++m) W[m] = c_ttc[k][side][m];
+            if (ydir) { const double t = W[1]; W[1] = W[2]; W[2] = t; }
+            ic_put<R>(I.Q[0][q], I, c, ic_cons(g1, p1, W));
+            break;
+        }
+        case 2: {   // circular explosion: 10x10 sub-cell samples inside r = 0.4 about (1, 1)
+            const double L[4] = {1.0, 0.0, 0.0, 1.0}, Rr[4] = {0.125, 0.0, 0.0, 0.1};
+            const int N = 10;
+            int inside = 0;
+            const double bx = x - 0.5 * I.dx, by = y - 0.5 * I.dy, ddx = I.dx / N, ddy = I.dy / N;
+            for (int a = 0; a < N; ++a)
+                for (int b = 0; b < N; ++b) {
+                    const double sx = bx + (a + 0.5) * ddx - 1.0;
+                    const double sy = by + (b + 0.5) * ddy - 1.0;
+                    if (sqrt(sx * sx + sy * sy) <= 0.4) inside++;
+                }
+            const double f = (double)inside / (N * N);
+            double W[4];
+            for (int m = 0; m < 4; ++m) W[m] = f * L[m] + (1.0 - f) * Rr[m];
+            ic_put<R>(I.Q[0][q], I, c, ic_cons(g1, p1, W));
+            break;
+        }
+        case 3: {   // shocked bubble (sim_twofluid.cpp:774-859): fluid 1 = bubble, phi < 0 inside
+            ic_put<R>(I.Q[0][q], I, c, ic_cons(g1, p1, D.w1));
+            ic_put<R>(I.Q[1][q], I, c, ic_cons(g2, p2, x < D.shock_x ? D.w2 : D.w3));
+            const double ddx = x - D.xc, ddy = y - D.yc;
+            phi = sqrt(ddx * ddx + ddy * ddy) - D.R;
+            break;
+        }
+        case 4: {   // RMI interface x_I(y) = x_if - amp cos(2 pi modes (y - y0) / Ly)
+            const double Ly = I.Ny * I.dy;
+            const double kk = 2.0 * 3.14159265358979323846 * D.modes / Ly;
+            const double arg = kk * (y - I.y0);
+            const double xi = D.x_if - D.amp * cos(arg);
+            const double dxi = D.amp * kk * sin(arg);
+            ic_put<R>(I.Q[0][q], I, c, ic_cons(g1, p1, x < D.shock_x ? D.w1 : D.w2));
+            ic_put<R>(I.Q[1][q], I, c, ic_cons(g2, p2, D.w3));
+            phi = (xi - x) / sqrt(1.0 + dxi * dxi);
+            break;
+        }
+        default: {  // planar two-material tube (sim_twofluid.cpp:675-742)
+            ic_put<R>(I.Q[0][q], I, c, ic_cons(g1, p1, D.w1));
+            ic_put<R>(I.Q[1][q], I, c, ic_cons(g2, p2, D.w2));
+            phi = D.axis == 0 ? x - D.pos : y - D.pos;
+            break;
+        }
+    }
+    if (twofluid) gmem((R*)I.phi)[c] = (R)phi;
+}
+
+template <typename R>
+void launch_ic(const LaunchCtx& L, const IcDesc* descs, const int* map, int q, int twofluid) {
+    hipLaunchKernelGGL((k_ic<R>), dim3((unsigned)L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, descs, map, q,
+                       twofluid);
+}
+
+// ---------------------------------------------------------------------------
 // A15 level set: TVD-RK3 stages of HOUC advection with the stored velocity
 // (update_interface_coupled, sim_twofluid.cpp:72), BCs folded in as a gather.
 //   stage 1: A = phi + dt L(phi)
@@ -2517,13 +2634,13 @@ void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes, int co
 }
 
 template <typename R>
-void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive) {
+void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive, int qorig) {
     if (muscl)
         hipLaunchKernelGGL((k_fs<R, true>), GRID(L.n_fs), dim3(FS_THREADS), 0, L.stream, L.insts, L.w_fs, qin, qout,
-                           copy_inactive);
+                           copy_inactive, qorig);
     else
         hipLaunchKernelGGL((k_fs<R, false>), GRID(L.n_fs), dim3(FS_THREADS), 0, L.stream, L.insts, L.w_fs, qin, qout,
-                           copy_inactive);
+                           copy_inactive, qorig);
 }
 
 template <typename R>
@@ -2718,7 +2835,7 @@ void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, doub
 
 #define INST(R)                                                               \
     template void launch_dt<R>(const LaunchCtx&, int, int, int, int);              \
-    template void launch_fs<R>(const LaunchCtx&, bool, int, int, int);             \
+    template void launch_fs<R>(const LaunchCtx&, bool, int, int, int, int);        \
     template void launch_extrap<R>(const LaunchCtx&, int, int, int);          \
     template bool launch_extrap_tb<R>(const LaunchCtx&, int, int, int, int, int); \
     template void launch_iface<R>(const LaunchCtx&, int, int, int);           \
@@ -2727,7 +2844,8 @@ void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, doub
     template void launch_ls<R>(const LaunchCtx&, const Houc&, int, int);      \
     template void launch_reinit_now<R>(const LaunchCtx&, int);                \
     template void launch_bc<R>(const LaunchCtx&, int, int, int);               \
-    template void launch_reset<R>(const LaunchCtx&, int);
+    template void launch_reset<R>(const LaunchCtx&, int);                      \
+    template void launch_ic<R>(const LaunchCtx&, const IcDesc*, const int*, int, int);
 INST(double)
 INST(float)
 

@@ -30,7 +30,9 @@ constexpr int FS_TILE_X = 32, FS_TILE_Y = 16;   // flux-update tile (one cell pe
 constexpr int SEG_PER_WG = 4;   // 256-cell row segments per workgroup (one per wavefront)
 
 template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes, int count_iface);
-template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive);
+// qorig: the ghost fill's input buffer in a whole two-fluid step (-1: none)
+template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive,
+                                     int qorig = -1);
 template <typename R> void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout);
 template <typename R> bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout);
 template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted);
@@ -43,6 +45,14 @@ template <typename R> void launch_ls(const LaunchCtx& L, const Houc& H, int ever
 template <typename R> void launch_reinit_now(const LaunchCtx& L, int iters);
 template <typename R> void launch_bc(const LaunchCtx& L, int q, int nfluid, int advance);
 template <typename R> void launch_reset(const LaunchCtx& L, int narrow_boxes);
+// initial conditions on the device: instance k gets descs[map[k]] (map[k] < 0: untouched), its
+// conserved grids Q[.][q] and phi
+struct IcDesc {   // the layout of gfm_ic_desc (include/gfm_hip.h)
+    int kind, test_case, axis, modes;
+    double xc, yc, R, x_if, amp, pos, shock_x;
+    double w1[4], w2[4], w3[4];
+};
+template <typename R> void launch_ic(const LaunchCtx& L, const IcDesc* descs, const int* map, int q, int twofluid);
 void launch_pure_rs(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR,
                     double* F, unsigned* st);
 // p[0..n) = v (a kernel, so that it may be captured in a graph)
