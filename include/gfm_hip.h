@@ -199,6 +199,32 @@ int gfm_ic_planar(const gfm_sim_info* info, const gfm_eos* eos, int axis, double
 int gfm_shock_state(double gamma, double pinf, const double* pre_prims, double mach, int direction,
                     double* post_prims);
 
+/* GPU initial-condition library: the same initial conditions, computed on
+ * the device directly into a committed context's state (conserved grids and
+ * phi; one-fluid contexts: the current grid), for n listed instances in one
+ * launch, and t = 0, step 0.  FP64 arithmetic as the host functions above
+ * (rounded to FP32 in a mixed-precision context), so that both give the same
+ * values (cos / sin of the RMI interface to within their last bits). */
+enum {
+    GFM_IC_TTC = 1,             /* test_case 1..5 = TTC1..5_x, 11..15 = TTC1..5_y (one fluid)   */
+    GFM_IC_EXPLOSION = 2,       /* circular explosion, 10x10 sub-cell mixing (one fluid)        */
+    GFM_IC_BUBBLE = 3,          /* gfm_ic_bubble: xc, yc, R, w1 bubble, w2 pre, w3 post, shock_x  */
+    GFM_IC_RMI = 4,             /* gfm_ic_rmi: x_if, amp, modes, w1 light, w2 post, w3 heavy, shock_x */
+    GFM_IC_PLANAR = 5           /* gfm_ic_planar: axis, pos, w1 fluid 1, w2 fluid 2            */
+};
+typedef struct {
+    int kind;                   /* GFM_IC_* */
+    int test_case;              /* GFM_IC_TTC */
+    int axis;                   /* GFM_IC_PLANAR: 0 x, 1 y */
+    int modes;                  /* GFM_IC_RMI */
+    double xc, yc, R;           /* GFM_IC_BUBBLE */
+    double x_if, amp;           /* GFM_IC_RMI */
+    double pos;                 /* GFM_IC_PLANAR */
+    double shock_x;             /* GFM_IC_BUBBLE, GFM_IC_RMI */
+    double w1[4], w2[4], w3[4]; /* primitive states (rho, u, v, p) */
+} gfm_ic_desc;
+int gfm_ic_device(gfm_ctx* ctx, int n, const int* inst, const gfm_ic_desc* desc);
+
 /* last HIP error string (thread local) */
 const char* gfm_last_error(void);
 

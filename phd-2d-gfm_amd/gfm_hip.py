@@ -33,7 +33,7 @@ EXPORTS = [
     "gfm_extrapolate", "gfm_ls_reinit", "gfm_get_bbox", "gfm_set_bbox",
     "gfm_step", "gfm_step_graph", "gfm_sync", "gfm_profile_steps", "gfm_status", "gfm_mass_sums", "gfm_mass_history", "gfm_iface_info",
     "gfm_pure_rs_solve", "gfm_pure_rs_solve_f32", "gfm_mixed_rs_solve", "gfm_pure_rs_solve_noalloc",
-    "gfm_ic_onefluid", "gfm_ic_bubble", "gfm_ic_rmi", "gfm_ic_planar", "gfm_shock_state",
+    "gfm_ic_onefluid", "gfm_ic_bubble", "gfm_ic_rmi", "gfm_ic_planar", "gfm_shock_state", "gfm_ic_device",
     "gfm_last_error",
 ]
 
@@ -56,6 +56,27 @@ class Eos(C.Structure):
 class Methods(C.Structure):
     _fields_ = [("sim_type", C.c_int), ("fs", C.c_int), ("gfm", C.c_int), ("ls_order", C.c_int),
                 ("reinit_every", C.c_int), ("reinit_iters", C.c_int), ("precision", C.c_int)]
+
+
+IC_TTC, IC_EXPLOSION, IC_BUBBLE, IC_RMI, IC_PLANAR = 1, 2, 3, 4, 5
+
+
+class IcDesc(C.Structure):
+    """gfm_ic_desc: an initial condition computed on the device (gfm_ic_device)"""
+    _fields_ = [("kind", C.c_int), ("test_case", C.c_int), ("axis", C.c_int), ("modes", C.c_int),
+                ("xc", C.c_double), ("yc", C.c_double), ("R", C.c_double), ("x_if", C.c_double),
+                ("amp", C.c_double), ("pos", C.c_double), ("shock_x", C.c_double),
+                ("w1", C.c_double * 4), ("w2", C.c_double * 4), ("w3", C.c_double * 4)]
+
+
+def ic_desc(kind, w1=(0, 0, 0, 0), w2=(0, 0, 0, 0), w3=(0, 0, 0, 0), **kw):
+    d = IcDesc()
+    d.kind = kind
+    for k, v in kw.items():
+        setattr(d, k, v)
+    for name, w in (("w1", w1), ("w2", w2), ("w3", w3)):
+        getattr(d, name)[:] = [float(x) for x in w]
+    return d
 
 
 class GfmError(RuntimeError):
@@ -112,6 +133,7 @@ def load():
                        C.c_double, _D, _D, _D],
         "gfm_ic_planar": [C.POINTER(SimInfo), C.POINTER(Eos), C.c_int, C.c_double, _D, _D, _D, _D, _D],
         "gfm_shock_state": [C.c_double, C.c_double, _D, C.c_double, C.c_int, _D],
+        "gfm_ic_device": [vp, C.c_int, _I, C.POINTER(IcDesc)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -189,6 +211,13 @@ class Context:
         phi = np.zeros((H, W)) if self.twofluid else None
         _check(load().gfm_download_state(self.h, k, _dp(g1), _dp(g2), _dp(phi)), "gfm_download_state")
         return (g1, g2, phi) if self.twofluid else g1
+
+    def ic_device(self, items):
+        """initial conditions computed on the device: items = [(instance, IcDesc), ...]; t = 0, step 0"""
+        n = len(items)
+        inst = (C.c_int * max(n, 1))(*[k for k, _ in items])
+        descs = (IcDesc * max(n, 1))(*[d for _, d in items])
+        _check(load().gfm_ic_device(self.h, n, inst, descs), "gfm_ic_device")
 
     def upload_vfield(self, k, vf):
         _check(load().gfm_upload_vfield(self.h, k, _dp(vf)), "gfm_upload_vfield")

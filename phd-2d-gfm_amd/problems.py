@@ -82,6 +82,32 @@ def rmi_batch(n=256, seed=20261204, smin=1.0, smax=4.0):
     return out
 
 
+def shocked_bubble_ic(Nx=1024, Ny=288, bubble_gamma=1.648, bubble_rho=0.182, mach=1.22, R=0.025, p0=1e5,
+                      numGC=8):
+    """shocked_bubble's instance as (info, eos, gfm_ic_desc) for the device initial conditions"""
+    Lx, Ly = 0.325, 0.089
+    info = G.SimInfo(Nx, Ny, numGC, 0.0, 0.0, Lx / Nx, Ly / Ny,
+                     G.BC_TRANSMISSIVE, G.BC_TRANSMISSIVE, G.BC_REFLECTIVE, G.BC_REFLECTIVE)
+    eos = G.Eos(bubble_gamma, 0.0, 1.4, 0.0)
+    pre = [1.0, 0.0, 0.0, p0]
+    post = G.shock_state(1.4, 0.0, pre, mach, -1)
+    desc = G.ic_desc(G.IC_BUBBLE, [bubble_rho, 0.0, 0.0, p0], pre, post, xc=0.175, yc=0.0445, R=R, shock_x=0.225)
+    return info, eos, desc
+
+
+def rmi_ic(Nx=512, Ny=128, amp_over_lambda=0.05, modes=1, mach=1.2, numGC=8):
+    """rmi's instance as (info, eos, gfm_ic_desc) for the device initial conditions"""
+    Lx, Ly = 4.0, 1.0
+    info = G.SimInfo(Nx, Ny, numGC, 0.0, 0.0, Lx / Nx, Ly / Ny,
+                     G.BC_TRANSMISSIVE, G.BC_TRANSMISSIVE, G.BC_PERIODIC, G.BC_PERIODIC)
+    eos = G.Eos(1.4, 0.0, 1.093, 0.0)
+    light = [1.0, 0.0, 0.0, 1.0]
+    post = G.shock_state(1.4, 0.0, light, mach, -1)
+    desc = G.ic_desc(G.IC_RMI, light, post, [5.04, 0.0, 0.0, 1.0], x_if=2.9, amp=amp_over_lambda * Ly / modes,
+                     modes=modes, shock_x=3.2)
+    return info, eos, desc
+
+
 def interior(info, a):
     g = info.numGC
     return a[g:g + info.Ny, g:g + info.Nx]
