@@ -54,7 +54,7 @@ def reference(n, cfl):
     ctx = G.Context(G.methods(sim_type=G.SIM_ONEFLUID, fs=G.FS_MUSCL, precision=G.FP64))
     ctx.add(info, eos, cfl, T_END, 1 << 30)
     ctx.commit()
-    ctx.upload(0, G.ic_onefluid(info, eos, 100))
+    ctx.ic_device([(0, G.ic_desc(G.IC_EXPLOSION))])   # 10x10 sub-cell mixing, on the device
     steps = run_to_end(ctx)
     status = ctx.status(0)
     rho = ctx.download(0)[..., 0].copy()
@@ -78,12 +78,12 @@ def bilinear(info, f, x, y):
 def gfm_error(gfm, n, cfl, ref_info, ref_rho, ls_order, reinit_every, reinit_iters):
     info = G.SimInfo(n, n, 6, 0.0, 0.0, 2.0 / n, 2.0 / n, TR, TR, TR, TR)
     eos = G.Eos(1.4, 0.0, 1.4, 0.0)
-    g1, g2, phi = G.ic_bubble(info, eos, 1.0, 1.0, 0.4, INSIDE, OUTSIDE, OUTSIDE, 1e300)
     ctx = G.Context(G.methods(sim_type=G.SIM_TWOFLUID, fs=G.FS_MUSCL, gfm=GFMS[gfm], ls_order=ls_order,
                               reinit_every=reinit_every, reinit_iters=reinit_iters, precision=G.FP64))
     ctx.add(info, eos, cfl, T_END, 1 << 30)
     ctx.commit()
-    ctx.upload(0, g1, g2, phi)
+    # fluid 1 the inside state everywhere, fluid 2 the outside one, phi the distance to the circle
+    ctx.ic_device([(0, G.ic_desc(G.IC_BUBBLE, INSIDE, OUTSIDE, OUTSIDE, xc=1.0, yc=1.0, R=0.4, shock_x=1e300))])
     steps = run_to_end(ctx)
     status = ctx.status(0)
     g1, g2, phi = ctx.download(0)
