@@ -157,6 +157,34 @@ int gfm_mass_history(gfm_ctx* ctx, int inst, int n, double* out3);
 /* number of interfacial cells and dependency rounds of the last ghost fill */
 int gfm_iface_info(gfm_ctx* ctx, int inst, int* ncells, int* rounds);
 
+/* Output timing (sim_twofluid.cpp:54-83: output at t = 0, then after each step
+ * with t - lastoutputtime > T / output_freq).  While interval >= 0, the
+ * instance takes no further step once t - last > interval holds, so that a
+ * host driving whole graphs of steps still stops at the step the reference
+ * writes its output after; the host writes, moves `last` to t and steps on.
+ * interval < 0 turns the hold off (the default).  Two-fluid and one-fluid. */
+int gfm_set_output_hold(gfm_ctx* ctx, int inst, double last, double interval);
+/* The perturbation-amplitude tracers of sim_twofluid.cpp:1831-1887 (RMI_pert):
+ * n <= 4 points (xy = x0 y0 x1 y1 ...), each moved after every step by one
+ * classical RK4 step of that step's dt in the velocity field its ghost fill
+ * stored (time_step_RK4 and the solved velocity field, external: restated as
+ * cell-centred bilinear interpolation); keep_y puts each back on its initial
+ * y afterwards (:1882-1884).  Called before gfm_ctx_commit to declare them
+ * (the history is allocated there), after it to reset their positions.
+ * Two-fluid contexts only. */
+int gfm_set_tracers(gfm_ctx* ctx, int inst, int n, const double* xy, int keep_y);
+/* entries 0 .. n-1 of 1 + 4 ntr doubles: entry k (k >= 1) holds t_k and, per
+ * tracer, its position and velocity after step k, before that step's RK4 move
+ * (the values output_perturbation_amplitude prints, :1869-1874); entry 0 is
+ * zero.  Capacity as gfm_mass_history. */
+int gfm_tracer_history(gfm_ctx* ctx, int inst, int n, double* out);
+/* z (sim_twofluid.cpp:199, the level set's get_z, external): the fraction of
+ * each cell's area on fluid 1's side of the plane through the cell centre with
+ * the level set's value and central-difference unit normal there, over the
+ * padded grid (the outermost ring 1 or 0 by the sign of phi).  Uses the level
+ * set's stage buffer: call between steps. */
+int gfm_download_z(gfm_ctx* ctx, int inst, double* z);
+
 /* Riemann solvers over n faces / interfaces, host pointers, AoS x4.
  * pure_RS_base::solve_RP (pure_RS_base.hpp:13-20): conserved L/R -> flux.
  * mixed_RS_base::solve_mixed_RP (mixed_RS_base.hpp:12-24): conserved L/R

@@ -769,6 +769,85 @@ void or_bilinear(const or_grid* G, const double* grid, double x, double y, doubl
         out[k] = (1.0 - ty) * ((1.0 - tx) * a[k] + tx * b[k]) + ty * ((1.0 - tx) * c[k] + tx * d[k]);
 }
 
+/* Velocity at (x, y) in an interleaved velocity field: the bilinear stencil
+ * of or_bilinear on its two components (the solved velocity field's
+ * get_velocity, external; sim_twofluid.cpp:1874) */
+static void vfield_at(const or_grid* G, const double* vf, double x, double y, double* u, double* v)
+{
+    int H = H_OF(G), Wd = W_OF(G);
+    double fj = (x - G->x0) / G->dx - 0.5 + G->g;
+    double fi = (y - G->y0) / G->dy - 0.5 + G->g;
+    int j0 = (int)floor(fj), i0 = (int)floor(fi);
+    if (j0 < 0) j0 = 0;
+    if (j0 > Wd - 2) j0 = Wd - 2;
+    if (i0 < 0) i0 = 0;
+    if (i0 > H - 2) i0 = H - 2;
+    double tx = fj - j0, ty = fi - i0;
+    tx = tx < 0.0 ? 0.0 : (tx > 1.0 ? 1.0 : tx);
+    ty = ty < 0.0 ? 0.0 : (ty > 1.0 ? 1.0 : ty);
+    const double* a = vf + IDX(G, i0, j0) * 2;
+    const double* b = vf + IDX(G, i0, j0 + 1) * 2;
+    const double* c = vf + IDX(G, i0 + 1, j0) * 2;
+    const double* d = vf + IDX(G, i0 + 1, j0 + 1) * 2;
+    *u = (1.0 - ty) * ((1.0 - tx) * a[0] + tx * b[0]) + ty * ((1.0 - tx) * c[0] + tx * d[0]);
+    *v = (1.0 - ty) * ((1.0 - tx) * a[1] + tx * b[1]) + ty * ((1.0 - tx) * c[1] + tx * d[1]);
+}
+
+/* output_perturbation_amplitude after step n (sim_twofluid.cpp:1861-1885):
+ * entry = {t, then x, y, u, v per tracer}; then each tracer moves by one
+ * classical RK4 step of dt in the frozen field (time_step_RK4, external),
+ * and returns to y0 if keep_y (:1882-1884) */
+void or_tracers_step(const or_grid* G, const double* vfield, int n, double* pos, const double* pos0,
+                     int keep_y, double t, double dt, double* entry)
+{
+    entry[0] = t;
+    for (int k = 0; k < n; ++k) {
+        double x = pos[2 * k], y = pos[2 * k + 1];
+        double u1, v1, u2, v2, u3, v3, u4, v4;
+        vfield_at(G, vfield, x, y, &u1, &v1);
+        entry[1 + 4 * k] = x;
+        entry[2 + 4 * k] = y;
+        entry[3 + 4 * k] = u1;
+        entry[4 + 4 * k] = v1;
+        vfield_at(G, vfield, x + 0.5 * dt * u1, y + 0.5 * dt * v1, &u2, &v2);
+        vfield_at(G, vfield, x + 0.5 * dt * u2, y + 0.5 * dt * v2, &u3, &v3);
+        vfield_at(G, vfield, x + dt * u3, y + dt * v3, &u4, &v4);
+        pos[2 * k] = x + dt / 6.0 * (u1 + 2.0 * u2 + 2.0 * u3 + u4);
+        pos[2 * k + 1] = keep_y ? pos0[2 * k + 1] : y + dt / 6.0 * (v1 + 2.0 * v2 + 2.0 * v3 + v4);
+    }
+}
+
+/* z (sim_twofluid.cpp:199; the level set's get_z is external): area
+ * fraction of each cell on fluid 1's side (phi <= 0) of the plane through
+ * the centre with phi's value and unit normal (unit_normal) there.  The
+ * area of {a' u + b' w <= m} in the unit-scaled rectangle (Scardovelli and
+ * Zaleski 2000, 2D): with a = |nx| dx <= b = |ny| dy and m = -phi + (a+b)/2,
+ * 0 | m^2/(2ab) | (2m - a)/(2b) | 1 - (a+b-m)^2/(2ab) | 1 on m <= 0 | < a |
+ * < b | < a+b | beyond.  The outermost ring: 1 or 0 by the sign of phi. */
+void or_volume_fraction(const or_grid* G, const double* phi, double* z)
+{
+    int H = H_OF(G), Wd = W_OF(G);
+    for (int i = 0; i < H; ++i)
+        for (int j = 0; j < Wd; ++j) {
+            double p = phi[IDX(G, i, j)];
+            if (i == 0 || j == 0 || i == H - 1 || j == Wd - 1) {
+                z[IDX(G, i, j)] = p <= 0.0 ? 1.0 : 0.0;
+                continue;
+            }
+            double n[2];
+            unit_normal(G, phi, i, j, n);
+            double a = fabs(n[0]) * G->dx, b = fabs(n[1]) * G->dy;
+            if (a > b) { double s = a; a = b; b = s; }
+            double m = -p + 0.5 * (a + b), f;
+            if (m <= 0.0) f = 0.0;
+            else if (m >= a + b) f = 1.0;
+            else if (m < a) f = m * m / (2.0 * a * b);
+            else if (m < b) f = (2.0 * m - a) / (2.0 * b);
+            else f = 1.0 - (a + b - m) * (a + b - m) / (2.0 * a * b);
+            z[IDX(G, i, j)] = f;
+        }
+}
+
 /* HOUC upwind-biased first-derivative stencils for u > 0: offsets
  * offset0 .. offset0+order, coefficients c[m] / den (den folded in). */
 void or_houc_coeffs(int order, double* c, int* offset0)

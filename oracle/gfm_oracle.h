@@ -97,6 +97,10 @@ double or_fluid_mass(const or_grid* G, double interiorsign, const double* grid, 
 void or_ls_normal(const or_grid* G, const double* phi, int i, int j, double* n2);
 int or_ls_is_interfacial(const or_grid* G, const double* phi, int i, int j);
 void or_bilinear(const or_grid* G, const double* grid, double x, double y, double* out4);
+/* diagnostics: RK4 tracers in the velocity field, the volume fraction z */
+void or_tracers_step(const or_grid* G, const double* vfield, int n, double* pos, const double* pos0,
+                     int keep_y, double t, double dt, double* entry);
+void or_volume_fraction(const or_grid* G, const double* phi, double* z);
 void or_houc_coeffs(int order, double* c, int* offset0);
 void or_ls_advect(const or_grid* G, int order, double* phi, const double* vfield, double dt);
 void or_ls_reinit(const or_grid* G, double* phi, int iters);

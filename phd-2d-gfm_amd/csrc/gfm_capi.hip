@@ -41,6 +41,8 @@ struct InstHost {
     gfm_eos eos;
     double cfl, T;
     int max_steps;
+    int ntr = 0, tr_keep_y = 0;          // passive tracers (gfm_set_tracers)
+    double tr_hdr[gfm::TR_HDR] = {};     // positions, then initial positions
     void* mem = nullptr;   // one allocation per instance
     Inst d;                // descriptor as uploaded (device pointers)
 };
@@ -668,6 +670,9 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.hist_cap = c->twofluid() ? (int)std::min<long long>((long long)h.max_steps + 1, 1 << 20) : 0;
         if (d.hist_cap < 0) d.hist_cap = 0;
         const size_t ohist = take((size_t)d.hist_cap * 3 * sizeof(double) + 8);
+        d.ntr = h.ntr;
+        d.tr_keep_y = h.tr_keep_y;
+        const size_t otrk = h.ntr ? take((gfm::TR_HDR + (size_t)d.hist_cap * (1 + 4 * h.ntr)) * sizeof(double)) : 0;
         const size_t osd = take(gfm::SD_N * sizeof(double));
         const size_t osi = take(gfm::SI_N * sizeof(int));
         hipError_t e = hipMalloc(&h.mem, off);
@@ -687,6 +692,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
         }
         d.partials = (double*)(base + opart);
         d.mass_hist = (double*)(base + ohist);
+        d.trk = h.ntr ? (double*)(base + otrk) : nullptr;
+        if (h.ntr) HC(hipMemcpyAsync(d.trk, h.tr_hdr, sizeof(h.tr_hdr), hipMemcpyHostToDevice, c->stream));
         d.sd = (double*)(base + osd);
         d.si = (int*)(base + osi);
         h.d = d;
@@ -949,6 +956,61 @@ int gfm_mass_history(gfm_ctx* c, int inst, int n, double* out3) {
     HC(hipMemcpyAsync(out3, d.mass_hist, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HC(hipStreamSynchronize(c->stream));
     return GFM_OK;
+}
+
+int gfm_set_tracers(gfm_ctx* c, int inst, int n, const double* xy, int keep_y) {
+    if (!c) return fail_arg("null ctx");
+    if (inst < 0 || inst >= (int)c->inst.size()) return fail_arg("bad instance index");
+    if (!c->twofluid()) return fail_arg("tracers need a two-fluid context (they move in its velocity field)");
+    if (n < 0 || n > gfm::TR_MAX || (n > 0 && !xy)) return fail_arg("0 to 4 tracers, positions as x0 y0 x1 y1 ...");
+    InstHost& h = c->inst[inst];
+    if (c->committed && n != h.ntr) return fail_arg("the number of tracers is fixed at gfm_ctx_commit");
+    for (int k = 0; k < 2 * n; ++k) {
+        h.tr_hdr[k] = xy[k];
+        h.tr_hdr[2 * gfm::TR_MAX + k] = xy[k];
+    }
+    h.ntr = n;
+    h.tr_keep_y = keep_y != 0;
+    if (!c->committed || n == 0) return GFM_OK;
+    if (!keep_y != !h.d.tr_keep_y) return fail_arg("keep_y is fixed at gfm_ctx_commit");
+    HC(hipMemcpyAsync(h.d.trk, h.tr_hdr, sizeof(h.tr_hdr), hipMemcpyHostToDevice, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return GFM_OK;
+}
+
+int gfm_tracer_history(gfm_ctx* c, int inst, int n, double* out) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    const Inst& d = c->inst[inst].d;
+    if (d.ntr == 0) return fail_arg("instance has no tracers");
+    if (n < 0 || n > d.hist_cap || (n > 0 && !out)) return fail_arg("history length beyond capacity");
+    if (n == 0) return GFM_OK;
+    HC(hipMemcpyAsync(out, d.trk + gfm::TR_HDR, (size_t)n * (1 + 4 * d.ntr) * sizeof(double),
+                      hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return GFM_OK;
+}
+
+int gfm_set_output_hold(gfm_ctx* c, int inst, double last, double interval) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    const Inst& d = c->inst[inst].d;
+    const double v[2] = {last, interval};
+    const int on = interval >= 0.0 ? 1 : 0;
+    HC(hipMemcpyAsync(d.sd + gfm::SD_HOLD_LAST, v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+    HC(hipMemcpyAsync(d.si + gfm::SI_HOLD, &on, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return GFM_OK;
+}
+
+int gfm_download_z(gfm_ctx* c, int inst, double* z) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    if (!c->twofluid() || !z) return fail_arg("z needs a two-fluid context and an output array");
+    if (c->mixed()) gfm::launch_zfrac<float>(c->L);
+    else gfm::launch_zfrac<double>(c->L);
+    HC(hipGetLastError());
+    return xfer(c, inst, c->inst[inst].d.phiA, z, 1, false);
 }
 
 int gfm_mass_sums(gfm_ctx* c, int inst, double* out2) {

@@ -13,13 +13,17 @@ enum { BC_T = 0, BC_R = 1, BC_P = 2 };
 enum { V_ORIGINAL = 0, V_REAL = 1, V_RIEMANN = 2, V_MODIFIED = 3 };
 
 // double scalars per instance
-enum { SD_T = 0, SD_DT = 1, SD_MASS1 = 2, SD_MASS2 = 3, SD_N = 8 };
+// SD_HOLD_LAST / SD_HOLD_IVL: an instance with SI_HOLD set takes no step while
+// t - last > interval (the output test of sim_twofluid.cpp:79), until the
+// host moves `last` forward
+enum { SD_T = 0, SD_DT = 1, SD_MASS1 = 2, SD_MASS2 = 3, SD_HOLD_LAST = 4, SD_HOLD_IVL = 5, SD_N = 8 };
 // int scalars per instance
 enum {
     SI_NSTEP = 0, SI_ACTIVE = 1, SI_NIFACE = 2, SI_DTCOUNT = 3,
     SI_BBOX = 4,   // 8 ints: fluid1 imin,imax,jmin,jmax ; fluid2 ...
     SI_STATUS = 12, SI_ROUNDS = 13,
     SI_NLIST = 14,   // entries of the last interfacial list recorded in imap
+    SI_HOLD = 15,
     SI_N = 16
 };
 enum { ST_UNPHYS = 1, ST_RS = 2, ST_IOVF = 4, ST_IROUNDS = 8, ST_ILIST = 16 };
@@ -97,7 +101,14 @@ struct Inst {
     int* idone;                // 1 when an entry's star state is final after the parallel pass
     int* iseg;                 // interfacial cells per 256-cell row segment, then their offsets
     int nseg;                  // row segments (Ny x ceil(Nx / 256))
+    int ntr;                   // passive tracers (0..TR_MAX)
+    int tr_keep_y;             // tracers return to their initial y after each step
+    double* trk;               // TR_HDR doubles (positions, initial positions), then
+                               // hist_cap entries of 1 + 4 ntr doubles: t, (x, y, u, v) per tracer
 };
+
+constexpr int TR_MAX = 4;
+constexpr int TR_HDR = 4 * TR_MAX;
 
 // HOUC first-derivative stencil for positive velocity: offsets o0..o0+n-1
 struct Houc { double c[10]; int n, o0; };

@@ -232,7 +232,8 @@ __global__ __launch_bounds__(256) void k_dt_final(const Inst* __restrict__ insts
     const Inst& I = insts[work[blockIdx.x].x];
     const int nstep = gmem(I.si)[SI_NSTEP];
     const double t = gmem(I.sd)[SD_T];
-    const bool active = (t < I.T) && (nstep < I.max_steps);
+    const bool held = gmem(I.si)[SI_HOLD] != 0 && t - gmem(I.sd)[SD_HOLD_LAST] > gmem(I.sd)[SD_HOLD_IVL];
+    const bool active = (t < I.T) && (nstep < I.max_steps) && !held;
     __shared__ double red[4][256];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     for (int k = threadIdx.x; k < I.ntile_dt; k += 256) {
@@ -2537,6 +2538,48 @@ __device__ __forceinline__ bool frame_cell(const Inst& I, int strip, int idx, in
     return true;
 }
 
+// velocity at (x, y): the bilinear stencil of the probes on the two planes of
+// the velocity field (the solved velocity field's get_velocity, external)
+template <typename R>
+__device__ __forceinline__ void vel_at(const Inst& I, double x, double y, double& u, double& v) {
+#pragma clang fp contract(off)
+    const Probe p = bilinear_stencil(I, x, y);
+    const auto vel = gmem((const R*)I.vel);
+    const long long a = cidx(I, p.i0, p.j0), c = a + I.pitch;
+    const double sx = 1.0 - p.tx, sy = 1.0 - p.ty;
+    u = sy * (sx * (double)vel[a] + p.tx * (double)vel[a + 1]) +
+        p.ty * (sx * (double)vel[c] + p.tx * (double)vel[c + 1]);
+    v = sy * (sx * (double)vel[a + I.plane] + p.tx * (double)vel[a + 1 + I.plane]) +
+        p.ty * (sx * (double)vel[c + I.plane] + p.tx * (double)vel[c + 1 + I.plane]);
+}
+
+// the passive tracers of sim_twofluid.cpp:1861-1885 after step n (t, dt of
+// that step, the velocity field its ghost fill stored): record t and each
+// tracer's position and velocity, then move it by one classical RK4 step of
+// dt in that field (time_step_RK4, external), and put it back on its initial
+// y if asked (:1882-1884).  One thread.
+template <typename R>
+__device__ void advance_tracers(const Inst& I) {
+#pragma clang fp contract(off)
+    const int n = gmem(I.si)[SI_NSTEP];
+    const double t = gmem(I.sd)[SD_T], dt = gmem(I.sd)[SD_DT];
+    double* pos = I.trk;
+    const int stride = 1 + 4 * I.ntr;
+    double* h = n < I.hist_cap ? I.trk + TR_HDR + (long long)n * stride : nullptr;
+    if (h) h[0] = t;
+    for (int k = 0; k < I.ntr; ++k) {
+        const double x = pos[2 * k], y = pos[2 * k + 1];
+        double u1, v1, u2, v2, u3, v3, u4, v4;
+        vel_at<R>(I, x, y, u1, v1);
+        if (h) { h[1 + 4 * k] = x; h[2 + 4 * k] = y; h[3 + 4 * k] = u1; h[4 + 4 * k] = v1; }
+        vel_at<R>(I, x + 0.5 * dt * u1, y + 0.5 * dt * v1, u2, v2);
+        vel_at<R>(I, x + 0.5 * dt * u2, y + 0.5 * dt * v2, u3, v3);
+        vel_at<R>(I, x + dt * u3, y + dt * v3, u4, v4);
+        pos[2 * k] = x + dt / 6.0 * (u1 + 2.0 * u2 + 2.0 * u3 + u4);
+        pos[2 * k + 1] = I.tr_keep_y ? pos[2 * TR_MAX + 2 * k + 1] : y + dt / 6.0 * (v1 + 2.0 * v2 + 2.0 * v3 + v4);
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_bc(const Inst* __restrict__ insts, const int4* __restrict__ work,
                                             int q, int nfluid, int advance) {
@@ -2557,6 +2600,7 @@ __global__ __launch_bounds__(256) void k_bc(const Inst* __restrict__ insts, cons
     if (active && advance && w.y == 0 && w.z == 0 && threadIdx.x == 0) {
         gmem(I.sd)[SD_T] += gmem(I.sd)[SD_DT];
         gmem(I.si)[SI_NSTEP] += 1;
+        if (I.ntr > 0) advance_tracers<R>(I);
     }
 }
 
@@ -2812,6 +2856,56 @@ void launch_pure_rs(hipStream_t s, int n, int px, double g, double pinf, const d
     hipLaunchKernelGGL(k_pure_rs, dim3((n + 255) / 256), dim3(256), 0, s, n, px, g, pinf, UL, UR, F, st);
 }
 
+// ---------------------------------------------------------------------------
+// z: the fraction of each cell's area on fluid 1's side (phi <= 0), from the
+// plane through the cell centre with phi's value there and the level set's
+// unit normal (central differences, the (1,0) fallback of unit_normal).  The
+// area under a line in a rectangle (Scardovelli & Zaleski, J. Comput. Phys.
+// 164 (2000), 2D case): with a = |nx| dx <= b = |ny| dy (swapped if not) and
+// m = -phi + (a + b)/2, the fraction is 0 (m <= 0), m^2 / (2ab) (m < a),
+// (2m - a) / (2b) (m < b), 1 - (a + b - m)^2 / (2ab) (m < a + b), else 1.
+// The outermost ring of the padded grid takes 1 or 0 by phi's sign.
+// Written to phiA.  work: {inst, tile_y, tile_x, 0}, 64x4 tiles over the
+// padded array
+// ---------------------------------------------------------------------------
+__host__ __device__ inline double area_fraction(double phi, double nx, double ny, double dx, double dy) {
+#pragma clang fp contract(off)
+    double a = fabs(nx) * dx, b = fabs(ny) * dy;
+    if (a > b) { const double s = a; a = b; b = s; }
+    const double m = -phi + 0.5 * (a + b);
+    if (m <= 0.0) return 0.0;
+    if (m >= a + b) return 1.0;
+    if (m < a) return m * m / (2.0 * a * b);
+    if (m < b) return (2.0 * m - a) / (2.0 * b);
+    const double r = a + b - m;
+    return 1.0 - r * r / (2.0 * a * b);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_zfrac(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    const long long c = cidx(I, i, j);
+    const double ph = (double)ld1<R>(I.phi, c);
+    double z;
+    if (i == 0 || j == 0 || i == I.H - 1 || j == I.W - 1) {
+        z = ph <= 0.0 ? 1.0 : 0.0;
+    } else {
+        double nx, ny;
+        unit_normal<R>(I, i, j, nx, ny);
+        z = area_fraction(ph, nx, ny, I.dx, I.dy);
+    }
+    gmem((R*)I.phiA)[c] = (R)z;
+}
+
+template <typename R>
+void launch_zfrac(const LaunchCtx& L) {
+    hipLaunchKernelGGL((k_zfrac<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full);
+}
+
 __global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
@@ -2845,7 +2939,8 @@ void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, doub
     template void launch_reinit_now<R>(const LaunchCtx&, int);                \
     template void launch_bc<R>(const LaunchCtx&, int, int, int);               \
     template void launch_reset<R>(const LaunchCtx&, int);                      \
-    template void launch_ic<R>(const LaunchCtx&, const IcDesc*, const int*, int, int);
+    template void launch_ic<R>(const LaunchCtx&, const IcDesc*, const int*, int, int); \
+    template void launch_zfrac<R>(const LaunchCtx&);
 INST(double)
 INST(float)
 

@@ -45,6 +45,8 @@ template <typename R> void launch_ls(const LaunchCtx& L, const Houc& H, int ever
 template <typename R> void launch_reinit_now(const LaunchCtx& L, int iters);
 template <typename R> void launch_bc(const LaunchCtx& L, int q, int nfluid, int advance);
 template <typename R> void launch_reset(const LaunchCtx& L, int narrow_boxes);
+// z, the fraction of each cell on fluid 1's side of phi's local plane, into phiA
+template <typename R> void launch_zfrac(const LaunchCtx& L);
 // initial conditions on the device: instance k gets descs[map[k]] (map[k] < 0: untouched), its
 // conserved grids Q[.][q] and phi
 struct IcDesc {   // the layout of gfm_ic_desc (include/gfm_hip.h)

@@ -32,6 +32,7 @@ EXPORTS = [
     "gfm_compute_dt", "gfm_set_ghost_states", "gfm_fs_update", "gfm_ls_update", "gfm_apply_bcs",
     "gfm_extrapolate", "gfm_ls_reinit", "gfm_get_bbox", "gfm_set_bbox",
     "gfm_step", "gfm_step_graph", "gfm_sync", "gfm_profile_steps", "gfm_status", "gfm_mass_sums", "gfm_mass_history", "gfm_iface_info",
+    "gfm_set_output_hold", "gfm_set_tracers", "gfm_tracer_history", "gfm_download_z",
     "gfm_pure_rs_solve", "gfm_pure_rs_solve_f32", "gfm_mixed_rs_solve", "gfm_pure_rs_solve_noalloc",
     "gfm_ic_onefluid", "gfm_ic_bubble", "gfm_ic_rmi", "gfm_ic_planar", "gfm_shock_state", "gfm_ic_device",
     "gfm_last_error",
@@ -122,6 +123,10 @@ def load():
         "gfm_status": [vp, C.c_int, _U], "gfm_mass_sums": [vp, C.c_int, _D],
         "gfm_mass_history": [vp, C.c_int, C.c_int, _D],
         "gfm_iface_info": [vp, C.c_int, _I, _I],
+        "gfm_set_output_hold": [vp, C.c_int, C.c_double, C.c_double],
+        "gfm_set_tracers": [vp, C.c_int, C.c_int, _D, C.c_int],
+        "gfm_tracer_history": [vp, C.c_int, C.c_int, _D],
+        "gfm_download_z": [vp, C.c_int, _D],
         "gfm_pure_rs_solve": [C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _D, _U],
         "gfm_pure_rs_solve_f32": [C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _D, _U],
         "gfm_pure_rs_solve_noalloc": [C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _D, _U],
@@ -171,6 +176,7 @@ class Context:
         self.h = C.c_void_p()
         _check(L.gfm_ctx_create(C.byref(m), C.byref(self.h)), "gfm_ctx_create")
         self.infos = []
+        self.ntr = {}
 
     def close(self):
         if self.h:
@@ -292,6 +298,28 @@ class Context:
         out = np.zeros((n, 3))
         _check(load().gfm_mass_history(self.h, k, n, _dp(out)), "gfm_mass_history")
         return out
+
+    def set_output_hold(self, k, last, interval):
+        """instance k takes no step while t - last > interval (interval < 0: off)"""
+        _check(load().gfm_set_output_hold(self.h, k, last, interval), "gfm_set_output_hold")
+
+    def set_tracers(self, k, xy, keep_y=False):
+        """passive RK4 tracers of instance k; xy: (n, 2) positions, n <= 4"""
+        xy = np.ascontiguousarray(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
+        _check(load().gfm_set_tracers(self.h, k, len(xy), _dp(xy), int(keep_y)), "gfm_set_tracers")
+        self.ntr[k] = len(xy)
+
+    def tracer_history(self, k, n):
+        """(n, 1 + 4 ntr): t, then x, y, u, v per tracer, for steps 0 .. n-1"""
+        out = np.zeros((n, 1 + 4 * self.ntr.get(k, 0)))
+        _check(load().gfm_tracer_history(self.h, k, n, _dp(out)), "gfm_tracer_history")
+        return out
+
+    def download_z(self, k):
+        H, W = self.infos[k].shape
+        z = np.zeros((H, W))
+        _check(load().gfm_download_z(self.h, k, _dp(z)), "gfm_download_z")
+        return z
 
     def iface_info(self, k):
         n, r = C.c_int(0), C.c_int(0)

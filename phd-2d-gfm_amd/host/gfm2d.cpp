@@ -326,6 +326,10 @@ void sim_onefluid::run_sim(GFM_settingsfile SF) {
     std::vector<double> g(4 * n);
     check(gfm_ic_onefluid(&s, &ce, onefluid_case_id(SF.test_case), g.data()), "gfm_ic_onefluid");
     check(gfm_upload_state(c, 0, g.data(), nullptr, nullptr), "gfm_upload_state");
+    // output at step 0 and at the end (:34, :60); the per-step test
+    // numsteps % output_freq (:54) never holds with output_freq 10^9
+    write_prims(SF.basename + "-0-prims.dat", p, e, g, nullptr, nullptr, 0, p.Ny + 2 * p.numGC, 0,
+                p.Nx + 2 * p.numGC);
     std::printf("[%s] Initialisation complete. Beginning time iterations with CFL = %g.\n", SF.basename.c_str(), SF.CFL);
     const double t0 = now();
     double t = 0.0, dt = 0.0;
@@ -361,8 +365,8 @@ void set_twofluid_parameters(const GFM_settingsfile& SF, sim_info& p, binarySGpa
     } else if (tc == "shocked_helium_bubble") {
         e.gamma1 = 1.667; p.dx = 325.0 / p.Nx; p.dy = 89.0 / p.Ny; p.T = 280.0;
         p.BC_T = p.BC_B = "reflective"; p.output_freq = 2;
-    } else if (tc == "RMI") {
-        e.gamma2 = 1.093; p.dx = 4.0 / p.Nx; p.dy = 1.0 / p.Ny; p.T = 10.0;
+    } else if (tc == "RMI" || tc == "RMI_pert" || tc == "RMI_unpert") {   // :430-442
+        e.gamma2 = 1.093; p.dx = 4.0 / p.Nx; p.dy = 1.0 / p.Ny; p.T = tc == "RMI_pert" ? 4.0 : 10.0;
         p.BC_T = p.BC_B = "reflective"; p.output_freq = 10;
     } else if (tc == "shocked_bubble") {          // SI units, parameterised gas
         e.gamma1 = SF.bubble_gamma; p.dx = 0.325 / p.Nx; p.dy = 0.089 / p.Ny; p.T = 1e300;
@@ -373,6 +377,72 @@ void set_twofluid_parameters(const GFM_settingsfile& SF, sim_info& p, binarySGpa
     } else {
         throw std::runtime_error("Invalid test_case choice.");
     }
+}
+
+// sim_twofluid::output (sim_twofluid.cpp:163-295): <stem>-prims.dat,
+// -ls.dat, -vfield.dat, -z.dat, -schlieren.dat and <base>-GDAerr.dat
+static void write_twofluid_output(const std::string& stem, const std::string& base, const sim_info& p,
+                                  const binarySGparams& e, const std::vector<double>& g1,
+                                  const std::vector<double>& g2, const std::vector<double>& phi,
+                                  const std::vector<double>& vf, const std::vector<double>& z) {
+    const int H = p.Ny + 2 * p.numGC, W = p.Nx + 2 * p.numGC, g = p.numGC;
+    write_prims(stem + "-prims.dat", p, e, g1, &g2, &phi, 2, H - 2, 2, W - 2);
+    {
+        std::ofstream ls(stem + "-ls.dat"), v(stem + "-vfield.dat"), zo(stem + "-z.dat");
+        for (int i = 2; i < H - 2; ++i) {
+            for (int j = 2; j < W - 2; ++j) {
+                const vec2 cc = p.cellcentre_coord(i, j);
+                const size_t q = (size_t)i * W + j;
+                ls << cc[0] << " " << cc[1] << " " << phi[q] << "\n";
+                v << cc[0] << " " << cc[1] << " " << vf[2 * q] << " " << vf[2 * q + 1] << "\n";
+                zo << cc[0] << " " << cc[1] << " " << z[q] << "\n";
+            }
+            ls << "\n";
+            v << "\n";
+            zo << "\n";
+        }
+    }
+    // numerical schlieren (:226-271): |grad rho| by central differences over
+    // the interior, each neighbour's density from the fluid real there
+    // (phi <= 0: fluid 1), over its maximum
+    auto rho = [&](int i, int j) {
+        const size_t q = (size_t)i * W + j;
+        return phi[q] <= 0.0 ? g1[4 * q] : g2[4 * q];
+    };
+    std::vector<double> grad;
+    grad.reserve((size_t)p.Nx * p.Ny);
+    for (int i = g; i < p.Ny + g; ++i)
+        for (int j = g; j < p.Nx + g; ++j) {
+            const double gx = (rho(i, j + 1) - rho(i, j - 1)) / (2.0 * p.dx);
+            const double gy = (rho(i + 1, j) - rho(i - 1, j)) / (2.0 * p.dy);
+            grad.push_back(std::sqrt(gx * gx + gy * gy));
+        }
+    const double gmax = *std::max_element(grad.begin(), grad.end());
+    {
+        std::ofstream out(stem + "-schlieren.dat");
+        size_t k = 0;
+        for (int i = g; i < p.Ny + g; ++i) {
+            for (int j = g; j < p.Nx + g; ++j) {
+                const vec2 cc = p.cellcentre_coord(i, j);
+                out << cc[0] << " " << cc[1] << " " << grad[k++] / gmax << "\n";
+            }
+            out << "\n";
+        }
+    }
+    // L1 distance of fluid 1's density to the Gaussian density bump of the
+    // advection test, 1 + 1000 exp(-|x - (0.5, 0.5)|^2 / (2 0.1^2)), written
+    // for every case and overwritten at every output, as the reference does
+    // (:273-291)
+    double l1 = 0.0;
+    for (int i = g; i < p.Ny + g; ++i)
+        for (int j = g; j < p.Nx + g; ++j) {
+            const vec2 cc = p.cellcentre_coord(i, j);
+            const double r2 = (cc[0] - 0.5) * (cc[0] - 0.5) + (cc[1] - 0.5) * (cc[1] - 0.5);
+            const double exact = 1.0 + 1000.0 * std::exp(-r2 / (2.0 * 0.1 * 0.1));
+            l1 += std::fabs(g1[4 * ((size_t)i * W + j)] - exact) * p.dx * p.dy;
+        }
+    std::ofstream err(base + "-GDAerr.dat");
+    err << p.Nx << " " << l1 << "\n";
 }
 
 static int ls_order(const std::string& itm) {
@@ -406,6 +476,11 @@ void sim_twofluid::run_sim(GFM_settingsfile SF) {
     const gfm_eos ce{e.gamma1, e.pinf1, e.gamma2, e.pinf2};
     const int max_steps = SF.steps > 0 ? SF.steps : (1 << 30);
     check(gfm_instance_add(c, &s, &ce, SF.CFL, p.T, max_steps, nullptr), "gfm_instance_add");
+    // the two points of output_perturbation_amplitude (:1852-1853), kept on
+    // their rows (:1882-1884)
+    const bool tracers = SF.test_case == "RMI_pert";
+    const double xy[4] = {2.88, 1.0, 2.92, 0.5};
+    if (tracers) check(gfm_set_tracers(c, 0, 2, xy, 1), "gfm_set_tracers");
     check(gfm_ctx_commit(c), "gfm_ctx_commit");
     const size_t n = (size_t)(p.Ny + 2 * p.numGC) * (p.Nx + 2 * p.numGC);
     std::vector<double> g1(4 * n), g2(4 * n), phi(n);
@@ -424,10 +499,12 @@ void sim_twofluid::run_sim(GFM_settingsfile SF) {
         const double he[4] = {0.138, 0.0, 0.0, 1.0}, pre[4] = {1.0, 0.0, 0.0, 1.0};
         const double post[4] = {1.3764, -0.394, 0.0, 1.5698};
         check(gfm_ic_bubble(&s, &ce, 175.0, 44.5, 25.0, he, pre, post, 225.0, g1.data(), g2.data(), phi.data()), "ic");
-    } else if (tc == "RMI" || tc == "RMI_periodic") {   // sim_twofluid.cpp:1245-1347
+    } else if (tc == "RMI" || tc == "RMI_pert" || tc == "RMI_unpert" || tc == "RMI_periodic") {
+        // sim_twofluid.cpp:1245-1347 (RMI), :1348-1453 (RMI_pert), :1454-1553 (RMI_unpert): the
+        // interface x = 2.9 - eps cos(2 pi y) with eps 0.2, 0.02, 0
         const double air[4] = {1.0, 0.0, 0.0, 1.0}, sf6[4] = {5.04, 0.0, 0.0, 1.0};
         double post[4] = {1.411, -0.39, 0.0, 1.628};
-        double amp = 0.2;
+        double amp = tc == "RMI_pert" ? 0.02 : (tc == "RMI_unpert" ? 0.0 : 0.2);
         int modes = 1;
         if (tc == "RMI_periodic") {
             check(gfm_shock_state(1.4, 0.0, air, SF.mach, -1, post), "shock");
@@ -446,95 +523,66 @@ void sim_twofluid::run_sim(GFM_settingsfile SF) {
     }
     check(gfm_upload_state(c, 0, g1.data(), g2.data(), phi.data()), "gfm_upload_state");
     check(gfm_ls_reinit(c, init_reinit), "gfm_ls_reinit");   // ls->initialisation() (:47)
+    const int H = p.Ny + 2 * p.numGC, W = p.Nx + 2 * p.numGC;
+    std::vector<double> vf(2 * (size_t)H * W), z((size_t)H * W);
+    // sim_twofluid::output (:163-295) of the current state
+    auto output = [&](int nstep) {
+        check(gfm_download_state(c, 0, g1.data(), g2.data(), phi.data()), "gfm_download_state");
+        // the velocity field of the last ghost fill, the one the level set was
+        // advected with (:181,198); zero before the first step
+        check(gfm_download_vfield(c, 0, vf.data()), "gfm_download_vfield");
+        check(gfm_download_z(c, 0, z.data()), "gfm_download_z");
+        write_twofluid_output(SF.basename + "-" + std::to_string(nstep), SF.basename, p, e, g1, g2, phi, vf, z);
+    };
+    output(0);
+    const double interval = p.T / p.output_freq;
+    const bool periodic_output = std::isfinite(interval);
+    double last = 0.0;
+    if (periodic_output) check(gfm_set_output_hold(c, 0, last, interval), "gfm_set_output_hold");
     std::printf("[%s] Initialisation complete. Beginning time iterations with CFL = %g.\n", SF.basename.c_str(), SF.CFL);
     const double t0 = now();
     double t = 0.0, dt = 0.0;
     int nstep = 0, prev = -1;
+    // whole graphs of steps; the output hold stops the instance after the step
+    // the reference writes its output after (t - lastoutputtime > interval,
+    // :79-83), so the host writes it and moves `last` on
     while (nstep != prev) {
         prev = nstep;
         check(gfm_step_graph(c, 50), "gfm_step_graph");
         check(gfm_get_time(c, 0, &t, &nstep, &dt), "gfm_get_time");
+        if (periodic_output && t - last > interval) {
+            output(nstep);
+            last = t;
+            check(gfm_set_output_hold(c, 0, last, interval), "gfm_set_output_hold");
+            prev = -1;
+        }
     }
     const double t1 = now();
     check(gfm_compute_dt(c), "gfm_compute_dt");   // records the final masses
     unsigned st = 0;
     check(gfm_status(c, 0, &st), "gfm_status");
-    check(gfm_download_state(c, 0, g1.data(), g2.data(), phi.data()), "gfm_download_state");
-    const int H = p.Ny + 2 * p.numGC, W = p.Nx + 2 * p.numGC;
-    write_prims(SF.basename + "-" + std::to_string(nstep) + "-prims.dat", p, e, g1, &g2, &phi, 2, H - 2, 2, W - 2);
+    output(nstep);                                 // :94
     {
-        std::ofstream out(SF.basename + "-" + std::to_string(nstep) + "-ls.dat");
-        for (int i = 2; i < H - 2; ++i) {
-            for (int j = 2; j < W - 2; ++j) {
-                const vec2 cc = p.cellcentre_coord(i, j);
-                out << cc[0] << " " << cc[1] << " " << phi[(size_t)i * W + j] << "\n";
-            }
-            out << "\n";
-        }
-    }
-    {
-        // the velocity field of the last ghost fill, the one the level set
-        // was advected with (sim_twofluid.cpp:181,198)
-        std::vector<double> vf(2 * (size_t)H * W);
-        check(gfm_download_vfield(c, 0, vf.data()), "gfm_download_vfield");
-        std::ofstream out(SF.basename + "-" + std::to_string(nstep) + "-vfield.dat");
-        for (int i = 2; i < H - 2; ++i) {
-            for (int j = 2; j < W - 2; ++j) {
-                const vec2 cc = p.cellcentre_coord(i, j);
-                const size_t q = (size_t)i * W + j;
-                out << cc[0] << " " << cc[1] << " " << vf[2 * q] << " " << vf[2 * q + 1] << "\n";
-            }
-            out << "\n";
-        }
-    }
-    {
-        // numerical schlieren (sim_twofluid.cpp:226-271): |grad rho| by
-        // central differences over the interior, each neighbour's density
-        // from the fluid real there (phi <= 0: fluid 1), over its maximum
-        const int g = p.numGC;
-        auto rho = [&](int i, int j) {
-            const size_t q = (size_t)i * W + j;
-            return phi[q] <= 0.0 ? g1[4 * q] : g2[4 * q];
-        };
-        std::vector<double> grad;
-        grad.reserve((size_t)p.Nx * p.Ny);
-        for (int i = g; i < p.Ny + g; ++i)
-            for (int j = g; j < p.Nx + g; ++j) {
-                const double gx = (rho(i, j + 1) - rho(i, j - 1)) / (2.0 * p.dx);
-                const double gy = (rho(i + 1, j) - rho(i - 1, j)) / (2.0 * p.dy);
-                grad.push_back(std::sqrt(gx * gx + gy * gy));
-            }
-        const double gmax = *std::max_element(grad.begin(), grad.end());
-        std::ofstream out(SF.basename + "-" + std::to_string(nstep) + "-schlieren.dat");
-        size_t k = 0;
-        for (int i = g; i < p.Ny + g; ++i) {
-            for (int j = g; j < p.Nx + g; ++j) {
-                const vec2 cc = p.cellcentre_coord(i, j);
-                out << cc[0] << " " << cc[1] << " " << grad[k++] / gmax << "\n";
-            }
-            out << "\n";
-        }
-        // L1 distance of fluid 1's density to the Gaussian density bump of
-        // the advection test, 1 + 1000 exp(-|x - (0.5, 0.5)|^2 / (2 0.1^2)),
-        // written for every case as the reference does (:273-291)
-        double l1 = 0.0;
-        for (int i = g; i < p.Ny + g; ++i)
-            for (int j = g; j < p.Nx + g; ++j) {
-                const vec2 cc = p.cellcentre_coord(i, j);
-                const double r2 = (cc[0] - 0.5) * (cc[0] - 0.5) + (cc[1] - 0.5) * (cc[1] - 0.5);
-                const double exact = 1.0 + 1000.0 * std::exp(-r2 / (2.0 * 0.1 * 0.1));
-                l1 += std::fabs(g1[4 * ((size_t)i * W + j)] - exact) * p.dx * p.dy;
-            }
-        std::ofstream err(SF.basename + "-GDAerr.dat");
-        err << p.Nx << " " << l1 << "\n";
-    }
-    {
-        const int nh = std::min(nstep + 1, max_steps + 1);
+        const int nh = std::min({nstep + 1, max_steps + 1, 1 << 20});   // the histories' capacity
         std::vector<double> h(3 * (size_t)nh);
         check(gfm_mass_history(c, 0, nh, h.data()), "gfm_mass_history");
         std::ofstream out(SF.basename + "-masschange.dat");
         for (int k = 0; k < nh; ++k)
             out << h[3 * k] << " " << (h[3 * k + 1] - h[1]) / h[1] << " " << (h[3 * k + 2] - h[2]) / h[2] << "\n";
+    }
+    if (tracers) {
+        // output_perturbation_amplitude (:1831-1887), one line per step:
+        // t, the amplitude 0.5 (x1 - x0) and both points' x, then the same
+        // for their x velocities
+        const int nh = std::min({nstep + 1, max_steps + 1, 1 << 20});   // the histories' capacity
+        std::vector<double> h(9 * (size_t)nh);
+        check(gfm_tracer_history(c, 0, nh, h.data()), "gfm_tracer_history");
+        std::ofstream amp(SF.basename + "-pertamp.dat"), der(SF.basename + "-pertampderiv.dat");
+        for (int k = 1; k < nh; ++k) {
+            const double* r = &h[9 * (size_t)k];
+            amp << r[0] << " " << 0.5 * (r[5] - r[1]) << " " << r[5] << " " << r[1] << "\n";
+            der << r[0] << " " << 0.5 * (r[7] - r[3]) << " " << r[7] << " " << r[3] << "\n";
+        }
     }
     {
         std::ofstream out(SF.basename + "-runtime.dat");

@@ -105,6 +105,8 @@ def _bind(L):
         "or_ls_normal": (None, [C.POINTER(Grid), D, i32, i32, D]),
         "or_ls_is_interfacial": (i32, [C.POINTER(Grid), D, i32, i32]),
         "or_bilinear": (None, [C.POINTER(Grid), D, dbl, dbl, D]),
+        "or_tracers_step": (None, [C.POINTER(Grid), D, i32, D, D, i32, dbl, dbl, D]),
+        "or_volume_fraction": (None, [C.POINTER(Grid), D, D]),
         "or_houc_coeffs": (None, [i32, D, I]),
         "or_ls_advect": (None, [C.POINTER(Grid), i32, D, D, dbl]),
         "or_ls_reinit": (None, [C.POINTER(Grid), D, i32]),

@@ -27,19 +27,26 @@ def read_dat(path):
     return np.array(rows, dtype=np.float64)
 
 
+def output_steps(tmp_path, suffix="-prims.dat"):
+    """{step: file name} of the per-output files <base>-<step><suffix>"""
+    return {int(p[:-len(suffix)].split("-")[-1]): p for p in os.listdir(tmp_path) if p.endswith(suffix)}
+
+
 def test_gfm2d_onefluid_circular_explosion(tmp_path, oracle):
     f = settings(tmp_path, Nx=101, Ny=101, CFL=0.7, sim_type="onefluid", FS="MUSCL", test_case="circularexplosion")
     out = subprocess.run([GFM2D, f], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     assert out.returncode == 0
-    files = [p for p in os.listdir(tmp_path) if p.endswith("-prims.dat")]
-    assert len(files) == 1
-    nsteps = int(files[0].split("-")[-2])
-    got = read_dat(os.path.join(tmp_path, files[0]))
+    files = output_steps(tmp_path)
+    assert len(files) == 2 and 0 in files         # step 0 and the end (sim_onefluid.cpp:34, :60)
+    nsteps = max(files)
+    got = read_dat(os.path.join(tmp_path, files[nsteps]))
     G = oracle.Grid(101, 101, 2, 0.0, 0.0, 2 / 101, 2 / 101, 0, 0, 0, 0)
     H, W = 105, 105
     grid = np.zeros((H, W, 4))
     oracle.lib().or_ic_onefluid(G, 1.4, 0.0, 100, oracle._p(grid))
+    first = read_dat(os.path.join(tmp_path, files[0]))
+    assert np.allclose(first[:, 2], grid[..., 0].reshape(-1), rtol=2e-5, atol=0)
     s = np.zeros(1, dtype=np.int32)
     t = np.zeros(1)
     assert oracle.lib().or_onefluid_run(G, 1.4, 0.0, 0, 0.7, 0.25, 100000, oracle._p(grid), oracle._p(s),
@@ -58,7 +65,9 @@ def test_gfm2d_helium_bubble(tmp_path, oracle):
     out = subprocess.run([GFM2D, f], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     assert out.returncode == 0 and "status 0" in out.stdout
-    prims = [p for p in os.listdir(tmp_path) if p.endswith("-prims.dat")][0]
+    files = output_steps(tmp_path)
+    assert sorted(files) == [0, 40]               # T / output_freq = 140 is not reached in 40 steps
+    prims = files[40]
     got = read_dat(os.path.join(tmp_path, prims))
     G = oracle.Grid(130, 36, 8, 0.0, 0.0, 325.0 / 130, 89.0 / 36, 0, 0, 1, 1)
     E = oracle.Eos(1.667, 0.0, 1.4, 0.0)
@@ -125,3 +134,67 @@ def test_gfm2d_helium_bubble_thesis_ls1(tmp_path):
     assert steps == 3083
     assert mass[-1, 0] == pytest.approx(280.0)
     assert abs(mass[-1, 1] - (-0.208)) <= 0.01
+
+
+def test_gfm2d_rmi_pert_outputs(tmp_path, oracle):
+    """RMI_pert (sim_twofluid.cpp:430-442, :1348-1453) at 128x32, 120 steps: the periodic output after
+    every step with t - lastoutputtime > T / output_freq = 0.4 (:79-83) plus step 0 and the end, the -z
+    files (fluid 1's area, 1.1, at step 0), and -pertamp / -pertampderiv (:1831-1887) against the
+    oracle's run with its tracers moved in each step's velocity field."""
+    n = 120
+    f = settings(tmp_path, Nx=128, Ny=32, CFL=0.5, sim_type="twofluid", FS="MUSCL", test_case="RMI_pert",
+                 ITM="LS5", GFM="riemannGFM", steps=n)
+    out = subprocess.run([GFM2D, f], capture_output=True, text=True, timeout=120)
+    print(out.stdout[-2000:], out.stderr)
+    assert out.returncode == 0 and "status 0" in out.stdout
+    mass = read_dat(os.path.join(tmp_path, [p for p in os.listdir(tmp_path) if p.endswith("masschange.dat")][0]))
+    assert mass.shape == (n + 1, 3)
+    # the output steps by the reference's rule, from the printed times (6 digits: the rule is applied
+    # to the oracle's times below as well)
+    G_ = oracle.Grid(128, 32, 8, 0.0, 0.0, 4.0 / 128, 1.0 / 32, 0, 0, 1, 1)
+    E = oracle.Eos(1.4, 0.0, 1.093, 0.0)
+    H, W = 48, 144
+    g1, g2, phi = np.zeros((H, W, 4)), np.zeros((H, W, 4)), np.zeros((H, W))
+    oracle.lib().or_ic_rmi(G_, E, 2.9, 0.02, 1, oracle._p(oracle.arr(1, 0, 0, 1)),
+                           oracle._p(oracle.arr(1.411, -0.39, 0, 1.628)), oracle._p(oracle.arr(5.04, 0, 0, 1)),
+                           3.2, oracle._p(g1), oracle._p(g2), oracle._p(phi))
+    oracle.lib().or_ls_reinit(G_, oracle._p(phi), 40)
+    run = oracle.Run(2, 0, 5, 5, 10, 0.5, 4.0, n)
+    t = np.zeros(1)
+    dt = np.zeros(1)
+    vf, s1, s2 = np.zeros((H, W, 2)), np.zeros((H, W, 4)), np.zeros((H, W, 4))
+    pos = np.array([[2.88, 1.0], [2.92, 0.5]])
+    p0 = pos.copy()
+    rows, times = [], [0.0]
+    for s in range(n):
+        assert oracle.lib().or_twofluid_step(G_, E, run, s, oracle._p(t), oracle._p(g1), oracle._p(g2),
+                                             oracle._p(phi), oracle._p(vf), oracle._p(s1), oracle._p(s2),
+                                             oracle._p(dt)) == 0
+        e = np.zeros(9)
+        oracle.lib().or_tracers_step(G_, oracle._p(vf), 2, oracle._p(pos), oracle._p(p0), 1, t[0], dt[0],
+                                     oracle._p(e))
+        rows.append(e)
+        times.append(t[0])
+    expect, last = {0, n}, 0.0
+    for s in range(1, n + 1):
+        if times[s] - last > 0.4:
+            expect.add(s)
+            last = times[s]
+    assert len(expect) >= 4
+    for suffix in ("-prims.dat", "-ls.dat", "-vfield.dat", "-z.dat", "-schlieren.dat"):
+        assert set(output_steps(tmp_path, suffix)) == expect, suffix
+    z0 = read_dat(os.path.join(tmp_path, output_steps(tmp_path, "-z.dat")[0]))
+    inside = (z0[:, 0] > 0) & (z0[:, 0] < 4) & (z0[:, 1] > 0) & (z0[:, 1] < 1)
+    assert abs(z0[inside, 2].sum() * G_.dx * G_.dy - 1.1) <= 2e-3
+    amp = read_dat(os.path.join(tmp_path, [p for p in os.listdir(tmp_path) if p.endswith("-pertamp.dat")][0]))
+    der = read_dat(os.path.join(tmp_path, [p for p in os.listdir(tmp_path) if p.endswith("-pertampderiv.dat")][0]))
+    rows = np.array(rows)
+    assert amp.shape == (n, 4) and der.shape == (n, 4)
+    assert np.allclose(amp[0], [rows[0, 0], 0.02, 2.92, 2.88], rtol=2e-5, atol=0)
+    ref_amp = np.stack([rows[:, 0], 0.5 * (rows[:, 5] - rows[:, 1]), rows[:, 5], rows[:, 1]], axis=1)
+    assert np.allclose(amp, ref_amp, rtol=2e-5, atol=1e-7)
+    ref_der = np.stack([rows[:, 0], 0.5 * (rows[:, 7] - rows[:, 3]), rows[:, 7], rows[:, 3]], axis=1)
+    umax = np.abs(rows[:, [3, 7]]).max()
+    assert umax > 0.1                       # the shock has reached both points
+    assert np.allclose(der[:, 0], ref_der[:, 0], rtol=2e-5, atol=0)
+    assert np.abs(der[:, 1:] - ref_der[:, 1:]).max() <= 2e-5 * umax
