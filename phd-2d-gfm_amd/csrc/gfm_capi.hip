@@ -1051,6 +1051,9 @@ int gfm_fs_update(gfm_ctx* c) {
     if (r) return r;
     const bool muscl = c->m.fs == GFM_FS_MUSCL;
     const int q = c->twofluid() ? 0 : c->cur;
+    // the update writes the cells of the realcells boxes only; every other
+    // cell keeps its input value (FS_MUSCL.cpp:162-169, :296-302)
+    if ((r = copy_q(c, q, 1 - q))) return r;
     if (c->mixed()) gfm::launch_fs<float>(c->L, muscl, q, 1 - q, 0);
     else gfm::launch_fs<double>(c->L, muscl, q, 1 - q, 0);
     if ((r = post_launch(c))) return r;

@@ -186,6 +186,14 @@ vec2 Levelset::get_normal(int i, int j) const {
             (get_sdf(i + 1, j) - get_sdf(i - 1, j)) / (2.0 * params.dy)};
 }
 
+vec2 Levelset::get_normal(const vec2& pos) const {
+    const int j = (int)std::floor((pos[0] - params.x0) / params.dx) + params.numGC;
+    const int i = (int)std::floor((pos[1] - params.y0) / params.dy) + params.numGC;
+    const int H = params.Ny + 2 * params.numGC;
+    if (i < 1 || j < 1 || i > H - 2 || j > W - 2) throw std::runtime_error("get_normal: position outside the grid");
+    return get_normal(i, j);
+}
+
 bool Levelset::is_interfacial_cell(int i, int j) const {
     const bool s = get_sdf(i, j) <= 0.0;
     return ((get_sdf(i, j + 1) <= 0.0) != s) || ((get_sdf(i, j - 1) <= 0.0) != s) ||

@@ -132,6 +132,9 @@ public:
     virtual double get_sdf(int i, int j) const = 0;
     virtual void set_sdf(int i, int j, double v) = 0;
     virtual vec2 get_normal(int i, int j) const = 0;
+    // at a position, as the reference calls it (GFM_base.cpp:19,
+    // GFM_real.cpp:59); every call site passes a cell centre
+    virtual vec2 get_normal(const vec2& pos) const = 0;
     virtual bool is_interfacial_cell(int i, int j) const = 0;
     virtual void initialisation() = 0;
     virtual void update_interface_coupled(const solved_velocity_field& vfield, double t, double dt) = 0;
@@ -145,6 +148,7 @@ public:
     double get_sdf(int i, int j) const override { return phi[(size_t)i * W + j]; }
     void set_sdf(int i, int j, double v) override { phi[(size_t)i * W + j] = v; }
     vec2 get_normal(int i, int j) const override;
+    vec2 get_normal(const vec2& pos) const override;   // the normal of the cell containing pos
     bool is_interfacial_cell(int i, int j) const override;
     void initialisation() override {}
     void update_interface_coupled(const solved_velocity_field& vfield, double t, double dt) override;

@@ -77,9 +77,12 @@ def run_thesis(tmp_path, ls):
     out = subprocess.run([GFM2D, str(f)], capture_output=True, text=True, timeout=300)
     print(out.stdout[-500:], out.stderr[-500:])
     assert out.returncode == 0 and "status 0" in out.stdout
-    ls_files = [p for p in os.listdir(tmp_path) if p.endswith("-ls.dat")]
-    assert len(ls_files) == 1
-    return int(ls_files[0].split("-")[-2]), read_ls(os.path.join(tmp_path, ls_files[0]))
+    # outputs at step 0, after the step passing t = T / output_freq = 140 (sim_twofluid.cpp:79-83) and at
+    # the end; the thesis contours are of the end
+    ls_files = {int(p.split("-")[-2]): p for p in os.listdir(tmp_path) if p.endswith("-ls.dat")}
+    assert len(ls_files) == 3 and 0 in ls_files
+    n = max(ls_files)
+    return n, read_ls(os.path.join(tmp_path, ls_files[n]))
 
 
 @pytest.mark.parametrize("ls,fixture,steps,mean_tol,haus_tol", [("LS1", "zerocont1.dat", 3083, 1.0, 5.0),
