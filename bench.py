@@ -47,7 +47,10 @@ cell-update (F = 2 fluids, L = 1 level set, s = word size).
 For mixed-precision workloads the FP64 path runs the same job the same way
 (`fp64_plain`) and the states after the timed window are compared
 (`accuracy`: relative L1 / Linf per conserved variable of the composite
-real-fluid state, interface-contour L1); --perturb adds the same comparison
+real-fluid state, interface-contour L1), beside the same comparison for the
+FP64 job with its state rounded to FP32 after every step
+(`fp64_f32storage`: what FP32 storage alone does to the job; the mixed path
+stores its state in FP32 too); --perturb adds the same comparison
 between FP64 and FP64 from an initial state perturbed by 1e-7 relative,
 which measures how much the flow itself amplifies a last-digit change.
 `cpu_baseline` times the FP64 CPU oracle on a bounded sample of the same
@@ -269,8 +272,9 @@ def accuracy(infos, got, ref):
     return A.aggregate(res, [i.Nx * i.Ny for i in infos])
 
 
-def run_job(setup, states, variant, precision, steps, init_reinit):
-    ctx = new_ctx(setup, variant, precision, steps + 1)
+def run_job(setup, states, variant, precision, steps, init_reinit, env=None):
+    with T.environment(**(env or {})):   # the library reads its switches at commit
+        ctx = new_ctx(setup, variant, precision, steps + 1)
     load_ic(ctx, states, init_reinit)
     ctx.step_graph(steps)
     ctx.sync()
@@ -427,6 +431,13 @@ def main():
         total = advance + a.warmup + a.steps
         acc = {"vs": "FP64 GPU path, same job, after %d steps from the initial condition" % total,
                GFM_LABEL[variant]: accuracy(infos, final, ref)}
+        # the same FP64 job with its state rounded to FP32 after every step:
+        # what FP32 storage alone does, beside which the mixed error is judged
+        st32 = run_job(setup, states, variant, G.FP64, total, init_reinit, {"GFM_ROUND_STATE_F32": "1"})
+        acc["fp64_f32storage"] = accuracy(infos, st32, ref)
+        acc["mixed_over_f32storage_L1_rho"] = (acc[GFM_LABEL[variant]]["L1_rho"] /
+                                               max(acc["fp64_f32storage"]["L1_rho"], 1e-300))
+        del st32
         if a.perturb:
             _, pstates = make_setup(builder, cases, perturb=1e-7)
             pert = run_job(setup, pstates, variant, G.FP64, total, init_reinit)

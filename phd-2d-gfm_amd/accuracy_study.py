@@ -8,7 +8,11 @@ For one workload and GFM variant, these jobs run side by side on the GPU from th
     conserved variables of both fluids), once per seed: how far the flow itself carries a change in the
     last digits.  Shocked interfaces are unstable, so after some steps any perturbation grows; an error
     of the mixed path within the spread of these runs is error the flow amplifies, not error the FP32
-    stages add.
+    stages add;
+  - the FP64 path with its state (both fluids' conserved grids and phi) rounded to FP32 after every
+    step (GFM_ROUND_STATE_F32=1, `fp64_f32storage`): FP32 storage alone, which perturbs the state by
+    up to half an FP32 ulp on every step rather than once.  The mixed path stores its state in FP32
+    too, so the part of its error above this job's is what its FP32 arithmetic adds.
 At each mark every job's state is compared with the FP64 job's (accuracy.py: relative L1 / Linf per
 conserved variable of the composite real-fluid state, and the interface-contour L1; cell-weighted mean
 over instances, max for Linf).
@@ -64,12 +68,22 @@ def perturbed(states, rel, seed):
     return out
 
 
-def job(setup, states, variant, precision, steps, init_reinit):
+def job(setup, states, variant, precision, steps, init_reinit, env=None):
     ctx = G.Context(G.methods(sim_type=G.SIM_TWOFLUID, fs=G.FS_MUSCL, gfm=variant, ls_order=5,
                               reinit_every=5, reinit_iters=10, precision=precision))
     for info, eos in setup:
         ctx.add(info, eos, 0.6, 1e300, steps + 1)
-    ctx.commit()
+    env = env or {}
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        ctx.commit()   # the library reads its switches here
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     for k, st in enumerate(states):
         ctx.upload(k, *st)
         ctx.set_time(k, 0.0, 0)
@@ -91,7 +105,9 @@ def study(name, variant_name=None, steps=None, marks=10, seeds=3, rel=1e-7, n_in
     w = [i.Nx * i.Ny for i in infos]
     log("initial conditions of %d instances built" % len(setup))
     runs = {"fp64": job(setup, states, variant, G.FP64, steps, init_reinit),
-            "mixed": job(setup, states, variant, G.MIXED, steps, init_reinit)}
+            "mixed": job(setup, states, variant, G.MIXED, steps, init_reinit),
+            "fp64_f32storage": job(setup, states, variant, G.FP64, steps, init_reinit,
+                                   {"GFM_ROUND_STATE_F32": "1"})}
     for s in range(seeds):
         runs["fp64_perturbed_%d" % (s + 1)] = job(setup, perturbed(states, rel, 1000 + s), variant, G.FP64, steps,
                                                   init_reinit)

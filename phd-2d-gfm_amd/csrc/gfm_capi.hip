@@ -70,6 +70,23 @@ void houc_coeffs(int order, gfm::Houc& H) {
 
 size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
 
+// index of the tile shape named by environment variable `var` ("XxY") in a
+// table of n shapes of `stride` ints each ({x, y, ...}); unset: shape 0
+int tile_shape(const char* var, const int* table, int stride, int n, int* out) {
+    *out = 0;
+    const char* v = std::getenv(var);
+    if (!v || !v[0]) return GFM_OK;
+    int x = 0, y = 0;
+    char tail = 0;
+    if (std::sscanf(v, "%dx%d%c", &x, &y, &tail) == 2)
+        for (int k = 0; k < n; ++k)
+            if (table[k * stride] == x && table[k * stride + 1] == y) { *out = k; return GFM_OK; }
+    std::string shapes;
+    for (int k = 0; k < n; ++k)
+        shapes += (k ? ", " : "") + std::to_string(table[k * stride]) + "x" + std::to_string(table[k * stride + 1]);
+    return fail_arg((std::string(var) + "=" + v + " is not one of the compiled tile shapes: " + shapes).c_str());
+}
+
 }  // namespace
 
 struct gfm_ctx {
@@ -87,6 +104,9 @@ struct gfm_ctx {
     bool mgfm_extrapolate = false;     // GFM_MGFM_EXTRAPOLATE=1 (GFM_modified only; not the reference)
     bool use_extension_vfield = false; // GFM_base::use_extension_vfield (sim_twofluid.cpp:551 sets it false)
     bool use_slip_BCs = false;         // GFM_base::use_slip_BCs
+    int fs_shape = 0;                  // flux tile gfm::FS_SHAPES[fs_shape] (GFM_FS_TILE, e.g. "32x16")
+    int tb_shape = 0;                  // blocked extrapolation tile gfm::TB_SHAPES[tb_shape] (GFM_TB_TILE)
+    bool round_f32 = false;            // GFM_ROUND_STATE_F32=1 (FP64 two-fluid only; analysis, not the reference)
     // two-fluid step path: the instances in groups, each group's step on its
     // own stream (group 0 on `stream`), forked from and joined back to
     // `stream`, so that the latency-bound phases of one group (the ordered
@@ -210,6 +230,9 @@ int twofluid_step(gfm_ctx* c, const gfm::LaunchCtx& L, hipEvent_t mark) {
     gfm::launch_fs<R>(L, muscl, 1, 0, 0, 0);   // the fill read Q[.][0]
     gfm::launch_ls<R>(L, c->houc, c->m.reinit_every, c->m.reinit_iters);
     gfm::launch_bc<R>(L, 0, 2, 1);
+    // FP32 storage of the step's result on the FP64 path: what storing the
+    // state in FP32 alone does to a job, without the mixed path's FP32 sweeps
+    if (c->round_f32) gfm::launch_round_state_f32(L, 0);
     return GFM_OK;
 }
 
@@ -428,7 +451,8 @@ TableOffsets append_tables(const gfm_ctx* c, const std::vector<Inst>& descs, con
         int li = 0;
         for (int ty = 0; ty < tiy; ++ty)
             for (int tx = 0; tx < tix; ++tx) w_int.push_back(make_int4(ki, ty, tx, li++));
-        const int sx = (s.Nx + gfm::FS_TILE_X - 1) / gfm::FS_TILE_X, sy = (s.Ny + gfm::FS_TILE_Y - 1) / gfm::FS_TILE_Y;
+        const int fsx = gfm::FS_SHAPES[c->fs_shape][0], fsy = gfm::FS_SHAPES[c->fs_shape][1];
+        const int sx = (s.Nx + fsx - 1) / fsx, sy = (s.Ny + fsy - 1) / fsy;
         for (int f = 0; f < nf; ++f)
             for (int ty = 0; ty < sy; ++ty)
                 for (int tx = 0; tx < sx; ++tx) w_fs.push_back(make_int4(ki, ty, tx, f));
@@ -436,9 +460,11 @@ TableOffsets append_tables(const gfm_ctx* c, const std::vector<Inst>& descs, con
         for (int st = 0; st < 4; ++st)
             for (int o = 0; o < strip_n[st]; o += 256) w_frame.push_back(make_int4(ki, st, o, 0));
         w_inst.push_back(make_int4(ki, 0, 0, 0));
-        if (c->twofluid())
-            for (int ty = 0; ty < (s.Ny + 15) / 16; ++ty)
-                for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_tb.push_back(make_int4(ki, ty, tx, 0));
+        if (c->twofluid()) {
+            const int tbx = gfm::TB_SHAPES[c->tb_shape][0], tby = gfm::TB_SHAPES[c->tb_shape][1];
+            for (int ty = 0; ty < (s.Ny + tby - 1) / tby; ++ty)
+                for (int tx = 0; tx < (s.Nx + tbx - 1) / tbx; ++tx) w_tb.push_back(make_int4(ki, ty, tx, 0));
+        }
         {
             const int nsx = (s.Nx + 255) / 256;
             for (int r = 0; r < s.Ny; ++r)
@@ -487,6 +513,8 @@ gfm::LaunchCtx launch_ctx(const gfm_ctx* c, const TableOffsets& t, hipStream_t s
         else L.mirror_y = 1;
     }
     L.iface_plain = iface_plain;
+    L.fs_shape = c->fs_shape;
+    L.tb_shape = c->tb_shape;
     return L;
 }
 
@@ -621,6 +649,11 @@ int gfm_ctx_commit(gfm_ctx* c) {
     if (!c) return fail_arg("null ctx");
     if (c->committed) return GFM_OK;
     if (c->inst.empty()) return fail_arg("no instances");
+    // tile shapes (the autotuner's choice, or the defaults): "XxY" from the
+    // shape tables; anything else is an error rather than a silent default
+    int r = tile_shape("GFM_FS_TILE", &gfm::FS_SHAPES[0][0], 2, gfm::FS_NSHAPE, &c->fs_shape);
+    if (!r) r = tile_shape("GFM_TB_TILE", &gfm::TB_SHAPES[0][0], 3, gfm::TB_NSHAPE, &c->tb_shape);
+    if (r) return r;
     const size_t es = c->es();
     std::vector<Inst> descs;
     bool periodic_x = false;
@@ -702,6 +735,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
     }
     const char* mx = std::getenv("GFM_MGFM_EXTRAPOLATE");
     c->mgfm_extrapolate = c->m.gfm == GFM_MODIFIED && mx && mx[0] == '1';
+    const char* rf = std::getenv("GFM_ROUND_STATE_F32");
+    c->round_f32 = c->twofluid() && !c->mixed() && rf && rf[0] == '1';
     const char* no_tb = std::getenv("GFM_NO_TEMPORAL_BLOCKING");
     // the blocked kernels wrap the rows of y-periodic instances themselves
     c->tb = c->twofluid() && !periodic_x && !(no_tb && no_tb[0] == '1');

@@ -315,20 +315,24 @@ __device__ __forceinline__ void tb_put(V4A<R>* s, int k, const V4<R>& W) {
 //  D  flux-difference update in the reference's accumulation order
 // work: {inst, tile_y, tile_x, fluid}
 // ---------------------------------------------------------------------------
-constexpr int FTX = FS_TILE_X, FTY = FS_TILE_Y, FS_THREADS = FTX * FTY;
-constexpr int FWX = FTX + 4, FWY = FTY + 4;
-constexpr int FEX = FTX + 2, FEY = FTY + 2;
-constexpr int NFX = FTY * (FTX + 1);
-constexpr int NFY = (FTY + 1) * FTX;
-constexpr int NFACE = NFX + NFY;
+// the tile's cells and the sizes that follow from them: primitives with a
+// 2-cell halo (W), cells that evolve a face state (E: tile + 1), faces
+template <int TX, int TY>
+struct FsTile {
+    static constexpr int X = TX, Y = TY, THREADS = TX * TY;
+    static constexpr int WX = TX + 4, WY = TY + 4, EX = TX + 2, EY = TY + 2;
+    static constexpr int NFX = TY * (TX + 1), NFY = (TY + 1) * TX, NFACE = NFX + NFY;
+};
 
 template <typename T>
 __device__ __forceinline__ T minmod1(T dL, T dR) {
     return dR > T(0) ? fmax(T(0), fmin(dL, dR)) : fmin(T(0), fmax(dL, dR));
 }
 
-// face c of a tile: its two cells (lo = left / below) in tile coordinates
+// face c of a TX x TY tile: its two cells (lo = left / below) in tile coordinates
+template <int TX, int TY>
 __device__ __forceinline__ void face_cells(int c, bool& px, int& ra, int& ca, int& rb, int& cb) {
+    constexpr int FTX = TX, NFX = FsTile<TX, TY>::NFX;
     if (c < NFX) {
         const int r = c / (FTX + 1), fc = c - r * (FTX + 1);
         px = true; ra = r; ca = fc - 1; rb = r; cb = fc;
@@ -362,10 +366,17 @@ __device__ __forceinline__ void face_input_godunov(const Inst& I, int i0, int j0
 }
 
 // minimum waves per SIMD (the second __launch_bounds__ argument on AMD
-// GPUs).  FP32 storage: 6, i.e. 3 workgroups of 512 per CU (48 KB of LDS
-// each) with at most 80 VGPRs; FP64 is insensitive to the cap
-template <typename R>
-struct FsWaves { static constexpr int value = sizeof(R) == 4 ? 6 : 1; };
+// GPUs).  FP32 storage: as many as the LDS lets a CU hold, at most 6 (the
+// 32x16 tile: 3 workgroups of 512 per CU, 48 KB of LDS each, with at most 80
+// VGPRs); FP64 is insensitive to the cap
+template <typename R, int TX, int TY>
+struct FsWaves {
+    typedef FsTile<TX, TY> S;
+    static constexpr int lds = 4 * S::WY * S::WX * (int)sizeof(R) + 2 * S::NFACE * (int)sizeof(V4A<R>) +
+                               S::NFACE * (int)sizeof(short) + (sizeof(R) == 4 ? S::NFACE * 4 : 4) + 64;
+    static constexpr int per_cu = (160 * 1024 / lds) * (S::THREADS / 64) / 4;
+    static constexpr int value = sizeof(R) == 4 ? (per_cu < 6 ? per_cu : 6) : 1;
+};
 
 // qorig >= 0 (FP32 storage, whole two-fluid steps): Q[.][qorig] holds the
 // ghost fill's input.  A cell whose fluid f is real there and not interfacial
@@ -374,11 +385,14 @@ struct FsWaves { static constexpr int value = sizeof(R) == 4 ? 6 : 1; };
 // every step (E - rho |u|^2 / 2 and back): the cell's own state for the update
 // D is therefore taken from Q[.][qorig], exactly as stored.  The
 // reconstruction keeps reading Q[.][qin].  FP64 takes every state from qin.
-template <typename R, bool MUSCL>
-__global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst* __restrict__ insts,
-                                                                         const int4* __restrict__ work, int qin,
-                                                                         int qout, int copy_inactive, int qorig) {
+template <typename R, bool MUSCL, int TX, int TY>
+__global__ __launch_bounds__(TX * TY, (FsWaves<R, TX, TY>::value)) void k_fs(const Inst* __restrict__ insts,
+                                                                      const int4* __restrict__ work, int qin,
+                                                                      int qout, int copy_inactive, int qorig) {
     typedef R T;
+    typedef FsTile<TX, TY> S;
+    constexpr int FTX = S::X, FTY = S::Y, FS_THREADS = S::THREADS, FWX = S::WX, FWY = S::WY;
+    constexpr int FEX = S::EX, FEY = S::EY, NFX = S::NFX, NFACE = S::NFACE;
     __shared__ T sW[4][FWY][FWX];
     __shared__ V4A<T> sF[2][NFACE];   // per face: inputs lo / hi, then sF[0] = flux
     __shared__ short sHard[NFACE];
@@ -544,7 +558,7 @@ __global__ __launch_bounds__(FS_THREADS, FsWaves<R>::value) void k_fs(const Inst
     for (int c = tid; c < NFACE; c += FS_THREADS) {
         bool px;
         int ra, ca, rb, cb;
-        face_cells(c, px, ra, ca, rb, cb);
+        face_cells<TX, TY>(c, px, ra, ca, rb, cb);
         const bool lo_upd = i0 + ra >= lo_i && i0 + ra < lim_i && j0 + ca >= lo_j && j0 + ca < lim_j;
         const bool hi_upd = i0 + rb >= lo_i && i0 + rb < lim_i && j0 + cb >= lo_j && j0 + cb < lim_j;
         if (!lo_upd && !hi_upd) continue;   // no updated cell of the tile needs this face
@@ -940,7 +954,6 @@ __global__ __launch_bounds__(256) void k_extrap(const Inst* __restrict__ insts, 
 // P[.][bout]).
 // work: {inst, tile_y, tile_x, 0}, TBX x TBY interior tiles
 // ---------------------------------------------------------------------------
-constexpr int TBX = 64, TBY = 16, TB_THREADS = 512;
 
 // new ghost-fluid primitives of an LDS cell (extrap_cell's arithmetic) from
 // its current ones c0 and those of its upwind x and y neighbours at LDS
@@ -950,10 +963,11 @@ __device__ __forceinline__ V4<R> tb_update(const V4A<R>* s, const V4<R>& c0, int
     return extrap_combine<R>(wx, wy, c0, tb_get<R>(s, ox), tb_get<R>(s, oy));
 }
 
-template <typename R, int NIT, int MIN, int MOUT, bool PY>
-__global__ __launch_bounds__(TB_THREADS) void k_extrap_tb(const Inst* __restrict__ insts,
-                                                          const int4* __restrict__ work, int bin, int bout) {
+template <typename R, int NIT, int MIN, int MOUT, bool PY, int SH>
+__global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __restrict__ insts,
+                                                                const int4* __restrict__ work, int bin, int bout) {
     static_assert(NIT >= 2, "the first iteration is not the last");
+    constexpr int TBX = TB_SHAPES[SH][0], TBY = TB_SHAPES[SH][1], TB_THREADS = TB_SHAPES[SH][2];
     constexpr int RX = TBX + 2 * NIT, RY = TBY + 2 * NIT, NC = RX * RY;
     constexpr int SLOTS = (NC + TB_THREADS - 1) / TB_THREADS;
     constexpr int LM = MIN & (MF_FIRST | MF_INJECT);
@@ -2677,14 +2691,27 @@ void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes, int co
     hipLaunchKernelGGL(k_dt_final, GRID(L.n_inst), dim3(256), 0, L.stream, L.insts, L.w_inst, twofluid, narrow_boxes);
 }
 
+template <typename R, int TX, int TY>
+void launch_fs_shape(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive, int qorig) {
+    if (muscl)
+        hipLaunchKernelGGL((k_fs<R, true, TX, TY>), GRID(L.n_fs), dim3(TX * TY), 0, L.stream, L.insts, L.w_fs, qin,
+                           qout, copy_inactive, qorig);
+    else
+        hipLaunchKernelGGL((k_fs<R, false, TX, TY>), GRID(L.n_fs), dim3(TX * TY), 0, L.stream, L.insts, L.w_fs, qin,
+                           qout, copy_inactive, qorig);
+}
+
+// the context's flux tile shape (FS_SHAPES[L.fs_shape]; its work table w_fs
+// was built for that shape)
 template <typename R>
 void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive, int qorig) {
-    if (muscl)
-        hipLaunchKernelGGL((k_fs<R, true>), GRID(L.n_fs), dim3(FS_THREADS), 0, L.stream, L.insts, L.w_fs, qin, qout,
-                           copy_inactive, qorig);
-    else
-        hipLaunchKernelGGL((k_fs<R, false>), GRID(L.n_fs), dim3(FS_THREADS), 0, L.stream, L.insts, L.w_fs, qin, qout,
-                           copy_inactive, qorig);
+    static_assert(FS_NSHAPE == 4, "one case per shape");
+    switch (L.fs_shape) {
+        case 1: launch_fs_shape<R, FS_SHAPES[1][0], FS_SHAPES[1][1]>(L, muscl, qin, qout, copy_inactive, qorig); break;
+        case 2: launch_fs_shape<R, FS_SHAPES[2][0], FS_SHAPES[2][1]>(L, muscl, qin, qout, copy_inactive, qorig); break;
+        case 3: launch_fs_shape<R, FS_SHAPES[3][0], FS_SHAPES[3][1]>(L, muscl, qin, qout, copy_inactive, qorig); break;
+        default: launch_fs_shape<R, FS_SHAPES[0][0], FS_SHAPES[0][1]>(L, muscl, qin, qout, copy_inactive, qorig); break;
+    }
 }
 
 template <typename R>
@@ -2713,20 +2740,33 @@ void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout) {
 #undef EXC
 }
 
+// the blocked extrapolation on tile shape TB_SHAPES[SH] (the context's w_tb
+// was built for it): the mirror-y and the periodic-y variant, each for the
+// instances it serves
+template <typename R, int N, int MI, int MO, int SH>
+void launch_extrap_tb_shape(const LaunchCtx& L, int bin, int bout) {
+    if (L.mirror_y)
+        hipLaunchKernelGGL((k_extrap_tb<R, N, MI, MO, false, SH>), GRID(L.n_tb), dim3(TB_SHAPES[SH][2]), 0, L.stream,
+                           L.insts, L.w_tb, bin, bout);
+    if (L.periodic_y)
+        hipLaunchKernelGGL((k_extrap_tb<R, N, MI, MO, true, SH>), GRID(L.n_tb), dim3(TB_SHAPES[SH][2]), 0, L.stream,
+                           L.insts, L.w_tb, bin, bout);
+}
+
 // NIT iterations of launch_extrap in one launch; false when the combination
 // has no temporally blocked instance (the caller then launches per iteration)
 template <typename R>
 bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout) {
-#define EXT(N, MI, MO)                                                                                              \
-    if (nit == N && min == (MI) && mout == (MO)) {                                                                  \
-        if (L.mirror_y)                                                                                             \
-            hipLaunchKernelGGL((k_extrap_tb<R, N, MI, MO, false>), GRID(L.n_tb), dim3(TB_THREADS), 0, L.stream,     \
-                               L.insts, L.w_tb, bin, bout);                                                         \
-        if (L.periodic_y)                                                                                           \
-            hipLaunchKernelGGL((k_extrap_tb<R, N, MI, MO, true>), GRID(L.n_tb), dim3(TB_THREADS), 0, L.stream,      \
-                               L.insts, L.w_tb, bin, bout);                                                         \
-        return true;                                                                                                \
+#define EXT(N, MI, MO)                                                                  \
+    if (nit == N && min == (MI) && mout == (MO)) {                                      \
+        switch (L.tb_shape) {                                                           \
+            case 1: launch_extrap_tb_shape<R, N, MI, MO, 1>(L, bin, bout); break;       \
+            case 2: launch_extrap_tb_shape<R, N, MI, MO, 2>(L, bin, bout); break;       \
+            default: launch_extrap_tb_shape<R, N, MI, MO, 0>(L, bin, bout); break;      \
+        }                                                                               \
+        return true;                                                                    \
     }
+    static_assert(TB_NSHAPE == 3, "one case per shape");
     EXT(6, MF_FIRST, MF_LAST | MF_OGFM)           // GFM_original
     EXT(6, MF_FIRST | MF_INJECT, MF_LAST)         // GFM_real
     EXT(2, MF_FIRST, 0)                           // GFM_riemann / GFM_modified, before the star states
@@ -2904,6 +2944,33 @@ __global__ __launch_bounds__(256) void k_zfrac(const Inst* __restrict__ insts, c
 template <typename R>
 void launch_zfrac(const LaunchCtx& L) {
     hipLaunchKernelGGL((k_zfrac<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full);
+}
+
+// FP32 storage on the FP64 path (analysis): the conserved grids Q[.][q] of
+// both fluids and phi rounded to FP32 and back, every padded cell.  Applied
+// after each step, it separates what FP32 storage alone does to a job from
+// what the FP32 arithmetic of the mixed path adds.
+// work: {inst, tile_y, tile_x, 0}, 64x4 tiles over the padded array
+__global__ __launch_bounds__(256) void k_round_state_f32(const Inst* __restrict__ insts,
+                                                         const int4* __restrict__ work, int q) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    const long long c = cidx(I, i, j);
+    for (int f = 0; f < 2; ++f) {
+        V4<double> U = ld4<double>(I.Q[f][q], I, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) U.v[k] = (double)(float)U.v[k];
+        st4<double>(I.Q[f][q], I, c, U);
+    }
+    double* ph = (double*)I.phi;
+    ph[c] = (double)(float)ph[c];
+}
+
+void launch_round_state_f32(const LaunchCtx& L, int q) {
+    hipLaunchKernelGGL(k_round_state_f32, GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, q);
 }
 
 __global__ void k_fill_u32(unsigned* p, int n, unsigned v) {

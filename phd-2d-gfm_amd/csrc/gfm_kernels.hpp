@@ -12,21 +12,30 @@ struct LaunchCtx {
     const Inst* insts;
     const int4* w_full;  int n_full;    // 64x4 tiles over the padded arrays
     const int4* w_int;   int n_int;     // 64x4 tiles over the interior (.w = tile index in instance)
-    const int4* w_fs;    int n_fs;      // 32x8 interior tiles x fluids (.w = fluid)
+    const int4* w_fs;    int n_fs;      // flux tiles (FS_SHAPES) x fluids (.w = fluid)
     const int4* w_frame; int n_frame;   // 256-cell chunks of the ghost frame
     const int4* w_inst;  int n_inst;    // one entry per instance
     const int4* w_ich;   int n_ich;     // 256-entry chunks of the interfacial lists
-    const int4* w_tb;    int n_tb;      // 64x16 interior tiles (temporally blocked extrapolation)
+    const int4* w_tb;    int n_tb;      // TB_SHAPES interior tiles (temporally blocked extrapolation)
     const int4* w_rt;    int n_rt;      // 64x32 interior tiles (temporally blocked reinitialisation)
     const int4* w_seg;   int n_seg;     // 256-cell interior row segments (dt, interfacial list)
     int tb;                             // temporally blocked kernels (no x-periodic instance)
     int mirror_y, periodic_y;           // some instance of the group has mirror / periodic y BCs
     int iface_plain;                    // interfacial solve by plain rounds instead of levels
+    int fs_shape;                       // flux tile: FS_SHAPES[fs_shape]
+    int tb_shape;                       // blocked extrapolation tile: TB_SHAPES[tb_shape]
 };
 
 struct Houc;
 constexpr int IFREC_BYTES = 384;   // >= sizeof(IfRec), checked in the kernels
-constexpr int FS_TILE_X = 32, FS_TILE_Y = 16;   // flux-update tile (one cell per thread)
+// flux-update tile shapes {x, y} (one cell per thread); a context uses one,
+// FS_SHAPES[LaunchCtx::fs_shape], chosen at commit (GFM_FS_TILE)
+constexpr int FS_NSHAPE = 4;
+constexpr int FS_SHAPES[FS_NSHAPE][2] = {{32, 16}, {64, 8}, {16, 32}, {32, 8}};
+// tile shapes {x, y, threads} of the temporally blocked extrapolation
+// (TB_SHAPES[LaunchCtx::tb_shape], chosen at commit: GFM_TB_TILE)
+constexpr int TB_NSHAPE = 3;
+constexpr int TB_SHAPES[TB_NSHAPE][3] = {{64, 16, 512}, {32, 16, 256}, {64, 8, 256}};
 constexpr int SEG_PER_WG = 4;   // 256-cell row segments per workgroup (one per wavefront)
 
 template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes, int count_iface);
@@ -57,6 +66,8 @@ struct IcDesc {   // the layout of gfm_ic_desc (include/gfm_hip.h)
 template <typename R> void launch_ic(const LaunchCtx& L, const IcDesc* descs, const int* map, int q, int twofluid);
 void launch_pure_rs(hipStream_t s, int n, int px, double g, double pinf, const double* UL, const double* UR,
                     double* F, unsigned* st);
+// FP64 two-fluid contexts: Q[.][q] of both fluids and phi rounded to FP32 and back
+void launch_round_state_f32(const LaunchCtx& L, int q);
 // p[0..n) = v (a kernel, so that it may be captured in a graph)
 void launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
 // the mixed-precision path's face solver on the same problem: FP32 states

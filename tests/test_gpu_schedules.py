@@ -29,9 +29,9 @@ T, R = G.BC_TRANSMISSIVE, G.BC_REFLECTIVE
 BCS = [(T, T, R, R), (R, R, T, T), (R, T, R, T), (T, R, T, R)]
 
 
-def make_ctx(cases, variant, precision, blocked, plain_rounds=False, groups=4):
+def make_ctx(cases, variant, precision, blocked, plain_rounds=False, groups=4, fs_tile="32x16", tb_tile="64x16"):
     env = {"GFM_NO_TEMPORAL_BLOCKING": "0" if blocked else "1", "GFM_IFACE_PLAIN_ROUNDS": "1" if plain_rounds else "0",
-           "GFM_STEP_GROUPS": str(groups)}
+           "GFM_STEP_GROUPS": str(groups), "GFM_FS_TILE": fs_tile, "GFM_TB_TILE": tb_tile}
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
@@ -212,9 +212,10 @@ def test_instance_groups_match_one_stream(variant):
 
 
 def test_autotune_small_batch():
-    """the schedule autotuner (autotune.py) on a real batch: five trials (four group counts with the
-    blocked sweeps, then the one-sweep schedule at the best count), and a context on the chosen
-    schedule steps the batch to the same bits as one stream"""
+    """the schedule and tile autotuner (autotune.py) on a real batch: ten trials (four group counts
+    with the blocked sweeps, the one-sweep schedule at the best count, then the three other flux tiles
+    and, with the blocked schedule, the two other extrapolation tiles), and a context on the chosen
+    settings steps the batch to the same bits as one stream"""
     print("CASE autotune", flush=True)
     import autotune as AT
     sizes = [(96, 40), (64, 24), (130, 33), (72, 72), (40, 20)]
@@ -236,9 +237,8 @@ def test_autotune_small_batch():
     before = {k: os.environ.get(k) for k in AT.KEYS}
     choice = AT.tune(build, load, len(cases), steps=6, warmup=2)
     assert {k: os.environ.get(k) for k in AT.KEYS} == before
-    assert len(choice["trials"]) == 5
-    assert {"groups": choice["groups"], "blocking": choice["blocking"], "ms_per_step": choice["ms_per_step"]} \
-        in choice["trials"]
+    assert len(choice["trials"]) == (10 if choice["blocking"] else 8)
+    assert {k: choice[k] for k in ("groups", "blocking", "fs_tile", "tb_tile", "ms_per_step")} in choice["trials"]
     with AT.applied(choice):
         ca = build()
     load(ca)
@@ -247,3 +247,77 @@ def test_autotune_small_batch():
         c.step_graph(12)
         c.sync()
     compare(ca, cb, len(cases))
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.MIXED], ids=["fp64", "mixed"])
+def test_tile_shapes_match_default(precision):
+    """every compiled flux tile (32x16, 64x8, 16x32, 32x8) and blocked-extrapolation tile (64x16,
+    32x16, 64x8) steps a batch of uneven instances (partial tiles, every boundary combination of the
+    blocked path) to the same bits as the default tiles: the per-cell and per-face arithmetic and the
+    update's accumulation order do not depend on the tile"""
+    import autotune as AT
+    sizes = [(96, 40), (70, 24), (130, 33), (72, 72)]
+    cases = [perturbed_bubble(nx, ny, BCS[k % len(BCS)], 70 + k) for k, (nx, ny) in enumerate(sizes)]
+    ref = make_ctx(cases, G.GFM_RIEMANN, precision, True, groups=1)
+    ref.step_graph(8)
+    ref.sync()
+    for fs in AT.FS_TILES[1:]:
+        print("CASE tile fs=%s precision=%d" % (fs, precision), flush=True)
+        c = make_ctx(cases, G.GFM_RIEMANN, precision, True, groups=1, fs_tile=fs)
+        c.step_graph(8)
+        c.sync()
+        compare(c, ref, len(cases))
+        c.close()
+    for tb in AT.TB_TILES[1:]:
+        print("CASE tile tb=%s precision=%d" % (tb, precision), flush=True)
+        c = make_ctx(cases, G.GFM_RIEMANN, precision, True, groups=1, tb_tile=tb)
+        c.step_graph(8)
+        c.sync()
+        compare(c, ref, len(cases))
+        c.close()
+    ref.close()
+
+
+def test_unknown_tile_shape_is_an_error():
+    """a tile shape the library was not compiled for is refused at commit, naming the shapes it has"""
+    print("CASE tile error", flush=True)
+    cases = [perturbed_bubble(40, 20, BCS[0], 3)]
+    with pytest.raises(G.GfmError, match="32x16, 64x8, 16x32, 32x8"):
+        make_ctx(cases, G.GFM_RIEMANN, G.MIXED, True, fs_tile="48x16")
+    with pytest.raises(G.GfmError, match="64x16, 32x16, 64x8"):
+        make_ctx(cases, G.GFM_RIEMANN, G.MIXED, True, tb_tile="7x7")
+
+
+def test_round_state_f32_switch():
+    """GFM_ROUND_STATE_F32=1 (FP64 path, analysis): after every step both fluids' conserved grids and
+    phi hold FP32 values; the first step matches the plain FP64 step rounded to FP32, and later steps
+    differ from the plain FP64 run by no more than FP32 rounding carried a few steps"""
+    print("CASE round f32", flush=True)
+    cases = [perturbed_bubble(96, 40, BCS[0], 11), perturbed_bubble(70, 24, BCS[1], 12)]
+    old = os.environ.get("GFM_ROUND_STATE_F32")
+    os.environ["GFM_ROUND_STATE_F32"] = "1"
+    try:
+        cr = make_ctx(cases, G.GFM_RIEMANN, G.FP64, True, groups=1)
+    finally:
+        if old is None:
+            del os.environ["GFM_ROUND_STATE_F32"]
+        else:
+            os.environ["GFM_ROUND_STATE_F32"] = old
+    cp = make_ctx(cases, G.GFM_RIEMANN, G.FP64, True, groups=1)
+    for c in (cr, cp):
+        c.step(1)
+        c.sync()
+    for k in range(len(cases)):
+        for a, b in zip(cr.download(k), cp.download(k)):
+            assert np.array_equal(a, b.astype(np.float32).astype(np.float64))
+    for c in (cr, cp):
+        c.step_graph(9)
+        c.sync()
+    for k in range(len(cases)):
+        for a, b in zip(cr.download(k), cp.download(k)):
+            assert np.array_equal(a, a.astype(np.float32).astype(np.float64))
+            scale = np.abs(b).max()
+            assert np.abs(a - b).max() <= 1e-5 * scale
+        assert cr.status(k) == 0
+    cr.close()
+    cp.close()
