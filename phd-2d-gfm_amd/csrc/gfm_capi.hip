@@ -70,6 +70,14 @@ void houc_coeffs(int order, gfm::Houc& H) {
 
 size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
 
+// The mixed path's ghost band in cells.  The states the interface reads
+// (the ghost fluid within two or three cells of it) come from a fill that
+// extrapolates from the real fluid and keeps part of each ghost state's own
+// value; the flux update moves values two cells per step, so a ghost state
+// far out reaches them only over many steps, each of which scales its weight
+// down.
+constexpr int GHOST_BAND_DEFAULT = 24;
+
 // index of the tile shape named by environment variable `var` ("XxY") in a
 // table of n shapes of `stride` ints each ({x, y, ...}); unset: shape 0
 int tile_shape(const char* var, const int* table, int stride, int n, int* out) {
@@ -106,6 +114,8 @@ struct gfm_ctx {
     bool use_slip_BCs = false;         // GFM_base::use_slip_BCs
     int fs_shape = 0;                  // flux tile gfm::FS_SHAPES[fs_shape] (GFM_FS_TILE, e.g. "32x16")
     int tb_shape = 0;                  // blocked extrapolation tile gfm::TB_SHAPES[tb_shape] (GFM_TB_TILE)
+    int ghost_band = 0;                // mixed path: ghost states updated within this many cells of their
+                                       // fluid's real region (GFM_GHOST_BAND)
     bool round_f32 = false;            // GFM_ROUND_STATE_F32=1 (FP64 two-fluid only; analysis, not the reference)
     // two-fluid step path: the instances in groups, each group's step on its
     // own stream (group 0 on `stream`), forked from and joined back to
@@ -651,6 +661,12 @@ int gfm_ctx_commit(gfm_ctx* c) {
     if (c->inst.empty()) return fail_arg("no instances");
     // tile shapes (the autotuner's choice, or the defaults): "XxY" from the
     // shape tables; anything else is an error rather than a silent default
+    {
+        // the mixed path's ghost band, in cells (GFM_GHOST_BAND; 0: update every ghost state)
+        const char* gb = std::getenv("GFM_GHOST_BAND");
+        c->ghost_band = gb ? std::atoi(gb) : GHOST_BAND_DEFAULT;
+        if (c->ghost_band < 0) return fail_arg("GFM_GHOST_BAND must be >= 0");
+    }
     int r = tile_shape("GFM_FS_TILE", &gfm::FS_SHAPES[0][0], 2, gfm::FS_NSHAPE, &c->fs_shape);
     if (!r) r = tile_shape("GFM_TB_TILE", &gfm::TB_SHAPES[0][0], 3, gfm::TB_NSHAPE, &c->tb_shape);
     if (r) return r;
@@ -675,6 +691,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.cfl = h.cfl; d.T = h.T; d.max_steps = h.max_steps;
         d.use_bbox = 1;   // boxes are reset to the full domain each step unless OGFM narrows them
         d.use_slip = c->use_slip_BCs ? 1 : 0;
+        d.band = c->mixed() && c->twofluid() ? c->ghost_band * std::max(s.dx, s.dy) : 0.0;
         d.ntile_dt = s.Ny * ((s.Nx + 255) / 256);   // dt partials: one per 256-cell row segment
         d.icap = c->twofluid() ? std::min(std::min(s.Nx * s.Ny, 16 * (s.Nx + s.Ny) + 1024), 49152) : 1;
         // layout of the single allocation

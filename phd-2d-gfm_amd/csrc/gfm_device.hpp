@@ -83,6 +83,8 @@ struct Inst {
     int icap;                  // interfacial list capacity
     int use_bbox;              // OGFM narrows the per-fluid update boxes
     int use_slip;              // GFM_base::use_slip_BCs (GFM_modified: the ghost keeps its tangential velocity)
+    double band;               // FP32 whole steps: ghost states farther than this (phi) from their
+                               // fluid's real region are not updated by the flux kernel (0: all are)
     void* Q[2][2];             // conserved [fluid][buffer]
     void* P[2][2];             // primitives [fluid][buffer]
     void* phi;

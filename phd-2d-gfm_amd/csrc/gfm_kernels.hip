@@ -452,6 +452,16 @@ __global__ __launch_bounds__(TX * TY, (FsWaves<R, TX, TY>::value)) void k_fs(con
         ph_own = ld1<R>(I.phi, cown);
         im_own = gmem(I.imap)[cown];
     }
+    // FP32 whole steps with a ghost band (I.band > 0): a tile on which fluid
+    // f is a ghost at more than I.band (phi distance) from its real region
+    // is not updated; those ghost states keep their values (no real cell
+    // reads them within the step, and their weight in the states the
+    // interface does read decays geometrically from step to step)
+    if (sizeof(R) == 4 && qorig >= 0 && I.band > 0.0) {
+        const R band = (R)I.band;
+        const bool near = own_in && (f == 0 ? ph_own < band : ph_own > -band);
+        if (!__syncthreads_or(near)) return;
+    }
     auto load_own = [&]() {
         if (!own_in) return;
         const bool orig = pick && ((ph_own <= R(0)) == (f == 0)) && im_own < 0;
