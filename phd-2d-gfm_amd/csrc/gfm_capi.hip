@@ -150,7 +150,7 @@ void dispatch_dt(gfm_ctx* c, int q) { gfm::launch_dt<R>(c->L, c->twofluid() ? 1 
 // which there is no blocked kernel.
 template <typename R>
 int extrap_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int N, bool from_q, int qin, int pstart, bool inject_first,
-                 bool to_q, int qout, int last_extra, int all_extra = 0) {
+                 bool to_q, int qout, int last_extra, int all_extra = 0, int band = 0) {
     if (L.tb && N >= 2 && all_extra == 0) {
         // one launch for all N iterations; it writes only the last one's
         // destination, which is the same buffer as in the loop below
@@ -158,7 +158,8 @@ int extrap_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int N, bool from_q, int qi
         for (int k = 0; k < N && !(k == N - 1 && to_q); ++k) cur_p = (k == 0 && from_q) ? 0 : 1 - cur_p;
         const int min = (from_q ? 1 : 0) | (inject_first ? 2 : 0);
         const int mout = to_q ? (4 | last_extra) : 0;
-        if (gfm::launch_extrap_tb<R>(L, N, min, mout, from_q ? qin : pstart, to_q ? qout : cur_p)) return cur_p;
+        if (gfm::launch_extrap_tb<R>(L, N, min, mout, from_q ? qin : pstart, to_q ? qout : cur_p, band))
+            return cur_p;
     }
     int cur_p = pstart;
     for (int k = 0; k < N; ++k) {
@@ -184,9 +185,12 @@ int extrap_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int N, bool from_q, int qi
 
 // GFM_base::set_ghost_states for the context's variant: Q[.][0] -> Q[.][1].
 // `mark`, if given, is recorded on L's stream once the interfacial solve (or,
-// for GFM_original, the whole chain) is launched.
+// for GFM_original, the whole chain) is launched.  `band`: the fill of a
+// whole mixed-precision step, whose blocked extrapolation passes the tiles
+// beyond the ghost band through (the flux kernel does not update their
+// ghost states)
 template <typename R>
-int ghost_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int counted = 0, hipEvent_t mark = nullptr) {
+int ghost_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int counted = 0, hipEvent_t mark = nullptr, int band = 0) {
     switch (c->m.gfm) {
         case GFM_ORIGINAL:   // GFM_original.cpp:34-165
             extrap_chain<R>(c, L, 6, true, 0, 0, false, true, 1, 8);
@@ -195,18 +199,18 @@ int ghost_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int counted = 0, hipEvent_t
         case GFM_REAL: {     // GFM_real.cpp:32-247
             gfm::launch_iface<R>(L, gfm::V_REAL, 0, counted);
             if (mark) HC(hipEventRecord(mark, L.stream));
-            extrap_chain<R>(c, L, 6, true, 0, 0, true, true, 1, 0);
+            extrap_chain<R>(c, L, 6, true, 0, 0, true, true, 1, 0, 0, band);
             break;
         }
         case GFM_RIEMANN: {  // GFM_riemann.cpp:33-187
-            const int p = extrap_chain<R>(c, L, 2, true, 0, 0, false, false, 0, 0);
+            const int p = extrap_chain<R>(c, L, 2, true, 0, 0, false, false, 0, 0, 0, band);
             gfm::launch_iface<R>(L, gfm::V_RIEMANN, p, counted);
             if (mark) HC(hipEventRecord(mark, L.stream));
-            extrap_chain<R>(c, L, 6, false, 0, p, true, true, 1, 0);
+            extrap_chain<R>(c, L, 6, false, 0, p, true, true, 1, 0, 0, band);
             break;
         }
         case GFM_MODIFIED: { // GFM_modified.cpp:37-239
-            const int p = extrap_chain<R>(c, L, 2, true, 0, 0, false, false, 0, 0);
+            const int p = extrap_chain<R>(c, L, 2, true, 0, 0, false, false, 0, 0, 0, band);
             gfm::launch_iface<R>(L, gfm::V_MODIFIED, p, counted);
             if (mark) HC(hipEventRecord(mark, L.stream));
             if (c->mgfm_extrapolate) {
@@ -235,7 +239,7 @@ int twofluid_step(gfm_ctx* c, const gfm::LaunchCtx& L, hipEvent_t mark) {
     // the dt pass also counts the interfacial cells when the variant lists them
     const int count = c->m.gfm != GFM_ORIGINAL;
     gfm::launch_dt<R>(L, 1, 0, c->narrow(), count);
-    int r = ghost_chain<R>(c, L, count, mark);
+    int r = ghost_chain<R>(c, L, count, mark, c->mixed() ? 1 : 0);
     if (r) return r;
     gfm::launch_fs<R>(L, muscl, 1, 0, 0, 0);   // the fill read Q[.][0]
     gfm::launch_ls<R>(L, c->houc, c->m.reinit_every, c->m.reinit_iters);
@@ -1216,7 +1220,7 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
             if (c->mixed()) {
                 gfm::launch_dt<float>(c->L, 1, 0, c->narrow(), c->m.gfm != GFM_ORIGINAL);
                 HC(hipEventRecord(ev[1], c->stream));
-                ghost_chain<float>(c, c->L, c->m.gfm != GFM_ORIGINAL);
+                ghost_chain<float>(c, c->L, c->m.gfm != GFM_ORIGINAL, nullptr, 1);
                 HC(hipEventRecord(ev[2], c->stream));
                 gfm::launch_fs<float>(c->L, muscl, 1, 0, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));

@@ -973,9 +973,33 @@ __device__ __forceinline__ V4<R> tb_update(const V4A<R>* s, const V4<R>& c0, int
     return extrap_combine<R>(wx, wy, c0, tb_get<R>(s, ox), tb_get<R>(s, oy));
 }
 
+// the output of an owned cell (i, j), and the ghost cells it is the source of
+// (inverse of src_row / src_col): rows {i, 2g-1-i, 2(Ny+g)-1-i} x columns
+// {j, 2g-1-j, 2(Nx+g)-1-j} where those are ghost cells, which take oa
+template <typename R, int MOUT, bool PY>
+__device__ __forceinline__ void tb_put_owned(const Inst& I, int i, int j, int bout, const ExOut<R>& o,
+                                             const ExOut<R>& oa, bool rfb, bool rft, bool rfx) {
+    const int g = I.g;
+    extrap_put<R, MOUT>(I, i, j, bout, o, false, false);
+    if (i < 2 * g || i >= I.Ny || j < 2 * g || j >= I.Nx) {
+#pragma unroll 1
+        for (int m = 1; m < 9; ++m) {
+            const int pr = m / 3, pc = m - 3 * (m / 3);
+            int r = i, cc = j;
+            bool fv = false, fu = false;
+            if (pr == 1) { if (i >= 2 * g) continue; r = PY ? i + I.Ny : 2 * g - 1 - i; fv = !PY && rfb; }
+            if (pr == 2) { if (i < I.Ny) continue; r = PY ? i - I.Ny : 2 * (I.Ny + g) - 1 - i; fv = !PY && rft; }
+            if (pc == 1) { if (j >= 2 * g) continue; cc = 2 * g - 1 - j; fu = rfx; }
+            if (pc == 2) { if (j < I.Nx) continue; cc = 2 * (I.Nx + g) - 1 - j; fu = rfx; }
+            extrap_put<R, MOUT & MF_LAST>(I, r, cc, bout, oa, fu, fv);
+        }
+    }
+}
+
 template <typename R, int NIT, int MIN, int MOUT, bool PY, int SH>
 __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __restrict__ insts,
-                                                                const int4* __restrict__ work, int bin, int bout) {
+                                                                const int4* __restrict__ work, int bin, int bout,
+                                                                int band) {
     static_assert(NIT >= 2, "the first iteration is not the last");
     constexpr int TBX = TB_SHAPES[SH][0], TBY = TB_SHAPES[SH][1], TB_THREADS = TB_SHAPES[SH][2];
     constexpr int RX = TBX + 2 * NIT, RY = TBY + 2 * NIT, NC = RX * RY;
@@ -995,6 +1019,51 @@ __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __re
     }
     const void* in0 = (MIN & MF_FIRST) ? I.Q[0][bin] : I.P[0][bin];
     const void* in1 = (MIN & MF_FIRST) ? I.Q[1][bin] : I.P[1][bin];
+    const bool rfb = I.bc[2] == BC_R, rft = I.bc[3] == BC_R, rfx = I.bc[0] == BC_R;
+    if constexpr (sizeof(R) == 4 && !(MOUT & MF_OGFM)) {
+        // FP32 whole steps with a ghost band: a tile whose cells are all
+        // farther than I.band (phi) from the interface holds only ghost states
+        // the band leaves alone; its cells pass through with the real fluid
+        // converted exactly as below and the ghost fluid as it came in
+        if (band && I.band > 0.0) {
+            const R bnd = (R)I.band;
+            const auto ph = gmem((const R*)I.phi);
+            bool near = false;
+            for (int q = threadIdx.x; q < TBX * TBY; q += TB_THREADS) {
+                const int i = g + w.y * TBY + q / TBX, j = g + w.z * TBX + q % TBX;
+                if (interior(I, i, j)) near = near || fabs(ph[cidx(I, i, j)]) < bnd;
+            }
+            if (!__syncthreads_or(near)) {
+                bool bad = false;
+                for (int q = threadIdx.x; q < TBX * TBY; q += TB_THREADS) {
+                    const int i = g + w.y * TBY + q / TBX, j = g + w.z * TBX + q % TBX;
+                    if (!interior(I, i, j)) continue;
+                    const long long c = cidx(I, i, j);
+                    V4<R> W1 = ld4<R>(in0, I, c), W2 = ld4<R>(in1, I, c);
+                    if (LM & MF_FIRST) {
+                        W1 = cons2prims_f(fluid_f(I.rsf[0]), W1);
+                        W2 = cons2prims_f(fluid_f(I.rsf[1]), W2);
+                    }
+                    const int m = (LM & MF_INJECT) ? gmem(I.imap)[c] : -1;
+                    const int fr = ph[c] <= R(0) ? 0 : 1;
+                    if (m >= 0) {
+                        const auto sd = gmem(I.side) + 8 * (long long)m;
+                        V4<R> Ws;
+#pragma unroll
+                        for (int qq = 0; qq < 4; ++qq) Ws.v[qq] = (R)sd[qq];
+                        if (fr == 0) W1 = Ws; else W2 = Ws;
+                    }
+                    const V4<R>& Wr = fr == 0 ? W1 : W2;
+                    const V4<R>& Wg = fr == 0 ? W2 : W1;
+                    if (!(Wg.v[0] >= R(0))) bad = true;
+                    const ExOut<R> o = extrap_out<R, MOUT>(I, fr, Wr, Wg);
+                    tb_put_owned<R, MOUT, PY>(I, i, j, bout, o, o, rfb, rft, rfx);
+                }
+                if (bad) flag(I, ST_UNPHYS);
+                return;
+            }
+        }
+    }
     const ExK<R> K = ex_k<R>(I);
     // per cell: the update weights, the ghost fluid's current primitives
     // (only this thread changes them), the LDS cells of the ghost fluid at
@@ -1006,7 +1075,6 @@ __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __re
     V4<R> cur[SLOTS];
     int oc[SLOTS], ox[SLOTS], oy[SLOTS];
     int meta[SLOTS];
-    const bool rfb = I.bc[2] == BC_R, rft = I.bc[3] == BC_R, rfx = I.bc[0] == BC_R;
     const int ib = g, it = g + I.Ny - 1, jl = g, jr = g + I.Nx - 1;
     // in two passes, so that every load for the thread's cells is in flight
     // before any is used: the raw values (states, phi and its neighbours,
@@ -1188,29 +1256,13 @@ __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __re
             atomicMin(&bb[2], j - 2); atomicMax(&bb[3], j + 2);
         }
         const ExOut<R> o = extrap_out<R, MOUT>(I, fr, Wr, Wo);
-        extrap_put<R, MOUT>(I, i, j, bout, o, false, false);
-        if (i < 2 * g || i >= I.Ny || j < 2 * g || j >= I.Nx) {
-            // ghost cells sourced here (inverse of src_row / src_col): rows
-            // {i, 2g-1-i, 2(Ny+g)-1-i} x columns {j, 2g-1-j, 2(Nx+g)-1-j}
-            // where those are ghost cells.  They take the plain extrapolated
-            // ghost fluid (GFM_original's rule applies to interior cells).
-            ExOut<R> oa = o;
-            if (MOUT & MF_OGFM) {
-                const V4<R> Ug = (MOUT & MF_LAST) ? to_cons<R>(I, fg, Wg) : Wg;
-                if (fr == 0) oa.b = Ug; else oa.a = Ug;
-            }
-#pragma unroll 1
-            for (int m = 1; m < 9; ++m) {
-                const int pr = m / 3, pc = m - 3 * (m / 3);
-                int r = i, cc = j;
-                bool fv = false, fu = false;
-                if (pr == 1) { if (i >= 2 * g) continue; r = PY ? i + I.Ny : 2 * g - 1 - i; fv = !PY && rfb; }
-                if (pr == 2) { if (i < I.Ny) continue; r = PY ? i - I.Ny : 2 * (I.Ny + g) - 1 - i; fv = !PY && rft; }
-                if (pc == 1) { if (j >= 2 * g) continue; cc = 2 * g - 1 - j; fu = rfx; }
-                if (pc == 2) { if (j < I.Nx) continue; cc = 2 * (I.Nx + g) - 1 - j; fu = rfx; }
-                extrap_put<R, MOUT & MF_LAST>(I, r, cc, bout, oa, fu, fv);
-            }
+        ExOut<R> oa = o;
+        if ((MOUT & MF_OGFM) && (i < 2 * g || i >= I.Ny || j < 2 * g || j >= I.Nx)) {
+            // GFM_original's rule applies to interior cells only
+            const V4<R> Ug = (MOUT & MF_LAST) ? to_cons<R>(I, fg, Wg) : Wg;
+            if (fr == 0) oa.b = Ug; else oa.a = Ug;
         }
+        tb_put_owned<R, MOUT, PY>(I, i, j, bout, o, oa, rfb, rft, rfx);
     }
     if (bad) flag(I, ST_UNPHYS);
     if (MOUT & MF_OGFM) {
@@ -2754,25 +2806,25 @@ void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout) {
 // was built for it): the mirror-y and the periodic-y variant, each for the
 // instances it serves
 template <typename R, int N, int MI, int MO, int SH>
-void launch_extrap_tb_shape(const LaunchCtx& L, int bin, int bout) {
+void launch_extrap_tb_shape(const LaunchCtx& L, int bin, int bout, int band) {
     if (L.mirror_y)
         hipLaunchKernelGGL((k_extrap_tb<R, N, MI, MO, false, SH>), GRID(L.n_tb), dim3(TB_SHAPES[SH][2]), 0, L.stream,
-                           L.insts, L.w_tb, bin, bout);
+                           L.insts, L.w_tb, bin, bout, band);
     if (L.periodic_y)
         hipLaunchKernelGGL((k_extrap_tb<R, N, MI, MO, true, SH>), GRID(L.n_tb), dim3(TB_SHAPES[SH][2]), 0, L.stream,
-                           L.insts, L.w_tb, bin, bout);
+                           L.insts, L.w_tb, bin, bout, band);
 }
 
 // NIT iterations of launch_extrap in one launch; false when the combination
 // has no temporally blocked instance (the caller then launches per iteration)
 template <typename R>
-bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout) {
+bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout, int band) {
 #define EXT(N, MI, MO)                                                                  \
     if (nit == N && min == (MI) && mout == (MO)) {                                      \
         switch (L.tb_shape) {                                                           \
-            case 1: launch_extrap_tb_shape<R, N, MI, MO, 1>(L, bin, bout); break;       \
-            case 2: launch_extrap_tb_shape<R, N, MI, MO, 2>(L, bin, bout); break;       \
-            default: launch_extrap_tb_shape<R, N, MI, MO, 0>(L, bin, bout); break;      \
+            case 1: launch_extrap_tb_shape<R, N, MI, MO, 1>(L, bin, bout, band); break; \
+            case 2: launch_extrap_tb_shape<R, N, MI, MO, 2>(L, bin, bout, band); break; \
+            default: launch_extrap_tb_shape<R, N, MI, MO, 0>(L, bin, bout, band); break;\
         }                                                                               \
         return true;                                                                    \
     }
@@ -3008,7 +3060,7 @@ void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, doub
     template void launch_dt<R>(const LaunchCtx&, int, int, int, int);              \
     template void launch_fs<R>(const LaunchCtx&, bool, int, int, int, int);        \
     template void launch_extrap<R>(const LaunchCtx&, int, int, int);          \
-    template bool launch_extrap_tb<R>(const LaunchCtx&, int, int, int, int, int); \
+    template bool launch_extrap_tb<R>(const LaunchCtx&, int, int, int, int, int, int); \
     template void launch_iface<R>(const LaunchCtx&, int, int, int);           \
     template void launch_mgfm_final<R>(const LaunchCtx&, int, int, bool);     \
     template void launch_vext<R>(const LaunchCtx&, bool);                       \

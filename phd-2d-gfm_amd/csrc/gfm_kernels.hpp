@@ -43,7 +43,10 @@ template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q, in
 template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive,
                                      int qorig = -1);
 template <typename R> void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout);
-template <typename R> bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout);
+// band: a whole FP32 step's fill, in which tiles farther than Inst::band from
+// the interface pass their states through without extrapolating
+template <typename R> bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout,
+                                            int band = 0);
 template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted);
 template <typename R> void launch_mgfm_final(const LaunchCtx& L, int qin, int qout, bool prims_out = false);
 // the extension velocity field of a ghost fill (use_extension_vfield): the

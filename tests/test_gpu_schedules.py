@@ -13,6 +13,10 @@
    multi-step graphs and staggered group starts) against one stream.
 4. The blocked extrapolation and level-set kernels on periodic-in-y
    instances (wrapped region rows) against the per-iteration ones.
+The mixed path's ghost band is off in this module (GFM_GHOST_BAND=0): the
+band leaves ghost states far from the interface alone, and which of them
+depends on the schedule and the tiles (the blocked extrapolation passes whole
+tiles through, the one-sweep kernels extrapolate every cell).
 """
 import os
 
@@ -25,6 +29,11 @@ import gfm_hip as G
 import problems as P
 
 VARIANTS = [G.GFM_ORIGINAL, G.GFM_REAL, G.GFM_RIEMANN, G.GFM_MODIFIED]
+
+
+@pytest.fixture(autouse=True)
+def no_ghost_band(monkeypatch):
+    monkeypatch.setenv("GFM_GHOST_BAND", "0")
 T, R = G.BC_TRANSMISSIVE, G.BC_REFLECTIVE
 BCS = [(T, T, R, R), (R, R, T, T), (R, T, R, T), (T, R, T, R)]
 
@@ -250,15 +259,12 @@ def test_autotune_small_batch():
 
 
 @pytest.mark.parametrize("precision", [G.FP64, G.MIXED], ids=["fp64", "mixed"])
-def test_tile_shapes_match_default(precision, monkeypatch):
+def test_tile_shapes_match_default(precision):
     """every compiled flux tile (32x16, 64x8, 16x32, 32x8) and blocked-extrapolation tile (64x16,
     32x16, 64x8) steps a batch of uneven instances (partial tiles, every boundary combination of the
     blocked path) to the same bits as the default tiles: the per-cell and per-face arithmetic and the
-    update's accumulation order do not depend on the tile.  The mixed path's ghost band is off here
-    (GFM_GHOST_BAND=0): it skips whole flux tiles, so which far ghost states keep older values depends
-    on the tile shape"""
+    update's accumulation order do not depend on the tile"""
     import autotune as AT
-    monkeypatch.setenv("GFM_GHOST_BAND", "0")
     sizes = [(96, 40), (70, 24), (130, 33), (72, 72)]
     cases = [perturbed_bubble(nx, ny, BCS[k % len(BCS)], 70 + k) for k, (nx, ny) in enumerate(sizes)]
     ref = make_ctx(cases, G.GFM_RIEMANN, precision, True, groups=1)
