@@ -91,6 +91,14 @@ K_FS_COUNTERS = {"batch": {"fetch_kib": 4.241e5, "write_kib": 7.982e5, "factors"
                            "measured_window": [760, 860], "source": "profiles/batch64_window_counters.md"}}
 
 
+def log(msg):
+    """progress on stderr, one line per stage, so that a long run shows it is alive"""
+    print("[bench %.1fs] %s" % (time.perf_counter() - T0, msg), file=sys.stderr, flush=True)
+
+
+T0 = time.perf_counter()
+
+
 def make_cases(workload, n_inst):
     """(problem builder, per-instance kwargs, variant, precision, job steps, name, init reinit iterations)"""
     if workload == "batch":
@@ -117,7 +125,9 @@ def make_setup(builder, cases, perturb=0.0):
     """per instance (sim info, EOS) and the initial host grids"""
     setup, states = [], []
     rng = np.random.default_rng(7)
-    for c in cases:
+    for k, c in enumerate(cases):
+        if k and k % 32 == 0:
+            log("  %d of %d built" % (k, len(cases)))
         info, eos, st = builder(**c)
         if perturb:
             st = tuple(np.ascontiguousarray(a * (1.0 + perturb * rng.uniform(-1.0, 1.0, a.shape)))
@@ -222,33 +232,48 @@ def oracle_job(builder_args):
     return info.Nx * info.Ny * int(s[0])
 
 
-def cpu_baseline(setup, ck, variant, steps, n_serial, n_threads):
+CPU_BUDGET = 1.5e7   # cell-steps per CPU leg: about 10 s of one core at the oracle's ~1.5e6 cell-updates/s
+
+
+def cpu_baseline(setup, ck, variant, max_steps, n_serial, n_threads):
     """FP64 CPU oracle on the smallest instances' checkpointed mid-job states: serially (one core, as the
-    reference runs) and with one host thread per instance"""
+    reference runs) and with one host thread per instance; each leg runs about CPU_BUDGET cell-steps
+    (at most max_steps steps per instance)"""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     O.lib()
     order = sorted(range(len(setup)), key=lambda k: setup[k][0].Nx * setup[k][0].Ny)
-    jobs = [(setup[k][0], setup[k][1], ck[k][0], variant, steps) for k in order]
+    size = lambda k: setup[k][0].Nx * setup[k][0].Ny  # noqa: E731
+    ser = order[:n_serial]
+    steps_s = int(max(1, min(max_steps, round(CPU_BUDGET / sum(size(k) for k in ser)))))
+    jobs = [(setup[k][0], setup[k][1], ck[k][0], variant, steps_s) for k in ser]
     t0 = time.perf_counter()
-    cells_s = sum(oracle_job(j) for j in jobs[:n_serial])
+    cells_s = sum(oracle_job(j) for j in jobs)
     el_s = time.perf_counter() - t0
-    threads = max(1, min(n_threads, HOST_THREADS_MAX, os.cpu_count() or 1))
-    par = jobs[:max(threads, 1)]
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(threads) as ex:
-        cells_p = sum(ex.map(oracle_job, par))
-    el_p = time.perf_counter() - t0
     desc = lambda js: ", ".join("%dx%d" % (j[0].Nx, j[0].Ny) for j in js)  # noqa: E731
     start = ck[0][1][1]
-    return {"value": cells_s / el_s, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "FP64 oracle, one thread, %d steps from the step-%d checkpoint of %s: %.1f s"
-                      % (steps, start, desc(jobs[:n_serial]), el_s),
-            "all_cores": {"value": cells_p / el_p, "unit": "cell-updates/s", "cores": threads,
-                          "nproc": os.cpu_count(),
-                          "sample": "FP64 oracle, one host thread per instance, %d steps from the step-%d "
-                                    "checkpoint of %d instances (%s): %.1f s"
-                                    % (steps, start, len(par), desc(par), el_p)}}
+    out = {"value": cells_s / el_s, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+           "sample": "FP64 oracle, one thread, %d steps from the step-%d checkpoint of %s: %.1f s"
+                     % (steps_s, start, desc(jobs), el_s)}
+    threads = max(1, min(n_threads, HOST_THREADS_MAX, os.cpu_count() or 1, len(setup)))
+    if threads < 2:
+        out["all_cores"] = {"value": None, "cores": 1, "nproc": os.cpu_count(),
+                            "sample": "one instance: the oracle is serial, so there is no instance-level "
+                                      "parallelism to time"}
+        return out
+    par = order[:threads]
+    steps_p = int(max(1, min(max_steps, round(CPU_BUDGET / max(size(k) for k in par)))))
+    pjobs = [(setup[k][0], setup[k][1], ck[k][0], variant, steps_p) for k in par]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        cells_p = sum(ex.map(oracle_job, pjobs))
+    el_p = time.perf_counter() - t0
+    out["all_cores"] = {"value": cells_p / el_p, "unit": "cell-updates/s", "cores": threads,
+                        "nproc": os.cpu_count(),
+                        "sample": "FP64 oracle, one host thread per instance, %d steps from the step-%d "
+                                  "checkpoint of %d instances (%s): %.1f s"
+                                  % (steps_p, start, len(pjobs), desc(pjobs), el_p)}
+    return out
 
 
 def host_io_run(ctx, ck, steps):
@@ -315,7 +340,7 @@ def main():
     ap.add_argument("--job-steps", type=int, default=None, help="job length whose middle is timed")
     ap.add_argument("--workload", default="batch", choices=["batch", "bubble", "large", "rmi"])
     ap.add_argument("--instances", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=200, help="most oracle steps per instance in the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run of mixed workloads")
     ap.add_argument("--no-replay", action="store_true", help="skip the per-phase HIP-event replay of the window")
@@ -350,14 +375,19 @@ def main():
     advance = job // 2 if a.advance is None else a.advance
     cap = advance + a.warmup + max(a.steps, a.tune_steps or 0) + 8
 
+    log("building the initial conditions of %d instances" % len(cases))
     setup, states = make_setup(builder, cases)
     infos = [s[0] for s in setup]
     cells = sum(i.Nx * i.Ny for i in infos)
     n = len(setup)
 
+    log("advancing %d steps" % advance)
     ck, st_adv = advanced(setup, states, variant, prec, advance, cap, init_reinit, a.eager)
+    log("autotuning")
     ctx, schedule = tuned_ctx(a, setup, ck, variant, prec, cap)
+    log("timing %d steps" % a.steps)
     el, win, final = timed_window(ctx, ck, a.steps, a.warmup, a.eager)
+    log("timed: %.3f ms per step" % (el * 1e3 / a.steps))
     status = sorted({ctx.status(k) for k in range(n)} | set(st_adv))
     value = cells * a.steps / el
 
@@ -421,6 +451,7 @@ def main():
     ctx.close()
 
     if prec == G.MIXED and not a.no_fp64:
+        log("FP64 job")
         ck64, st64 = advanced(setup, states, variant, G.FP64, advance, cap, init_reinit, a.eager)
         ctx64, sched64 = tuned_ctx(a, setup, ck64, variant, G.FP64, cap)
         el64, _, ref = timed_window(ctx64, ck64, a.steps, a.warmup, a.eager)
@@ -433,22 +464,26 @@ def main():
                GFM_LABEL[variant]: accuracy(infos, final, ref)}
         # the same FP64 job with its state rounded to FP32 after every step:
         # what FP32 storage alone does, beside which the mixed error is judged
+        log("FP64 job with FP32 storage")
         st32 = run_job(setup, states, variant, G.FP64, total, init_reinit, {"GFM_ROUND_STATE_F32": "1"})
         acc["fp64_f32storage"] = accuracy(infos, st32, ref)
         acc["mixed_over_f32storage_L1_rho"] = (acc[GFM_LABEL[variant]]["L1_rho"] /
                                                max(acc["fp64_f32storage"]["L1_rho"], 1e-300))
         del st32
         if a.perturb:
+            log("FP64 job from a perturbed start")
             _, pstates = make_setup(builder, cases, perturb=1e-7)
             pert = run_job(setup, pstates, variant, G.FP64, total, init_reinit)
             acc["fp64_perturbed_1e-7"] = accuracy(infos, pert, ref)
         if a.accuracy and a.workload == "large":
             for v in (G.GFM_ORIGINAL, G.GFM_REAL):
+                log("variant %s" % GFM_LABEL[v])
                 mixed = run_job(setup, states, v, G.MIXED, total, init_reinit)
                 fp64 = run_job(setup, states, v, G.FP64, total, init_reinit)
                 acc[GFM_LABEL[v]] = accuracy(infos, mixed, fp64)
         out["accuracy"] = acc
     if not a.no_cpu:
+        log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(setup, ck, variant, a.cpu_steps, 4, HOST_THREADS_MAX)
     print(json.dumps(out))
 

@@ -114,6 +114,8 @@ struct gfm_ctx {
     bool use_slip_BCs = false;         // GFM_base::use_slip_BCs
     int fs_shape = 0;                  // flux tile gfm::FS_SHAPES[fs_shape] (GFM_FS_TILE, e.g. "32x16")
     int tb_shape = 0;                  // blocked extrapolation tile gfm::TB_SHAPES[tb_shape] (GFM_TB_TILE)
+    int fs_persist = 0;                // GFM_FS_PERSISTENT=1: the persistent flux kernel (the GPU's CU count)
+    unsigned* d_fs_next = nullptr;     // its queue counters, 8 per launch context
     int ghost_band = 0;                // mixed path: ghost states updated within this many cells of their
                                        // fluid's real region (GFM_GHOST_BAND)
     bool round_f32 = false;            // GFM_ROUND_STATE_F32=1 (FP64 two-fluid only; analysis, not the reference)
@@ -528,6 +530,8 @@ gfm::LaunchCtx launch_ctx(const gfm_ctx* c, const TableOffsets& t, hipStream_t s
     }
     L.iface_plain = iface_plain;
     L.fs_shape = c->fs_shape;
+    L.fs_persist = c->fs_persist;
+    L.fs_next = nullptr;   // set by the caller
     L.tb_shape = c->tb_shape;
     return L;
 }
@@ -610,6 +614,7 @@ int gfm_ctx_destroy(gfm_ctx* c) {
     if (c->d_ic) (void)hipFree(c->d_ic);
     if (c->d_icmap) (void)hipFree(c->d_icmap);
     if (c->d_work) (void)hipFree(c->d_work);
+    if (c->d_fs_next) (void)hipFree(c->d_fs_next);
     for (size_t g = 1; g < c->gstream.size(); ++g) (void)hipStreamDestroy(c->gstream[g]);
     for (size_t g = 1; g < c->ev_join.size(); ++g) (void)hipEventDestroy(c->ev_join[g]);
     for (auto ev : c->ev_stag) (void)hipEventDestroy(ev);
@@ -808,7 +813,21 @@ int gfm_ctx_commit(gfm_ctx* c) {
         for (int g = 0; g < ng; ++g) offs.push_back(append_tables(c, descs, members[g], all));
     HC(hipMalloc((void**)&c->d_work, all.size() * sizeof(int4)));
     HC(hipMemcpyAsync(c->d_work, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    {
+        // the persistent flux kernel (GFM_FS_PERSISTENT=1): its grid spans the
+        // GPU's CUs; queue counters for the context and for each group
+        const char* fp = std::getenv("GFM_FS_PERSISTENT");
+        c->fs_persist = 0;
+        if (fp && fp[0] == '1') {
+            int dev = 0, ncu = 0;
+            HC(hipGetDevice(&dev));
+            HC(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            c->fs_persist = std::max(ncu, 1);
+        }
+        HC(hipMalloc((void**)&c->d_fs_next, 8 * sizeof(unsigned) * (ng + 1)));
+    }
     c->L = launch_ctx(c, offs[0], c->stream, iface_plain, everyone);
+    c->L.fs_next = c->d_fs_next;
     {
         // steps per graph of a grouped context (GFM_GRAPH_STEPS, default 100)
         const char* gs = std::getenv("GFM_GRAPH_STEPS");
@@ -831,6 +850,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
                 c->ev_join.push_back(ev);
             }
             c->GL.push_back(launch_ctx(c, offs[1 + g], c->gstream[g], iface_plain, members[g]));
+            c->GL.back().fs_next = c->d_fs_next + 8 * (1 + g);
         }
     }
     // initial scalars: t = 0, step 0, active

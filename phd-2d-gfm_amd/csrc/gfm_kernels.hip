@@ -7,6 +7,8 @@
 // R is the storage type (double, or float in mixed precision).
 #include "gfm_kernels.hpp"
 
+#include <algorithm>
+
 namespace gfm {
 
 // every array of an instance is device global memory: pointers cast to the
@@ -385,21 +387,33 @@ struct FsWaves {
 // every step (E - rho |u|^2 / 2 and back): the cell's own state for the update
 // D is therefore taken from Q[.][qorig], exactly as stored.  The
 // reconstruction keeps reading Q[.][qin].  FP64 takes every state from qin.
+// the flux kernel's LDS for one tile
+template <typename T, int TX, int TY>
+struct FsLds {
+    typedef FsTile<TX, TY> S;
+    T sW[4][S::WY][S::WX];
+    V4A<T> sF[2][S::NFACE];   // per face: inputs lo / hi, then sF[0] = flux
+    short sHard[S::NFACE];
+    float sP0[sizeof(T) == 4 ? S::NFACE : 1];   // FP32 storage: start of each hard face's FP64 Newton
+    T sLam[2];                                  // dt / dx, dt / dy for D
+    int nHard;
+};
+
+// one work item {inst, tile_y, tile_x, fluid} of the flux update; every
+// early return is taken by the whole workgroup
 template <typename R, bool MUSCL, int TX, int TY>
-__global__ __launch_bounds__(TX * TY, (FsWaves<R, TX, TY>::value)) void k_fs(const Inst* __restrict__ insts,
-                                                                      const int4* __restrict__ work, int qin,
-                                                                      int qout, int copy_inactive, int qorig) {
+__device__ __forceinline__ void fs_tile(const Inst* __restrict__ insts, const int4 w, int qin, int qout,
+                                        int copy_inactive, int qorig, FsLds<R, TX, TY>& L) {
     typedef R T;
     typedef FsTile<TX, TY> S;
     constexpr int FTX = S::X, FTY = S::Y, FS_THREADS = S::THREADS, FWX = S::WX, FWY = S::WY;
     constexpr int FEX = S::EX, FEY = S::EY, NFX = S::NFX, NFACE = S::NFACE;
-    __shared__ T sW[4][FWY][FWX];
-    __shared__ V4A<T> sF[2][NFACE];   // per face: inputs lo / hi, then sF[0] = flux
-    __shared__ short sHard[NFACE];
-    __shared__ float sP0[sizeof(T) == 4 ? NFACE : 1];   // FP32 storage: start of each hard face's FP64 Newton
-    __shared__ T sLam[2];                               // dt / dx, dt / dy for D
-    __shared__ int nHard;
-    const int4 w = work[blockIdx.x];
+    auto& sW = L.sW;
+    auto& sF = L.sF;
+    auto& sHard = L.sHard;
+    auto& sP0 = L.sP0;
+    auto& sLam = L.sLam;
+    int& nHard = L.nHard;
     const Inst& I = insts[w.x];
     const int f = w.w;
     const int tid = threadIdx.x;
@@ -677,6 +691,51 @@ __global__ __launch_bounds__(TX * TY, (FsWaves<R, TX, TY>::value)) void k_fs(con
             }
         }
         st4<R>(Qo, I, c, U);
+    }
+}
+
+template <typename R, bool MUSCL, int TX, int TY>
+__global__ __launch_bounds__(TX * TY, (FsWaves<R, TX, TY>::value)) void k_fs(const Inst* __restrict__ insts,
+                                                                      const int4* __restrict__ work, int qin,
+                                                                      int qout, int copy_inactive, int qorig) {
+    __shared__ FsLds<R, TX, TY> L;
+    fs_tile<R, MUSCL, TX, TY>(insts, work[blockIdx.x], qin, qout, copy_inactive, qorig, L);
+}
+
+// Persistent form of k_fs: a grid of as many workgroups as the GPU holds at
+// once drains the work table.  The table is in XCD order (entry p belongs to
+// XCD p mod 8), so the workgroups of XCD x first take x, x + 8, x + 16, ...
+// from queue x (counter next[x], zero at launch), then help the other
+// queues in turn.  Thread 0 keeps the queue it is on and claims its next
+// item before processing the current one; the claim's counter value is
+// looked at only after that item, so the atomic's latency overlaps it.
+// Every workgroup stops once all eight queues are empty.
+template <typename R, bool MUSCL, int TX, int TY>
+__global__ __launch_bounds__(TX * TY, (FsWaves<R, TX, TY>::value)) void k_fs_persist(
+        const Inst* __restrict__ insts, const int4* __restrict__ work, int n, unsigned* __restrict__ next, int qin,
+        int qout, int copy_inactive, int qorig) {
+    __shared__ FsLds<R, TX, TY> L;
+    __shared__ int s_item;
+    const unsigned x = blockIdx.x & 7u;
+    unsigned tried = 0;   // thread 0: queues found empty (x, x + 1, ... mod 8)
+    unsigned pend = 0;    // thread 0: counter value of the claim on queue x + tried
+    if (threadIdx.x == 0) pend = atomicAdd(&next[x], 1u);
+    for (;;) {
+        __syncthreads();   // the previous item's LDS and s_item are no longer read
+        if (threadIdx.x == 0) {
+            int p = -1;
+            while (tried < 8u) {
+                const long long cand = (long long)((x + tried) & 7u) + 8ll * pend;
+                if (cand < n) { p = (int)cand; break; }
+                if (++tried < 8u) pend = atomicAdd(&next[(x + tried) & 7u], 1u);
+            }
+            s_item = p;
+            if (p >= 0) pend = atomicAdd(&next[(x + tried) & 7u], 1u);   // the next claim, read next round
+        }
+        __syncthreads();
+        const int p = s_item;
+        if (p < 0) return;
+        fs_tile<R, MUSCL, TX, TY>(insts, work[p], qin, qout, copy_inactive, qorig, L);
     }
 }
 
@@ -2755,6 +2814,20 @@ void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes, int co
 
 template <typename R, int TX, int TY>
 void launch_fs_shape(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive, int qorig) {
+    if (L.fs_persist > 0 && L.fs_next) {
+        // eight workgroups per CU, more than fit at once: the ones that do
+        // not fit start as the first ones leave, find the queues empty and
+        // leave too, so the grid needs no occupancy figure
+        const int grid = std::min(8 * L.fs_persist, (L.n_fs + 7) / 8 * 8);
+        launch_fill_u32(L.stream, L.fs_next, 8, 0u);
+        if (muscl)
+            hipLaunchKernelGGL((k_fs_persist<R, true, TX, TY>), GRID(grid), dim3(TX * TY), 0, L.stream, L.insts,
+                               L.w_fs, L.n_fs, L.fs_next, qin, qout, copy_inactive, qorig);
+        else
+            hipLaunchKernelGGL((k_fs_persist<R, false, TX, TY>), GRID(grid), dim3(TX * TY), 0, L.stream, L.insts,
+                               L.w_fs, L.n_fs, L.fs_next, qin, qout, copy_inactive, qorig);
+        return;
+    }
     if (muscl)
         hipLaunchKernelGGL((k_fs<R, true, TX, TY>), GRID(L.n_fs), dim3(TX * TY), 0, L.stream, L.insts, L.w_fs, qin,
                            qout, copy_inactive, qorig);

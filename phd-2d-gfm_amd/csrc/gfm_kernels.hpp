@@ -23,6 +23,9 @@ struct LaunchCtx {
     int mirror_y, periodic_y;           // some instance of the group has mirror / periodic y BCs
     int iface_plain;                    // interfacial solve by plain rounds instead of levels
     int fs_shape;                       // flux tile: FS_SHAPES[fs_shape]
+    int fs_persist;                     // > 0: the persistent flux kernel, on fs_persist CUs' worth of
+                                        // workgroups (the GPU's CU count); 0: one workgroup per tile
+    unsigned* fs_next;                  // its eight queue counters (device, one set per launch context)
     int tb_shape;                       // blocked extrapolation tile: TB_SHAPES[tb_shape]
 };
 

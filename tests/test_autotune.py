@@ -1,7 +1,7 @@
 """Schedule and tile autotuner (phd-2d-gfm_amd/autotune.py) on a fake context and a fake clock: it
 must try each group count with the blocked sweeps, then the one-sweep schedule at the best count, then
-the other flux tiles, then (blocked schedule only) the other extrapolation tiles, pick the fastest, and
-leave the caller's environment as it was."""
+the other flux tiles, then (blocked schedule only) the other extrapolation tiles, then the persistent
+flux kernel, pick the fastest, and leave the caller's environment as it was."""
 import os
 import sys
 
@@ -29,11 +29,12 @@ class FakeCtx:
         self.blocking = os.environ.get("GFM_NO_TEMPORAL_BLOCKING") != "1"
         self.fs = os.environ.get("GFM_FS_TILE", "32x16")
         self.tb = os.environ.get("GFM_TB_TILE", "64x16")
+        self.persistent = os.environ.get("GFM_FS_PERSISTENT") == "1"
         self.closed = False
         built.append((self.groups, self.blocking, self.fs, self.tb))
 
     def factor(self):
-        return FS[self.fs] * TB[self.tb]
+        return FS[self.fs] * TB[self.tb] * (1.01 if self.persistent else 1.0)
 
     def step_graph(self, n):
         self.clock.t += n * MS[self.groups] * (1.0 if self.blocking else 2.0) * self.factor() * 1e-3
@@ -67,8 +68,10 @@ def test_picks_fastest_schedule(clock, monkeypatch):
     d = ("32x16", "64x16")
     assert built == [(1, True) + d, (2, True) + d, (3, True) + d, (4, True) + d, (2, False) + d,
                      (2, True, "64x8", "64x16"), (2, True, "16x32", "64x16"), (2, True, "32x8", "64x16"),
-                     (2, True, "16x32", "32x16"), (2, True, "16x32", "64x8")]
-    assert len(choice["trials"]) == 10 and all(c.closed for c in ctxs)
+                     (2, True, "16x32", "32x16"), (2, True, "16x32", "64x8"), (2, True, "16x32", "64x8")]
+    assert len(choice["trials"]) == 11 and all(c.closed for c in ctxs)
+    assert [t["persistent"] for t in choice["trials"]] == [False] * 10 + [True]
+    assert choice["persistent"] is False                 # 1% slower here
     assert len(loads) == 2 * len(ctxs)                  # warmup from the initial state, then the timed run
     assert os.environ["GFM_STEP_GROUPS"] == "3"          # caller's environment restored
     assert "GFM_NO_TEMPORAL_BLOCKING" not in os.environ
@@ -78,13 +81,14 @@ def test_picks_fastest_schedule(clock, monkeypatch):
         assert (os.environ["GFM_FS_TILE"], os.environ["GFM_TB_TILE"]) == ("16x32", "64x8")
     assert os.environ["GFM_STEP_GROUPS"] == "3"
     assert "GFM_FS_TILE" not in os.environ and "GFM_TB_TILE" not in os.environ
+    assert "GFM_FS_PERSISTENT" not in os.environ
 
 
 def test_single_instance_tries_one_group(clock, monkeypatch):
     monkeypatch.delenv("GFM_STEP_GROUPS", raising=False)
     built = []
     choice = T.tune(lambda: FakeCtx(clock, built), lambda c: None, n_instances=1, steps=10, warmup=2)
-    assert [b[:2] for b in built] == [(1, True), (1, False)] + [(1, True)] * 5
+    assert [b[:2] for b in built] == [(1, True), (1, False)] + [(1, True)] * 6
     assert (choice["groups"], choice["blocking"]) == (1, True)
     assert "GFM_STEP_GROUPS" not in os.environ
 
@@ -100,6 +104,6 @@ def test_blocking_off_wins_when_faster(clock, monkeypatch):
     choice = T.tune(lambda: Inverted(clock, built), lambda c: None, n_instances=8, steps=10, warmup=2)
     assert (choice["groups"], choice["blocking"]) == (2, False)
     # the one-sweep schedule has no blocked extrapolation: its tile is not tried
-    assert len(built) == 5 + 3 and all(b[3] == "64x16" for b in built)
+    assert len(built) == 5 + 3 + 1 and all(b[3] == "64x16" for b in built)
     with T.applied(choice):
         assert os.environ["GFM_NO_TEMPORAL_BLOCKING"] == "1"

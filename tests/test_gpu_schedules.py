@@ -38,9 +38,11 @@ T, R = G.BC_TRANSMISSIVE, G.BC_REFLECTIVE
 BCS = [(T, T, R, R), (R, R, T, T), (R, T, R, T), (T, R, T, R)]
 
 
-def make_ctx(cases, variant, precision, blocked, plain_rounds=False, groups=4, fs_tile="32x16", tb_tile="64x16"):
+def make_ctx(cases, variant, precision, blocked, plain_rounds=False, groups=4, fs_tile="32x16", tb_tile="64x16",
+             persistent=False):
     env = {"GFM_NO_TEMPORAL_BLOCKING": "0" if blocked else "1", "GFM_IFACE_PLAIN_ROUNDS": "1" if plain_rounds else "0",
-           "GFM_STEP_GROUPS": str(groups), "GFM_FS_TILE": fs_tile, "GFM_TB_TILE": tb_tile}
+           "GFM_STEP_GROUPS": str(groups), "GFM_FS_TILE": fs_tile, "GFM_TB_TILE": tb_tile,
+           "GFM_FS_PERSISTENT": "1" if persistent else "0"}
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
@@ -221,10 +223,10 @@ def test_instance_groups_match_one_stream(variant):
 
 
 def test_autotune_small_batch():
-    """the schedule and tile autotuner (autotune.py) on a real batch: ten trials (four group counts
-    with the blocked sweeps, the one-sweep schedule at the best count, then the three other flux tiles
-    and, with the blocked schedule, the two other extrapolation tiles), and a context on the chosen
-    settings steps the batch to the same bits as one stream"""
+    """the schedule and tile autotuner (autotune.py) on a real batch: eleven trials (four group counts
+    with the blocked sweeps, the one-sweep schedule at the best count, then the three other flux tiles,
+    with the blocked schedule the two other extrapolation tiles, and the persistent flux kernel), and a
+    context on the chosen settings steps the batch to the same bits as one stream"""
     print("CASE autotune", flush=True)
     import autotune as AT
     sizes = [(96, 40), (64, 24), (130, 33), (72, 72), (40, 20)]
@@ -246,8 +248,9 @@ def test_autotune_small_batch():
     before = {k: os.environ.get(k) for k in AT.KEYS}
     choice = AT.tune(build, load, len(cases), steps=6, warmup=2)
     assert {k: os.environ.get(k) for k in AT.KEYS} == before
-    assert len(choice["trials"]) == (10 if choice["blocking"] else 8)
-    assert {k: choice[k] for k in ("groups", "blocking", "fs_tile", "tb_tile", "ms_per_step")} in choice["trials"]
+    assert len(choice["trials"]) == (11 if choice["blocking"] else 9)
+    assert {k: choice[k] for k in ("groups", "blocking", "fs_tile", "tb_tile", "persistent", "ms_per_step")} \
+        in choice["trials"]
     with AT.applied(choice):
         ca = build()
     load(ca)
@@ -330,3 +333,26 @@ def test_round_state_f32_switch():
         assert cr.status(k) == 0
     cr.close()
     cp.close()
+
+
+@pytest.mark.parametrize("precision", [G.FP64, G.MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("groups", [1, 2])
+def test_persistent_flux_matches_per_tile(precision, groups, monkeypatch):
+    """the persistent flux kernel (GFM_FS_PERSISTENT=1: as many workgroups as the GPU holds, draining
+    the tiles through eight per-XCD queues) against one workgroup per tile, bit for bit, on a batch of
+    uneven instances in one or two instance groups (each group its own queue counters), with the
+    default tile and the 16x32 tile, and in mixed precision with the ghost band on"""
+    sizes = [(96, 40), (70, 24), (130, 33), (72, 72), (200, 64)]
+    cases = [perturbed_bubble(nx, ny, BCS[k % len(BCS)], 90 + k) for k, (nx, ny) in enumerate(sizes)]
+    if precision == G.MIXED:
+        monkeypatch.setenv("GFM_GHOST_BAND", "4")
+    for fs in ("32x16", "16x32"):
+        print("CASE persistent flux precision=%d groups=%d tile=%s" % (precision, groups, fs), flush=True)
+        ref = make_ctx(cases, G.GFM_RIEMANN, precision, True, groups=groups, fs_tile=fs)
+        per = make_ctx(cases, G.GFM_RIEMANN, precision, True, groups=groups, fs_tile=fs, persistent=True)
+        for c in (ref, per):
+            c.step_graph(10)
+            c.sync()
+        compare(per, ref, len(cases))
+        ref.close()
+        per.close()
