@@ -212,7 +212,11 @@ int ghost_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int counted = 0, hipEvent_t
             break;
         }
         case GFM_MODIFIED: { // GFM_modified.cpp:37-239
-            const int p = extrap_chain<R>(c, L, 2, true, 0, 0, false, false, 0, 0, 0, band);
+            // the two iterations feed only the probes of the interfacial
+            // cells (the write-back of :230-239 takes the unextrapolated
+            // states), so with the band, tiles beyond it write nothing
+            const int b1 = (band && !c->mgfm_extrapolate) ? 2 : band;
+            const int p = extrap_chain<R>(c, L, 2, true, 0, 0, false, false, 0, 0, 0, b1);
             gfm::launch_iface<R>(L, gfm::V_MODIFIED, p, counted);
             if (mark) HC(hipEventRecord(mark, L.stream));
             if (c->mgfm_extrapolate) {

@@ -1093,6 +1093,7 @@ __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __re
                 if (interior(I, i, j)) near = near || fabs(ph[cidx(I, i, j)]) < bnd;
             }
             if (!__syncthreads_or(near)) {
+                if (band == 2) return;   // nothing reads this chain's output there
                 bool bad = false;
                 for (int q = threadIdx.x; q < TBX * TBY; q += TB_THREADS) {
                     const int i = g + w.y * TBY + q / TBX, j = g + w.z * TBX + q % TBX;

@@ -47,7 +47,8 @@ template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, in
                                      int qorig = -1);
 template <typename R> void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout);
 // band: a whole FP32 step's fill, in which tiles farther than Inst::band from
-// the interface pass their states through without extrapolating
+// the interface pass their states through without extrapolating (1), or
+// write nothing, where nothing reads the chain's output there (2)
 template <typename R> bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout,
                                             int band = 0);
 template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted);
