@@ -37,10 +37,10 @@ steps: the checkpoint taken at the start of the timed window is restored
 and the same steps are launched eagerly with events around each phase
 (`replay_bitwise` says whether the replay reproduced the timed run's final
 state bit for bit).  `traffic` is HBM bytes per k_fs launch from rocprof
-FETCH_SIZE / WRITE_SIZE per launch (--traffic-kib; for the default workload
-the values measured over its default window are built in), scaled by what
-those counters read for a cold device copy of known size
-(--copy-calibration): FETCH_SIZE x 2, WRITE_SIZE x 1.  `step_roofline` is the
+FETCH_SIZE / WRITE_SIZE per launch (--traffic-kib, with --copy-factors: what those counters read for a cold
+device copy of known size, --copy-calibration); for the default workload
+the values measured over its default window are built in, with the read
+bytes taken from the L2-to-fabric read requests by size.  `step_roofline` is the
 whole step against the compulsory state traffic 2 (4 F + L) s bytes per
 cell-update (F = 2 fluids, L = 1 level set, s = word size).
 
@@ -84,10 +84,11 @@ SEED = 20261203
 GFM_LABEL = {G.GFM_ORIGINAL: "OGFM", G.GFM_REAL: "realGFM", G.GFM_RIEMANN: "riemannGFM", G.GFM_MODIFIED: "MGFM"}
 HOST_THREADS_MAX = 16   # the host cores one GPU's share of a box offers
 # rocprof counters of the flux kernel per launch over the default workload's window (steps 760-860),
-# profiles/batch64_window_counters.md: FETCH_SIZE and WRITE_SIZE in KiB, with the factors a cold 2 GiB
-# device copy gives them (FETCH_SIZE counts gfx950's 128-byte read requests as 64 bytes), VALU busy
-K_FS_COUNTERS = {"batch": {"fetch_kib": 4.241e5, "write_kib": 7.982e5, "factors": (2.0, 1.0),
-                           "valu_busy": 0.962, "lds_bank_conflict_per_lds_cycle": 0.71,
+# profiles/batch64_window_counters.md: read bytes from the L2-to-fabric read requests by size, raw
+# FETCH_SIZE and WRITE_SIZE in KiB (FETCH_SIZE counts gfx950's 128-byte read requests as 64 bytes),
+# VALU busy, LDS bank conflicts per active LDS instruction
+K_FS_COUNTERS = {"batch": {"read_bytes": 1.3057e9, "fetch_kib_raw": 6.4096e5, "write_kib": 4.3430e5,
+                           "valu_busy": 0.785, "lds_bank_conflict_per_lds_inst": 0.68,
                            "measured_window": [760, 860], "source": "profiles/batch64_window_counters.md"}}
 
 
@@ -424,8 +425,11 @@ def main():
         elif a.workload in K_FS_COUNTERS and prec == G.MIXED and n == 64:
             counters = K_FS_COUNTERS[a.workload]
         if counters:
-            traffic = 1024.0 * (counters["fetch_kib"] * counters["factors"][0]
-                                + counters["write_kib"] * counters["factors"][1])
+            if "read_bytes" in counters:
+                traffic = counters["read_bytes"] + 1024.0 * counters["write_kib"]
+            else:
+                traffic = 1024.0 * (counters["fetch_kib"] * counters["factors"][0]
+                                    + counters["write_kib"] * counters["factors"][1])
         out["roofline"] = {"bound": "hbm", "kernel": "k_fs (MUSCL-Hancock + exact RS, both fluids)",
                            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
