@@ -81,7 +81,9 @@ import problems as P  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 SEED = 20261203
-GFM_LABEL = {G.GFM_ORIGINAL: "OGFM", G.GFM_REAL: "realGFM", G.GFM_RIEMANN: "riemannGFM", G.GFM_MODIFIED: "MGFM"}
+GFM_LABEL = {v: k for k, v in G.GFM_NAMES.items()}   # settings-file names: OGFM, realGFM, ..., OGFMVOF
+GFM_CLASS = {G.GFM_ORIGINAL: "GFM_original", G.GFM_REAL: "GFM_real", G.GFM_RIEMANN: "GFM_riemann",
+             G.GFM_MODIFIED: "GFM_modified", G.GFM_VOFGFM: "GFM_VOF", G.GFM_OGFMVOF: "OGFM_VOF"}
 HOST_THREADS_MAX = 16   # the host cores one GPU's share of a box offers
 # rocprof counters of the flux kernel per launch over the default workload's window (steps 760-860),
 # profiles/batch64_window_counters.md: read bytes from the L2-to-fabric read requests by size, raw
@@ -105,20 +107,20 @@ def make_cases(workload, n_inst):
     if workload == "batch":
         cases = P.bubble_batch(n_inst)
         return (P.shocked_bubble, cases, G.GFM_RIEMANN, G.MIXED, 1500,
-                "%d seeded shocked bubbles s*(256x72), s~U[1,8], GFM_riemann, mixed FP32/FP64" % len(cases), 0)
+                "%d seeded shocked bubbles s*(256x72), s~U[1,8], {gfm}, mixed FP32/FP64" % len(cases), 0)
     if workload == "bubble":
         return (P.shocked_bubble, [dict(Nx=1024, Ny=288)], G.GFM_RIEMANN, G.FP64, 2000,
-                "shocked bubble 1024x288, GFM_riemann, plain FP64", 0)
+                "shocked bubble 1024x288, {gfm}, plain FP64", 0)
     if workload == "large":
         rng = np.random.default_rng(SEED)
         c = dict(Nx=4096, Ny=1152, bubble_gamma=float(rng.uniform(1.09, 1.67)),
                  bubble_rho=float(rng.uniform(0.18, 5.0)))
         return (P.shocked_bubble, [c], G.GFM_RIEMANN, G.MIXED, 2000,
-                "shocked bubble 4096x1152 (gamma %.4f, rho %.4f, seed %d), GFM_riemann, mixed FP32/FP64"
+                "shocked bubble 4096x1152 (gamma %.4f, rho %.4f, seed %d), {gfm}, mixed FP32/FP64"
                 % (c["bubble_gamma"], c["bubble_rho"], SEED), 0)
     cases = P.rmi_batch(n_inst)
     return (P.rmi, cases, G.GFM_MODIFIED, G.MIXED, 300,
-            "%d seeded single-mode RMI s*(512x128), s~U[1,4], periodic y, GFM_modified, mixed FP32/FP64"
+            "%d seeded single-mode RMI s*(512x128), s~U[1,4], periodic y, {gfm}, mixed FP32/FP64"
             % len(cases), 40)
 
 
@@ -341,6 +343,8 @@ def main():
     ap.add_argument("--job-steps", type=int, default=None, help="job length whose middle is timed")
     ap.add_argument("--workload", default="batch", choices=["batch", "bubble", "large", "rmi"])
     ap.add_argument("--instances", type=int, default=None)
+    ap.add_argument("--gfm", default=None, choices=sorted(G.GFM_NAMES),
+                    help="the workload with another ghost fluid method (settings-file name, e.g. VOFGFM)")
     ap.add_argument("--cpu-steps", type=int, default=200, help="most oracle steps per instance in the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run of mixed workloads")
@@ -372,6 +376,9 @@ def main():
 
     n_inst = a.instances or (64 if a.workload == "batch" else 256)
     builder, cases, variant, prec, job, name, init_reinit = make_cases(a.workload, n_inst)
+    if a.gfm:
+        variant = G.GFM_NAMES[a.gfm]
+    name = name.format(gfm=GFM_CLASS[variant])
     job = a.job_steps or job
     advance = job // 2 if a.advance is None else a.advance
     cap = advance + a.warmup + max(a.steps, a.tune_steps or 0) + 8

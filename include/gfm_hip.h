@@ -32,8 +32,11 @@ enum { GFM_BC_TRANSMISSIVE = 0, GFM_BC_REFLECTIVE = 1, GFM_BC_PERIODIC = 2 };
 enum { GFM_SIM_ONEFLUID = 0, GFM_SIM_TWOFLUID = 1 };
 /* flux solvers (GFM_settingsfile::FS) */
 enum { GFM_FS_MUSCL = 0, GFM_FS_GODUNOV = 1 };
-/* ghost fluid methods (GFM_settingsfile::GFM) */
-enum { GFM_ORIGINAL = 0, GFM_REAL = 1, GFM_RIEMANN = 2, GFM_MODIFIED = 3 };
+/* ghost fluid methods (GFM_settingsfile::GFM; "OGFM", "realGFM", "riemannGFM", "MGFM", "VOFGFM",
+ * "OGFMVOF", sim_twofluid.cpp:522-549).  GFM_VOFGFM and GFM_OGFMVOF (GFM_VOF.cpp, OGFM_VOF.cpp)
+ * read the volume fraction, mixed-cell normal and interface location of the level set's PLIC
+ * reconstruction (the reference takes them from external VOF trackers) and need numGC >= 6. */
+enum { GFM_ORIGINAL = 0, GFM_REAL = 1, GFM_RIEMANN = 2, GFM_MODIFIED = 3, GFM_VOFGFM = 4, GFM_OGFMVOF = 5 };
 /* precision */
 enum { GFM_FP64 = 0, GFM_MIXED = 1 };
 

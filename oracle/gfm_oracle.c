@@ -817,34 +817,86 @@ void or_tracers_step(const or_grid* G, const double* vfield, int n, double* pos,
     }
 }
 
-/* z (sim_twofluid.cpp:199; the level set's get_z is external): area
- * fraction of each cell on fluid 1's side (phi <= 0) of the plane through
- * the centre with phi's value and unit normal (unit_normal) there.  The
- * area of {a' u + b' w <= m} in the unit-scaled rectangle (Scardovelli and
- * Zaleski 2000, 2D): with a = |nx| dx <= b = |ny| dy and m = -phi + (a+b)/2,
- * 0 | m^2/(2ab) | (2m - a)/(2b) | 1 - (a+b-m)^2/(2ab) | 1 on m <= 0 | < a |
- * < b | < a+b | beyond.  The outermost ring: 1 or 0 by the sign of phi. */
+/* One cell of the level set's PLIC reconstruction: the line through the
+ * cell centre c with phi's value and unit normal n there (unit_normal),
+ * n . (x - c) + phi = 0, fluid 1 on the side n . (x - c) + phi <= 0.
+ *  - z: the area fraction of the cell on fluid 1's side.  The area of
+ *    {a' u + b' w <= m} in the unit-scaled rectangle (Scardovelli and
+ *    Zaleski 2000, 2D): with a = |nx| dx <= b = |ny| dy and m = -phi +
+ *    (a+b)/2, 0 | m^2/(2ab) | (2m - a)/(2b) | 1 - (a+b-m)^2/(2ab) | 1 on
+ *    m <= 0 | < a | < b | < a+b | beyond;
+ *  - nrm: n, pointing into fluid 2 (phi > 0);
+ *  - loc: the midpoint of the line's chord through the cell: the foot point
+ *    c - phi n moved along the tangent (-ny, nx) to the middle of the
+ *    parameter interval the cell's four sides clip; the foot point itself
+ *    if the line misses the cell.
+ * The outermost ring has no centred gradient: z is 1 or 0 by the sign of
+ * phi, nrm (1, 0) and loc the cell centre. */
+static void vof_cell(const or_grid* G, const double* phi, int i, int j, double* zc, double* n, double* loc)
+{
+    int H = H_OF(G), Wd = W_OF(G);
+    double p = phi[IDX(G, i, j)];
+    double cx, cy;
+    or_cellcentre(G, i, j, &cx, &cy);
+    if (i == 0 || j == 0 || i == H - 1 || j == Wd - 1) {
+        *zc = p <= 0.0 ? 1.0 : 0.0;
+        n[0] = 1.0; n[1] = 0.0;
+        loc[0] = cx; loc[1] = cy;
+        return;
+    }
+    unit_normal(G, phi, i, j, n);
+    double a = fabs(n[0]) * G->dx, b = fabs(n[1]) * G->dy;
+    if (a > b) { double s = a; a = b; b = s; }
+    double m = -p + 0.5 * (a + b), f;
+    if (m <= 0.0) f = 0.0;
+    else if (m >= a + b) f = 1.0;
+    else if (m < a) f = m * m / (2.0 * a * b);
+    else if (m < b) f = (2.0 * m - a) / (2.0 * b);
+    else f = 1.0 - (a + b - m) * (a + b - m) / (2.0 * a * b);
+    *zc = f;
+    /* chord: X(s) = -phi nx - s ny, Y(s) = -phi ny + s nx, |X| <= dx/2, |Y| <= dy/2 */
+    double fx = -p * n[0], fy = -p * n[1];
+    double lo = -1e300, hi = 1e300;
+    if (n[1] != 0.0) {
+        double s1 = (fx - 0.5 * G->dx) / n[1], s2 = (fx + 0.5 * G->dx) / n[1];
+        lo = fmax(lo, fmin(s1, s2));
+        hi = fmin(hi, fmax(s1, s2));
+    }
+    if (n[0] != 0.0) {
+        double s1 = (-0.5 * G->dy - fy) / n[0], s2 = (0.5 * G->dy - fy) / n[0];
+        lo = fmax(lo, fmin(s1, s2));
+        hi = fmin(hi, fmax(s1, s2));
+    }
+    double sm = lo <= hi ? 0.5 * (lo + hi) : 0.0;
+    loc[0] = cx + (fx - sm * n[1]);
+    loc[1] = cy + (fy + sm * n[0]);
+}
+
+/* z (sim_twofluid.cpp:199; the level set's get_z is external): vof_cell's
+ * area fraction of every cell */
 void or_volume_fraction(const or_grid* G, const double* phi, double* z)
 {
     int H = H_OF(G), Wd = W_OF(G);
     for (int i = 0; i < H; ++i)
         for (int j = 0; j < Wd; ++j) {
-            double p = phi[IDX(G, i, j)];
-            if (i == 0 || j == 0 || i == H - 1 || j == Wd - 1) {
-                z[IDX(G, i, j)] = p <= 0.0 ? 1.0 : 0.0;
-                continue;
-            }
-            double n[2];
-            unit_normal(G, phi, i, j, n);
-            double a = fabs(n[0]) * G->dx, b = fabs(n[1]) * G->dy;
-            if (a > b) { double s = a; a = b; b = s; }
-            double m = -p + 0.5 * (a + b), f;
-            if (m <= 0.0) f = 0.0;
-            else if (m >= a + b) f = 1.0;
-            else if (m < a) f = m * m / (2.0 * a * b);
-            else if (m < b) f = (2.0 * m - a) / (2.0 * b);
-            else f = 1.0 - (a + b - m) * (a + b - m) / (2.0 * a * b);
-            z[IDX(G, i, j)] = f;
+            double n[2], loc[2];
+            vof_cell(G, phi, i, j, z + IDX(G, i, j), n, loc);
+        }
+}
+
+/* The GFM_VOF_interface the VOF-based GFMs take (GFM_VOF.cpp:20,54,103-104;
+ * OGFM_VOF.cpp:23,59,140-141: get_z, get_mixedcell_normal,
+ * get_interfacelocation), restated from the level set: its trackers (VOF,
+ * CLSVOF, MOF; sim_twofluid.cpp:571-589) live in the external
+ * PhD-Interface-tracking-methods library.  Per cell (vof_cell): z,
+ * nrm[2*q..], loc[2*q..]. */
+void or_vof_interface(const or_grid* G, const double* phi, double* z, double* nrm, double* loc)
+{
+    int H = H_OF(G), Wd = W_OF(G);
+    for (int i = 0; i < H; ++i)
+        for (int j = 0; j < Wd; ++j) {
+            size_t q = IDX(G, i, j);
+            vof_cell(G, phi, i, j, z + q, nrm + 2 * q, loc + 2 * q);
         }
 }
 
@@ -1155,6 +1207,217 @@ static void store_vfield_real(const or_grid* G, const double* phi, const double*
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* A18: the VOF-based GFMs, GFM_VOF.cpp:8-325 and OGFM_VOF.cpp:8-274,   */
+/* on the interface of or_vof_interface                                 */
+/* ------------------------------------------------------------------ */
+/* inverse-distance weight of cell (ii, jj) seen from (i, j)
+ * (GFM_VOF.cpp:244-246: 1 / |centre' - centre|) */
+static double idw_weight(const or_grid* G, int i, int j, int ii, int jj)
+{
+    double cx, cy, ox, oy;
+    or_cellcentre(G, i, j, &cx, &cy);
+    or_cellcentre(G, ii, jj, &ox, &oy);
+    double ex = ox - cx, ey = oy - cy;
+    return 1.0 / sqrt(ex * ex + ey * ey);
+}
+
+/* flag of GFM_VOF.cpp:42-99 / OGFM_VOF.cpp:96-136 for a cell whose 7x7
+ * neighbourhood lies in the mixed-cell loop's range: -1 with no mixed cell
+ * within three cells, else 1 (z = 0), 2 (z = 1) or 0 (mixed) */
+static int vof_flag(const or_grid* G, const double* z, int i, int j)
+{
+    int near = 0;
+    for (int xi = -3; xi <= 3 && !near; ++xi)
+        for (int yi = -3; yi <= 3; ++yi) {
+            double zz = z[IDX(G, i + yi, j + xi)];
+            if (0.0 < zz && zz < 1.0) { near = 1; break; }
+        }
+    if (!near) return -1;
+    double zc = z[IDX(G, i, j)];
+    return zc == 0.0 ? 1 : (zc == 1.0 ? 2 : 0);
+}
+
+/* inverse-distance average over the mixed cells of the 7x7 neighbourhood
+ * of (i, j), x offsets outer, y offsets inner (GFM_VOF.cpp:233-254) */
+static void vof_idw(const or_grid* G, const double* z, const double* mrp, int i, int j, double* out)
+{
+    double sw = 0.0, sp[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int xi = -3; xi <= 3; ++xi)
+        for (int yi = -3; yi <= 3; ++yi) {
+            size_t q = IDX(G, i + yi, j + xi);
+            if (!(0.0 < z[q] && z[q] < 1.0)) continue;
+            double w = idw_weight(G, i, j, i + yi, j + xi);
+            sw += w;
+            for (int k = 0; k < 4; ++k) sp[k] += w * mrp[q * 4 + k];
+        }
+    for (int k = 0; k < 4; ++k) out[k] = sp[k] / sw;
+}
+
+/* variant OR_GFM_VOF (GFM_VOF.cpp:8-325) or OR_GFM_OGFM_VOF (OGFM_VOF.cpp:8-274) */
+static int vof_ghost_states(const or_grid* G, const or_eos* E, int variant, const double* phi,
+                            double* grid1, double* grid2, int* bb1, int* bb2, double* vfield, int use_slip_BCs)
+{
+    static _Thread_local buf_t bp1, bp2, bz, bn, bl, bm1, bm2;
+    int st = 0;
+    int H = H_OF(G), Wd = W_OF(G), g = G->g;
+    int ogfm = variant == OR_GFM_OGFM_VOF;
+    size_t ncell = (size_t)H * Wd;
+    if (ogfm && use_slip_BCs) return OR_ST_BADARG;     /* OGFM_VOF.cpp:20-21 */
+    if (g < 6) return OR_ST_BADARG;                     /* the flag loop reaches six cells out */
+    double* prims1 = buf_get(&bp1, ncell * 4);
+    double* prims2 = buf_get(&bp2, ncell * 4);
+    double* z = buf_get(&bz, ncell);
+    double* nrm = buf_get(&bn, ncell * 2);
+    double* loc = buf_get(&bl, ncell * 2);
+    double* mrp1 = buf_get(&bm1, ncell * 4);            /* MRPsoln1 / MRPsoln2 */
+    double* mrp2 = buf_get(&bm2, ncell * 4);
+    double ds = fmin(G->dx, G->dy);
+    or_vof_interface(G, phi, z, nrm, loc);
+
+    if (ogfm) {                                          /* OGFM_VOF.cpp:45-53 */
+        bb1[0] = g + G->Ny; bb1[1] = g; bb1[2] = g + G->Nx; bb1[3] = g;
+        bb2[0] = g + G->Ny; bb2[1] = g; bb2[2] = g + G->Nx; bb2[3] = g;
+    } else {
+        bb1[0] = g; bb1[1] = g + G->Ny; bb1[2] = g; bb1[3] = g + G->Nx;
+        bb2[0] = g; bb2[1] = g + G->Ny; bb2[2] = g; bb2[3] = g + G->Nx;
+    }
+    for (int i = 0; i < H; ++i)                          /* GFM_VOF.cpp:50-68, OGFM_VOF.cpp:55-83 */
+        for (int j = 0; j < Wd; ++j) {
+            size_t q = IDX(G, i, j);
+            vfield[q * 2] = vfield[q * 2 + 1] = 0.0;
+            or_cons_to_prims(E->gamma1, E->pinf1, grid1 + q * 4, prims1 + q * 4);
+            or_cons_to_prims(E->gamma2, E->pinf2, grid2 + q * 4, prims2 + q * 4);
+            int* b;
+            if (z[q] >= 0.5) {
+                if (!is_physical_state(grid1 + q * 4)) st |= OR_ST_UNPHYSICAL;
+                b = bb1;
+            } else {
+                if (!is_physical_state(grid2 + q * 4)) st |= OR_ST_UNPHYSICAL;
+                b = bb2;
+            }
+            if (ogfm) {
+                if (i - 2 < b[0]) b[0] = i - 2;
+                if (i + 2 > b[1]) b[1] = i + 2;
+                if (j - 2 < b[2]) b[2] = j - 2;
+                if (j + 2 > b[3]) b[3] = j + 2;
+            }
+        }
+    if (ogfm) {                                          /* OGFM_VOF.cpp:85-93 */
+        int* bs[2] = {bb1, bb2};
+        for (int f = 0; f < 2; ++f) {
+            int* b = bs[f];
+            if (b[0] < g) b[0] = g;
+            if (b[1] > g + G->Ny) b[1] = g + G->Ny;
+            if (b[2] < g) b[2] = g;
+            if (b[3] > g + G->Nx) b[3] = g + G->Nx;
+        }
+    }
+
+    /* mixed cells within three cells of the domain, row-major, in place
+     * (GFM_VOF.cpp:73-216, OGFM_VOF.cpp:108-182) */
+    for (int i = g - 3; i < G->Ny + g + 3; ++i)
+        for (int j = g - 3; j < G->Nx + g + 3; ++j) {
+            size_t q = IDX(G, i, j);
+            double zc = z[q];
+            double* P1 = prims1 + q * 4;
+            double* P2 = prims2 + q * 4;
+            if (0.0 < zc && zc < 1.0) {
+                const double* n = nrm + 2 * q;
+                const double* l = loc + 2 * q;
+                double ip1[4], ip2[4];
+                /* GFM_base.cpp:36-56: fluid 2 at +n, fluid 1 at -n */
+                or_bilinear(G, prims2, l[0] + ds * n[0], l[1] + ds * n[1], ip2);
+                or_bilinear(G, prims1, l[0] - ds * n[0], l[1] - ds * n[1], ip1);
+                if (ogfm) {
+                    memcpy(mrp1 + q * 4, ip1, 32);       /* :148-149 */
+                    memcpy(mrp2 + q * 4, ip2, 32);
+                    if (zc > 0.5) {                      /* :154-167 */
+                        P2[0] = or_eos_isentropic(E->gamma2, E->pinf2, ip2[0], ip2[3], P1[3]);
+                        P2[1] = P1[1]; P2[2] = P1[2]; P2[3] = P1[3];
+                    } else {
+                        P1[0] = or_eos_isentropic(E->gamma1, E->pinf1, ip1[0], ip1[3], P2[3]);
+                        P1[1] = P2[1]; P1[2] = P2[2]; P1[3] = P2[3];
+                    }
+                } else {
+                    double star[4], t1[2], t2[2];
+                    st |= interface_mrp(E, ip1, ip2, n, star, t1, t2);   /* :112-153 */
+                    /* :161-178: slip keeps each fluid's tangential part, else
+                     * both take the majority fluid's */
+                    const double* ta = use_slip_BCs ? t1 : (zc > 0.5 ? t1 : t2);
+                    const double* tb = use_slip_BCs ? t2 : (zc > 0.5 ? t1 : t2);
+                    double* M1 = mrp1 + q * 4;
+                    double* M2 = mrp2 + q * 4;
+                    M1[0] = star[2]; M1[1] = star[1] * n[0] + ta[0]; M1[2] = star[1] * n[1] + ta[1]; M1[3] = star[0];
+                    M2[0] = star[3]; M2[1] = star[1] * n[0] + tb[0]; M2[2] = star[1] * n[1] + tb[1]; M2[3] = star[0];
+                    if (zc > 0.5) {                      /* :196-213 */
+                        P1[0] = or_eos_isentropic(E->gamma1, E->pinf1, star[2], star[0], P1[3]);
+                        memcpy(P2, M2, 32);
+                    } else {
+                        P2[0] = or_eos_isentropic(E->gamma2, E->pinf2, star[3], star[0], P2[3]);
+                        memcpy(P1, M1, 32);
+                    }
+                }
+            }
+            if (ogfm) {                                  /* OGFM_VOF.cpp:171-180 */
+                const double* s = zc >= 0.5 ? P1 : P2;
+                vfield[q * 2] = s[1];
+                vfield[q * 2 + 1] = s[2];
+            }
+        }
+
+    /* inverse-distance fill of the pure cells within three cells of a mixed
+     * cell (GFM_VOF.cpp:221-287, OGFM_VOF.cpp:188-268) */
+    for (int i = g; i < G->Ny + g; ++i)
+        for (int j = g; j < G->Nx + g; ++j) {
+            size_t q = IDX(G, i, j);
+            double* P1 = prims1 + q * 4;
+            double* P2 = prims2 + q * 4;
+            int fl = vof_flag(G, z, i, j);
+            if (fl == 1) {
+                double sp[4];
+                vof_idw(G, z, mrp1, i, j, sp);
+                if (ogfm) {
+                    P1[0] = or_eos_isentropic(E->gamma1, E->pinf1, sp[0], sp[3], P2[3]);
+                    P1[1] = P2[1]; P1[2] = P2[2]; P1[3] = P2[3];
+                } else {
+                    memcpy(P1, sp, 32);
+                }
+            } else if (fl == 2) {
+                double sp[4];
+                vof_idw(G, z, mrp2, i, j, sp);
+                if (ogfm) {
+                    P2[0] = or_eos_isentropic(E->gamma2, E->pinf2, sp[0], sp[3], P1[3]);
+                    P2[1] = P1[1]; P2[2] = P1[2]; P2[3] = P1[3];
+                } else {
+                    memcpy(P2, sp, 32);
+                }
+            }
+            if (ogfm) {                                  /* :263-266, interior cells only */
+                or_prims_to_cons(E->gamma1, E->pinf1, P1, grid1 + q * 4);
+                or_prims_to_cons(E->gamma2, E->pinf2, P2, grid2 + q * 4);
+                if (!is_physical_state(grid1 + q * 4) || !is_physical_state(grid2 + q * 4)) st |= OR_ST_UNPHYSICAL;
+            }
+        }
+    if (!ogfm) {                                         /* GFM_VOF.cpp:290-299, every cell */
+        for (size_t q = 0; q < ncell; ++q) {
+            or_prims_to_cons(E->gamma1, E->pinf1, prims1 + q * 4, grid1 + q * 4);
+            or_prims_to_cons(E->gamma2, E->pinf2, prims2 + q * 4, grid2 + q * 4);
+            if (!is_physical_state(grid1 + q * 4) || !is_physical_state(grid2 + q * 4)) st |= OR_ST_UNPHYSICAL;
+        }
+    }
+    or_apply_bcs(G, grid1);                              /* GFM_VOF.cpp:302-303, OGFM_VOF.cpp:271-272 */
+    or_apply_bcs(G, grid2);
+    if (!ogfm) {                                         /* GFM_VOF.cpp:306-319 */
+        for (size_t q = 0; q < ncell; ++q) {
+            const double* s = z[q] >= 0.5 ? prims1 + q * 4 : prims2 + q * 4;
+            vfield[q * 2] = s[1];
+            vfield[q * 2 + 1] = s[2];
+        }
+    }
+    return st;
+}
+
 /* A8-A11, with the GFM_base flags use_extension_vfield and use_slip_BCs
  * (GFM_base.hpp:20-21, defaults true and false; sim_twofluid.cpp:551-552
  * sets both false) */
@@ -1170,6 +1433,8 @@ int or_set_ghost_states_flags(const or_grid* G, const or_eos* E, int variant, co
     double* prims2 = buf_get(&bp2, ncell * 4);
     double* iv = buf_get(&biv, ncell * 2);   /* newvelocities of the interfacial cells */
     double ds = fmin(G->dx, G->dy);
+    if (variant == OR_GFM_VOF || variant == OR_GFM_OGFM_VOF)
+        return vof_ghost_states(G, E, variant, phi, grid1, grid2, bb1, bb2, vfield, use_slip_BCs);
     if (variant == OR_GFM_ORIGINAL && use_slip_BCs) return OR_ST_BADARG;   /* GFM_original.cpp:21 */
 
     for (size_t q = 0; q < ncell; ++q) {

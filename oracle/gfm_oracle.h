@@ -20,7 +20,8 @@ extern "C" {
 #endif
 
 enum { OR_BC_TRANSMISSIVE = 0, OR_BC_REFLECTIVE = 1, OR_BC_PERIODIC = 2 };
-enum { OR_GFM_ORIGINAL = 0, OR_GFM_REAL = 1, OR_GFM_RIEMANN = 2, OR_GFM_MODIFIED = 3 };
+enum { OR_GFM_ORIGINAL = 0, OR_GFM_REAL = 1, OR_GFM_RIEMANN = 2, OR_GFM_MODIFIED = 3,
+       OR_GFM_VOF = 4, OR_GFM_OGFM_VOF = 5 };
 enum { OR_FS_MUSCL = 0, OR_FS_GODUNOV = 1 };
 
 /* status bits (the reference aborts on assert; the oracle records instead) */
@@ -101,6 +102,8 @@ void or_bilinear(const or_grid* G, const double* grid, double x, double y, doubl
 void or_tracers_step(const or_grid* G, const double* vfield, int n, double* pos, const double* pos0,
                      int keep_y, double t, double dt, double* entry);
 void or_volume_fraction(const or_grid* G, const double* phi, double* z);
+/* the VOF interface the VOF-based GFMs read, from the level set: z, unit normal, chord midpoint */
+void or_vof_interface(const or_grid* G, const double* phi, double* z, double* nrm2, double* loc2);
 void or_houc_coeffs(int order, double* c, int* offset0);
 void or_ls_advect(const or_grid* G, int order, double* phi, const double* vfield, double dt);
 void or_ls_reinit(const or_grid* G, double* phi, int iters);

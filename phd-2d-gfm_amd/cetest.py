@@ -27,11 +27,14 @@ import numpy as np
 
 import gfm_hip as G
 
-# density L1 errors at N = 50, 100, 200, 400 (CEtest_GFM.eps:650-681)
+# density L1 errors at N = 50, 100, 200, 400 (CEtest_GFM.eps:650-681; VOFGFM: CEtest_VOFGFM.eps:661-664,
+# run with the reference's VOF tracker, "PYVOF")
 THESIS = {"OGFM": [0.0506, 0.0266, 0.0134, 0.00732],
           "riemannGFM": [0.0902, 0.0445, 0.0240, 0.0126],
-          "MGFM": [0.0413, 0.0236, 0.0119, 0.00667]}
-GFMS = {"OGFM": G.GFM_ORIGINAL, "realGFM": G.GFM_REAL, "riemannGFM": G.GFM_RIEMANN, "MGFM": G.GFM_MODIFIED}
+          "MGFM": [0.0413, 0.0236, 0.0119, 0.00667],
+          "VOFGFM": [0.0433, 0.0256, 0.0120, 0.00642]}
+GFMS = {"OGFM": G.GFM_ORIGINAL, "realGFM": G.GFM_REAL, "riemannGFM": G.GFM_RIEMANN, "MGFM": G.GFM_MODIFIED,
+        "VOFGFM": G.GFM_VOFGFM, "OGFMVOF": G.GFM_OGFMVOF}
 INSIDE, OUTSIDE = (1.0, 0.0, 0.0, 1.0), (0.125, 0.0, 0.0, 0.1)
 T_END = 0.25
 TR = G.BC_TRANSMISSIVE

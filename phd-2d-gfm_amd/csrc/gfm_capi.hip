@@ -136,7 +136,11 @@ struct gfm_ctx {
     bool twofluid() const { return m.sim_type == GFM_SIM_TWOFLUID; }
     int nfluid() const { return twofluid() ? 2 : 1; }
     size_t es() const { return mixed() ? 4 : 8; }
-    int narrow() const { return twofluid() && m.gfm == GFM_ORIGINAL ? 1 : 0; }
+    // OGFM and OGFM_VOF narrow the update boxes to each fluid's real cells
+    int narrow() const { return twofluid() && (m.gfm == GFM_ORIGINAL || m.gfm == GFM_OGFMVOF) ? 1 : 0; }
+    bool vof() const { return twofluid() && (m.gfm == GFM_VOFGFM || m.gfm == GFM_OGFMVOF); }
+    // the variants whose interfacial list the dt pass counts
+    int counted() const { return twofluid() && m.gfm != GFM_ORIGINAL && !vof() ? 1 : 0; }
 };
 
 namespace {
@@ -230,6 +234,15 @@ int ghost_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int counted = 0, hipEvent_t
             }
             break;
         }
+        case GFM_VOFGFM:     // GFM_VOF.cpp:8-325
+        case GFM_OGFMVOF:   // OGFM_VOF.cpp:8-274
+            gfm::launch_vof<R>(L, c->m.gfm == GFM_OGFMVOF);
+            if (mark) HC(hipEventRecord(mark, L.stream));
+            // both apply the BCs to their result (GFM_VOF.cpp:302-303,
+            // OGFM_VOF.cpp:271-272) and store the real fluid's velocity,
+            // whatever use_extension_vfield says
+            gfm::launch_bc<R>(L, 1, 2, 0);
+            return GFM_OK;
         default: break;
     }
     // the extension velocity field replaces the real fluid's velocity the
@@ -243,7 +256,7 @@ template <typename R>
 int twofluid_step(gfm_ctx* c, const gfm::LaunchCtx& L, hipEvent_t mark) {
     const bool muscl = c->m.fs == GFM_FS_MUSCL;
     // the dt pass also counts the interfacial cells when the variant lists them
-    const int count = c->m.gfm != GFM_ORIGINAL;
+    const int count = c->counted();
     gfm::launch_dt<R>(L, 1, 0, c->narrow(), count);
     int r = ghost_chain<R>(c, L, count, mark, c->mixed() ? 1 : 0);
     if (r) return r;
@@ -453,13 +466,13 @@ namespace {
 
 // work tables of a set of instances, appended to `all`
 struct TableOffsets {
-    size_t o[9];
-    int n[9];
+    size_t o[10];
+    int n[10];
 };
 
 TableOffsets append_tables(const gfm_ctx* c, const std::vector<Inst>& descs, const std::vector<int>& members,
                            std::vector<int4>& all) {
-    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich, w_tb, w_rt, w_seg;
+    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich, w_tb, w_rt, w_seg, w_vseg;
     const int nf = c->nfluid();
     for (int ki : members) {
         const Inst& d = descs[ki];
@@ -495,16 +508,23 @@ TableOffsets append_tables(const gfm_ctx* c, const std::vector<Inst>& descs, con
                 for (int tx = 0; tx < (s.Nx + 63) / 64; ++tx) w_rt.push_back(make_int4(ki, ty, tx, 0));
         if (c->twofluid())
             for (int o = 0; o < d.icap; o += 256) w_ich.push_back(make_int4(ki, o, 0, 0));
+        if (c->vof()) {
+            const int nsx = (s.Nx + 6 + 255) / 256;
+            for (int r = 0; r < s.Ny + 6; ++r)
+                for (int x = 0; x < nsx; ++x) w_vseg.push_back(make_int4(ki, r, x, r * nsx + x));
+        }
     }
     xcd_order(w_full, 0);
     xcd_order(w_fs, 0);
     xcd_order(w_tb, 0);
     xcd_order(w_rt, 0);
     xcd_order(w_seg, 0, gfm::SEG_PER_WG);
+    xcd_order(w_vseg, 0, gfm::SEG_PER_WG);
     if (w_ich.empty()) w_ich.push_back(make_int4(members.empty() ? 0 : members[0], 1 << 30, 0, 0));
-    const std::vector<int4>* t[9] = {&w_full, &w_int, &w_fs, &w_frame, &w_inst, &w_ich, &w_tb, &w_rt, &w_seg};
+    const std::vector<int4>* t[10] = {&w_full, &w_int, &w_fs, &w_frame, &w_inst, &w_ich, &w_tb, &w_rt, &w_seg,
+                                      &w_vseg};
     TableOffsets r{};
-    for (int q = 0; q < 9; ++q) {
+    for (int q = 0; q < 10; ++q) {
         r.o[q] = all.size();
         r.n[q] = (int)t[q]->size();
         all.insert(all.end(), t[q]->begin(), t[q]->end());
@@ -527,6 +547,7 @@ gfm::LaunchCtx launch_ctx(const gfm_ctx* c, const TableOffsets& t, hipStream_t s
     L.w_tb = w + t.o[6]; L.n_tb = t.n[6];
     L.w_rt = w + t.o[7]; L.n_rt = t.n[7];
     L.w_seg = w + t.o[8]; L.n_seg = t.n[8];
+    L.w_vseg = w + t.o[9]; L.n_vseg = t.n[9];
     L.tb = c->tb ? 1 : 0;
     for (int k : members) {
         if (c->inst[k].info.BC_B == GFM_BC_PERIODIC) L.periodic_y = 1;
@@ -588,7 +609,7 @@ int gfm_ctx_create(const gfm_methods* methods, gfm_ctx** out) {
     if (!methods || !out) return fail_arg("null argument");
     if (methods->sim_type != GFM_SIM_ONEFLUID && methods->sim_type != GFM_SIM_TWOFLUID) return fail_arg("sim_type");
     if (methods->fs != GFM_FS_MUSCL && methods->fs != GFM_FS_GODUNOV) return fail_arg("fs");
-    if (methods->gfm < GFM_ORIGINAL || methods->gfm > GFM_MODIFIED) return fail_arg("gfm");
+    if (methods->gfm < GFM_ORIGINAL || methods->gfm > GFM_OGFMVOF) return fail_arg("gfm");
     if (methods->precision != GFM_FP64 && methods->precision != GFM_MIXED) return fail_arg("precision");
     const int o = methods->ls_order;
     if (o != 1 && o != 3 && o != 5 && o != 7 && o != 9) return fail_arg("ls_order must be 1,3,5,7 or 9");
@@ -663,6 +684,7 @@ int gfm_set_gfm_flags(gfm_ctx* c, int use_extension_vfield, int use_slip_BCs) {
     if (c->committed) return fail_arg("the GFM flags are set before gfm_ctx_commit");
     if (!c->twofluid() && (use_extension_vfield || use_slip_BCs)) return fail_arg("GFM flags need a two-fluid context");
     if (use_slip_BCs && c->m.gfm == GFM_ORIGINAL) return fail_arg("GFM_original has no slip BCs (GFM_original.cpp:21)");
+    if (use_slip_BCs && c->m.gfm == GFM_OGFMVOF) return fail_arg("OGFM_VOF has no slip BCs (OGFM_VOF.cpp:20-21)");
     c->use_extension_vfield = use_extension_vfield != 0;
     c->use_slip_BCs = use_slip_BCs != 0;
     return GFM_OK;
@@ -704,9 +726,12 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.cfl = h.cfl; d.T = h.T; d.max_steps = h.max_steps;
         d.use_bbox = 1;   // boxes are reset to the full domain each step unless OGFM narrows them
         d.use_slip = c->use_slip_BCs ? 1 : 0;
-        d.band = c->mixed() && c->twofluid() ? c->ghost_band * std::max(s.dx, s.dy) : 0.0;
+        // the ghost band is the extrapolating fills' (the VOF fills update only cells near the interface)
+        d.band = c->mixed() && c->twofluid() && !c->vof() ? c->ghost_band * std::max(s.dx, s.dy) : 0.0;
         d.ntile_dt = s.Ny * ((s.Nx + 255) / 256);   // dt partials: one per 256-cell row segment
-        d.icap = c->twofluid() ? std::min(std::min(s.Nx * s.Ny, 16 * (s.Nx + s.Ny) + 1024), 49152) : 1;
+        // the VOF GFMs list the mixed cells of the interior grown by three cells
+        const int lnx = c->vof() ? s.Nx + 6 : s.Nx, lny = c->vof() ? s.Ny + 6 : s.Ny;
+        d.icap = c->twofluid() ? std::min(std::min(lnx * lny, 16 * (lnx + lny) + 1024), 49152) : 1;
         // layout of the single allocation
         const size_t P = (size_t)d.plane;
         const int nf = c->nfluid();
@@ -716,7 +741,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         for (int f = 0; f < nf; ++f)
             for (int b = 0; b < 2; ++b) oQ[f][b] = take(4 * P * es);
         size_t ophi = 0, oA = 0, oB = 0, ovel = 0, oimap = 0, oilist = 0, oside = 0, orec = 0, odone = 0, oseg = 0;
-        d.nseg = s.Ny * ((s.Nx + 255) / 256);
+        d.nseg = lny * ((lnx + 255) / 256);
         if (c->twofluid()) {
             for (int f = 0; f < 2; ++f)
                 for (int b = 0; b < 2; ++b) oP[f][b] = take(4 * P * es);
@@ -724,8 +749,8 @@ int gfm_ctx_commit(gfm_ctx* c) {
             ovel = take(2 * P * es);
             oimap = take(P * sizeof(int));
             oilist = take((size_t)d.icap * sizeof(int));
-            oside = take((size_t)d.icap * 8 * sizeof(double));
-            orec = take((size_t)d.icap * gfm::IFREC_BYTES);
+            oside = take((size_t)d.icap * (c->vof() ? gfm::VSIDE : 8) * sizeof(double));
+            orec = take((size_t)d.icap * (c->vof() ? gfm::VOFREC_BYTES : gfm::IFREC_BYTES));
             odone = take((size_t)d.icap * sizeof(int));
             oseg = take((size_t)d.nseg * sizeof(int));   // interfacial counts per segment
         }
@@ -1242,9 +1267,9 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
         HC(hipEventRecord(ev[0], c->stream));
         if (c->twofluid()) {
             if (c->mixed()) {
-                gfm::launch_dt<float>(c->L, 1, 0, c->narrow(), c->m.gfm != GFM_ORIGINAL);
+                gfm::launch_dt<float>(c->L, 1, 0, c->narrow(), c->counted());
                 HC(hipEventRecord(ev[1], c->stream));
-                ghost_chain<float>(c, c->L, c->m.gfm != GFM_ORIGINAL, nullptr, 1);
+                ghost_chain<float>(c, c->L, c->counted(), nullptr, 1);
                 HC(hipEventRecord(ev[2], c->stream));
                 gfm::launch_fs<float>(c->L, muscl, 1, 0, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));
@@ -1252,9 +1277,9 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
                 HC(hipEventRecord(ev[4], c->stream));
                 gfm::launch_bc<float>(c->L, 0, 2, 1);
             } else {
-                gfm::launch_dt<double>(c->L, 1, 0, c->narrow(), c->m.gfm != GFM_ORIGINAL);
+                gfm::launch_dt<double>(c->L, 1, 0, c->narrow(), c->counted());
                 HC(hipEventRecord(ev[1], c->stream));
-                ghost_chain<double>(c, c->L, c->m.gfm != GFM_ORIGINAL);
+                ghost_chain<double>(c, c->L, c->counted());
                 HC(hipEventRecord(ev[2], c->stream));
                 gfm::launch_fs<double>(c->L, muscl, 1, 0, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));

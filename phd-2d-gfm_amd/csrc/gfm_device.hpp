@@ -10,7 +10,7 @@
 namespace gfm {
 
 enum { BC_T = 0, BC_R = 1, BC_P = 2 };
-enum { V_ORIGINAL = 0, V_REAL = 1, V_RIEMANN = 2, V_MODIFIED = 3 };
+enum { V_ORIGINAL = 0, V_REAL = 1, V_RIEMANN = 2, V_MODIFIED = 3, V_VOF = 4, V_OGFM_VOF = 5 };
 
 // double scalars per instance
 // SD_HOLD_LAST / SD_HOLD_IVL: an instance with SI_HOLD set takes no step while
@@ -93,7 +93,7 @@ struct Inst {
     void* vel;                 // 2 planes
     int* imap;                 // interfacial list index per cell, -1 if none
     int* ilist;                // row-major linear index (i*W+j) per entry
-    double* side;              // 8 doubles per interfacial entry
+    double* side;              // 8 doubles per interfacial entry (VSIDE per mixed cell for the VOF GFMs)
     double* partials;          // 4 doubles per dt tile
     double* sd;                // SD_* scalars
     int* si;                   // SI_* scalars

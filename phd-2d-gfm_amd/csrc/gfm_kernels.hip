@@ -1465,6 +1465,26 @@ __global__ __launch_bounds__(256) void k_iface_mark(const Inst* __restrict__ ins
 // ---------------------------------------------------------------------------
 constexpr int ICAP_LDS = 49152;
 
+// area fraction on the side n . (x - c) + phi <= 0 of a dx x dy cell, for the
+// line through the cell centre c with phi's value and unit normal (nx, ny)
+// there (Scardovelli and Zaleski, J. Comput. Phys. 164 (2000), 2D case): with
+// a = |nx| dx <= b = |ny| dy (swapped if not) and m = -phi + (a + b)/2, the
+// fraction is 0 (m <= 0), m^2 / (2ab) (m < a), (2m - a) / (2b) (m < b),
+// 1 - (a + b - m)^2 / (2ab) (m < a + b), else 1
+__host__ __device__ inline double area_fraction(double phi, double nx, double ny, double dx, double dy) {
+#pragma clang fp contract(off)
+    double a = fabs(nx) * dx, b = fabs(ny) * dy;
+    if (a > b) { const double s = a; a = b; b = s; }
+    const double m = -phi + 0.5 * (a + b);
+    if (m <= 0.0) return 0.0;
+    if (m >= a + b) return 1.0;
+    if (m < a) return m * m / (2.0 * a * b);
+    if (m < b) return (2.0 * m - a) / (2.0 * b);
+    const double r = a + b - m;
+    return 1.0 - r * r / (2.0 * a * b);
+}
+
+
 struct Probe { int i0, j0; double tx, ty; };
 
 __device__ __forceinline__ Probe bilinear_stencil(const Inst& I, double x, double y) {
@@ -1585,6 +1605,19 @@ struct alignas(16) IfRec {
     int dep[2][4];
 };
 static_assert(sizeof(IfRec) <= IFREC_BYTES, "interfacial record larger than its slot");
+
+// a mixed cell of the VOF GFMs: its unit normal and z, its original
+// primitives of both fluids, and its probes (index 0: fluid 1 at -n, 1:
+// fluid 2 at +n) with the corner primitives and which corners are earlier
+// mixed cells, whose new primitives replace the stored ones
+struct alignas(16) VofRec {
+    double nx, ny, z, pspec;
+    double own[2][4];
+    double tx[2], ty[2];
+    double cv[2][4][4];
+    int dep[2][4];
+};
+static_assert(sizeof(VofRec) <= VOFREC_BYTES, "mixed-cell record larger than its slot");
 
 __device__ __forceinline__ void bilerp(const double (*cv)[4], double tx, double ty, double* out) {
     for (int k = 0; k < 4; ++k)
@@ -1751,14 +1784,21 @@ __global__ __launch_bounds__(256) void k_iface_prep(const Inst* __restrict__ ins
 
 // star state of pending entry k once every earlier interfacial cell it reads
 // has its own (the side buffer holds those)
+template <bool OG, bool FAST>
+__device__ __forceinline__ unsigned vof_solve_pending(const Inst& I, const VofRec& r, int k);
+
 template <int VAR, bool FAST, typename Rec>
 __device__ __forceinline__ unsigned iface_solve_pending(const Inst& I, const Rec& r, int k) {
-    double ip1[4], ip2[4], out[8], ps;
-    probe_values_side<VAR>(I, r, ip1, ip2);
-    const unsigned st = iface_finish<VAR, FAST>(I, r, ip1, ip2, r.pspec, out, &ps);
-    const auto o = gmem(I.side) + 8 * (long long)k;
-    for (int q = 0; q < 6; ++q) o[q] = out[q];
-    return st;
+    if constexpr (VAR == V_VOF || VAR == V_OGFM_VOF) {
+        return vof_solve_pending<VAR == V_OGFM_VOF, FAST>(I, r, k);
+    } else {
+        double ip1[4], ip2[4], out[8], ps;
+        probe_values_side<VAR>(I, r, ip1, ip2);
+        const unsigned st = iface_finish<VAR, FAST>(I, r, ip1, ip2, r.pspec, out, &ps);
+        const auto o = gmem(I.side) + 8 * (long long)k;
+        for (int q = 0; q < 6; ++q) o[q] = out[q];
+        return st;
+    }
 }
 
 // Pass 2, one workgroup per instance.  Entries are in row-major cell order
@@ -1772,6 +1812,10 @@ __device__ __forceinline__ unsigned iface_solve_pending(const Inst& I, const Rec
 // repeat.  Both give the same bits.
 constexpr unsigned IFACE_SPIN_LIMIT = 1u << 24;
 
+template <int VAR> struct RecOf { using type = IfRec; };
+template <> struct RecOf<V_VOF> { using type = VofRec; };
+template <> struct RecOf<V_OGFM_VOF> { using type = VofRec; };
+
 template <typename R, int VAR>
 __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ insts, const int4* __restrict__ work,
                                                       int plain) {
@@ -1784,7 +1828,8 @@ __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ i
     if (!gmem(I.si)[SI_ACTIVE]) return;
     const int n = min(gmem(I.si)[SI_NIFACE], I.icap);
     if (n > ICAP_LDS) { if (threadIdx.x == 0) flag(I, ST_IOVF); return; }
-    const auto recs = gmem((const IfRec*)I.irec);
+    using Rec = typename RecOf<VAR>::type;
+    const auto recs = gmem((const Rec*)I.irec);
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int k = tid; k < n; k += nt) fin[k] = I.idone[k] ? 1 : 0;
     if (tid == 0) nclaim = nt;
@@ -1912,6 +1957,439 @@ __global__ __launch_bounds__(256) void k_mgfm_final(const Inst* __restrict__ ins
     }
     st4<R>(I.Q[0][qout], I, c, prims2cons<R>((R)I.gam[0], (R)I.pinf[0], W1));
     st4<R>(I.Q[1][qout], I, c, prims2cons<R>((R)I.gam[1], (R)I.pinf[1], W2));
+}
+
+// ---------------------------------------------------------------------------
+// VOF-based GFMs: GFM_VOF (GFM_VOF.cpp:8-325) and OGFM_VOF (OGFM_VOF.cpp:8-274).
+// Their interface (GFM_VOF_interface: get_z, get_mixedcell_normal,
+// get_interfacelocation; the reference's trackers are external) is the level
+// set's PLIC reconstruction: in each cell the line through its centre with
+// phi's value and unit normal there.  z is the area fraction on fluid 1's side
+// (area_fraction), the normal points into fluid 2 and the interface location
+// is the midpoint of the line's chord through the cell.
+//
+// The mixed cells (0 < z < 1) within three cells of the interior are listed in
+// row-major order, the order of the reference's loops (GFM_VOF.cpp:73-216,
+// OGFM_VOF.cpp:108-182), and solved in that order: a mixed cell's probes read
+// the primitives of earlier mixed cells as those modified them in place (both
+// fluids for GFM_VOF, the ghost fluid for OGFM_VOF), through the dataflow pass
+// of the interfacial solve (k_iface_rounds).  A listed cell's side-buffer
+// entry holds VSIDE doubles: its MRP states of fluid 1 and fluid 2 (GFM_VOF:
+// the star states; OGFM_VOF: the probe states), then its new primitives of
+// fluid 1 and fluid 2.  The final pass writes every cell: a listed cell's new
+// primitives; a pure interior cell within three cells of listed ones the
+// inverse-distance average of their MRP states of its ghost fluid (OGFM_VOF:
+// isentropic to the real fluid's pressure, with its velocity); any other cell
+// its own state.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool vof_range(const Inst& I, int i, int j) {
+    return i >= I.g - 3 && i < I.g + I.Ny + 3 && j >= I.g - 3 && j < I.g + I.Nx + 3;
+}
+
+// z of a cell; the outermost ring has no centred gradient and takes 1 or 0 by
+// phi's sign
+template <typename R>
+__device__ __forceinline__ double vof_z(const Inst& I, int i, int j) {
+    const double ph = (double)ld1<R>(I.phi, cidx(I, i, j));
+    if (i == 0 || j == 0 || i == I.H - 1 || j == I.W - 1) return ph <= 0.0 ? 1.0 : 0.0;
+    double nx, ny;
+    unit_normal<R>(I, i, j, nx, ny);
+    return area_fraction(ph, nx, ny, I.dx, I.dy);
+}
+
+struct VofGeom { double z, nx, ny, lx, ly; };
+
+// z, unit normal and interface location of a cell: the chord midpoint is the
+// foot point c - phi n moved along the tangent (-ny, nx) to the middle of the
+// parameter interval that the cell's sides clip (the foot point if the line
+// misses the cell)
+template <typename R>
+__device__ __forceinline__ VofGeom vof_geom(const Inst& I, int i, int j) {
+#pragma clang fp contract(off)
+    VofGeom gm;
+    const double p = (double)ld1<R>(I.phi, cidx(I, i, j));
+    const double cx = I.x0 + (j - I.g + 0.5) * I.dx, cy = I.y0 + (i - I.g + 0.5) * I.dy;
+    if (i == 0 || j == 0 || i == I.H - 1 || j == I.W - 1) {
+        gm.z = p <= 0.0 ? 1.0 : 0.0;
+        gm.nx = 1.0; gm.ny = 0.0; gm.lx = cx; gm.ly = cy;
+        return gm;
+    }
+    unit_normal<R>(I, i, j, gm.nx, gm.ny);
+    gm.z = area_fraction(p, gm.nx, gm.ny, I.dx, I.dy);
+    const double fx = -p * gm.nx, fy = -p * gm.ny;
+    double lo = -1e300, hi = 1e300;
+    if (gm.ny != 0.0) {
+        const double s1 = (fx - 0.5 * I.dx) / gm.ny, s2 = (fx + 0.5 * I.dx) / gm.ny;
+        lo = fmax(lo, fmin(s1, s2));
+        hi = fmin(hi, fmax(s1, s2));
+    }
+    if (gm.nx != 0.0) {
+        const double s1 = (-0.5 * I.dy - fy) / gm.nx, s2 = (0.5 * I.dy - fy) / gm.nx;
+        lo = fmax(lo, fmin(s1, s2));
+        hi = fmin(hi, fmax(s1, s2));
+    }
+    const double sm = lo <= hi ? 0.5 * (lo + hi) : 0.0;
+    gm.lx = cx + (fx - sm * gm.ny);
+    gm.ly = cy + (fy + sm * gm.nx);
+    return gm;
+}
+
+// FP64 primitives of fluid f at a cell, from the stored conserved state
+template <typename R>
+__device__ __forceinline__ void vof_prims(const Inst& I, int f, int i, int j, double* W) {
+    const V4<R> U = ld4<R>(I.Q[f][0], I, cidx(I, i, j));
+    V4<double> Ud;
+    for (int k = 0; k < 4; ++k) Ud.v[k] = (double)U.v[k];
+    const V4<double> P = cons2prims<double>(I.gam[f], I.pinf[f], Ud);
+    for (int k = 0; k < 4; ++k) W[k] = P.v[k];
+}
+
+// isentropic_extrapolation (OGFM_VOF.cpp:156, GFM_VOF.cpp:198)
+template <bool FAST>
+__device__ __forceinline__ double vof_isen(const Inst& I, int f, double rho_old, double p_old, double p_new) {
+    return rho_old * rpow<FAST>((p_new + I.pinf[f]) / (p_old + I.pinf[f]), 1.0 / I.gam[f]);
+}
+
+// the mixed cell's MRP states and new primitives from its probe states
+// (GFM_VOF.cpp:112-213; OGFM_VOF.cpp:148-167); out: VSIDE doubles
+template <bool OG, bool FAST>
+__device__ __forceinline__ unsigned vof_finish(const Inst& I, const VofRec& r, const double* ip1, const double* ip2,
+                                               double p0, double* out, double* pstar) {
+#pragma clang fp contract(off)
+    double* M1 = out;
+    double* M2 = out + 4;
+    double* N1 = out + 8;
+    double* N2 = out + 12;
+    for (int k = 0; k < 4; ++k) { N1[k] = r.own[0][k]; N2[k] = r.own[1][k]; }
+    if (OG) {
+        for (int k = 0; k < 4; ++k) { M1[k] = ip1[k]; M2[k] = ip2[k]; }
+        // the ghost fluid: the real fluid's velocity and pressure, its own
+        // probe's density made isentropic to that pressure
+        if (r.z > 0.5) {
+            N2[0] = vof_isen<FAST>(I, 1, ip2[0], ip2[3], r.own[0][3]);
+            N2[1] = r.own[0][1]; N2[2] = r.own[0][2]; N2[3] = r.own[0][3];
+        } else {
+            N1[0] = vof_isen<FAST>(I, 0, ip1[0], ip1[3], r.own[1][3]);
+            N1[1] = r.own[1][1]; N1[2] = r.own[1][2]; N1[3] = r.own[1][3];
+        }
+        *pstar = 0.0;
+        return 0;
+    }
+    Star S;
+    double t1[2], t2[2];
+    unsigned st = iface_mrp<FAST>(I, ip1, ip2, r.nx, r.ny, S, t1, t2, p0);
+    // slip: each fluid keeps its tangential velocity; no-slip: both take the
+    // majority fluid's (GFM_VOF.cpp:161-178)
+    const double* ta = I.use_slip ? t1 : (r.z > 0.5 ? t1 : t2);
+    const double* tb = I.use_slip ? t2 : (r.z > 0.5 ? t1 : t2);
+    M1[0] = S.rl; M1[1] = S.u * r.nx + ta[0]; M1[2] = S.u * r.ny + ta[1]; M1[3] = S.p;
+    M2[0] = S.rr; M2[1] = S.u * r.nx + tb[0]; M2[2] = S.u * r.ny + tb[1]; M2[3] = S.p;
+    // the majority fluid keeps its state but for the density, isentropic
+    // from its star state to its own pressure; the other takes its star state
+    if (r.z > 0.5) {
+        N1[0] = vof_isen<FAST>(I, 0, S.rl, S.p, r.own[0][3]);
+        for (int k = 0; k < 4; ++k) N2[k] = M2[k];
+    } else {
+        N2[0] = vof_isen<FAST>(I, 1, S.rr, S.p, r.own[1][3]);
+        for (int k = 0; k < 4; ++k) N1[k] = M1[k];
+    }
+    if (S.p != S.p) st |= ST_RS;
+    *pstar = S.p;
+    return st;
+}
+
+// probe states of a record: a dependent corner reads the new primitives of
+// the earlier mixed cell from the side buffer (GFM_base.cpp:36-56)
+template <bool SIDE>
+__device__ __forceinline__ void vof_probe_values(const Inst& I, const VofRec& r, double* ip1, double* ip2) {
+    double cv[2][4][4];
+    const auto side = gmem(I.side);
+    for (int p = 0; p < 2; ++p)
+        for (int q = 0; q < 4; ++q) {
+            const int d = SIDE ? r.dep[p][q] : -1;
+            for (int k = 0; k < 4; ++k) cv[p][q][k] = d >= 0 ? side[(long long)VSIDE * d + 8 + 4 * p + k] : r.cv[p][q][k];
+        }
+    bilerp(cv[0], r.tx[0], r.ty[0], ip1);
+    bilerp(cv[1], r.tx[1], r.ty[1], ip2);
+}
+
+template <bool OG, bool FAST>
+__device__ __forceinline__ unsigned vof_solve_pending(const Inst& I, const VofRec& r, int k) {
+    double ip1[4], ip2[4], out[VSIDE], ps;
+    vof_probe_values<true>(I, r, ip1, ip2);
+    const unsigned st = vof_finish<OG, FAST>(I, r, ip1, ip2, r.pspec, out, &ps);
+    const auto o = gmem(I.side) + (long long)VSIDE * k;
+    for (int q = 0; q < VSIDE; ++q) o[q] = out[q];
+    return st;
+}
+
+// mixed cells of a row segment of the grown interior: bit q for cell q*64 + lane
+template <typename R>
+__device__ __forceinline__ unsigned vof_seg_mixed(const Inst& I, const int4& w, int lane) {
+    const int i = I.g - 3 + w.y, jend = I.g + I.Nx + 3;
+    unsigned m = 0;
+#pragma unroll
+    for (int q = 0; q < SEG_Q; ++q) {
+        const int j = I.g - 3 + w.z * SEGW + q * 64 + lane;
+        if (j < jend) {
+            const double z = vof_z<R>(I, i, j);
+            if (0.0 < z && z < 1.0) m |= 1u << q;
+        }
+    }
+    return m;
+}
+
+// erases the previous mixed-cell list from imap (entries [s SEGW, (s+1) SEGW)
+// for segment s, which covers the list: at most one entry per cell of the
+// grown interior) and counts the segment's mixed cells
+// work: {inst, row of the grown interior, segment column, segment index}
+template <typename R>
+__global__ __launch_bounds__(256) void k_vof_count(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                   int nwork) {
+    const int sw = blockIdx.x * SEG_PER_WG + (int)(threadIdx.x >> 6);
+    if (sw >= nwork) return;
+    const int4 w = work[sw];
+    const Inst& I = insts[w.x];
+    const int lane = threadIdx.x & 63;
+    {
+        const int n = gmem(I.si)[SI_NLIST];
+#pragma unroll
+        for (int q = 0; q < SEG_Q; ++q) {
+            const int k = w.w * SEGW + q * 64 + lane;
+            if (k < n) {
+                const int v = I.ilist[k];
+                const int i = v / I.W, j = v - (v / I.W) * I.W;
+                if (v >= 0 && vof_range(I, i, j)) I.imap[cidx(I, i, j)] = -1;
+            }
+        }
+    }
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const unsigned m = vof_seg_mixed<R>(I, w, lane);
+    int nc = 0;
+#pragma unroll
+    for (int k = 0; k < SEG_Q; ++k) nc += __popcll(__ballot((m >> k) & 1u));
+    if (lane == 0) I.iseg[w.w] = nc;
+}
+
+// each segment with mixed cells lists them in column order from its offset
+template <typename R>
+__global__ __launch_bounds__(256) void k_vof_mark(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                  int nwork) {
+    const int sw = blockIdx.x * SEG_PER_WG + (int)(threadIdx.x >> 6);
+    if (sw >= nwork) return;
+    const int4 w = work[sw];
+    const Inst& I = insts[w.x];
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const int lane = threadIdx.x & 63;
+    int k = I.iseg[w.w];
+    const int next = w.w + 1 < I.nseg ? I.iseg[w.w + 1] : gmem(I.si)[SI_NIFACE];
+    if (next == k) return;
+    const unsigned m = vof_seg_mixed<R>(I, w, lane);
+    const int i = I.g - 3 + w.y;
+    bool ovf = false, bad = false;
+#pragma unroll
+    for (int q = 0; q < SEG_Q; ++q) {
+        const bool f = (m >> q) & 1u;
+        const unsigned long long mask = __ballot(f);
+        if (f) {
+            const int kk = k + __popcll(mask & ((1ull << lane) - 1ull));
+            const int j = I.g - 3 + w.z * SEGW + q * 64 + lane;
+            if (kk >= next) {
+                bad = true;
+            } else if (kk < I.icap) {
+                I.ilist[kk] = i * I.W + j;
+                I.imap[cidx(I, i, j)] = kk;
+            } else {
+                ovf = true;
+            }
+        }
+        k += __popcll(mask);
+    }
+    if (k < next) bad = true;
+    if (ovf) flag(I, ST_IOVF);
+    if (bad) flag(I, ST_ILIST);
+}
+
+// a read of fluid f at corner (ii, jj) by the mixed cell with linear index
+// lin depends on an earlier mixed cell if that cell changed fluid f: both
+// fluids for GFM_VOF, the ghost fluid (fluid 2 where z > 1/2) for OGFM_VOF
+template <typename R, bool OG>
+__device__ __forceinline__ int vof_dep(const Inst& I, int ii, int jj, int f, int lin) {
+    const int k = I.imap[cidx(I, ii, jj)];
+    if (k < 0 || ii * I.W + jj >= lin) return -1;
+    if (OG) {
+        const int fg = vof_z<R>(I, ii, jj) > 0.5 ? 1 : 0;
+        if (fg != f) return -1;
+    }
+    return k;
+}
+
+// Pass 1, parallel over the mixed cells: records; a cell that reads no
+// earlier mixed cell is final here, the others get a speculative p* (GFM_VOF)
+// work: {inst, first entry of a 256-entry chunk, 0, 0}
+template <typename R, bool OG>
+__global__ __launch_bounds__(256) void k_vof_prep(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+#pragma clang fp contract(off)
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const int n = min(gmem(I.si)[SI_NIFACE], I.icap);
+    const int k = w.y + threadIdx.x;
+    if (k >= n) return;
+    int lin = I.ilist[k];
+    int i = lin / I.W, j = lin % I.W;
+    if (lin < 0 || !vof_range(I, i, j)) {
+        flag(I, ST_ILIST);   // only if the counts and the list disagreed
+        i = I.g; j = I.g; lin = i * I.W + j;
+    }
+    VofRec r;
+    const VofGeom gm = vof_geom<R>(I, i, j);
+    r.nx = gm.nx; r.ny = gm.ny; r.z = gm.z;
+    vof_prims<R>(I, 0, i, j, r.own[0]);
+    vof_prims<R>(I, 1, i, j, r.own[1]);
+    const double ds = fmin(I.dx, I.dy);
+    Probe pr[2];
+    pr[0] = bilinear_stencil(I, gm.lx - ds * gm.nx, gm.ly - ds * gm.ny);   // fluid 1 at -n
+    pr[1] = bilinear_stencil(I, gm.lx + ds * gm.nx, gm.ly + ds * gm.ny);   // fluid 2 at +n
+    bool has_dep = false;
+    for (int p = 0; p < 2; ++p) {
+        r.tx[p] = pr[p].tx;
+        r.ty[p] = pr[p].ty;
+        for (int q = 0; q < 4; ++q) {
+            const int ii = pr[p].i0 + (q >> 1), jj = pr[p].j0 + (q & 1);
+            vof_prims<R>(I, p, ii, jj, r.cv[p][q]);
+            r.dep[p][q] = vof_dep<R, OG>(I, ii, jj, p, lin);
+            has_dep |= r.dep[p][q] >= 0;
+        }
+    }
+    double ip1[4], ip2[4], out[VSIDE], ps;
+    vof_probe_values<false>(I, r, ip1, ip2);
+    const unsigned st = vof_finish<OG, (sizeof(R) == 4)>(I, r, ip1, ip2, -1e300, out, &ps);
+    r.pspec = ps;
+    gmem((VofRec*)I.irec)[k] = r;
+    gmem(I.idone)[k] = has_dep ? 0 : 1;
+    if (!has_dep) {
+        const auto o = gmem(I.side) + (long long)VSIDE * k;
+        for (int q = 0; q < VSIDE; ++q) o[q] = out[q];
+    }
+    flag(I, st);
+}
+
+// Final pass (GFM_VOF.cpp:221-319, OGFM_VOF.cpp:55-93, :171-180, :188-268):
+// the new primitives of every cell, W->U into Q[.][1], the velocity field
+// and, for OGFM_VOF, the update boxes.  A 64x4 tile keeps the list indices of
+// its cells and a three-cell halo in LDS for the inverse-distance fill.
+// work: full-array 64x4 tiles
+template <typename R, bool OG>
+__global__ __launch_bounds__(256) void k_vof_final(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+#pragma clang fp contract(off)
+    constexpr int TW = 64 + 6, TH = 4 + 6;
+    __shared__ int smap[TH][TW];
+    __shared__ int sbb[8];
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const int ti = w.y * 4, tj = w.z * 64;
+    bool any = false;
+    for (int q = threadIdx.x; q < TH * TW; q += 256) {
+        const int a = q / TW, b = q - a * TW;
+        const int ii = ti - 3 + a, jj = tj - 3 + b;
+        int v = -1;
+        if (ii >= 0 && ii < I.H && jj >= 0 && jj < I.W) v = I.imap[cidx(I, ii, jj)];
+        smap[a][b] = v;
+        any |= v >= 0;
+    }
+    if (OG && threadIdx.x < 8) sbb[threadIdx.x] = (threadIdx.x & 1) ? INT_MIN : INT_MAX;
+    any = __syncthreads_or(any);
+    const int li = threadIdx.x >> 6, lj = threadIdx.x & 63;
+    const int i = ti + li, j = tj + lj;
+    unsigned st = 0;
+    if (i < I.H && j < I.W) {
+        const long long c = cidx(I, i, j);
+        const V4<R> U1 = ld4<R>(I.Q[0][0], I, c), U2 = ld4<R>(I.Q[1][0], I, c);
+        const double z = vof_z<R>(I, i, j);
+        // GFM_VOF.cpp:59-66, OGFM_VOF.cpp:64-81: the real fluid's state
+        if (!phys_state(z >= 0.5 ? U1 : U2)) st |= ST_UNPHYS;
+        const int fr = z >= 0.5 ? 0 : 1;
+        double W[2][4];
+        vof_prims<R>(I, 0, i, j, W[0]);
+        vof_prims<R>(I, 1, i, j, W[1]);
+        const int self = smap[li + 3][lj + 3];
+        bool changed = false;
+        if (self >= 0) {
+            const double* s = gmem(I.side) + (long long)VSIDE * self;
+            for (int k = 0; k < 4; ++k) { W[0][k] = s[8 + k]; W[1][k] = s[12 + k]; }
+            changed = true;
+        } else if (any && interior(I, i, j) && (z == 0.0 || z == 1.0)) {
+            // the fluid whose ghost state is set: fluid 1 where z = 0 (flag 1)
+            const int fg = z == 0.0 ? 0 : 1;
+            double sw = 0.0, sp[4] = {0.0, 0.0, 0.0, 0.0};
+            bool near = false;
+            const double cx = I.x0 + (j - I.g + 0.5) * I.dx, cy = I.y0 + (i - I.g + 0.5) * I.dy;
+            for (int xi = -3; xi <= 3; ++xi)
+                for (int yi = -3; yi <= 3; ++yi) {
+                    const int kk = smap[li + 3 + yi][lj + 3 + xi];
+                    if (kk < 0) continue;
+                    near = true;
+                    const double ox = I.x0 + (j + xi - I.g + 0.5) * I.dx, oy = I.y0 + (i + yi - I.g + 0.5) * I.dy;
+                    const double ex = ox - cx, ey = oy - cy;
+                    const double wt = 1.0 / sqrt(ex * ex + ey * ey);
+                    const double* s = gmem(I.side) + (long long)VSIDE * kk + 4 * fg;
+                    sw += wt;
+                    for (int k = 0; k < 4; ++k) sp[k] += wt * s[k];
+                }
+            if (near) {
+                for (int k = 0; k < 4; ++k) sp[k] /= sw;
+                double* G = W[fg];
+                const double* Rr = W[1 - fg];
+                if (OG) {
+                    G[0] = vof_isen<(sizeof(R) == 4)>(I, fg, sp[0], sp[3], Rr[3]);
+                    G[1] = Rr[1]; G[2] = Rr[2]; G[3] = Rr[3];
+                } else {
+                    for (int k = 0; k < 4; ++k) G[k] = sp[k];
+                }
+                changed = true;
+            }
+        }
+        // the real fluid's velocity (GFM_VOF.cpp:306-319; OGFM_VOF.cpp:171-180,
+        // within three cells of the interior only, zero beyond)
+        const auto vel = gmem((R*)I.vel);
+        const bool vin = !OG || vof_range(I, i, j);
+        vel[c] = vin ? (R)W[fr][1] : R(0);
+        vel[c + I.plane] = vin ? (R)W[fr][2] : R(0);
+        // GFM_VOF converts every cell, OGFM_VOF the interior (:263-266); the
+        // ghost frame is overwritten by the BCs that follow either way.  FP32
+        // storage: an unchanged cell goes out as it came in (the W->U would
+        // return it up to an FP32 round trip)
+        const bool convert = (!OG || interior(I, i, j)) && (sizeof(R) == 8 || changed);
+        for (int f = 0; f < 2; ++f) {
+            V4<R> Uo = f == 0 ? U1 : U2;
+            if (convert) {
+                V4<double> Wd;
+                for (int k = 0; k < 4; ++k) Wd.v[k] = W[f][k];
+                const V4<double> Ud = prims2cons<double>(I.gam[f], I.pinf[f], Wd);
+                if (!phys_state(Ud)) st |= ST_UNPHYS;
+                for (int k = 0; k < 4; ++k) Uo.v[k] = (R)Ud.v[k];
+            }
+            st4<R>(I.Q[f][1], I, c, Uo);
+        }
+        if (OG) {
+            // OGFM_VOF.cpp:64-81: each fluid's real cells grown by two
+            int* bb = sbb + 4 * fr;
+            atomicMin(&bb[0], i - 2); atomicMax(&bb[1], i + 2);
+            atomicMin(&bb[2], j - 2); atomicMax(&bb[3], j + 2);
+        }
+    }
+    flag(I, st);
+    if (OG) {
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            const int v = sbb[threadIdx.x];
+            int* dst = I.si + SI_BBOX + threadIdx.x;
+            if (threadIdx.x & 1) { if (v != INT_MIN) atomicMax(dst, v); }
+            else if (v != INT_MAX) atomicMin(dst, v);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2937,6 +3415,26 @@ void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted) {
 }
 
 template <typename R>
+void launch_vof(const LaunchCtx& L, bool ogfm) {
+    const int nsb = (L.n_vseg + SEG_PER_WG - 1) / SEG_PER_WG;
+    hipLaunchKernelGGL((k_vof_count<R>), dim3(nsb), dim3(256), 0, L.stream, L.insts, L.w_vseg, L.n_vseg);
+    hipLaunchKernelGGL(k_iface_scan, GRID(L.n_inst), dim3(1024), 0, L.stream, L.insts, L.w_inst);
+    hipLaunchKernelGGL((k_vof_mark<R>), dim3(nsb), dim3(256), 0, L.stream, L.insts, L.w_vseg, L.n_vseg);
+    if (ogfm) {
+        hipLaunchKernelGGL((k_vof_prep<R, true>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich);
+        hipLaunchKernelGGL((k_iface_rounds<R, V_OGFM_VOF>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts,
+                           L.w_inst, L.iface_plain);
+        hipLaunchKernelGGL((k_vof_final<R, true>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full);
+    } else {
+        hipLaunchKernelGGL((k_vof_prep<R, false>), GRID(L.n_ich), dim3(256), 0, L.stream, L.insts, L.w_ich);
+        hipLaunchKernelGGL((k_iface_rounds<R, V_VOF>), GRID(L.n_inst), dim3(512), 0, L.stream, L.insts,
+                           L.w_inst, L.iface_plain);
+        hipLaunchKernelGGL((k_vof_final<R, false>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full);
+    }
+}
+
+
+template <typename R>
 void launch_mgfm_final(const LaunchCtx& L, int qin, int qout, bool prims_out) {
     if (prims_out)
         hipLaunchKernelGGL((k_mgfm_final<R, true>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, qin, qout);
@@ -3044,19 +3542,6 @@ void launch_pure_rs(hipStream_t s, int n, int px, double g, double pinf, const d
 // Written to phiA.  work: {inst, tile_y, tile_x, 0}, 64x4 tiles over the
 // padded array
 // ---------------------------------------------------------------------------
-__host__ __device__ inline double area_fraction(double phi, double nx, double ny, double dx, double dy) {
-#pragma clang fp contract(off)
-    double a = fabs(nx) * dx, b = fabs(ny) * dy;
-    if (a > b) { const double s = a; a = b; b = s; }
-    const double m = -phi + 0.5 * (a + b);
-    if (m <= 0.0) return 0.0;
-    if (m >= a + b) return 1.0;
-    if (m < a) return m * m / (2.0 * a * b);
-    if (m < b) return (2.0 * m - a) / (2.0 * b);
-    const double r = a + b - m;
-    return 1.0 - r * r / (2.0 * a * b);
-}
-
 template <typename R>
 __global__ __launch_bounds__(256) void k_zfrac(const Inst* __restrict__ insts, const int4* __restrict__ work) {
     const int4 w = work[blockIdx.x];
@@ -3141,6 +3626,7 @@ void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, doub
     template void launch_ls<R>(const LaunchCtx&, const Houc&, int, int);      \
     template void launch_reinit_now<R>(const LaunchCtx&, int);                \
     template void launch_bc<R>(const LaunchCtx&, int, int, int);               \
+    template void launch_vof<R>(const LaunchCtx&, bool);                       \
     template void launch_reset<R>(const LaunchCtx&, int);                      \
     template void launch_ic<R>(const LaunchCtx&, const IcDesc*, const int*, int, int); \
     template void launch_zfrac<R>(const LaunchCtx&);

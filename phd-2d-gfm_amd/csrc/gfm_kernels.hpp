@@ -19,6 +19,8 @@ struct LaunchCtx {
     const int4* w_tb;    int n_tb;      // TB_SHAPES interior tiles (temporally blocked extrapolation)
     const int4* w_rt;    int n_rt;      // 64x32 interior tiles (temporally blocked reinitialisation)
     const int4* w_seg;   int n_seg;     // 256-cell interior row segments (dt, interfacial list)
+    const int4* w_vseg;  int n_vseg;    // 256-cell row segments of the interior grown by three cells
+                                        // (the VOF GFMs' mixed-cell list; .w = segment index)
     int tb;                             // temporally blocked kernels (no x-periodic instance)
     int mirror_y, periodic_y;           // some instance of the group has mirror / periodic y BCs
     int iface_plain;                    // interfacial solve by plain rounds instead of levels
@@ -31,6 +33,8 @@ struct LaunchCtx {
 
 struct Houc;
 constexpr int IFREC_BYTES = 384;   // >= sizeof(IfRec), checked in the kernels
+constexpr int VOFREC_BYTES = 448;  // >= sizeof(VofRec): a mixed cell's record (VOF GFMs)
+constexpr int VSIDE = 16;          // side-buffer doubles per mixed cell (VOF GFMs)
 // flux-update tile shapes {x, y} (one cell per thread); a context uses one,
 // FS_SHAPES[LaunchCtx::fs_shape], chosen at commit (GFM_FS_TILE)
 constexpr int FS_NSHAPE = 4;
@@ -53,6 +57,9 @@ template <typename R> bool launch_extrap_tb(const LaunchCtx& L, int nit, int min
                                             int band = 0);
 template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted);
 template <typename R> void launch_mgfm_final(const LaunchCtx& L, int qin, int qout, bool prims_out = false);
+// the ghost fill of GFM_VOF (ogfm false) or OGFM_VOF (true): Q[.][0] -> Q[.][1] and the velocity
+// field (OGFM_VOF also narrows the update boxes); the BCs of the result are the caller's
+template <typename R> void launch_vof(const LaunchCtx& L, bool ogfm);
 // the extension velocity field of a ghost fill (use_extension_vfield): the
 // interfacial cells' velocity carried into the rest of the grid by 20
 // iterations, written to the velocity field

@@ -17,11 +17,12 @@ LIB_PATH = os.environ.get("GFM_HIP_LIB", os.path.join(HERE, "libgfm_hip.so"))
 BC_TRANSMISSIVE, BC_REFLECTIVE, BC_PERIODIC = 0, 1, 2
 SIM_ONEFLUID, SIM_TWOFLUID = 0, 1
 FS_MUSCL, FS_GODUNOV = 0, 1
-GFM_ORIGINAL, GFM_REAL, GFM_RIEMANN, GFM_MODIFIED = 0, 1, 2, 3
+GFM_ORIGINAL, GFM_REAL, GFM_RIEMANN, GFM_MODIFIED, GFM_VOFGFM, GFM_OGFMVOF = 0, 1, 2, 3, 4, 5
 FP64, MIXED = 0, 1
 ST_UNPHYSICAL, ST_RS_FAIL, ST_IFACE_OVERFLOW, ST_IFACE_ROUNDS, ST_IFACE_LIST = 1, 2, 4, 8, 16
 
-GFM_NAMES = {"OGFM": GFM_ORIGINAL, "realGFM": GFM_REAL, "riemannGFM": GFM_RIEMANN, "MGFM": GFM_MODIFIED}
+GFM_NAMES = {"OGFM": GFM_ORIGINAL, "realGFM": GFM_REAL, "riemannGFM": GFM_RIEMANN, "MGFM": GFM_MODIFIED,
+             "VOFGFM": GFM_VOFGFM, "OGFMVOF": GFM_OGFMVOF}
 
 # every entry point declared in include/gfm_hip.h
 EXPORTS = [

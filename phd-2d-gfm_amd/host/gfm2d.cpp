@@ -466,7 +466,9 @@ static int gfm_id(const std::string& g) {
     if (g == "realGFM") return GFM_REAL;
     if (g == "riemannGFM") return GFM_RIEMANN;
     if (g == "MGFM") return GFM_MODIFIED;
-    throw std::runtime_error("Invalid GFM choice (OGFM, realGFM, riemannGFM, MGFM)");
+    if (g == "VOFGFM") return GFM_VOFGFM;
+    if (g == "OGFMVOF") return GFM_OGFMVOF;
+    throw std::runtime_error("Invalid GFM choice (OGFM, realGFM, riemannGFM, MGFM, VOFGFM, OGFMVOF)");
 }
 
 // sim_twofluid::run_sim (sim_twofluid.cpp:27-103) with the whole loop on the device

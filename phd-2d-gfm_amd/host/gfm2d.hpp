@@ -194,6 +194,10 @@ GFM2D_VARIANT(GFM_original, GFM_ORIGINAL)
 GFM2D_VARIANT(GFM_real, GFM_REAL)
 GFM2D_VARIANT(GFM_riemann, GFM_RIEMANN)
 GFM2D_VARIANT(GFM_modified, GFM_MODIFIED)
+// GFM_VOF.hpp / OGFM_VOF.hpp: the VOF-based fills, on the volume fraction, mixed-cell normal and
+// interface location of the level set's PLIC reconstruction (the tracker `ls` is a Levelset here)
+GFM2D_VARIANT(GFM_VOF, ::GFM_VOFGFM)
+GFM2D_VARIANT(OGFM_VOF, ::GFM_OGFMVOF)
 #undef GFM2D_VARIANT
 
 // extrapolate_vector.hpp:8-15 (primitive grids)

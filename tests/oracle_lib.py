@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
 BC_T, BC_R, BC_P = 0, 1, 2
-GFM_ORIGINAL, GFM_REAL, GFM_RIEMANN, GFM_MODIFIED = 0, 1, 2, 3
+GFM_ORIGINAL, GFM_REAL, GFM_RIEMANN, GFM_MODIFIED, GFM_VOF, GFM_OGFM_VOF = 0, 1, 2, 3, 4, 5
 FS_MUSCL, FS_GODUNOV = 0, 1
 
 
@@ -107,6 +107,7 @@ def _bind(L):
         "or_bilinear": (None, [C.POINTER(Grid), D, dbl, dbl, D]),
         "or_tracers_step": (None, [C.POINTER(Grid), D, i32, D, D, i32, dbl, dbl, D]),
         "or_volume_fraction": (None, [C.POINTER(Grid), D, D]),
+        "or_vof_interface": (None, [C.POINTER(Grid), D, D, D, D]),
         "or_houc_coeffs": (None, [i32, D, I]),
         "or_ls_advect": (None, [C.POINTER(Grid), i32, D, D, dbl]),
         "or_ls_reinit": (None, [C.POINTER(Grid), D, i32]),

@@ -138,7 +138,8 @@ def test_gda_godunov_matches_exact_discrete_solution(oracle):
     assert np.abs(p - 1e-4).max() <= 1e-9
 
 
-def tube_run(case, N, variant, lib=None, cfl=0.5):
+def tube_run(case, N, variant, lib=None, cfl=0.5, shift=0.0):
+    """shift: moves the initial interface by that distance"""
     g = 8
     if case == "ST1_x":
         G = O.Grid(N, 4, g, 0.0, 0.0, 1.0 / N, 0.25, 0, 0, 0, 0)
@@ -148,6 +149,7 @@ def tube_run(case, N, variant, lib=None, cfl=0.5):
         G = O.Grid(4, N, g, -2.0, -2.0, 1.0, 4.0 / N, 0, 0, 0, 0)
         E = O.Eos(4.4, 6e8, 1.4, 0.0)
         W1, W2, T, axis, pos = [1000.0, 0, 0, 1e9], [50.0, 0, 0, 1e5], 0.0009, 1, 0.7
+    pos += shift
     L = O.lib(lib)
     H, W = G.shape
     g1, g2, phi = np.zeros((H, W, 4)), np.zeros((H, W, 4)), np.zeros((H, W))
