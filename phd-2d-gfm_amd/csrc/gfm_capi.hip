@@ -726,8 +726,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.cfl = h.cfl; d.T = h.T; d.max_steps = h.max_steps;
         d.use_bbox = 1;   // boxes are reset to the full domain each step unless OGFM narrows them
         d.use_slip = c->use_slip_BCs ? 1 : 0;
-        // the ghost band is the extrapolating fills' (the VOF fills update only cells near the interface)
-        d.band = c->mixed() && c->twofluid() && !c->vof() ? c->ghost_band * std::max(s.dx, s.dy) : 0.0;
+        d.band = c->mixed() && c->twofluid() ? c->ghost_band * std::max(s.dx, s.dy) : 0.0;
         d.ntile_dt = s.Ny * ((s.Nx + 255) / 256);   // dt partials: one per 256-cell row segment
         // the VOF GFMs list the mixed cells of the interior grown by three cells
         const int lnx = c->vof() ? s.Nx + 6 : s.Nx, lny = c->vof() ? s.Ny + 6 : s.Ny;

@@ -1987,11 +1987,15 @@ __device__ __forceinline__ bool vof_range(const Inst& I, int i, int j) {
 }
 
 // z of a cell; the outermost ring has no centred gradient and takes 1 or 0 by
-// phi's sign
+// phi's sign.  With |phi| >= (dx + dy)/2 >= (a + b)/2 the line misses the
+// cell and area_fraction returns exactly 0 or 1 (m <= 0 or m >= a + b in
+// floating point too: a <= |nx| dx and b <= |ny| dy round monotonically), so
+// the normal is computed only nearer the interface.
 template <typename R>
 __device__ __forceinline__ double vof_z(const Inst& I, int i, int j) {
     const double ph = (double)ld1<R>(I.phi, cidx(I, i, j));
     if (i == 0 || j == 0 || i == I.H - 1 || j == I.W - 1) return ph <= 0.0 ? 1.0 : 0.0;
+    if (fabs(ph) >= 0.5 * (I.dx + I.dy)) return ph <= 0.0 ? 1.0 : 0.0;
     double nx, ny;
     unit_normal<R>(I, i, j, nx, ny);
     return area_fraction(ph, nx, ny, I.dx, I.dy);
@@ -2032,6 +2036,15 @@ __device__ __forceinline__ VofGeom vof_geom(const Inst& I, int i, int j) {
     gm.lx = cx + (fx - sm * gm.ny);
     gm.ly = cy + (fy + sm * gm.nx);
     return gm;
+}
+
+// FP64 primitives of fluid f from its conserved state U
+template <typename R>
+__device__ __forceinline__ void vof_prims_of(const Inst& I, int f, const V4<R>& U, double* W) {
+    V4<double> Ud;
+    for (int k = 0; k < 4; ++k) Ud.v[k] = (double)U.v[k];
+    const V4<double> P = cons2prims<double>(I.gam[f], I.pinf[f], Ud);
+    for (int k = 0; k < 4; ++k) W[k] = P.v[k];
 }
 
 // FP64 primitives of fluid f at a cell, from the stored conserved state
@@ -2306,20 +2319,19 @@ __global__ __launch_bounds__(256) void k_vof_final(const Inst* __restrict__ inst
     unsigned st = 0;
     if (i < I.H && j < I.W) {
         const long long c = cidx(I, i, j);
-        const V4<R> U1 = ld4<R>(I.Q[0][0], I, c), U2 = ld4<R>(I.Q[1][0], I, c);
+        const V4<R> U[2] = {ld4<R>(I.Q[0][0], I, c), ld4<R>(I.Q[1][0], I, c)};
         const double z = vof_z<R>(I, i, j);
         // GFM_VOF.cpp:59-66, OGFM_VOF.cpp:64-81: the real fluid's state
-        if (!phys_state(z >= 0.5 ? U1 : U2)) st |= ST_UNPHYS;
         const int fr = z >= 0.5 ? 0 : 1;
+        if (!phys_state(U[fr])) st |= ST_UNPHYS;
+        // W[f]: fluid f's new primitives where changed[f], else converted on demand
         double W[2][4];
-        vof_prims<R>(I, 0, i, j, W[0]);
-        vof_prims<R>(I, 1, i, j, W[1]);
+        bool have[2] = {false, false}, changed[2] = {false, false};
         const int self = smap[li + 3][lj + 3];
-        bool changed = false;
         if (self >= 0) {
             const double* s = gmem(I.side) + (long long)VSIDE * self;
             for (int k = 0; k < 4; ++k) { W[0][k] = s[8 + k]; W[1][k] = s[12 + k]; }
-            changed = true;
+            have[0] = have[1] = changed[0] = changed[1] = true;
         } else if (any && interior(I, i, j) && (z == 0.0 || z == 1.0)) {
             // the fluid whose ghost state is set: fluid 1 where z = 0 (flag 1)
             const int fg = z == 0.0 ? 0 : 1;
@@ -2341,29 +2353,30 @@ __global__ __launch_bounds__(256) void k_vof_final(const Inst* __restrict__ inst
             if (near) {
                 for (int k = 0; k < 4; ++k) sp[k] /= sw;
                 double* G = W[fg];
-                const double* Rr = W[1 - fg];
                 if (OG) {
-                    G[0] = vof_isen<(sizeof(R) == 4)>(I, fg, sp[0], sp[3], Rr[3]);
-                    G[1] = Rr[1]; G[2] = Rr[2]; G[3] = Rr[3];
+                    const int fo = 1 - fg;
+                    vof_prims_of<R>(I, fo, U[fo], W[fo]);
+                    have[fo] = true;
+                    G[0] = vof_isen<(sizeof(R) == 4)>(I, fg, sp[0], sp[3], W[fo][3]);
+                    G[1] = W[fo][1]; G[2] = W[fo][2]; G[3] = W[fo][3];
                 } else {
                     for (int k = 0; k < 4; ++k) G[k] = sp[k];
                 }
-                changed = true;
+                have[fg] = changed[fg] = true;
             }
         }
-        // the real fluid's velocity (GFM_VOF.cpp:306-319; OGFM_VOF.cpp:171-180,
-        // within three cells of the interior only, zero beyond)
-        const auto vel = gmem((R*)I.vel);
-        const bool vin = !OG || vof_range(I, i, j);
-        vel[c] = vin ? (R)W[fr][1] : R(0);
-        vel[c + I.plane] = vin ? (R)W[fr][2] : R(0);
         // GFM_VOF converts every cell, OGFM_VOF the interior (:263-266); the
         // ghost frame is overwritten by the BCs that follow either way.  FP32
-        // storage: an unchanged cell goes out as it came in (the W->U would
+        // storage: an unchanged state goes out as it came in (the W->U would
         // return it up to an FP32 round trip)
-        const bool convert = (!OG || interior(I, i, j)) && (sizeof(R) == 8 || changed);
+        const bool conv_cell = !OG || interior(I, i, j);
         for (int f = 0; f < 2; ++f) {
-            V4<R> Uo = f == 0 ? U1 : U2;
+            const bool convert = conv_cell && (sizeof(R) == 8 || changed[f]);
+            if ((convert || f == fr) && !have[f]) {
+                vof_prims_of<R>(I, f, U[f], W[f]);
+                have[f] = true;
+            }
+            V4<R> Uo = U[f];
             if (convert) {
                 V4<double> Wd;
                 for (int k = 0; k < 4; ++k) Wd.v[k] = W[f][k];
@@ -2373,6 +2386,12 @@ __global__ __launch_bounds__(256) void k_vof_final(const Inst* __restrict__ inst
             }
             st4<R>(I.Q[f][1], I, c, Uo);
         }
+        // the real fluid's velocity (GFM_VOF.cpp:306-319; OGFM_VOF.cpp:171-180,
+        // within three cells of the interior only, zero beyond)
+        const auto vel = gmem((R*)I.vel);
+        const bool vin = !OG || vof_range(I, i, j);
+        vel[c] = vin ? (R)W[fr][1] : R(0);
+        vel[c + I.plane] = vin ? (R)W[fr][2] : R(0);
         if (OG) {
             // OGFM_VOF.cpp:64-81: each fluid's real cells grown by two
             int* bb = sbb + 4 * fr;
