@@ -192,9 +192,9 @@ int extrap_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int N, bool from_q, int qi
 // GFM_base::set_ghost_states for the context's variant: Q[.][0] -> Q[.][1].
 // `mark`, if given, is recorded on L's stream once the interfacial solve (or,
 // for GFM_original, the whole chain) is launched.  `band`: the fill of a
-// whole mixed-precision step, whose blocked extrapolation passes the tiles
-// beyond the ghost band through (the flux kernel does not update their
-// ghost states)
+// whole mixed-precision step, whose blocked extrapolation skips the tiles
+// beyond the ghost band in its first chain and passes them through in its
+// last (the flux kernel does not update their ghost states)
 template <typename R>
 int ghost_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int counted = 0, hipEvent_t mark = nullptr, int band = 0) {
     switch (c->m.gfm) {
@@ -216,11 +216,7 @@ int ghost_chain(gfm_ctx* c, const gfm::LaunchCtx& L, int counted = 0, hipEvent_t
             break;
         }
         case GFM_MODIFIED: { // GFM_modified.cpp:37-239
-            // the two iterations feed only the probes of the interfacial
-            // cells (the write-back of :230-239 takes the unextrapolated
-            // states), so with the band, tiles beyond it write nothing
-            const int b1 = (band && !c->mgfm_extrapolate) ? 2 : band;
-            const int p = extrap_chain<R>(c, L, 2, true, 0, 0, false, false, 0, 0, 0, b1);
+            const int p = extrap_chain<R>(c, L, 2, true, 0, 0, false, false, 0, 0, 0, band);
             gfm::launch_iface<R>(L, gfm::V_MODIFIED, p, counted);
             if (mark) HC(hipEventRecord(mark, L.stream));
             if (c->mgfm_extrapolate) {

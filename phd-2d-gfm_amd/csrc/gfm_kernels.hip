@@ -1055,6 +1055,33 @@ __device__ __forceinline__ void tb_put_owned(const Inst& I, int i, int j, int bo
     }
 }
 
+// Whole FP32 steps with a ghost band: a chain that starts from conserved
+// states writes its primitives only on tiles with a cell nearer the
+// interface than Inst::band, so the chain after it reads the fill's input
+// Q[.][0] at the cells farther than that.  A ghost-frame cell stands for its
+// BC source cell, whose input is read with the BC's sign change (c: the
+// source's index; fu, fv: the velocity component the BC negates).  True when
+// the cell is such a cell.
+template <typename R>
+__device__ __forceinline__ bool far_src(const Inst& I, int i, int j, R bnd, long long& c, bool& fu, bool& fv) {
+    const int si = src_row(I, i, fv), sj = src_col(I, j, fu);
+    c = cidx(I, si, sj);
+    return fabs(ld1<R>(I.phi, c)) >= bnd;
+}
+
+// primitives of both fluids from the fill's input at source cell c
+template <typename R>
+__device__ __forceinline__ void far_prims(const Inst& I, long long c, bool fu, bool fv, V4<R>& W1, V4<R>& W2) {
+    W1 = ld4<R>(I.Q[0][0], I, c);
+    W2 = ld4<R>(I.Q[1][0], I, c);
+    if constexpr (sizeof(R) == 4) {
+        W1 = cons2prims_f(fluid_f(I.rsf[0]), W1);
+        W2 = cons2prims_f(fluid_f(I.rsf[1]), W2);
+    }
+    if (fu) { W1.v[1] = -W1.v[1]; W2.v[1] = -W2.v[1]; }
+    if (fv) { W1.v[2] = -W1.v[2]; W2.v[2] = -W2.v[2]; }
+}
+
 template <typename R, int NIT, int MIN, int MOUT, bool PY, int SH>
 __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __restrict__ insts,
                                                                 const int4* __restrict__ work, int bin, int bout,
@@ -1079,13 +1106,18 @@ __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __re
     const void* in0 = (MIN & MF_FIRST) ? I.Q[0][bin] : I.P[0][bin];
     const void* in1 = (MIN & MF_FIRST) ? I.Q[1][bin] : I.P[1][bin];
     const bool rfb = I.bc[2] == BC_R, rft = I.bc[3] == BC_R, rfx = I.bc[0] == BC_R;
+    // FP32 whole steps with a ghost band (band != 0): a tile whose cells are
+    // all farther than I.band (phi) from the interface holds real states the
+    // fill does not change and ghost states the band leaves alone.  The first
+    // chain of a fill writes nothing there (the next chain reads the fill's
+    // input at such cells instead); the last chain passes the fill's input
+    // through, the real fluid converted exactly as below and the ghost fluid
+    // as it came in.  A chain that starts from primitives reads the fill's
+    // input wherever |phi| >= I.band (sub_in).
+    const bool subst = sizeof(R) == 4 && !(MOUT & MF_OGFM) && band && I.band > 0.0;
+    const R bnd = (R)I.band;
     if constexpr (sizeof(R) == 4 && !(MOUT & MF_OGFM)) {
-        // FP32 whole steps with a ghost band: a tile whose cells are all
-        // farther than I.band (phi) from the interface holds only ghost states
-        // the band leaves alone; its cells pass through with the real fluid
-        // converted exactly as below and the ghost fluid as it came in
-        if (band && I.band > 0.0) {
-            const R bnd = (R)I.band;
+        if (subst) {
             const auto ph = gmem((const R*)I.phi);
             bool near = false;
             for (int q = threadIdx.x; q < TBX * TBY; q += TB_THREADS) {
@@ -1093,26 +1125,15 @@ __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __re
                 if (interior(I, i, j)) near = near || fabs(ph[cidx(I, i, j)]) < bnd;
             }
             if (!__syncthreads_or(near)) {
-                if (band == 2) return;   // nothing reads this chain's output there
+                if (!(MOUT & MF_LAST)) return;
                 bool bad = false;
                 for (int q = threadIdx.x; q < TBX * TBY; q += TB_THREADS) {
                     const int i = g + w.y * TBY + q / TBX, j = g + w.z * TBX + q % TBX;
                     if (!interior(I, i, j)) continue;
                     const long long c = cidx(I, i, j);
-                    V4<R> W1 = ld4<R>(in0, I, c), W2 = ld4<R>(in1, I, c);
-                    if (LM & MF_FIRST) {
-                        W1 = cons2prims_f(fluid_f(I.rsf[0]), W1);
-                        W2 = cons2prims_f(fluid_f(I.rsf[1]), W2);
-                    }
-                    const int m = (LM & MF_INJECT) ? gmem(I.imap)[c] : -1;
+                    V4<R> W1, W2;
+                    far_prims<R>(I, c, false, false, W1, W2);
                     const int fr = ph[c] <= R(0) ? 0 : 1;
-                    if (m >= 0) {
-                        const auto sd = gmem(I.side) + 8 * (long long)m;
-                        V4<R> Ws;
-#pragma unroll
-                        for (int qq = 0; qq < 4; ++qq) Ws.v[qq] = (R)sd[qq];
-                        if (fr == 0) W1 = Ws; else W2 = Ws;
-                    }
                     const V4<R>& Wr = fr == 0 ? W1 : W2;
                     const V4<R>& Wg = fr == 0 ? W2 : W1;
                     if (!(Wg.v[0] >= R(0))) bad = true;
@@ -1124,6 +1145,7 @@ __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __re
             }
         }
     }
+    const bool sub_in = subst && !(LM & MF_FIRST);
     const ExK<R> K = ex_k<R>(I);
     // per cell: the update weights, the ghost fluid's current primitives
     // (only this thread changes them), the LDS cells of the ghost fluid at
@@ -1143,6 +1165,8 @@ __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __re
     V4<R> L1[SLOTS], L2[SLOTS];
     R pc[SLOTS], pe[SLOTS], pw[SLOTS], pn[SLOTS], ps[SLOTS];
     int im[SLOTS];
+    unsigned sbits = 0;   // bit k: slot k's primitives come from the fill's input (sub_in)
+    static_assert(SLOTS <= 32, "one bit per slot");
     const auto phig = gmem((const R*)I.phi);
     if constexpr (PY) {
         // the stored ghost rows g-1 and g+Ny as the first iteration reads them
@@ -1150,7 +1174,11 @@ __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __re
             const int side = q / RX, b = q - side * RX, j = rj + b;
             const int r = side ? g + I.Ny : g - 1;
             V4<R> W1 = {{R(0), R(0), R(0), R(0)}}, W2 = W1;
-            if (j >= 0 && j < I.W) {
+            long long cs = 0;
+            bool su = false, sv = false;
+            if (j >= 0 && j < I.W && sub_in && far_src<R>(I, r, j, bnd, cs, su, sv)) {
+                far_prims<R>(I, cs, su, sv, W1, W2);
+            } else if (j >= 0 && j < I.W) {
                 const long long c = cidx(I, r, j);
                 W1 = ld4<R>(in0, I, c);
                 W2 = ld4<R>(in1, I, c);
@@ -1190,8 +1218,15 @@ __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __re
         pc[k] = pe[k] = pw[k] = pn[k] = ps[k] = R(0);
         im[k] = -1;
         if (valid) {
-            L1[k] = ld4<R>(in0, I, c);
-            L2[k] = ld4<R>(in1, I, c);
+            long long cs = 0;
+            bool su = false, sv = false;
+            if (sub_in && far_src<R>(I, is, j, bnd, cs, su, sv)) {
+                far_prims<R>(I, cs, su, sv, L1[k], L2[k]);
+                sbits |= 1u << k;
+            } else {
+                L1[k] = ld4<R>(in0, I, c);
+                L2[k] = ld4<R>(in1, I, c);
+            }
             if (LM & MF_INJECT) im[k] = gmem(I.imap)[c];
             if ((LM & MF_INJECT) || inner) pc[k] = phig[c];
         }
@@ -1214,8 +1249,9 @@ __global__ __launch_bounds__(TB_SHAPES[SH][2]) void k_extrap_tb(const Inst* __re
             V4<R> W1 = {{R(0), R(0), R(0), R(0)}}, W2 = W1;
             if (is >= 0 && is < I.H && j >= 0 && j < I.W) {
                 // ex_load<R, LM> of both fluids on the loaded values
+                // (sub_in slots hold primitives already)
                 W1 = L1[k]; W2 = L2[k];
-                if (LM & MF_FIRST) {
+                if ((LM & MF_FIRST) && !((sbits >> k) & 1u)) {
                     if constexpr (sizeof(R) == 4) {
                         W1 = cons2prims_f(fluid_f(I.rsf[0]), L1[k]);
                         W2 = cons2prims_f(fluid_f(I.rsf[1]), L2[k]);

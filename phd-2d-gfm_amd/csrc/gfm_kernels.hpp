@@ -50,9 +50,10 @@ template <typename R> void launch_dt(const LaunchCtx& L, int twofluid, int q, in
 template <typename R> void launch_fs(const LaunchCtx& L, bool muscl, int qin, int qout, int copy_inactive,
                                      int qorig = -1);
 template <typename R> void launch_extrap(const LaunchCtx& L, int mode, int bin, int bout);
-// band: a whole FP32 step's fill, in which tiles farther than Inst::band from
-// the interface pass their states through without extrapolating (1), or
-// write nothing, where nothing reads the chain's output there (2)
+// band: a whole FP32 step's fill (input Q[.][0]): on tiles farther than
+// Inst::band from the interface the first chain writes nothing and the last
+// passes the fill's input through; a chain that starts from primitives reads
+// the fill's input at cells farther than that
 template <typename R> bool launch_extrap_tb(const LaunchCtx& L, int nit, int min, int mout, int bin, int bout,
                                             int band = 0);
 template <typename R> void launch_iface(const LaunchCtx& L, int variant, int pbuf, int counted);
