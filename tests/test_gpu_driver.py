@@ -118,6 +118,33 @@ def test_gfm2d_helium_bubble(tmp_path, oracle):
     assert abs(mass[-1, 0] - t[0]) <= 1e-5 * t[0]
 
 
+@pytest.mark.parametrize("gfm,variant", [("VOFGFM", 4), ("OGFMVOF", 5)])
+def test_gfm2d_vof_helium_bubble(tmp_path, oracle, gfm, variant):
+    """the settings names of the VOF-based GFMs (sim_twofluid.cpp:538-545) on the nondimensional
+    shocked_helium_bubble at 130x36, 40 steps, against the oracle's GFM_VOF / OGFM_VOF"""
+    f = settings(tmp_path, Nx=130, Ny=36, CFL=0.4, sim_type="twofluid", FS="MUSCL",
+                 test_case="shocked_helium_bubble", ITM="LS5", GFM=gfm, steps=40)
+    out = subprocess.run([GFM2D, f], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0 and "status 0" in out.stdout
+    files = output_steps(tmp_path)
+    got = read_dat(os.path.join(tmp_path, files[40]))
+    G = oracle.Grid(130, 36, 8, 0.0, 0.0, 325.0 / 130, 89.0 / 36, 0, 0, 1, 1)
+    E = oracle.Eos(1.667, 0.0, 1.4, 0.0)
+    H, W = 52, 146
+    g1, g2, phi = np.zeros((H, W, 4)), np.zeros((H, W, 4)), np.zeros((H, W))
+    oracle.lib().or_ic_bubble(G, E, 175.0, 44.5, 25.0, oracle._p(oracle.arr(0.138, 0, 0, 1)),
+                              oracle._p(oracle.arr(1, 0, 0, 1)), oracle._p(oracle.arr(1.3764, -0.394, 0, 1.5698)),
+                              225.0, oracle._p(g1), oracle._p(g2), oracle._p(phi))
+    run = oracle.Run(variant, 0, 5, 5, 10, 0.4, 280.0, 40)
+    s = np.zeros(1, dtype=np.int32)
+    t = np.zeros(1)
+    assert oracle.lib().or_twofluid_run(G, E, run, oracle._p(g1), oracle._p(g2), oracle._p(phi), oracle._p(s),
+                                        oracle._p(t)) == 0
+    rho = np.where(phi < 0, g1[..., 0], g2[..., 0])[2:H - 2, 2:W - 2].reshape(-1)
+    assert np.allclose(got[:, 2], rho, rtol=2e-5, atol=0)
+
+
 def test_gfm2d_helium_bubble_thesis_ls1(tmp_path):
     """The thesis result for the nondimensional shocked helium bubble at 325x89 (OGFM + LS1, MUSCL,
     CFL 0.4, t = 280): 3083 steps and a relative bubble-mass error of -0.208 at the end (decoded from
