@@ -238,12 +238,24 @@ __global__ __launch_bounds__(256) void k_dt_final(const Inst* __restrict__ insts
     const bool active = (t < I.T) && (nstep < I.max_steps) && !held;
     __shared__ double red[4][256];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (int k = threadIdx.x; k < I.ntile_dt; k += 256) {
-        const double* pp = I.partials + 4 * (long long)k;
-        a0 = fmax(a0, pp[0]);
-        a1 = fmax(a1, pp[1]);
-        a2 += pp[2];
-        a3 += pp[3];
+    // eight partials' loads in flight at once; accumulated in the same order
+    constexpr int U = 8;
+    for (int k0 = threadIdx.x; k0 < I.ntile_dt; k0 += 256 * U) {
+        double v[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + 256 * u;
+            const double* pp = I.partials + 4 * (long long)k;
+            for (int q = 0; q < 4; ++q) v[u][q] = k < I.ntile_dt ? pp[q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k0 + 256 * u >= I.ntile_dt) break;
+            a0 = fmax(a0, v[u][0]);
+            a1 = fmax(a1, v[u][1]);
+            a2 += v[u][2];
+            a3 += v[u][3];
+        }
     }
     red[0][threadIdx.x] = a0; red[1][threadIdx.x] = a1; red[2][threadIdx.x] = a2; red[3][threadIdx.x] = a3;
     __syncthreads();
@@ -1419,31 +1431,63 @@ __global__ __launch_bounds__(256) void k_iface_count(const Inst* __restrict__ in
 // one workgroup per instance: exclusive scan of the segment counts in place;
 // the total is the number of interfacial cells
 __global__ __launch_bounds__(1024) void k_iface_scan(const Inst* __restrict__ insts, const int4* __restrict__ work) {
-    __shared__ int part[1024];
+    __shared__ int wsum[16];   // inclusive prefix of the wavefront totals
     const Inst& I = insts[work[blockIdx.x].x];
     if (!gmem(I.si)[SI_ACTIVE]) return;
-    const int n = I.nseg, tid = threadIdx.x;
+    const int n = I.nseg, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int per = (n + 1023) / 1024;
     const int b0 = min(n, tid * per), b1 = min(n, b0 + per);
+    // the thread's chunk of counts: in registers (all loads in flight at
+    // once) when it has at most PC of them
+    constexpr int PC = 32;
+    int vals[PC];
     int sum = 0;
-    for (int q = b0; q < b1; ++q) sum += I.iseg[q];
-    part[tid] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {   // inclusive Hillis-Steele scan
-        const int v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    if (per <= PC) {
+#pragma unroll
+        for (int u = 0; u < PC; ++u) vals[u] = b0 + u < b1 ? I.iseg[b0 + u] : 0;
+#pragma unroll
+        for (int u = 0; u < PC; ++u) sum += vals[u];
+    } else {
+        for (int q = b0; q < b1; ++q) sum += I.iseg[q];
     }
-    int acc = part[tid] - sum;
-    for (int q = b0; q < b1; ++q) {
-        const int c = I.iseg[q];
-        I.iseg[q] = acc;
-        acc += c;
+    // inclusive scan of the chunk sums: within each wavefront by shuffles,
+    // then over the sixteen wavefront totals
+    int x = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        int t = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const int y = __shfl_up(t, d);
+            if (lane >= d) t += y;
+        }
+        if (lane < 16) wsum[lane] = t;
+    }
+    __syncthreads();
+    int acc = x - sum + (wid > 0 ? wsum[wid - 1] : 0);
+    if (per <= PC) {
+#pragma unroll
+        for (int u = 0; u < PC; ++u) {
+            if (b0 + u < b1) I.iseg[b0 + u] = acc;
+            acc += vals[u];
+        }
+    } else {
+        for (int q = b0; q < b1; ++q) {
+            const int c = I.iseg[q];
+            I.iseg[q] = acc;
+            acc += c;
+        }
     }
     if (tid == 1023) {
-        gmem(I.si)[SI_NIFACE] = part[1023];
-        gmem(I.si)[SI_NLIST] = min(part[1023], I.icap);   // the entries k_iface_mark records
+        const int total = wsum[15];
+        gmem(I.si)[SI_NIFACE] = total;
+        gmem(I.si)[SI_NLIST] = min(total, I.icap);   // the entries k_iface_mark records
     }
 }
 
