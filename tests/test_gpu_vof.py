@@ -130,6 +130,36 @@ def test_vof_steps(oracle, variant):
 
 
 @pytest.mark.parametrize("variant", VOF)
+def test_vof_steps_periodic_y(oracle, variant):
+    """single-mode Richtmyer-Meshkov instance, transmissive in x and periodic in y: the mixed cells of
+    the three ghost rows beyond each periodic side are solved too (GFM_VOF.cpp:73-75), 40 steps"""
+    print("CASE vof steps periodic variant=%d" % variant, flush=True)
+    nsteps = 40
+    info, eos, (g1, g2, phi) = P.rmi(128, 32)
+    ctx, k = ctx_of(info, eos, variant)
+    ctx.upload(k, g1, g2, phi)
+    ctx.step_graph(nsteps)
+    ctx.sync()
+    o1, o2, op = ctx.download(k)
+    t, n, _ = ctx.get_time(k)
+    st = ctx.status(k)
+    ctx.close()
+    assert n == nsteps and st == 0
+    run = O.Run(variant, 0, 5, 5, 10, 0.6, 1e300, nsteps)
+    s = np.zeros(1, dtype=np.int32)
+    to = np.zeros(1)
+    r1, r2, rp = g1.copy(), g2.copy(), phi.copy()
+    assert O.lib().or_twofluid_run(ogrid(info), oeos(eos), run, O._p(r1), O._p(r2), O._p(rp), O._p(s),
+                                   O._p(to)) == 0
+    assert abs(t - to[0]) <= 1e-12 * to[0]
+    for a, b in ((o1, r1), (o2, r2)):
+        linf, l1 = scaled_err(P.interior(info, a), P.interior(info, b))
+        assert linf <= 1e-8 and l1 <= 1e-10, (linf, l1)
+    pi, pr = P.interior(info, op), P.interior(info, rp)
+    assert np.array_equal(pi <= 0, pr <= 0)
+
+
+@pytest.mark.parametrize("variant", VOF)
 def test_vof_schedules_agree(variant, monkeypatch):
     """the dataflow solve of the mixed cells and its plain-rounds form give the same bits, and a
     batch of two instances on two streams gives each instance's single-context bits"""
