@@ -39,6 +39,13 @@ enum { GFM_FS_MUSCL = 0, GFM_FS_GODUNOV = 1 };
 enum { GFM_ORIGINAL = 0, GFM_REAL = 1, GFM_RIEMANN = 2, GFM_MODIFIED = 3, GFM_VOFGFM = 4, GFM_OGFMVOF = 5 };
 /* precision */
 enum { GFM_FP64 = 0, GFM_MIXED = 1 };
+/* interface tracker (settings ITM, sim_twofluid.cpp:555-589): the level set ("LS1".."LS9"), or
+ * the CLSVOF tracker ("CLSVOF", :571-574): the level set plus a volume fraction z of fluid 1
+ * advected conservatively by split PLIC fluxes (Weymouth and Yue 2010), phi given the PLIC line
+ * of z in every cell after each step (the reference's tracker classes are external; this is the
+ * published method).  z follows phi whenever the host sets or reinitialises phi (upload,
+ * gfm_ic_device, gfm_ls_reinit); gfm_download_z returns it. */
+enum { GFM_ITM_LS = 0, GFM_ITM_CLSVOF = 1 };
 
 /* error codes */
 enum {
@@ -81,6 +88,7 @@ typedef struct {
     int reinit_every;  /* reinitialise phi every N steps (0 = never) */
     int reinit_iters;  /* pseudo-time iterations per reinit          */
     int precision;     /* GFM_FP64 or GFM_MIXED                      */
+    int itm;           /* GFM_ITM_LS or GFM_ITM_CLSVOF               */
 } gfm_methods;
 
 typedef struct gfm_ctx gfm_ctx;

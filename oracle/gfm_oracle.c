@@ -900,6 +900,160 @@ void or_vof_interface(const or_grid* G, const double* phi, double* z, double* nr
         }
 }
 
+/* ------------------------------------------------------------------ */
+/* CLSVOF interface tracker (settings ITM "CLSVOF", sim_twofluid.cpp:      */
+/* 571-574).  The reference's class is in the external interface-tracking  */
+/* library, so this restates the published method, not the reference's    */
+/* code: a coupled level set and volume of fluid (Sussman and Puckett      */
+/* 2000).  The volume fraction z of fluid 1 is advected conservatively by  */
+/* directionally split geometric PLIC fluxes with the cell-centred         */
+/* divergence term of Weymouth and Yue (2010), the donor cells             */
+/* reconstructed with the level set's unit normals (unit_normal) and the   */
+/* line that leaves their z; the level set is advected and reinitialised   */
+/* as with "LS5", then given in every cell the PLIC line of z along its    */
+/* own normal wherever the two disagree.                                   */
+/* ------------------------------------------------------------------ */
+
+/* z within this of 0 or 1 counts as a pure cell (rounding in the flux sums
+ * leaves z a few ulp from 1 inside a region of fluid 1) */
+#define OR_ZEPS 1e-12
+
+/* area fraction on fluid 1's side (n . (x - c) + s <= 0) of a w x h
+ * rectangle centred at c, unit normal n: vof_cell's cell-area function */
+double or_plic_area(double s, double nx, double ny, double w, double h)
+{
+    double a = fabs(nx) * w, b = fabs(ny) * h;
+    if (a > b) { double t = a; a = b; b = t; }
+    double m = -s + 0.5 * (a + b);
+    if (m <= 0.0) return 0.0;
+    if (m >= a + b) return 1.0;
+    if (m < a) return m * m / (2.0 * a * b);
+    if (m < b) return (2.0 * m - a) / (2.0 * b);
+    double r = a + b - m;
+    return 1.0 - r * r / (2.0 * a * b);
+}
+
+/* its inverse in s for a dx x dy cell (Scardovelli and Zaleski 2000, 2D):
+ * m = sqrt(2abz) | bz + a/2 | a + b - sqrt(2ab(1-z)) on 2bz <= a | between |
+ * 2b(1-z) <= a, s = (a+b)/2 - m; z <= 0 and z >= 1 give the lines through
+ * the cell's corners */
+double or_plic_line(double z, double nx, double ny, double dx, double dy)
+{
+    double a = fabs(nx) * dx, b = fabs(ny) * dy;
+    if (a > b) { double t = a; a = b; b = t; }
+    double m;
+    if (z <= 0.0) m = 0.0;
+    else if (z >= 1.0) m = a + b;
+    else if (2.0 * b * z <= a) m = sqrt(2.0 * a * b * z);
+    else if (2.0 * b * (1.0 - z) <= a) m = a + b - sqrt(2.0 * a * b * (1.0 - z));
+    else m = b * z + 0.5 * a;
+    return 0.5 * (a + b) - m;
+}
+
+/* the signed volume fraction of fluid 1 through the face between cells A
+ * (lower index along d) and B at face velocity u over dt, in units of one
+ * cell: the donor's strip of width |u| dt next to the face */
+static double vof_face_flux(const or_grid* G, int d, int iA, int jA, int iB, int jB, double u, double dt,
+                            const double* zs, const double* phi)
+{
+    double h = d == 0 ? G->dx : G->dy, w = fabs(u) * dt;
+    if (w == 0.0) return 0.0;
+    int i = u > 0.0 ? iA : iB, j = u > 0.0 ? jA : jB;
+    double z = zs[IDX(G, i, j)], f;
+    if (z <= OR_ZEPS || z >= 1.0 - OR_ZEPS) {
+        f = z;   /* a pure donor: its strip holds z of its area */
+    } else {
+        double n[2];
+        unit_normal(G, phi, i, j, n);
+        double sl = or_plic_line(z, n[0], n[1], G->dx, G->dy);
+        double off = (u > 0.0 ? 0.5 : -0.5) * (h - w);   /* strip centre - donor centre along d */
+        double sc = sl + n[d] * off;
+        f = d == 0 ? or_plic_area(sc, n[0], n[1], w, G->dy) : or_plic_area(sc, n[0], n[1], G->dx, w);
+    }
+    return (u > 0.0 ? 1.0 : -1.0) * (w / h) * f;
+}
+
+/* one sweep along d (0: x, 1: y) over the interior: out = zs minus the
+ * flux difference plus c (uR - uL) dt / h, c = 1 where the step's input zn
+ * exceeds 1/2, clipped to [0, 1]; face velocities are the means of the
+ * cells' velocity components along d (the ghost frame's from the fill) */
+static void vof_sweep(const or_grid* G, int d, const double* zs, const double* zn, const double* phi,
+                      const double* vf, double dt, double* out)
+{
+    double h = d == 0 ? G->dx : G->dy;
+    int di = d == 1, dj = d == 0;
+    for (int i = G->g; i < G->Ny + G->g; ++i)
+        for (int j = G->g; j < G->Nx + G->g; ++j) {
+            size_t q = IDX(G, i, j);
+            double uc = vf[2 * q + d];
+            double uL = 0.5 * (vf[2 * IDX(G, i - di, j - dj) + d] + uc);
+            double uR = 0.5 * (uc + vf[2 * IDX(G, i + di, j + dj) + d]);
+            double fL = vof_face_flux(G, d, i - di, j - dj, i, j, uL, dt, zs, phi);
+            double fR = vof_face_flux(G, d, i, j, i + di, j + dj, uR, dt, zs, phi);
+            double c = zn[q] > 0.5 ? 1.0 : 0.0;
+            double v = zs[q] - (fR - fL) + c * ((uR - uL) * dt / h);
+            out[q] = fmin(fmax(v, 0.0), 1.0);
+        }
+}
+
+/* z over one step of the flow velocity vf (the fill's velocity field): x
+ * then y on even steps, y then x on odd ones; phi: the level set at the
+ * start of the step; the BCs of a scalar after each sweep */
+void or_vof_advect(const or_grid* G, int step_index, double* z, const double* phi, const double* vf, double dt)
+{
+    static _Thread_local buf_t b0, b1;
+    size_t n = (size_t)H_OF(G) * W_OF(G);
+    double* zn = buf_get(&b0, n);
+    double* zs = buf_get(&b1, n);
+    memcpy(zn, z, n * sizeof(double));
+    memcpy(zs, z, n * sizeof(double));
+    int d0 = (step_index & 1) ? 1 : 0;
+    vof_sweep(G, d0, zn, zn, phi, vf, dt, zs);
+    or_apply_bcs_scalar(G, zs);
+    vof_sweep(G, 1 - d0, zs, zn, phi, vf, dt, z);
+    or_apply_bcs_scalar(G, z);
+}
+
+/* the coupling: every interior cell's phi takes the PLIC line of its z
+ * along the uncorrected phi's unit normal where the two disagree (a mixed
+ * cell always; a pure one, z within OR_ZEPS of 0 or 1, where phi's line
+ * cuts it or has the other sign: phi then keeps its magnitude but at least
+ * half the cell's extent along n, with the sign of z's fluid); then the
+ * BCs of a scalar */
+void or_clsvof_correct(const or_grid* G, double* phi, const double* z)
+{
+    static _Thread_local buf_t b0;
+    size_t n = (size_t)H_OF(G) * W_OF(G);
+    double* p0 = buf_get(&b0, n);
+    memcpy(p0, phi, n * sizeof(double));
+    for (int i = G->g; i < G->Ny + G->g; ++i)
+        for (int j = G->g; j < G->Nx + G->g; ++j) {
+            size_t q = IDX(G, i, j);
+            double nv[2];
+            unit_normal(G, p0, i, j, nv);
+            double zc = z[q], p = p0[q];
+            if (zc > OR_ZEPS && zc < 1.0 - OR_ZEPS) {
+                phi[q] = or_plic_line(zc, nv[0], nv[1], G->dx, G->dy);
+            } else {
+                double zp = zc >= 0.5 ? 1.0 : 0.0;
+                if (or_plic_area(p, nv[0], nv[1], G->dx, G->dy) != zp) {
+                    double half = 0.5 * (fabs(nv[0]) * G->dx + fabs(nv[1]) * G->dy);
+                    double mag = fmax(fabs(p), half);
+                    phi[q] = zp == 1.0 ? -mag : mag;
+                }
+            }
+        }
+    or_apply_bcs_scalar(G, phi);
+}
+
+/* initialisation(): z from the level set's PLIC reconstruction
+ * (or_volume_fraction), the ghost frame by the BCs of a scalar */
+void or_clsvof_init(const or_grid* G, const double* phi, double* z)
+{
+    or_volume_fraction(G, phi, z);
+    or_apply_bcs_scalar(G, z);
+}
+
 /* HOUC upwind-biased first-derivative stencils for u > 0: offsets
  * offset0 .. offset0+order, coefficients c[m] / den (den folded in). */
 void or_houc_coeffs(int order, double* c, int* offset0)
@@ -1684,7 +1838,16 @@ int or_twofluid_step(const or_grid* G, const or_eos* E, const or_run* R, int ste
                      double* t, double* grid1, double* grid2, double* phi,
                      double* vfield, double* scratch1, double* scratch2, double* dt_out)
 {
+    return or_twofluid_step_z(G, E, R, step_index, t, grid1, grid2, phi, NULL, vfield, scratch1, scratch2,
+                              dt_out);
+}
+
+int or_twofluid_step_z(const or_grid* G, const or_eos* E, const or_run* R, int step_index,
+                       double* t, double* grid1, double* grid2, double* phi, double* z,
+                       double* vfield, double* scratch1, double* scratch2, double* dt_out)
+{
     int st = 0;
+    const int clsvof = R->itm == OR_ITM_CLSVOF && z;
     double cfl = step_index < 5 ? fmin(R->cfl, 0.2) : R->cfl;             /* :66 */
     double dt = or_compute_dt_twofluid(G, E, cfl, phi, grid1, grid2, R->T, *t);
     int bb1[4], bb2[4];
@@ -1697,9 +1860,13 @@ int or_twofluid_step(const or_grid* G, const or_eos* E, const or_run* R, int ste
         st |= or_fs_muscl(G, E->gamma1, E->pinf1, dt, bb1, grid1, scratch1);
         st |= or_fs_muscl(G, E->gamma2, E->pinf2, dt, bb2, grid2, scratch2);
     }
+    /* update_interface_coupled (:72); CLSVOF: z first, from the level set
+     * at the start of the step */
+    if (clsvof) or_vof_advect(G, step_index, z, phi, vfield, dt);
     or_ls_advect(G, R->ls_order, phi, vfield, dt);
     if (R->reinit_every > 0 && ((step_index + 1) % R->reinit_every) == 0)
         or_ls_reinit(G, phi, R->reinit_iters);
+    if (clsvof) or_clsvof_correct(G, phi, z);
     or_apply_bcs(G, grid1);
     or_apply_bcs(G, grid2);
     *t += dt;
@@ -1710,19 +1877,32 @@ int or_twofluid_step(const or_grid* G, const or_eos* E, const or_run* R, int ste
 int or_twofluid_run(const or_grid* G, const or_eos* E, const or_run* R,
                     double* grid1, double* grid2, double* phi, int* steps_done, double* t_out)
 {
+    return or_twofluid_run_z(G, E, R, grid1, grid2, phi, NULL, steps_done, t_out);
+}
+
+int or_twofluid_run_z(const or_grid* G, const or_eos* E, const or_run* R,
+                      double* grid1, double* grid2, double* phi, double* z, int* steps_done, double* t_out)
+{
     size_t ncell = (size_t)H_OF(G) * W_OF(G);
     double* s1 = (double*)calloc(ncell * 4, sizeof(double));
     double* s2 = (double*)calloc(ncell * 4, sizeof(double));
     double* vf = (double*)calloc(ncell * 2, sizeof(double));
+    double* zown = NULL;
+    if (R->itm == OR_ITM_CLSVOF && !z) {
+        zown = (double*)calloc(ncell, sizeof(double));
+        or_clsvof_init(G, phi, zown);
+        z = zown;
+    }
     int st = 0, s = 0;
     double t = 0.0;
     while (t < R->T && s < R->max_steps) {
-        st |= or_twofluid_step(G, E, R, s, &t, grid1, grid2, phi, vf, s1, s2, NULL);
+        st |= or_twofluid_step_z(G, E, R, s, &t, grid1, grid2, phi, z, vf, s1, s2, NULL);
         ++s;
     }
     free(s1);
     free(s2);
     free(vf);
+    free(zown);
     *steps_done = s;
     *t_out = t;
     return st;

@@ -53,7 +53,10 @@ typedef struct {
     int max_steps;
     int use_extension_vfield;   /* GFM_base flags (GFM_base.hpp:20-21); 0 as sim_twofluid.cpp:551 */
     int use_slip_BCs;
+    int itm;            /* OR_ITM_*: the interface tracker (settings ITM) */
 } or_run;
+
+enum { OR_ITM_LS = 0, OR_ITM_CLSVOF = 1 };
 
 /* A1/A2: EOS + conversions */
 double or_eos_pressure(double g, double pinf, double e, double rho);
@@ -127,6 +130,19 @@ int or_onefluid_step(const or_grid* G, double g, double pinf, int fs, double cfl
 int or_twofluid_step(const or_grid* G, const or_eos* E, const or_run* R, int step_index,
                      double* t, double* grid1, double* grid2, double* phi,
                      double* vfield, double* scratch1, double* scratch2, double* dt_out);
+/* CLSVOF tracker (restated method; see gfm_oracle.c) */
+double or_plic_line(double z, double nx, double ny, double dx, double dy);
+double or_plic_area(double s, double nx, double ny, double w, double h);
+void or_vof_advect(const or_grid* G, int step_index, double* z, const double* phi, const double* vf, double dt);
+void or_clsvof_correct(const or_grid* G, double* phi, const double* z);
+void or_clsvof_init(const or_grid* G, const double* phi, double* z);
+/* or_twofluid_step / or_twofluid_run with the tracker's volume fraction z
+ * (read and written when R->itm is OR_ITM_CLSVOF; run: NULL = from phi) */
+int or_twofluid_step_z(const or_grid* G, const or_eos* E, const or_run* R, int step_index,
+                       double* t, double* grid1, double* grid2, double* phi, double* z,
+                       double* vfield, double* scratch1, double* scratch2, double* dt_out);
+int or_twofluid_run_z(const or_grid* G, const or_eos* E, const or_run* R,
+                      double* grid1, double* grid2, double* phi, double* z, int* steps_done, double* t_out);
 int or_twofluid_run(const or_grid* G, const or_eos* E, const or_run* R,
                     double* grid1, double* grid2, double* phi,
                     int* steps_done, double* t_out);

@@ -134,6 +134,7 @@ struct gfm_ctx {
     int graph_steps = 0;
     bool mixed() const { return m.precision == GFM_MIXED; }
     bool twofluid() const { return m.sim_type == GFM_SIM_TWOFLUID; }
+    bool clsvof() const { return twofluid() && m.itm == GFM_ITM_CLSVOF; }
     int nfluid() const { return twofluid() ? 2 : 1; }
     size_t es() const { return mixed() ? 4 : 8; }
     // OGFM and OGFM_VOF narrow the update boxes to each fluid's real cells
@@ -257,7 +258,7 @@ int twofluid_step(gfm_ctx* c, const gfm::LaunchCtx& L, hipEvent_t mark) {
     int r = ghost_chain<R>(c, L, count, mark, c->mixed() ? 1 : 0);
     if (r) return r;
     gfm::launch_fs<R>(L, muscl, 1, 0, 0, 0);   // the fill read Q[.][0]
-    gfm::launch_ls<R>(L, c->houc, c->m.reinit_every, c->m.reinit_iters);
+    gfm::launch_ls<R>(L, c->houc, c->m.reinit_every, c->m.reinit_iters, c->clsvof());
     gfm::launch_bc<R>(L, 0, 2, 1);
     // FP32 storage of the step's result on the FP64 path: what storing the
     // state in FP32 alone does to a job, without the mixed path's FP32 sweeps
@@ -394,6 +395,16 @@ static int post_launch(gfm_ctx* c) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail("kernel launch", e);
     return GFM_OK;
+}
+
+// the CLSVOF tracker's initialisation() (z from the level set) for one
+// instance, or every instance with inst < 0: whenever the host sets or
+// reinitialises phi, z follows it
+static int clsvof_init(gfm_ctx* c, int inst) {
+    if (!c->clsvof()) return GFM_OK;
+    if (c->mixed()) gfm::launch_clsvof_init<float>(c->L, inst);
+    else gfm::launch_clsvof_init<double>(c->L, inst);
+    return post_launch(c);
 }
 
 // kind: 0 pure (FP64), 1 mixed, 2 pure with the mixed-precision face solver
@@ -610,6 +621,7 @@ int gfm_ctx_create(const gfm_methods* methods, gfm_ctx** out) {
     const int o = methods->ls_order;
     if (o != 1 && o != 3 && o != 5 && o != 7 && o != 9) return fail_arg("ls_order must be 1,3,5,7 or 9");
     if (methods->reinit_every < 0 || methods->reinit_iters < 0) return fail_arg("reinit");
+    if (methods->itm != GFM_ITM_LS && methods->itm != GFM_ITM_CLSVOF) return fail_arg("itm");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev == 0) return fail("hipGetDeviceCount (no GPU)", e == hipSuccess ? hipErrorNoDevice : e);
@@ -756,6 +768,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.ntr = h.ntr;
         d.tr_keep_y = h.tr_keep_y;
         const size_t otrk = h.ntr ? take((gfm::TR_HDR + (size_t)d.hist_cap * (1 + 4 * h.ntr)) * sizeof(double)) : 0;
+        const size_t ozv = c->clsvof() ? take(3 * P * sizeof(double)) : 0;   // CLSVOF: z, first sweep, flags
         const size_t osd = take(gfm::SD_N * sizeof(double));
         const size_t osi = take(gfm::SI_N * sizeof(int));
         hipError_t e = hipMalloc(&h.mem, off);
@@ -776,6 +789,7 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.partials = (double*)(base + opart);
         d.mass_hist = (double*)(base + ohist);
         d.trk = h.ntr ? (double*)(base + otrk) : nullptr;
+        d.zv = c->clsvof() ? (double*)(base + ozv) : nullptr;
         if (h.ntr) HC(hipMemcpyAsync(d.trk, h.tr_hdr, sizeof(h.tr_hdr), hipMemcpyHostToDevice, c->stream));
         d.sd = (double*)(base + osd);
         d.si = (int*)(base + osi);
@@ -904,6 +918,7 @@ int gfm_upload_state(gfm_ctx* c, int inst, const double* g1, const double* g2, c
         if (!g2 || !phi) return fail_arg("two-fluid upload needs grid2 and phi");
         if ((r = xfer(c, inst, h.d.Q[1][0], (double*)g2, 4, true))) return r;
         if ((r = xfer(c, inst, h.d.phi, (double*)phi, 1, true))) return r;
+        if ((r = clsvof_init(c, inst))) return r;
     }
     return GFM_OK;
 }
@@ -945,6 +960,8 @@ int gfm_ic_device(gfm_ctx* c, int n, const int* inst, const gfm_ic_desc* desc) {
     if (c->mixed()) gfm::launch_ic<float>(c->L, c->d_ic, c->d_icmap, q, c->twofluid() ? 1 : 0);
     else gfm::launch_ic<double>(c->L, c->d_ic, c->d_icmap, q, c->twofluid() ? 1 : 0);
     if ((r = post_launch(c))) return r;
+    for (int k = 0; k < n; ++k)
+        if ((r = clsvof_init(c, inst[k]))) return r;
     HC(hipStreamSynchronize(c->stream));
     for (int k = 0; k < n; ++k)
         if ((r = gfm_set_time(c, inst[k], 0.0, 0))) return r;
@@ -1107,6 +1124,8 @@ int gfm_download_z(gfm_ctx* c, int inst, double* z) {
     int r = check_inst(c, inst);
     if (r) return r;
     if (!c->twofluid() || !z) return fail_arg("z needs a two-fluid context and an output array");
+    // the CLSVOF tracker's own volume fraction (FP64 in both precisions)
+    if (c->clsvof()) return download_planes<double>(c, c->inst[inst], c->inst[inst].d.zv, z, 1);
     if (c->mixed()) gfm::launch_zfrac<float>(c->L);
     else gfm::launch_zfrac<double>(c->L);
     HC(hipGetLastError());
@@ -1164,8 +1183,8 @@ int gfm_ls_update(gfm_ctx* c) {
     int r = ready(c);
     if (r) return r;
     if (!c->twofluid()) return fail_arg("level set needs a two-fluid context");
-    if (c->mixed()) gfm::launch_ls<float>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
-    else gfm::launch_ls<double>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
+    if (c->mixed()) gfm::launch_ls<float>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters, c->clsvof());
+    else gfm::launch_ls<double>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters, c->clsvof());
     return post_launch(c);
 }
 
@@ -1176,7 +1195,8 @@ int gfm_ls_reinit(gfm_ctx* c, int iters) {
     if (iters <= 0) return GFM_OK;
     if (c->mixed()) gfm::launch_reinit_now<float>(c->L, iters);
     else gfm::launch_reinit_now<double>(c->L, iters);
-    return post_launch(c);
+    if ((r = post_launch(c))) return r;
+    return clsvof_init(c, -1);
 }
 
 int gfm_apply_bcs(gfm_ctx* c) {
@@ -1268,7 +1288,7 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
                 HC(hipEventRecord(ev[2], c->stream));
                 gfm::launch_fs<float>(c->L, muscl, 1, 0, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));
-                gfm::launch_ls<float>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
+                gfm::launch_ls<float>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters, c->clsvof());
                 HC(hipEventRecord(ev[4], c->stream));
                 gfm::launch_bc<float>(c->L, 0, 2, 1);
             } else {
@@ -1278,7 +1298,7 @@ int gfm_profile_steps(gfm_ctx* c, int nsteps, double* ms) {
                 HC(hipEventRecord(ev[2], c->stream));
                 gfm::launch_fs<double>(c->L, muscl, 1, 0, 0, 0);
                 HC(hipEventRecord(ev[3], c->stream));
-                gfm::launch_ls<double>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters);
+                gfm::launch_ls<double>(c->L, c->houc, c->m.reinit_every, c->m.reinit_iters, c->clsvof());
                 HC(hipEventRecord(ev[4], c->stream));
                 gfm::launch_bc<double>(c->L, 0, 2, 1);
             }

@@ -107,6 +107,7 @@ struct Inst {
     int tr_keep_y;             // tracers return to their initial y after each step
     double* trk;               // TR_HDR doubles (positions, initial positions), then
                                // hist_cap entries of 1 + 4 ntr doubles: t, (x, y, u, v) per tracer
+    double* zv;                // CLSVOF tracker: 3 FP64 planes (z, first sweep, flags); null otherwise
 };
 
 constexpr int TR_MAX = 4;

@@ -3383,6 +3383,167 @@ __global__ void k_reset(const Inst* __restrict__ insts, const int4* __restrict__
 // ---------------------------------------------------------------------------
 #define GRID(n) dim3((unsigned)(n))
 
+// ---------------------------------------------------------------------------
+// CLSVOF tracker (settings ITM "CLSVOF", sim_twofluid.cpp:571-574).  The
+// reference's tracker is a class of an external library, so this is the
+// published method, not a restatement of its code: a coupled level set and
+// volume of fluid (Sussman and Puckett 2000).  Inst::zv holds three FP64
+// planes: the volume fraction z of fluid 1, the first sweep's result, and
+// the divergence term's flag of the step's input.  Each step, between the
+// level set's advection (phi -> phiA) and its reinitialisation:
+//  - two directionally split sweeps (x first on even steps, y first on odd
+//    ones) move z by geometric PLIC fluxes with the cell-centred divergence
+//    term of Weymouth and Yue (2010); a donor cell is reconstructed with the
+//    start-of-step level set's unit normal and the line that leaves its z;
+// and after it:
+//  - every cell's phi takes the PLIC line of its z along its own normal
+//    wherever the two disagree (k_clsvof_fix, into phiA, then the copy).
+// Full-array kernels: a ghost-frame cell takes the result of its BC source
+// cell (the BCs of a scalar).
+// ---------------------------------------------------------------------------
+constexpr double CLS_ZEPS = 1e-12;   // z this near 0 or 1 counts as a pure cell
+
+// inverse of area_fraction in phi for a dx x dy cell (Scardovelli and
+// Zaleski 2000, 2D): m = sqrt(2abz) | bz + a/2 | a + b - sqrt(2ab(1-z)), the
+// line value (a + b)/2 - m
+__device__ __forceinline__ double plic_line(double z, double nx, double ny, double dx, double dy) {
+#pragma clang fp contract(off)
+    double a = fabs(nx) * dx, b = fabs(ny) * dy;
+    if (a > b) { const double t = a; a = b; b = t; }
+    double m;
+    if (z <= 0.0) m = 0.0;
+    else if (z >= 1.0) m = a + b;
+    else if (2.0 * b * z <= a) m = sqrt(2.0 * a * b * z);
+    else if (2.0 * b * (1.0 - z) <= a) m = a + b - sqrt(2.0 * a * b * (1.0 - z));
+    else m = b * z + 0.5 * a;
+    return 0.5 * (a + b) - m;
+}
+
+// signed volume fraction of fluid 1 through the face between cell (iA, jA)
+// and the next cell along d, at face velocity u over dt, in units of one
+// cell: the donor's strip of width |u| dt next to the face
+template <typename R>
+__device__ __forceinline__ double cls_face_flux(const Inst& I, int d, int iA, int jA, double u, double dt,
+                                                const double* zs) {
+#pragma clang fp contract(off)
+    const double h = d == 0 ? I.dx : I.dy, w = fabs(u) * dt;
+    if (w == 0.0) return 0.0;
+    const int i = u > 0.0 ? iA : iA + (d == 1 ? 1 : 0), j = u > 0.0 ? jA : jA + (d == 0 ? 1 : 0);
+    const double z = zs[cidx(I, i, j)];
+    double f;
+    if (z <= CLS_ZEPS || z >= 1.0 - CLS_ZEPS) {
+        f = z;   // a pure donor: its strip holds z of its area
+    } else {
+        double nx, ny;
+        unit_normal<R>(I, i, j, nx, ny);
+        const double sl = plic_line(z, nx, ny, I.dx, I.dy);
+        const double off = (u > 0.0 ? 0.5 : -0.5) * (h - w);   // strip centre - donor centre along d
+        const double sc = sl + (d == 0 ? nx : ny) * off;
+        f = d == 0 ? area_fraction(sc, nx, ny, w, I.dy) : area_fraction(sc, nx, ny, I.dx, w);
+    }
+    return (u > 0.0 ? 1.0 : -1.0) * (w / h) * f;
+}
+
+// one sweep: the first reads plane 0 (the step's input) and writes plane 1
+// and the flags (plane 2), the second reads plane 1 and the flags and
+// writes plane 0.  Face velocities are the means of the cells' velocity
+// components along d (the fill's velocity field, ghost frame included);
+// results clipped to [0, 1].  work: 64x4 tiles over the padded array
+template <typename R>
+__global__ __launch_bounds__(256) void k_cls_sweep(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                   int second) {
+#pragma clang fp contract(off)
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    bool fv, fu;
+    const int si = src_row(I, i, fv), sj = src_col(I, j, fu);
+    const int odd = gmem(I.si)[SI_NSTEP] & 1;
+    const int d = second ? 1 - odd : odd;
+    const double dt = gmem(I.sd)[SD_DT];
+    double* zv = gmem(I.zv);
+    const double* zs = zv + (second ? I.plane : 0);
+    double* out = zv + (second ? 0 : I.plane);
+    double* flags = zv + 2 * I.plane;
+    const long long c = cidx(I, i, j), q = cidx(I, si, sj);
+    double cf;
+    if (second) {
+        cf = flags[c];
+    } else {
+        cf = zs[q] > 0.5 ? 1.0 : 0.0;
+        flags[c] = cf;
+    }
+    const R* vel = gmem((const R*)I.vel) + (d == 0 ? 0 : I.plane);
+    const int di = d == 1 ? 1 : 0, dj = d == 0 ? 1 : 0;
+    const double uc = (double)vel[q];
+    const double uL = 0.5 * ((double)vel[cidx(I, si - di, sj - dj)] + uc);
+    const double uR = 0.5 * (uc + (double)vel[cidx(I, si + di, sj + dj)]);
+    const double fL = cls_face_flux<R>(I, d, si - di, sj - dj, uL, dt, zs);
+    const double fR = cls_face_flux<R>(I, d, si, sj, uR, dt, zs);
+    const double h = d == 0 ? I.dx : I.dy;
+    const double v = zs[q] - (fR - fL) + cf * ((uR - uL) * dt / h);
+    out[c] = fmin(fmax(v, 0.0), 1.0);
+}
+
+// the coupling: phi at a cell takes the PLIC line of its z along phi's unit
+// normal where the two disagree (a mixed cell always; a pure one where phi's
+// line cuts it or has the other sign: phi keeps its magnitude but at least
+// half the cell's extent along n, with the sign of z's fluid).  Written to
+// phiA (the copy follows).  work: 64x4 tiles over the padded array
+template <typename R>
+__global__ __launch_bounds__(256) void k_clsvof_fix(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+#pragma clang fp contract(off)
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    bool fv, fu;
+    const int si = src_row(I, i, fv), sj = src_col(I, j, fu);
+    const long long q = cidx(I, si, sj);
+    double nx, ny;
+    unit_normal<R>(I, si, sj, nx, ny);
+    const double zc = gmem(I.zv)[q], p = (double)ld1<R>(I.phi, q);
+    double out = p;
+    if (zc > CLS_ZEPS && zc < 1.0 - CLS_ZEPS) {
+        out = plic_line(zc, nx, ny, I.dx, I.dy);
+    } else {
+        const double zp = zc >= 0.5 ? 1.0 : 0.0;
+        if (area_fraction(p, nx, ny, I.dx, I.dy) != zp) {
+            const double half = 0.5 * (fabs(nx) * I.dx + fabs(ny) * I.dy);
+            const double mag = fmax(fabs(p), half);
+            out = zp == 1.0 ? -mag : mag;
+        }
+    }
+    gmem((R*)I.phiA)[cidx(I, i, j)] = (R)out;
+}
+
+// initialisation(): z from the level set's PLIC reconstruction (vof_z of
+// the cell, of its BC source in the ghost frame); only >= 0: that instance
+template <typename R>
+__global__ __launch_bounds__(256) void k_clsvof_init(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                     int only) {
+    const int4 w = work[blockIdx.x];
+    if (only >= 0 && w.x != only) return;
+    const Inst& I = insts[w.x];
+    const int i = w.y * 4 + (threadIdx.x >> 6);
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (i >= I.H || j >= I.W) return;
+    bool fv, fu;
+    const int si = src_row(I, i, fv), sj = src_col(I, j, fu);
+    gmem(I.zv)[cidx(I, i, j)] = vof_z<R>(I, si, sj);
+}
+
+template <typename R>
+void launch_clsvof_init(const LaunchCtx& L, int only) {
+    hipLaunchKernelGGL((k_clsvof_init<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, only);
+}
+
+
 template <typename R>
 void launch_dt(const LaunchCtx& L, int twofluid, int q, int narrow_boxes, int count_iface) {
     hipLaunchKernelGGL((k_dt<R>), GRID((L.n_seg + SEG_PER_WG - 1) / SEG_PER_WG), dim3(256), 0, L.stream, L.insts,
@@ -3578,8 +3739,20 @@ void launch_reinit_chain(const LaunchCtx& L, int K, int every, int force, int co
 }
 
 template <typename R>
-void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters) {
+void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters, bool clsvof) {
     if (iters <= 0) every = 0;   // no reinitialisation iterations: phi = A every step
+    // CLSVOF: z's sweeps read the level set at the start of the step (phi,
+    // before the copy or reinitialisation from A); the coupling follows
+    auto sweeps = [&]() {
+        if (!clsvof) return;
+        hipLaunchKernelGGL((k_cls_sweep<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, 0);
+        hipLaunchKernelGGL((k_cls_sweep<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, 1);
+    };
+    auto couple = [&]() {
+        if (!clsvof) return;
+        hipLaunchKernelGGL((k_clsvof_fix<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full);
+        hipLaunchKernelGGL((k_ls_copy<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, 0);
+    };
     if (L.tb) {
         static_assert(LTX == RTX && LTY == RTY, "the fused level-set kernel uses the reinitialisation tiles");
 #define RK3(O)                                                                                                         \
@@ -3597,15 +3770,19 @@ void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters) {
             default: RK3(9); break;
         }
 #undef RK3
+        sweeps();
         if (every > 0) launch_reinit_chain<R>(L, iters, every, 0, 1);
         else hipLaunchKernelGGL((k_ls_copy<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, every);
+        couple();
         return;
     }
     hipLaunchKernelGGL((k_ls_stage<R, 1>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
     hipLaunchKernelGGL((k_ls_stage<R, 2>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
     hipLaunchKernelGGL((k_ls_stage<R, 3>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, H);
+    sweeps();
     hipLaunchKernelGGL((k_ls_copy<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, every);
     if (every > 0) launch_reinit_chain<R>(L, iters, every, 0, 0);
+    couple();
 }
 
 template <typename R>
@@ -3722,7 +3899,8 @@ void launch_mixed_rs(hipStream_t s, int n, double gL, double pL, double gR, doub
     template void launch_iface<R>(const LaunchCtx&, int, int, int);           \
     template void launch_mgfm_final<R>(const LaunchCtx&, int, int, bool);     \
     template void launch_vext<R>(const LaunchCtx&, bool);                       \
-    template void launch_ls<R>(const LaunchCtx&, const Houc&, int, int);      \
+    template void launch_ls<R>(const LaunchCtx&, const Houc&, int, int, bool); \
+    template void launch_clsvof_init<R>(const LaunchCtx&, int);                 \
     template void launch_reinit_now<R>(const LaunchCtx&, int);                \
     template void launch_bc<R>(const LaunchCtx&, int, int, int);               \
     template void launch_vof<R>(const LaunchCtx&, bool);                       \

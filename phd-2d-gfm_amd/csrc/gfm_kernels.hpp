@@ -65,7 +65,11 @@ template <typename R> void launch_vof(const LaunchCtx& L, bool ogfm);
 // interfacial cells' velocity carried into the rest of the grid by 20
 // iterations, written to the velocity field
 template <typename R> void launch_vext(const LaunchCtx& L, bool ogfm);
-template <typename R> void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters);
+// clsvof: the CLSVOF tracker's z sweeps and coupling around the level set's
+// reinitialisation (Inst::zv)
+template <typename R> void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters, bool clsvof);
+// z of the CLSVOF tracker from the level set (only >= 0: that instance)
+template <typename R> void launch_clsvof_init(const LaunchCtx& L, int only);
 template <typename R> void launch_reinit_now(const LaunchCtx& L, int iters);
 template <typename R> void launch_bc(const LaunchCtx& L, int q, int nfluid, int advance);
 template <typename R> void launch_reset(const LaunchCtx& L, int narrow_boxes);

@@ -19,6 +19,7 @@ SIM_ONEFLUID, SIM_TWOFLUID = 0, 1
 FS_MUSCL, FS_GODUNOV = 0, 1
 GFM_ORIGINAL, GFM_REAL, GFM_RIEMANN, GFM_MODIFIED, GFM_VOFGFM, GFM_OGFMVOF = 0, 1, 2, 3, 4, 5
 FP64, MIXED = 0, 1
+ITM_LS, ITM_CLSVOF = 0, 1
 ST_UNPHYSICAL, ST_RS_FAIL, ST_IFACE_OVERFLOW, ST_IFACE_ROUNDS, ST_IFACE_LIST = 1, 2, 4, 8, 16
 
 GFM_NAMES = {"OGFM": GFM_ORIGINAL, "realGFM": GFM_REAL, "riemannGFM": GFM_RIEMANN, "MGFM": GFM_MODIFIED,
@@ -57,7 +58,7 @@ class Eos(C.Structure):
 
 class Methods(C.Structure):
     _fields_ = [("sim_type", C.c_int), ("fs", C.c_int), ("gfm", C.c_int), ("ls_order", C.c_int),
-                ("reinit_every", C.c_int), ("reinit_iters", C.c_int), ("precision", C.c_int)]
+                ("reinit_every", C.c_int), ("reinit_iters", C.c_int), ("precision", C.c_int), ("itm", C.c_int)]
 
 
 IC_TTC, IC_EXPLOSION, IC_BUBBLE, IC_RMI, IC_PLANAR = 1, 2, 3, 4, 5
@@ -164,8 +165,8 @@ def _dp(a):
 
 
 def methods(sim_type=SIM_TWOFLUID, fs=FS_MUSCL, gfm=GFM_RIEMANN, ls_order=5, reinit_every=5,
-            reinit_iters=10, precision=FP64):
-    return Methods(sim_type, fs, gfm, ls_order, reinit_every, reinit_iters, precision)
+            reinit_iters=10, precision=FP64, itm=ITM_LS):
+    return Methods(sim_type, fs, gfm, ls_order, reinit_every, reinit_iters, precision, itm)
 
 
 class Context:

@@ -122,10 +122,10 @@ static gfm_ctx* cached_ctx(const gfm_methods& m, const sim_info& p, const gfm_eo
 }
 
 static gfm_methods methods_of(int sim_type, int fs, int gfm, int order = 5, int every = 0, int iters = 0,
-                              int precision = GFM_FP64) {
+                              int precision = GFM_FP64, int itm = GFM_ITM_LS) {
     gfm_methods m;
     m.sim_type = sim_type; m.fs = fs; m.gfm = gfm; m.ls_order = order;
-    m.reinit_every = every; m.reinit_iters = iters; m.precision = precision;
+    m.reinit_every = every; m.reinit_iters = iters; m.precision = precision; m.itm = itm;
     return m;
 }
 
@@ -453,12 +453,15 @@ static void write_twofluid_output(const std::string& stem, const std::string& ba
     err << p.Nx << " " << l1 << "\n";
 }
 
+// the level set's HOUC order for settings ITM (sim_twofluid.cpp:555-589); CLSVOF carries a fifth-order
+// level set beside its volume fraction
 static int ls_order(const std::string& itm) {
     if (itm == "LS1") return 1;
     if (itm == "LS3") return 3;
-    if (itm == "LS5") return 5;
+    if (itm == "LS5" || itm == "CLSVOF") return 5;
     if (itm == "LS9") return 9;
-    throw std::runtime_error("Invalid ITM choice (level-set trackers LS1/LS3/LS5/LS9 only)");
+    throw std::runtime_error("Invalid ITM choice (LS1/LS3/LS5/LS9, CLSVOF; the PYVOF, MOF and EMOF2 trackers are "
+                             "not built)");
 }
 
 static int gfm_id(const std::string& g) {
@@ -479,7 +482,8 @@ void sim_twofluid::run_sim(GFM_settingsfile SF) {
     const int fs = SF.FS == "Godunov" ? GFM_FS_GODUNOV : (SF.FS == "MUSCL" ? GFM_FS_MUSCL : -1);
     if (fs < 0) throw std::runtime_error("Invalid FS choice");
     gfm_methods m = methods_of(GFM_SIM_TWOFLUID, fs, gfm_id(SF.GFM), ls_order(SF.ITM), SF.reinit_every,
-                               SF.reinit_iters, SF.precision == "mixed" ? GFM_MIXED : GFM_FP64);
+                               SF.reinit_iters, SF.precision == "mixed" ? GFM_MIXED : GFM_FP64,
+                               SF.ITM == "CLSVOF" ? GFM_ITM_CLSVOF : GFM_ITM_LS);
     gfm_ctx* c = nullptr;
     check(gfm_ctx_create(&m, &c), "gfm_ctx_create");
     const gfm_sim_info s = p.c_info();

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 BC_T, BC_R, BC_P = 0, 1, 2
 GFM_ORIGINAL, GFM_REAL, GFM_RIEMANN, GFM_MODIFIED, GFM_VOF, GFM_OGFM_VOF = 0, 1, 2, 3, 4, 5
 FS_MUSCL, FS_GODUNOV = 0, 1
+ITM_LS, ITM_CLSVOF = 0, 1
 
 
 class Grid(C.Structure):
@@ -39,7 +40,7 @@ class Run(C.Structure):
     _fields_ = [("gfm", C.c_int), ("fs", C.c_int), ("ls_order", C.c_int),
                 ("reinit_every", C.c_int), ("reinit_iters", C.c_int),
                 ("cfl", C.c_double), ("T", C.c_double), ("max_steps", C.c_int),
-                ("use_extension_vfield", C.c_int), ("use_slip_BCs", C.c_int)]
+                ("use_extension_vfield", C.c_int), ("use_slip_BCs", C.c_int), ("itm", C.c_int)]
 
 
 _lib = None
@@ -116,6 +117,14 @@ def _bind(L):
         "or_onefluid_step": (i32, [C.POINTER(Grid), dbl, dbl, i32, dbl, dbl, D, D, D, D]),
         "or_twofluid_step": (i32, [C.POINTER(Grid), C.POINTER(Eos), C.POINTER(Run), i32, D, D, D, D, D, D, D, D]),
         "or_twofluid_run": (i32, [C.POINTER(Grid), C.POINTER(Eos), C.POINTER(Run), D, D, D, I, D]),
+        "or_plic_line": (dbl, [dbl, dbl, dbl, dbl, dbl]),
+        "or_plic_area": (dbl, [dbl, dbl, dbl, dbl, dbl]),
+        "or_vof_advect": (None, [C.POINTER(Grid), i32, D, D, D, dbl]),
+        "or_clsvof_correct": (None, [C.POINTER(Grid), D, D]),
+        "or_clsvof_init": (None, [C.POINTER(Grid), D, D]),
+        "or_twofluid_step_z": (i32, [C.POINTER(Grid), C.POINTER(Eos), C.POINTER(Run), i32, D, D, D, D, D, D, D, D,
+                                     D]),
+        "or_twofluid_run_z": (i32, [C.POINTER(Grid), C.POINTER(Eos), C.POINTER(Run), D, D, D, D, I, D]),
         "or_onefluid_run": (i32, [C.POINTER(Grid), dbl, dbl, i32, dbl, dbl, i32, D, I, D]),
         "or_cellcentre": (None, [C.POINTER(Grid), i32, i32, D, D]),
         "or_ic_onefluid": (i32, [C.POINTER(Grid), dbl, dbl, i32, D]),
