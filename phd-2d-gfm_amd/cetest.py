@@ -78,11 +78,11 @@ def bilinear(info, f, x, y):
             + ty * ((1.0 - tx) * f[i0 + 1, j0] + tx * f[i0 + 1, j0 + 1]))
 
 
-def gfm_error(gfm, n, cfl, ref_info, ref_rho, ls_order, reinit_every, reinit_iters):
+def gfm_error(gfm, n, cfl, ref_info, ref_rho, ls_order, reinit_every, reinit_iters, itm=G.ITM_LS):
     info = G.SimInfo(n, n, 6, 0.0, 0.0, 2.0 / n, 2.0 / n, TR, TR, TR, TR)
     eos = G.Eos(1.4, 0.0, 1.4, 0.0)
     ctx = G.Context(G.methods(sim_type=G.SIM_TWOFLUID, fs=G.FS_MUSCL, gfm=GFMS[gfm], ls_order=ls_order,
-                              reinit_every=reinit_every, reinit_iters=reinit_iters, precision=G.FP64))
+                              reinit_every=reinit_every, reinit_iters=reinit_iters, precision=G.FP64, itm=itm))
     ctx.add(info, eos, cfl, T_END, 1 << 30)
     ctx.commit()
     # fluid 1 the inside state everywhere, fluid 2 the outside one, phi the distance to the circle

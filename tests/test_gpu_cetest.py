@@ -18,6 +18,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 import cetest as CT
+import gfm_hip as G
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 VARIANT = os.path.join(os.path.dirname(HERE), "oracle", "liboracle_mgfm_new.so")
@@ -97,3 +98,18 @@ def test_vof_gfm_thesis(ref):
         print("VOFGFM N=%d L1=%.5f thesis=%.5f" % (n, l1, CT.THESIS["VOFGFM"][k]), flush=True)
         assert status == 0
         assert abs(l1 / CT.THESIS["VOFGFM"][k] - 1.0) <= 0.15, (n, l1, CT.THESIS["VOFGFM"][k])
+
+
+def test_vof_gfm_on_clsvof_tracker_thesis(ref):
+    """GFM_VOF on the CLSVOF tracker (a level set with a conservatively advected volume fraction)
+    against the thesis's VOF-GFM errors, which came from a VOF tracker: within 15% at N = 50, 100, 200,
+    with first-order convergence"""
+    print("CASE cetest VOFGFM on CLSVOF", flush=True)
+    got = []
+    for k, n in enumerate((50, 100, 200)):
+        l1, steps, status = CT.gfm_error("VOFGFM", n, CFL, ref[0], ref[1], 9, 0, 10, itm=G.ITM_CLSVOF)
+        print("VOFGFM+CLSVOF N=%d L1=%.5f thesis=%.5f" % (n, l1, CT.THESIS["VOFGFM"][k]), flush=True)
+        assert status == 0
+        got.append(l1)
+        assert abs(l1 / CT.THESIS["VOFGFM"][k] - 1.0) <= 0.15, (n, l1, CT.THESIS["VOFGFM"][k])
+    assert got[0] > got[1] > got[2]

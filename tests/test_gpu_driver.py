@@ -118,12 +118,13 @@ def test_gfm2d_helium_bubble(tmp_path, oracle):
     assert abs(mass[-1, 0] - t[0]) <= 1e-5 * t[0]
 
 
-@pytest.mark.parametrize("gfm,variant", [("VOFGFM", 4), ("OGFMVOF", 5)])
-def test_gfm2d_vof_helium_bubble(tmp_path, oracle, gfm, variant):
-    """the settings names of the VOF-based GFMs (sim_twofluid.cpp:538-545) on the nondimensional
-    shocked_helium_bubble at 130x36, 40 steps, against the oracle's GFM_VOF / OGFM_VOF"""
+@pytest.mark.parametrize("gfm,variant,itm", [("VOFGFM", 4, "LS5"), ("OGFMVOF", 5, "LS5"), ("VOFGFM", 4, "CLSVOF")])
+def test_gfm2d_vof_helium_bubble(tmp_path, oracle, gfm, variant, itm):
+    """the settings names of the VOF-based GFMs (sim_twofluid.cpp:538-545) and of the CLSVOF tracker
+    (:571-574) on the nondimensional shocked_helium_bubble at 130x36, 40 steps, against the oracle's
+    GFM_VOF / OGFM_VOF on the level set or the CLSVOF tracker"""
     f = settings(tmp_path, Nx=130, Ny=36, CFL=0.4, sim_type="twofluid", FS="MUSCL",
-                 test_case="shocked_helium_bubble", ITM="LS5", GFM=gfm, steps=40)
+                 test_case="shocked_helium_bubble", ITM=itm, GFM=gfm, steps=40)
     out = subprocess.run([GFM2D, f], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     assert out.returncode == 0 and "status 0" in out.stdout
@@ -136,7 +137,8 @@ def test_gfm2d_vof_helium_bubble(tmp_path, oracle, gfm, variant):
     oracle.lib().or_ic_bubble(G, E, 175.0, 44.5, 25.0, oracle._p(oracle.arr(0.138, 0, 0, 1)),
                               oracle._p(oracle.arr(1, 0, 0, 1)), oracle._p(oracle.arr(1.3764, -0.394, 0, 1.5698)),
                               225.0, oracle._p(g1), oracle._p(g2), oracle._p(phi))
-    run = oracle.Run(variant, 0, 5, 5, 10, 0.4, 280.0, 40)
+    run = oracle.Run(variant, 0, 5, 5, 10, 0.4, 280.0, 40, 0, 0,
+                     oracle.ITM_CLSVOF if itm == "CLSVOF" else oracle.ITM_LS)
     s = np.zeros(1, dtype=np.int32)
     t = np.zeros(1)
     assert oracle.lib().or_twofluid_run(G, E, run, oracle._p(g1), oracle._p(g2), oracle._p(phi), oracle._p(s),
