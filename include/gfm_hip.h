@@ -193,7 +193,8 @@ int gfm_tracer_history(gfm_ctx* ctx, int inst, int n, double* out);
  * each cell's area on fluid 1's side of the plane through the cell centre with
  * the level set's value and central-difference unit normal there, over the
  * padded grid (the outermost ring 1 or 0 by the sign of phi).  Uses the level
- * set's stage buffer: call between steps. */
+ * set's stage buffer: call between steps.  With GFM_ITM_CLSVOF: the tracker's
+ * own volume fraction (the ghost frame by the BCs of a scalar). */
 int gfm_download_z(gfm_ctx* ctx, int inst, double* z);
 
 /* Riemann solvers over n faces / interfaces, host pointers, AoS x4.
