@@ -140,9 +140,12 @@ def make_setup(builder, cases, perturb=0.0):
     return setup, states
 
 
+ITM = G.ITM_LS   # the interface tracker of every context (--itm)
+
+
 def new_ctx(setup, variant, precision, max_steps):
     m = G.methods(sim_type=G.SIM_TWOFLUID, fs=G.FS_MUSCL, gfm=variant, ls_order=5,
-                  reinit_every=5, reinit_iters=10, precision=precision)
+                  reinit_every=5, reinit_iters=10, precision=precision, itm=ITM)
     ctx = G.Context(m)
     for info, eos in setup:
         ctx.add(info, eos, 0.6, 1e300, max_steps)
@@ -159,14 +162,17 @@ def load_ic(ctx, states, init_reinit):
 
 
 def checkpoint(ctx, n):
-    """every instance's grids, time and step count"""
+    """every instance's grids, time and step count, and the CLSVOF tracker's z"""
     ctx.sync()
-    return [(ctx.download(k), ctx.get_time(k)[:2]) for k in range(n)]
+    return [(ctx.download(k), ctx.get_time(k)[:2], ctx.download_z(k) if ITM == G.ITM_CLSVOF else None)
+            for k in range(n)]
 
 
 def restore(ctx, ck):
-    for k, (st, (t, nstep)) in enumerate(ck):
+    for k, (st, (t, nstep), z) in enumerate(ck):
         ctx.upload(k, *st)
+        if z is not None:
+            ctx.upload_z(k, z)
         ctx.set_time(k, t, nstep)
     ctx.sync()
 
@@ -228,7 +234,8 @@ def oracle_job(builder_args):
     Gr = O.Grid(info.Nx, info.Ny, info.numGC, info.x0, info.y0, info.dx, info.dy,
                 info.BC_L, info.BC_R, info.BC_B, info.BC_T)
     E = O.Eos(eos.gamma1, eos.pinf1, eos.gamma2, eos.pinf2)
-    run = O.Run(variant, 0, 5, 5, 10, 0.6, 1e300, steps)
+    run = O.Run(variant, 0, 5, 5, 10, 0.6, 1e300, steps, 0, 0,
+                O.ITM_CLSVOF if ITM == G.ITM_CLSVOF else O.ITM_LS)
     s = np.zeros(1, dtype=np.int32)
     t = np.zeros(1)
     O.lib().or_twofluid_run(Gr, E, run, O._p(g1), O._p(g2), O._p(phi), O._p(s), O._p(t))
@@ -345,6 +352,8 @@ def main():
     ap.add_argument("--instances", type=int, default=None)
     ap.add_argument("--gfm", default=None, choices=sorted(G.GFM_NAMES),
                     help="the workload with another ghost fluid method (settings-file name, e.g. VOFGFM)")
+    ap.add_argument("--itm", default="LS5", choices=["LS5", "CLSVOF"],
+                    help="interface tracker: the level set, or the CLSVOF tracker beside it")
     ap.add_argument("--cpu-steps", type=int, default=200, help="most oracle steps per instance in the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run of mixed workloads")
@@ -374,11 +383,13 @@ def main():
         print(json.dumps(copy_calibration(int(a.copy_calibration * (1 << 30)))))
         return
 
+    global ITM
+    ITM = G.ITM_CLSVOF if a.itm == "CLSVOF" else G.ITM_LS
     n_inst = a.instances or (64 if a.workload == "batch" else 256)
     builder, cases, variant, prec, job, name, init_reinit = make_cases(a.workload, n_inst)
     if a.gfm:
         variant = G.GFM_NAMES[a.gfm]
-    name = name.format(gfm=GFM_CLASS[variant])
+    name = name.format(gfm=GFM_CLASS[variant] + (" on the CLSVOF tracker" if ITM == G.ITM_CLSVOF else ""))
     job = a.job_steps or job
     advance = job // 2 if a.advance is None else a.advance
     cap = advance + a.warmup + max(a.steps, a.tune_steps or 0) + 8
@@ -415,7 +426,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": name, "instances": n, "cells": cells, "job_steps": job,
                    "window": [advance + a.warmup, advance + a.warmup + a.steps],
-                   "gfm": GFM_LABEL[variant], "fs": "MUSCL", "ls": "LS5", "reinit_every": 5, "cfl": 0.6},
+                   "gfm": GFM_LABEL[variant], "fs": "MUSCL", "ls": a.itm, "reinit_every": 5, "cfl": 0.6},
     }
     if not a.no_replay:
         # the same steps again from the window's checkpoint, eager, events around each phase

@@ -196,6 +196,10 @@ int gfm_tracer_history(gfm_ctx* ctx, int inst, int n, double* out);
  * set's stage buffer: call between steps.  With GFM_ITM_CLSVOF: the tracker's
  * own volume fraction (the ghost frame by the BCs of a scalar). */
 int gfm_download_z(gfm_ctx* ctx, int inst, double* z);
+/* No counterpart in the reference, whose trackers keep z to themselves: sets the CLSVOF tracker's
+ * volume fraction over the padded grid (after gfm_upload_state, which derives it from phi), so that a
+ * caller can restore a checkpoint taken with gfm_download_z.  GFM_ITM_CLSVOF contexts only. */
+int gfm_upload_z(gfm_ctx* ctx, int inst, const double* z);
 
 /* Riemann solvers over n faces / interfaces, host pointers, AoS x4.
  * pure_RS_base::solve_RP (pure_RS_base.hpp:13-20): conserved L/R -> flux.

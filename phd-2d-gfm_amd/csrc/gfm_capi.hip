@@ -1132,6 +1132,13 @@ int gfm_download_z(gfm_ctx* c, int inst, double* z) {
     return xfer(c, inst, c->inst[inst].d.phiA, z, 1, false);
 }
 
+int gfm_upload_z(gfm_ctx* c, int inst, const double* z) {
+    int r = check_inst(c, inst);
+    if (r) return r;
+    if (!c->clsvof() || !z) return fail_arg("z upload needs a CLSVOF context and an input array");
+    return upload_planes<double>(c, c->inst[inst], c->inst[inst].d.zv, z, 1);
+}
+
 int gfm_mass_sums(gfm_ctx* c, int inst, double* out2) {
     int r = check_inst(c, inst);
     if (r) return r;

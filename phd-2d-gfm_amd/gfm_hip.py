@@ -34,7 +34,7 @@ EXPORTS = [
     "gfm_compute_dt", "gfm_set_ghost_states", "gfm_fs_update", "gfm_ls_update", "gfm_apply_bcs",
     "gfm_extrapolate", "gfm_ls_reinit", "gfm_get_bbox", "gfm_set_bbox",
     "gfm_step", "gfm_step_graph", "gfm_sync", "gfm_profile_steps", "gfm_status", "gfm_mass_sums", "gfm_mass_history", "gfm_iface_info",
-    "gfm_set_output_hold", "gfm_set_tracers", "gfm_tracer_history", "gfm_download_z",
+    "gfm_set_output_hold", "gfm_set_tracers", "gfm_tracer_history", "gfm_download_z", "gfm_upload_z",
     "gfm_pure_rs_solve", "gfm_pure_rs_solve_f32", "gfm_mixed_rs_solve", "gfm_pure_rs_solve_noalloc",
     "gfm_ic_onefluid", "gfm_ic_bubble", "gfm_ic_rmi", "gfm_ic_planar", "gfm_shock_state", "gfm_ic_device",
     "gfm_last_error",
@@ -129,6 +129,7 @@ def load():
         "gfm_set_tracers": [vp, C.c_int, C.c_int, _D, C.c_int],
         "gfm_tracer_history": [vp, C.c_int, C.c_int, _D],
         "gfm_download_z": [vp, C.c_int, _D],
+        "gfm_upload_z": [vp, C.c_int, _D],
         "gfm_pure_rs_solve": [C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _D, _U],
         "gfm_pure_rs_solve_f32": [C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _D, _U],
         "gfm_pure_rs_solve_noalloc": [C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _D, _U],
@@ -322,6 +323,10 @@ class Context:
         z = np.zeros((H, W))
         _check(load().gfm_download_z(self.h, k, _dp(z)), "gfm_download_z")
         return z
+
+    def upload_z(self, k, z):
+        """the CLSVOF tracker's z (after upload, which derives it from phi)"""
+        _check(load().gfm_upload_z(self.h, k, _dp(np.ascontiguousarray(z, dtype=np.float64))), "gfm_upload_z")
 
     def iface_info(self, k):
         n, r = C.c_int(0), C.c_int(0)
