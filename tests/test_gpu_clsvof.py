@@ -200,3 +200,34 @@ def test_clsvof_periodic_x_matches_oracle():
     zi, zr = P.interior(info, oz), P.interior(info, rz)
     assert np.abs(zi - zr).max() <= 1e-8
     assert np.array_equal(P.interior(info, op) <= 0, P.interior(info, rp) <= 0)
+
+
+def test_upload_z_restores_the_tracker():
+    """gfm_upload_z sets the tracker's z (a checkpoint taken with gfm_download_z restores the steps'
+    bits); level-set contexts refuse it"""
+    print("CASE clsvof upload_z", flush=True)
+    info, eos, state = P.shocked_bubble(96, 28)
+    ctx, k = ctx_of(info, eos, G.GFM_RIEMANN)
+    ctx.upload(k, *state)
+    ctx.step_graph(12)
+    ctx.sync()
+    ck = (ctx.download(k), ctx.get_time(k)[:2], ctx.download_z(k))
+    ctx.step_graph(10)
+    ctx.sync()
+    ref = ctx.download(k) + (ctx.download_z(k),)
+    ctx.upload(k, *ck[0])
+    ctx.upload_z(k, ck[2])
+    ctx.set_time(k, *ck[1])
+    assert np.array_equal(ctx.download_z(k), ck[2])
+    ctx.step_graph(10)
+    ctx.sync()
+    got = ctx.download(k) + (ctx.download_z(k),)
+    ctx.close()
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    ls = G.Context(G.methods(gfm=G.GFM_RIEMANN))
+    kk = ls.add(info, eos, 0.6, 1e300, 1 << 30)
+    ls.commit()
+    with pytest.raises(G.GfmError):
+        ls.upload_z(kk, ck[2])
+    ls.close()
