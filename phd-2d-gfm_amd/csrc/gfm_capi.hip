@@ -473,13 +473,13 @@ namespace {
 
 // work tables of a set of instances, appended to `all`
 struct TableOffsets {
-    size_t o[10];
-    int n[10];
+    size_t o[11];
+    int n[11];
 };
 
 TableOffsets append_tables(const gfm_ctx* c, const std::vector<Inst>& descs, const std::vector<int>& members,
                            std::vector<int4>& all) {
-    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich, w_tb, w_rt, w_seg, w_vseg;
+    std::vector<int4> w_full, w_int, w_fs, w_frame, w_inst, w_ich, w_tb, w_rt, w_seg, w_vseg, w_p16;
     const int nf = c->nfluid();
     for (int ki : members) {
         const Inst& d = descs[ki];
@@ -487,6 +487,9 @@ TableOffsets append_tables(const gfm_ctx* c, const std::vector<Inst>& descs, con
         const int fx = (d.W + 63) / 64, fy = (d.H + 3) / 4;
         for (int ty = 0; ty < fy; ++ty)
             for (int tx = 0; tx < fx; ++tx) w_full.push_back(make_int4(ki, ty, tx, 0));
+        if (c->clsvof())   // the tracker's full-array kernels: 64x16 tiles, four rows per thread
+            for (int ty = 0; ty < (d.H + 15) / 16; ++ty)
+                for (int tx = 0; tx < fx; ++tx) w_p16.push_back(make_int4(ki, ty, tx, 0));
         const int tix = (s.Nx + 63) / 64, tiy = (s.Ny + 3) / 4;
         int li = 0;
         for (int ty = 0; ty < tiy; ++ty)
@@ -522,16 +525,17 @@ TableOffsets append_tables(const gfm_ctx* c, const std::vector<Inst>& descs, con
         }
     }
     xcd_order(w_full, 0);
+    xcd_order(w_p16, 0);
     xcd_order(w_fs, 0);
     xcd_order(w_tb, 0);
     xcd_order(w_rt, 0);
     xcd_order(w_seg, 0, gfm::SEG_PER_WG);
     xcd_order(w_vseg, 0, gfm::SEG_PER_WG);
     if (w_ich.empty()) w_ich.push_back(make_int4(members.empty() ? 0 : members[0], 1 << 30, 0, 0));
-    const std::vector<int4>* t[10] = {&w_full, &w_int, &w_fs, &w_frame, &w_inst, &w_ich, &w_tb, &w_rt, &w_seg,
-                                      &w_vseg};
+    const std::vector<int4>* t[11] = {&w_full, &w_int, &w_fs, &w_frame, &w_inst, &w_ich, &w_tb, &w_rt, &w_seg,
+                                      &w_vseg, &w_p16};
     TableOffsets r{};
-    for (int q = 0; q < 10; ++q) {
+    for (int q = 0; q < 11; ++q) {
         r.o[q] = all.size();
         r.n[q] = (int)t[q]->size();
         all.insert(all.end(), t[q]->begin(), t[q]->end());
@@ -555,6 +559,7 @@ gfm::LaunchCtx launch_ctx(const gfm_ctx* c, const TableOffsets& t, hipStream_t s
     L.w_rt = w + t.o[7]; L.n_rt = t.n[7];
     L.w_seg = w + t.o[8]; L.n_seg = t.n[8];
     L.w_vseg = w + t.o[9]; L.n_vseg = t.n[9];
+    L.w_p16 = w + t.o[10]; L.n_p16 = t.n[10];
     L.tb = c->tb ? 1 : 0;
     for (int k : members) {
         if (c->inst[k].info.BC_B == GFM_BC_PERIODIC) L.periodic_y = 1;

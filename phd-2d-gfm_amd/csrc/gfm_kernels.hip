@@ -3397,7 +3397,7 @@ __global__ void k_reset(const Inst* __restrict__ insts, const int4* __restrict__
 //    start-of-step level set's unit normal and the line that leaves its z;
 // and after it:
 //  - every cell's phi takes the PLIC line of its z along its own normal
-//    wherever the two disagree (k_clsvof_fix, into phiA, then the copy).
+//    wherever the two disagree (k_clsvof_fix, into phiA, then k_cls_copy).
 // Full-array kernels: a ghost-frame cell takes the result of its BC source
 // cell (the BCs of a scalar).
 // ---------------------------------------------------------------------------
@@ -3444,11 +3444,16 @@ __device__ __forceinline__ double cls_face_flux(const Inst& I, int d, int iA, in
     return (u > 0.0 ? 1.0 : -1.0) * (w / h) * f;
 }
 
+// rows per thread of the tracker's full-array kernels (64 x 4 CLS_ROWS tiles,
+// LaunchCtx::w_p16): a workgroup's chain of dependent loads (work item,
+// descriptor, scalars) is paid once for CLS_ROWS cells per thread
+constexpr int CLS_ROWS = 4;
+
 // one sweep: the first reads plane 0 (the step's input) and writes plane 1
 // and the flags (plane 2), the second reads plane 1 and the flags and
 // writes plane 0.  Face velocities are the means of the cells' velocity
 // components along d (the fill's velocity field, ghost frame included);
-// results clipped to [0, 1].  work: 64x4 tiles over the padded array
+// results clipped to [0, 1].  work: 64x16 tiles over the padded array
 template <typename R>
 __global__ __launch_bounds__(256) void k_cls_sweep(const Inst* __restrict__ insts, const int4* __restrict__ work,
                                                    int second) {
@@ -3456,11 +3461,8 @@ __global__ __launch_bounds__(256) void k_cls_sweep(const Inst* __restrict__ inst
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
     if (!gmem(I.si)[SI_ACTIVE]) return;
-    const int i = w.y * 4 + (threadIdx.x >> 6);
     const int j = w.z * 64 + (threadIdx.x & 63);
-    if (i >= I.H || j >= I.W) return;
-    bool fv, fu;
-    const int si = src_row(I, i, fv), sj = src_col(I, j, fu);
+    if (j >= I.W) return;
     const int odd = gmem(I.si)[SI_NSTEP] & 1;
     const int d = second ? 1 - odd : odd;
     const double dt = gmem(I.sd)[SD_DT];
@@ -3468,58 +3470,90 @@ __global__ __launch_bounds__(256) void k_cls_sweep(const Inst* __restrict__ inst
     const double* zs = zv + (second ? I.plane : 0);
     double* out = zv + (second ? 0 : I.plane);
     double* flags = zv + 2 * I.plane;
-    const long long c = cidx(I, i, j), q = cidx(I, si, sj);
-    double cf;
-    if (second) {
-        cf = flags[c];
-    } else {
-        cf = zs[q] > 0.5 ? 1.0 : 0.0;
-        flags[c] = cf;
-    }
     const R* vel = gmem((const R*)I.vel) + (d == 0 ? 0 : I.plane);
     const int di = d == 1 ? 1 : 0, dj = d == 0 ? 1 : 0;
-    const double uc = (double)vel[q];
-    const double uL = 0.5 * ((double)vel[cidx(I, si - di, sj - dj)] + uc);
-    const double uR = 0.5 * (uc + (double)vel[cidx(I, si + di, sj + dj)]);
-    const double fL = cls_face_flux<R>(I, d, si - di, sj - dj, uL, dt, zs);
-    const double fR = cls_face_flux<R>(I, d, si, sj, uR, dt, zs);
     const double h = d == 0 ? I.dx : I.dy;
-    const double v = zs[q] - (fR - fL) + cf * ((uR - uL) * dt / h);
-    out[c] = fmin(fmax(v, 0.0), 1.0);
+    bool fu;
+    const int sj = src_col(I, j, fu);
+#pragma unroll
+    for (int r = 0; r < CLS_ROWS; ++r) {
+        const int i = w.y * (4 * CLS_ROWS) + (threadIdx.x >> 6) + 4 * r;
+        if (i >= I.H) break;
+        bool fv;
+        const int si = src_row(I, i, fv);
+        const long long c = cidx(I, i, j), q = cidx(I, si, sj);
+        double cf;
+        if (second) {
+            cf = flags[c];
+        } else {
+            cf = zs[q] > 0.5 ? 1.0 : 0.0;
+            flags[c] = cf;
+        }
+        const double uc = (double)vel[q];
+        const double uL = 0.5 * ((double)vel[cidx(I, si - di, sj - dj)] + uc);
+        const double uR = 0.5 * (uc + (double)vel[cidx(I, si + di, sj + dj)]);
+        const double fL = cls_face_flux<R>(I, d, si - di, sj - dj, uL, dt, zs);
+        const double fR = cls_face_flux<R>(I, d, si, sj, uR, dt, zs);
+        const double v = zs[q] - (fR - fL) + cf * ((uR - uL) * dt / h);
+        out[c] = fmin(fmax(v, 0.0), 1.0);
+    }
 }
 
 // the coupling: phi at a cell takes the PLIC line of its z along phi's unit
 // normal where the two disagree (a mixed cell always; a pure one where phi's
 // line cuts it or has the other sign: phi keeps its magnitude but at least
 // half the cell's extent along n, with the sign of z's fluid).  Written to
-// phiA (the copy follows).  work: 64x4 tiles over the padded array
+// phiA (the copy follows).  work: 64x16 tiles over the padded array
 template <typename R>
 __global__ __launch_bounds__(256) void k_clsvof_fix(const Inst* __restrict__ insts, const int4* __restrict__ work) {
 #pragma clang fp contract(off)
     const int4 w = work[blockIdx.x];
     const Inst& I = insts[w.x];
     if (!gmem(I.si)[SI_ACTIVE]) return;
-    const int i = w.y * 4 + (threadIdx.x >> 6);
     const int j = w.z * 64 + (threadIdx.x & 63);
-    if (i >= I.H || j >= I.W) return;
-    bool fv, fu;
-    const int si = src_row(I, i, fv), sj = src_col(I, j, fu);
-    const long long q = cidx(I, si, sj);
-    double nx, ny;
-    unit_normal<R>(I, si, sj, nx, ny);
-    const double zc = gmem(I.zv)[q], p = (double)ld1<R>(I.phi, q);
-    double out = p;
-    if (zc > CLS_ZEPS && zc < 1.0 - CLS_ZEPS) {
-        out = plic_line(zc, nx, ny, I.dx, I.dy);
-    } else {
-        const double zp = zc >= 0.5 ? 1.0 : 0.0;
-        if (area_fraction(p, nx, ny, I.dx, I.dy) != zp) {
-            const double half = 0.5 * (fabs(nx) * I.dx + fabs(ny) * I.dy);
-            const double mag = fmax(fabs(p), half);
-            out = zp == 1.0 ? -mag : mag;
+    if (j >= I.W) return;
+    bool fu;
+    const int sj = src_col(I, j, fu);
+#pragma unroll
+    for (int r = 0; r < CLS_ROWS; ++r) {
+        const int i = w.y * (4 * CLS_ROWS) + (threadIdx.x >> 6) + 4 * r;
+        if (i >= I.H) break;
+        bool fv;
+        const int si = src_row(I, i, fv);
+        const long long q = cidx(I, si, sj);
+        double nx, ny;
+        unit_normal<R>(I, si, sj, nx, ny);
+        const double zc = gmem(I.zv)[q], p = (double)ld1<R>(I.phi, q);
+        double out = p;
+        if (zc > CLS_ZEPS && zc < 1.0 - CLS_ZEPS) {
+            out = plic_line(zc, nx, ny, I.dx, I.dy);
+        } else {
+            const double zp = zc >= 0.5 ? 1.0 : 0.0;
+            if (area_fraction(p, nx, ny, I.dx, I.dy) != zp) {
+                const double half = 0.5 * (fabs(nx) * I.dx + fabs(ny) * I.dy);
+                const double mag = fmax(fabs(p), half);
+                out = zp == 1.0 ? -mag : mag;
+            }
         }
+        gmem((R*)I.phiA)[cidx(I, i, j)] = (R)out;
     }
-    gmem((R*)I.phiA)[cidx(I, i, j)] = (R)out;
+}
+
+// phi = phiA after the coupling (k_ls_copy's work on 64x16 tiles)
+template <typename R>
+__global__ __launch_bounds__(256) void k_cls_copy(const Inst* __restrict__ insts, const int4* __restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const int j = w.z * 64 + (threadIdx.x & 63);
+    if (j >= I.W) return;
+#pragma unroll
+    for (int r = 0; r < CLS_ROWS; ++r) {
+        const int i = w.y * (4 * CLS_ROWS) + (threadIdx.x >> 6) + 4 * r;
+        if (i >= I.H) break;
+        const long long c = cidx(I, i, j);
+        gmem((R*)I.phi)[c] = gmem((const R*)I.phiA)[c];
+    }
 }
 
 // initialisation(): z from the level set's PLIC reconstruction (vof_z of
@@ -3745,13 +3779,13 @@ void launch_ls(const LaunchCtx& L, const Houc& H, int every, int iters, bool cls
     // before the copy or reinitialisation from A); the coupling follows
     auto sweeps = [&]() {
         if (!clsvof) return;
-        hipLaunchKernelGGL((k_cls_sweep<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, 0);
-        hipLaunchKernelGGL((k_cls_sweep<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, 1);
+        hipLaunchKernelGGL((k_cls_sweep<R>), GRID(L.n_p16), dim3(256), 0, L.stream, L.insts, L.w_p16, 0);
+        hipLaunchKernelGGL((k_cls_sweep<R>), GRID(L.n_p16), dim3(256), 0, L.stream, L.insts, L.w_p16, 1);
     };
     auto couple = [&]() {
         if (!clsvof) return;
-        hipLaunchKernelGGL((k_clsvof_fix<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full);
-        hipLaunchKernelGGL((k_ls_copy<R>), GRID(L.n_full), dim3(256), 0, L.stream, L.insts, L.w_full, 0);
+        hipLaunchKernelGGL((k_clsvof_fix<R>), GRID(L.n_p16), dim3(256), 0, L.stream, L.insts, L.w_p16);
+        hipLaunchKernelGGL((k_cls_copy<R>), GRID(L.n_p16), dim3(256), 0, L.stream, L.insts, L.w_p16);
     };
     if (L.tb) {
         static_assert(LTX == RTX && LTY == RTY, "the fused level-set kernel uses the reinitialisation tiles");

@@ -20,6 +20,7 @@ struct LaunchCtx {
     const int4* w_rt;    int n_rt;      // 64x32 interior tiles (temporally blocked reinitialisation)
     const int4* w_seg;   int n_seg;     // 256-cell interior row segments (dt, interfacial list)
     const int4* w_vseg;  int n_vseg;    // 256-cell row segments of the interior grown by three cells
+    const int4* w_p16;   int n_p16;     // 64x16 tiles of the padded array (CLSVOF contexts only)
                                         // (the VOF GFMs' mixed-cell list; .w = segment index)
     int tb;                             // temporally blocked kernels (no x-periodic instance)
     int mirror_y, periodic_y;           // some instance of the group has mirror / periodic y BCs
