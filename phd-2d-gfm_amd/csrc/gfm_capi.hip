@@ -743,7 +743,11 @@ int gfm_ctx_commit(gfm_ctx* c) {
         d.ntile_dt = s.Ny * ((s.Nx + 255) / 256);   // dt partials: one per 256-cell row segment
         // the VOF GFMs list the mixed cells of the interior grown by three cells
         const int lnx = c->vof() ? s.Nx + 6 : s.Nx, lny = c->vof() ? s.Ny + 6 : s.Ny;
-        d.icap = c->twofluid() ? std::min(std::min(lnx * lny, 16 * (lnx + lny) + 1024), 49152) : 1;
+        d.icap = c->twofluid() ? std::min(lnx * lny, 16 * (lnx + lny) + 1024) : 1;
+        // GFM_IFACE_LDS_ENTRIES lowers the longest list whose ordered solve keeps
+        // its flags in LDS (the tests run the global-memory form on small grids)
+        const char* il = std::getenv("GFM_IFACE_LDS_ENTRIES");
+        d.icap_lds = (il && il[0]) ? std::max(0, std::min(std::atoi(il), gfm::ICAP_LDS)) : gfm::ICAP_LDS;
         // layout of the single allocation
         const size_t P = (size_t)d.plane;
         const int nf = c->nfluid();

@@ -27,6 +27,8 @@ enum {
     SI_N = 16
 };
 enum { ST_UNPHYS = 1, ST_RS = 2, ST_IOVF = 4, ST_IROUNDS = 8, ST_ILIST = 16 };
+// interfacial entries whose flags the ordered solve holds in LDS (1 byte each)
+constexpr int ICAP_LDS = 49152;
 
 // Per-fluid constants of the mixed-precision face Riemann solver, set on the
 // host at commit: z = (g-1)/(2g), g6 = (g-1)/(g+1), a2 = 2/(g+1), ig = 1/g,
@@ -81,6 +83,7 @@ struct Inst {
     int max_steps;
     int ntile_dt;
     int icap;                  // interfacial list capacity
+    int icap_lds;              // lists up to this long keep the ordered solve's flags in LDS
     int use_bbox;              // OGFM narrows the per-fluid update boxes
     int use_slip;              // GFM_base::use_slip_BCs (GFM_modified: the ghost keeps its tangential velocity)
     double band;               // FP32 whole steps: ghost states farther than this (phi) from their

@@ -1543,7 +1543,6 @@ __global__ __launch_bounds__(256) void k_iface_mark(const Inst* __restrict__ ins
 // reads those cells' star states and every other cell's original value.
 // GFM_modified writes to separate arrays and has no such dependency.
 // ---------------------------------------------------------------------------
-constexpr int ICAP_LDS = 49152;
 
 // area fraction on the side n . (x - c) + phi <= 0 of a dx x dy cell, for the
 // line through the cell centre c with phi's value and unit normal (nx, ny)
@@ -1896,22 +1895,15 @@ template <int VAR> struct RecOf { using type = IfRec; };
 template <> struct RecOf<V_VOF> { using type = VofRec; };
 template <> struct RecOf<V_OGFM_VOF> { using type = VofRec; };
 
-template <typename R, int VAR>
-__global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ insts, const int4* __restrict__ work,
-                                                      int plain) {
+// fin[k] != 0: entry k's star state is final (LDS bytes, or for lists longer
+// than I.icap_lds pass 1's flags in I.idone, in global memory)
+template <typename R, int VAR, typename FL>
+__device__ __forceinline__ void iface_rounds_run(const Inst& I, int n, int plain, FL* fin, int (&nunk)[2],
+                                                 int& nclaim) {
     constexpr bool FAST = sizeof(R) == 4;
-    __shared__ unsigned char fin[ICAP_LDS];   // 1: star state final
-    __shared__ int nunk[2];                   // per round parity (plain rounds)
-    __shared__ int nclaim;                    // next unclaimed entry
-    const int4 w = work[blockIdx.x];
-    const Inst& I = insts[w.x];
-    if (!gmem(I.si)[SI_ACTIVE]) return;
-    const int n = min(gmem(I.si)[SI_NIFACE], I.icap);
-    if (n > ICAP_LDS) { if (threadIdx.x == 0) flag(I, ST_IOVF); return; }
     using Rec = typename RecOf<VAR>::type;
     const auto recs = gmem((const Rec*)I.irec);
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int k = tid; k < n; k += nt) fin[k] = I.idone[k] ? 1 : 0;
     if (tid == 0) nclaim = nt;
     __syncthreads();
     unsigned st = 0;
@@ -1944,7 +1936,7 @@ __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ i
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                     st |= iface_solve_pending<VAR, FAST>(I, recs[k], k);
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    __atomic_store_n(&fin[k], (unsigned char)1, __ATOMIC_RELAXED);
+                    __atomic_store_n(&fin[k], (FL)1, __ATOMIC_RELAXED);
                     k = atomicAdd(&nclaim, 1);
                     have = k < n;
                     claim();
@@ -1981,6 +1973,27 @@ __global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ i
     flag(I, st);
     if (tid == 0) gmem(I.si)[SI_ROUNDS] = rounds + 1;
 }
+
+template <typename R, int VAR>
+__global__ __launch_bounds__(512) void k_iface_rounds(const Inst* __restrict__ insts, const int4* __restrict__ work,
+                                                      int plain) {
+    __shared__ unsigned char fin[ICAP_LDS];   // 1: star state final
+    __shared__ int nunk[2];                   // per round parity (plain rounds)
+    __shared__ int nclaim;                    // next unclaimed entry
+    const int4 w = work[blockIdx.x];
+    const Inst& I = insts[w.x];
+    if (!gmem(I.si)[SI_ACTIVE]) return;
+    const int n = min(gmem(I.si)[SI_NIFACE], I.icap);
+    if (n <= I.icap_lds) {
+        for (int k = threadIdx.x; k < n; k += blockDim.x) fin[k] = I.idone[k] ? 1 : 0;
+        iface_rounds_run<R, VAR>(I, n, plain, fin, nunk, nclaim);
+    } else {
+        // a longer list keeps its flags where pass 1 wrote them (each entry's
+        // flag is written by the lane that solves it; no later kernel reads them)
+        iface_rounds_run<R, VAR>(I, n, plain, I.idone, nunk, nclaim);
+    }
+}
+
 
 // GFM_modified write-back (GFM_modified.cpp:37-47, :230-239): the grids are
 // W->U of the unextrapolated primitives with the interfacial changes; the

@@ -39,10 +39,11 @@ BCS = [(T, T, R, R), (R, R, T, T), (R, T, R, T), (T, R, T, R)]
 
 
 def make_ctx(cases, variant, precision, blocked, plain_rounds=False, groups=4, fs_tile="32x16", tb_tile="64x16",
-             persistent=False):
+             persistent=False, iface_lds=None):
     env = {"GFM_NO_TEMPORAL_BLOCKING": "0" if blocked else "1", "GFM_IFACE_PLAIN_ROUNDS": "1" if plain_rounds else "0",
            "GFM_STEP_GROUPS": str(groups), "GFM_FS_TILE": fs_tile, "GFM_TB_TILE": tb_tile,
-           "GFM_FS_PERSISTENT": "1" if persistent else "0"}
+           "GFM_FS_PERSISTENT": "1" if persistent else "0",
+           "GFM_IFACE_LDS_ENTRIES": "" if iface_lds is None else str(iface_lds)}
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
@@ -134,6 +135,26 @@ def test_iface_levels_match_plain_rounds(variant, precision):
     compare(ca, cb, len(cases))
     for k in range(len(cases)):
         assert ca.iface_info(k)[0] > 0
+
+
+@pytest.mark.parametrize("plain", [False, True])
+@pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
+@pytest.mark.parametrize("variant", [G.GFM_REAL, G.GFM_RIEMANN, G.GFM_VOFGFM])
+def test_iface_flags_in_global_memory_match_lds(variant, precision, plain):
+    """an interfacial list longer than the ordered solve's LDS flag array
+    keeps its flags in global memory (here every list: the array is cut to
+    16 entries); both schedules give the bits of the LDS form"""
+    print("CASE iface global flags variant=%d precision=%d plain=%d" % (variant, precision, plain), flush=True)
+    cases = [perturbed_bubble(300, 84, BCS[0], 5, R=0.03), perturbed_bubble(128, 40, BCS[1], 6)]
+    ca = make_ctx(cases, variant, precision, True)
+    cb = make_ctx(cases, variant, precision, True, plain_rounds=plain, iface_lds=16)
+    ca.step_graph(8)
+    cb.step_graph(8)
+    ca.sync()
+    cb.sync()
+    compare(ca, cb, len(cases))
+    for k in range(len(cases)):
+        assert ca.iface_info(k)[0] > 16
 
 
 @pytest.mark.parametrize("precision", [G.FP64, G.MIXED])
