@@ -18,6 +18,8 @@ After 100 steps:
     minus its mirror, to 1e-4 of each component's scale (of the momentum magnitude for both momenta;
     FP32 rounding is not symmetric).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -137,3 +139,42 @@ def test_rmi_fullsize_properties():
     ctx = run(cases, G.GFM_MODIFIED, init_reinit=40)
     assert check_budget(ctx, cases, [rmi_meta(c) for c in cs], "rmi") > 0
     ctx.close()
+
+
+def test_long_interface_flags_in_global_memory():
+    """A single 16384x4096 Richtmyer-Meshkov instance with a steep 16-mode interface (amplitude twice
+    the wavelength; about 65536 interfacial cells, 16 per row at 1024x256 on the host) from the
+    device initial conditions: its interfacial list is longer than the ordered solve's LDS flag array
+    (49152 entries), so that solve keeps its flags in global memory. After 20 steps, before the shock
+    (about 700 cells away) reaches the interface: no status bit, the whole list solved, and the
+    barrier-round schedule gives the same FP64 mass history bit for bit."""
+    steps = 20
+    hist = []
+    for plain in ("0", "1"):
+        print("CASE long interface rmi 16384x4096 modes 16 mixed GFM_riemann plain_rounds=%s" % plain, flush=True)
+        old = os.environ.get("GFM_IFACE_PLAIN_ROUNDS")
+        os.environ["GFM_IFACE_PLAIN_ROUNDS"] = plain
+        try:
+            ctx = G.Context(G.methods(sim_type=G.SIM_TWOFLUID, fs=G.FS_MUSCL, gfm=G.GFM_RIEMANN, ls_order=5,
+                                      reinit_every=5, reinit_iters=10, precision=G.MIXED))
+            info, eos, desc = P.rmi_ic(Nx=16384, Ny=4096, amp_over_lambda=2.0, modes=16)
+            ctx.add(info, eos, 0.6, 1e300, steps)
+            ctx.commit()
+        finally:
+            if old is None:
+                del os.environ["GFM_IFACE_PLAIN_ROUNDS"]
+            else:
+                os.environ["GFM_IFACE_PLAIN_ROUNDS"] = old
+        ctx.ic_device([(0, desc)])
+        ctx.step_graph(steps)
+        ctx.sync()
+        assert ctx.status(0) == 0, ctx.status(0)
+        n, _ = ctx.iface_info(0)
+        print("interfacial entries", n, flush=True)
+        assert n > 49152
+        assert ctx.get_time(0)[1] == steps
+        h = ctx.mass_history(0, steps)
+        assert np.isfinite(h).all() and (h[:, 1:] > 0).all()
+        hist.append(h)
+        ctx.close()
+    assert np.array_equal(hist[0], hist[1])
